@@ -1,0 +1,271 @@
+"""ela3d_amd -- MI355X-native SEISMIC_CPML 3-D isotropic elastic time loop.
+
+Python mirror of the C ABI in include/ela3d.h.  The reference program
+(seismicCPML_3d_isotropic_simpelmodel.cpp) exposes its loop only through the
+literals of main(); ``Config`` carries those same parameters (NIMX/NIMY/NIMZ,
+DELTAX/Y/Z, DELTAT, NSTEP, NPOINTS_PML, ...), ``Shot`` runs the loop on the GPU
+and returns seismograms and the total energy.  Errors raise ``Ela3dError``
+(the reference printed and exited, HOM:153-162).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field, replace
+from typing import Optional
+
+import numpy as np
+
+from ._lib import (ABI_VERSION, ConfigStruct, Ela3dError, InfoStruct, KernelStats, LayeredSpec,
+                   check, lib)
+
+__all__ = [
+    "Config", "Shot", "Ela3dError", "homogeneous_model", "layered_model", "device_count",
+    "copy_bench", "SRC_REFERENCE_SIGMA", "SRC_FORCE", "SRC_EXPLOSIVE", "SRC_FORCE3",
+    "WAVELET_GAUSSIAN", "WAVELET_DGAUSSIAN", "WAVELET_RICKER", "STORAGE_FP32", "STORAGE_FP16",
+    "STORAGE_FP64", "COMPAT_XHALF_A", "COMPAT_ZHALF_MIN", "COMPAT_GHOST", "COMPAT_REFERENCE",
+    "FIELD_NAMES",
+]
+
+SRC_REFERENCE_SIGMA, SRC_FORCE, SRC_EXPLOSIVE, SRC_FORCE3 = 0, 1, 2, 3
+WAVELET_GAUSSIAN, WAVELET_DGAUSSIAN, WAVELET_RICKER = 0, 1, 2
+STORAGE_FP32, STORAGE_FP16, STORAGE_FP64 = 0, 1, 2
+COMPAT_XHALF_A, COMPAT_ZHALF_MIN, COMPAT_GHOST, COMPAT_REFERENCE = 1, 2, 4, 7
+
+FIELD_NAMES = [
+    "vx", "vy", "vz", "sigmaxx", "sigmayy", "sigmazz", "sigmaxy", "sigmaxz", "sigmayz",
+    "memory_dvx_dx", "memory_dvy_dy", "memory_dvz_dz", "memory_dvy_dx", "memory_dvx_dy",
+    "memory_dvz_dx", "memory_dvx_dz", "memory_dvz_dy", "memory_dvy_dz",
+    "memory_dsigmaxx_dx", "memory_dsigmaxy_dy", "memory_dsigmaxz_dz", "memory_dsigmaxy_dx",
+    "memory_dsigmayy_dy", "memory_dsigmayz_dz", "memory_dsigmaxz_dx", "memory_dsigmayz_dy",
+    "memory_dsigmazz_dz",
+]
+
+
+@dataclass
+class Config:
+    """The reference's parameter block (CPU:19-78, 146-153) with 0-based source/receivers."""
+    nx: int = 101                # NIMX
+    ny: int = 101                # NIMY
+    nz: int = 101                # NIMZ
+    dx: float = 10.0             # DELTAX
+    dy: float = 10.0             # DELTAY
+    dz: float = 10.0             # DELTAZ
+    dt: float = 1.6e-3           # DELTATT
+    nstep: int = 500             # NSTEP
+    npoints_pml: int = 10        # NPOINTS_PML
+    npower: float = 2.0          # NPOWER
+    k_max_pml: float = 1.0       # K_MAX_PML
+    alpha_max_pml: float = 0.0   # ALPHA_MAX_PML, <= 0 -> pi*f0
+    rcoef: float = 1e-3          # Rcoef
+    vpml: float = 3000.0         # vpml
+    f0: float = 7.0
+    t0: float = 0.0              # <= 0 -> 1.2/f0
+    factor: float = 1e7
+    angle_force: float = 0.0     # ANGLE_FORCE [deg]
+    src_kind: int = SRC_FORCE
+    wavelet: int = WAVELET_RICKER
+    isrc: int = 50               # ISOURCE - 1
+    jsrc: int = 50               # JSOURCE - 1
+    ksrc: int = 50               # KSOURCE - 1
+    receivers: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), dtype=np.int32))
+    sample_every: int = 1        # sampgat
+    energy_every: int = 0
+    storage: int = STORAGE_FP32
+    compat_flags: int = 0
+    dirichlet: bool = True
+    fp16_rescale_every: int = 100
+    device: int = 0
+    use_graph: bool = True
+    block: int = 0
+    zseg: int = 0
+
+    @property
+    def t0_effective(self) -> float:
+        return self.t0 if self.t0 > 0 else 1.2 / self.f0
+
+    @property
+    def nrec(self) -> int:
+        return int(np.asarray(self.receivers).reshape(-1, 3).shape[0])
+
+    @property
+    def cells(self) -> int:
+        return int(self.nx) * int(self.ny) * int(self.nz)
+
+    def replace(self, **kw) -> "Config":
+        return replace(self, **kw)
+
+    @classmethod
+    def from_reference(cls, NIMX, NIMY, NIMZ, DELTAX, DELTAT, NSTEP, NPOINTS_PML=10, f0=35.0,
+                       factor=1e7, ISOURCE=None, JSOURCE=None, KSOURCE=11, ANGLE_FORCE=0.0,
+                       sampgat=2, vpml=3000.0, **kw) -> "Config":
+        """Build a Config from the reference's literal names (1-based source, CPU:20-72).
+
+        Defaults reproduce the reference program: sigma source (CPU:620-622),
+        derivative-of-Gaussian wavelet (CPU:610), all quirks (compat) on, no Dirichlet.
+        """
+        ISOURCE = (NIMX + 1) // 2 if ISOURCE is None else ISOURCE
+        JSOURCE = (NIMY + 1) // 2 if JSOURCE is None else JSOURCE
+        base = dict(nx=NIMX, ny=NIMY, nz=NIMZ, dx=DELTAX, dy=DELTAX, dz=DELTAX, dt=DELTAT,
+                    nstep=NSTEP, npoints_pml=NPOINTS_PML, f0=f0, factor=factor,
+                    isrc=ISOURCE - 1, jsrc=JSOURCE - 1, ksrc=KSOURCE - 1,
+                    angle_force=ANGLE_FORCE, sample_every=sampgat, vpml=vpml,
+                    src_kind=SRC_REFERENCE_SIGMA, wavelet=WAVELET_DGAUSSIAN,
+                    compat_flags=COMPAT_REFERENCE, dirichlet=False)
+        base.update(kw)
+        return cls(**base)
+
+    def to_struct(self):
+        """C struct + keepalive for the receiver array."""
+        rec = np.ascontiguousarray(np.asarray(self.receivers, dtype=np.int32).reshape(-1, 3))
+        s = ConfigStruct()
+        lib().ela3d_config_default(C.byref(s))
+        for name in ("nx", "ny", "nz", "nstep", "npoints_pml", "src_kind", "wavelet", "isrc",
+                     "jsrc", "ksrc", "sample_every", "energy_every", "storage",
+                     "fp16_rescale_every", "device", "block", "zseg"):
+            setattr(s, name, int(getattr(self, name)))
+        for name in ("dx", "dy", "dz", "dt", "npower", "k_max_pml", "alpha_max_pml", "rcoef",
+                     "vpml", "f0", "t0", "factor", "angle_force"):
+            setattr(s, name, float(getattr(self, name)))
+        s.compat_flags = int(self.compat_flags)
+        s.dirichlet = 1 if self.dirichlet else 0
+        s.use_graph = 1 if self.use_graph else 0
+        s.nrec = rec.shape[0]
+        s.rec = rec.ctypes.data_as(C.POINTER(C.c_int32)) if rec.size else C.POINTER(C.c_int32)()
+        return s, rec
+
+    def validate(self):
+        s, keep = self.to_struct()
+        check(lib().ela3d_validate(C.byref(s)))
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Shot:
+    """One forward shot resident on the GPU (ela3d_create ... ela3d_destroy)."""
+
+    def __init__(self, cfg: Config, cp, cs, rho):
+        self.cfg = cfg
+        shape = (cfg.nz, cfg.ny, cfg.nx)
+        self._model = [np.ascontiguousarray(np.asarray(a, dtype=np.float32)).reshape(shape)
+                       for a in (cp, cs, rho)]
+        s, keep = cfg.to_struct()
+        h = C.c_void_p()
+        check(lib().ela3d_create(C.byref(s), _fptr(self._model[0]), _fptr(self._model[1]),
+                                 _fptr(self._model[2]), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().ela3d_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self, n: int = 1):
+        check(lib().ela3d_step(self._h, int(n)))
+
+    def sync(self):
+        check(lib().ela3d_sync(self._h))
+
+    def run(self):
+        check(lib().ela3d_run(self._h))
+
+    def info(self) -> dict:
+        s = InfoStruct()
+        check(lib().ela3d_info_get(self._h, C.byref(s)))
+        d = {k: getattr(s, k) for k, _ in InfoStruct._fields_}
+        d["region_lo"] = list(s.region_lo)
+        d["region_hi"] = list(s.region_hi)
+        return d
+
+    def seismograms(self) -> np.ndarray:
+        inf = self.info()
+        out = np.zeros((inf["nsamples"], self.cfg.nrec, 3), dtype=np.float64)
+        check(lib().ela3d_seismograms(self._h, _dptr(out), out.size))
+        return out
+
+    def energy(self) -> np.ndarray:
+        inf = self.info()
+        out = np.zeros((inf["nenergy"], 2), dtype=np.float64)
+        check(lib().ela3d_energy(self._h, _dptr(out), out.size))
+        return out
+
+    def snapshot(self, fid: int) -> np.ndarray:
+        out = np.zeros((self.cfg.nz, self.cfg.ny, self.cfg.nx), dtype=np.float64)
+        check(lib().ela3d_snapshot(self._h, int(fid), _dptr(out), out.size))
+        return out
+
+    def set_field(self, fid: int, values):
+        a = np.ascontiguousarray(np.asarray(values, dtype=np.float64))
+        check(lib().ela3d_set_field(self._h, int(fid), _dptr(a), a.size))
+
+    def profile(self, nsteps: int):
+        st = (KernelStats * 8)()
+        nk = C.c_int32()
+        wall = C.c_double()
+        check(lib().ela3d_profile(self._h, int(nsteps), st, 8, C.byref(nk), C.byref(wall)))
+        ks = [dict(name=st[q].name.decode(), launches=st[q].launches, total_ms=st[q].total_ms,
+                   alg_bytes_per_launch=st[q].alg_bytes_per_launch) for q in range(nk.value)]
+        return ks, wall.value
+
+
+def device_count() -> int:
+    n = C.c_int32()
+    lib().ela3d_device_count(C.byref(n))
+    return n.value
+
+
+def copy_bench(nbytes: int, reps: int = 20, device: int = 0) -> float:
+    """ms per device-to-device copy of nbytes with the library's stream-copy kernel."""
+    ms = C.c_double()
+    check(lib().ela3d_copy_bench(int(device), int(nbytes), int(reps), C.byref(ms)))
+    return ms.value
+
+
+def homogeneous_model(nx, ny, nz, vp=3300.0, vs=3300.0 / 1.732, rho=2800.0):
+    shape = (nz, ny, nx)
+    cp = np.empty(shape, np.float32)
+    cs = np.empty(shape, np.float32)
+    rh = np.empty(shape, np.float32)
+    check(lib().ela3d_model_homogeneous(nx, ny, nz, vp, vs, rho, _fptr(cp), _fptr(cs), _fptr(rh)))
+    return cp, cs, rh
+
+
+def layered_model(nx=401, ny=401, nz=301, h=10.0, seed=None, **kw):
+    """Seeded 6-layer anticline model; returns (cp, cs, rho, vp_max)."""
+    s = LayeredSpec()
+    lib().ela3d_layered_default(C.byref(s))
+    s.nx, s.ny, s.nz, s.h = nx, ny, nz, h
+    if seed is not None:
+        s.seed = int(seed)
+    for k, v in kw.items():
+        setattr(s, k, v)
+    shape = (nz, ny, nx)
+    cp = np.empty(shape, np.float32)
+    cs = np.empty(shape, np.float32)
+    rh = np.empty(shape, np.float32)
+    vmax = C.c_double()
+    check(lib().ela3d_model_layered(C.byref(s), _fptr(cp), _fptr(cs), _fptr(rh), C.byref(vmax)))
+    return cp, cs, rh, vmax.value
+
+
+def courant_dt(h: float, vp_max: float, courant: float = 0.5) -> float:
+    """dt = courant * h / (vp_max * sqrt(3)) for the 3-D staggered scheme."""
+    return courant * h / (vp_max * math.sqrt(3.0))

@@ -1,0 +1,85 @@
+"""Compile the gfx950 library (libela3d.so) and the C++ command-line driver in-tree.
+
+Run as ``python -m ela3d_amd.build`` or through ``__graft_entry__.build()``.
+Objects go to ``ela3d_amd/_build/``; the shared library lands next to this file
+so it travels with the source tree.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libela3d.so")
+CLI = os.path.join(HERE, "bin", "ela3d_run")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+ARCH = os.environ.get("ELA3D_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# Packed FP32 (v_pk_*_f32) is disabled for all device code.
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{INCLUDE}"]
+
+LIB_SOURCES = ["core.cpp", "models.cpp", "capi.cpp", "shot.cpp", "kernels.hip", "serve.cpp",
+               "tune.cpp"]
+
+
+def _run(cmd):
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in p.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if p.returncode != 0:
+        raise RuntimeError(f"command failed ({p.returncode}): {' '.join(cmd)}\n{out}")
+    return out
+
+
+def _compile(src: str) -> str:
+    path = os.path.join(CSRC, src)
+    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, "-x", "hip", "-c", path, "-o", obj]
+    _run(cmd)
+    return obj
+
+
+def build(verbose: bool = True) -> str:
+    if not os.path.exists(HIPCC):
+        raise RuntimeError(f"hipcc not found at {HIPCC}")
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(os.path.dirname(CLI), exist_ok=True)
+    sources = [s for s in LIB_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(sources))) as ex:
+        objs = list(ex.map(_compile, sources))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    cli_src = os.path.join(CSRC, "ela3d_run.cpp")
+    if os.path.exists(cli_src):
+        _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", cli_src, "-o", CLI, f"-L{HERE}", "-lela3d",
+              f"-Wl,-rpath,$ORIGIN/.."])
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+def build_oracle(verbose: bool = True) -> str:
+    root = os.path.dirname(HERE)
+    odir = os.path.join(root, "oracle")
+    make = shutil.which("make")
+    if make:
+        _run([make, "-s", "-B", "-C", odir])
+    else:
+        _run(["gcc", "-O3", "-std=gnu99", "-fPIC", "-fopenmp", "-ffp-contract=off", "-shared", "-o",
+              os.path.join(odir, "libela3d_oracle.so"), os.path.join(odir, "ela3d_oracle.c"), "-lm"])
+    if verbose:
+        print(f"built {os.path.join(odir, 'libela3d_oracle.so')}")
+    return os.path.join(odir, "libela3d_oracle.so")
+
+
+if __name__ == "__main__":
+    build()
+    if "--oracle" in sys.argv:
+        build_oracle()

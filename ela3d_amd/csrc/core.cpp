@@ -1,0 +1,241 @@
+// core.cpp -- config validation, FP64 C-PML tables, wavelet and traffic accounting.
+#include "core.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace ela3d {
+
+static const double kPi = 3.141592653589793238462643;
+
+int storage_bytes(int storage)
+{
+    switch (storage) {
+    case ELA3D_STORAGE_FP16: return 2;
+    case ELA3D_STORAGE_FP64: return 8;
+    default: return 4;
+    }
+}
+
+static void fail(const std::string& m) { throw Error{ELA3D_E_INVALID, m}; }
+
+// C-PML profile of one axis, following CPU:142-361 in FP64.  compat quirks:
+// XHALF_A zeroes a_half on x (CPU:223); ZHALF_MIN normalises z by the y
+// thickness and tests the z-half min side against the stale y abscissa (CPU:325-357).
+PmlAxis make_axis(const ela3d_config& c, int axis)
+{
+    PmlAxis A;
+    const int n = axis == 0 ? c.nx : (axis == 1 ? c.ny : c.nz);
+    const double h = axis == 0 ? c.dx : (axis == 1 ? c.dy : c.dz);
+    A.n = n;
+    A.d.assign(n, 0.0); A.K.assign(n, 1.0); A.alpha.assign(n, 0.0); A.a.assign(n, 0.0);
+    A.b.assign(n, 1.0);
+    A.d_h.assign(n, 0.0); A.K_h.assign(n, 1.0); A.alpha_h.assign(n, 0.0); A.a_h.assign(n, 0.0);
+    A.b_h.assign(n, 1.0);
+    if (c.npoints_pml <= 0) {
+        A.lo = 0;
+        A.hi = n;
+        return A;
+    }
+    const double thick = c.npoints_pml * h;
+    const double amax = c.alpha_max_pml > 0 ? c.alpha_max_pml : kPi * c.f0;
+    const double d0 = -(c.npower + 1.0) * c.vpml * std::log(c.rcoef) / (2.0 * thick);
+    const bool zq = axis == 2 && (c.compat_flags & ELA3D_COMPAT_ZHALF_MIN);
+    const double thick_y = c.npoints_pml * c.dy;
+    const double norm_min_full = zq ? thick_y : thick;
+    const double norm_min_half = zq ? thick_y : thick;
+    const double norm_max_half = zq ? thick_y : thick;
+    const double origin_lo = thick, origin_hi = (n - 1) * h - thick;
+    auto profile = [&](double s, double& d, double& K, double& al) {
+        const double p = std::pow(s, c.npower);
+        d = d0 * p;
+        K = 1.0 + (c.k_max_pml - 1.0) * p;
+        al = amax * (1.0 - s) + 0.1 * amax;
+    };
+    for (int i = 0; i < n; ++i) {
+        const double x = h * i;
+        double ab = origin_lo - x;
+        if (ab >= 0.0) profile(ab / norm_min_full, A.d[i], A.K[i], A.alpha[i]);
+        ab = zq ? thick_y - (c.dy * (c.ny - 1) + c.dy / 2.0) : origin_lo - (x + h / 2.0);
+        if (ab >= 0.0) profile(ab / norm_min_half, A.d_h[i], A.K_h[i], A.alpha_h[i]);
+        ab = x - origin_hi;
+        if (ab >= 0.0) profile(ab / thick, A.d[i], A.K[i], A.alpha[i]);
+        ab = x + h / 2.0 - origin_hi;
+        if (ab >= 0.0) profile(ab / norm_max_half, A.d_h[i], A.K_h[i], A.alpha_h[i]);
+        A.alpha[i] = std::max(A.alpha[i], 0.0);
+        A.alpha_h[i] = std::max(A.alpha_h[i], 0.0);
+        A.b[i] = std::exp(-(A.d[i] / A.K[i] + A.alpha[i]) * c.dt);
+        A.b_h[i] = std::exp(-(A.d_h[i] / A.K_h[i] + A.alpha_h[i]) * c.dt);
+        if (std::fabs(A.d[i]) > 1e-6)
+            A.a[i] = A.d[i] * (A.b[i] - 1.0) / (A.K[i] * (A.d[i] + A.K[i] * A.alpha[i]));
+        if (std::fabs(A.d_h[i]) > 1e-6)
+            A.a_h[i] = A.d_h[i] * (A.b_h[i] - 1.0) / (A.K_h[i] * (A.d_h[i] + A.K_h[i] * A.alpha_h[i]));
+        if (axis == 0 && (c.compat_flags & ELA3D_COMPAT_XHALF_A)) A.a_h[i] = 0.0;
+    }
+    // compact region: every index with a non-zero a on either grid.  Outside it
+    // d = 0, alpha = 0, K = 1, so a = 0 and b = 1 and the memory variables stay 0.
+    int lo = 0, hi = n;
+    for (int i = 0; i < n; ++i)
+        if (A.a[i] != 0.0 || A.a_h[i] != 0.0) {
+            if (i < n / 2) lo = std::max(lo, i + 1);
+            else hi = std::min(hi, i);
+        }
+    A.lo = lo;
+    A.hi = hi;
+    return A;
+}
+
+double wavelet_value(const ela3d_config& c, double t0, int it)
+{
+    const double a = kPi * kPi * c.f0 * c.f0;
+    const double u = (it - 1) * c.dt - t0;
+    const double g = std::exp(-a * u * u);
+    switch (c.wavelet) {
+    case ELA3D_WAVELET_GAUSSIAN: return c.factor * g;                       // HOM:464
+    case ELA3D_WAVELET_DGAUSSIAN: return -c.factor * 2.0 * a * u * g;       // CPU:610
+    default: return c.factor * (1.0 - 2.0 * a * u * u) * g;                 // UNI:441
+    }
+}
+
+// cells of a memory variable's update range whose own table has a != 0
+static int64_t active_count(const std::vector<double>& a, int lo_i, int hi_i)
+{
+    int64_t n = 0;
+    for (int i = lo_i; i <= hi_i; ++i)
+        if (a[i] != 0.0) ++n;
+    return n;
+}
+
+ShotPlan make_plan(const ela3d_config& in)
+{
+    if (in.struct_size != sizeof(ela3d_config)) fail("config.struct_size does not match this library");
+    if (in.abi_version != ELA3D_ABI_VERSION) fail("config.abi_version does not match this library");
+    ShotPlan P;
+    P.cfg = in;
+    ela3d_config& c = P.cfg;
+    const int npml = c.npoints_pml;
+    if (npml < 0) fail("npoints_pml must be >= 0");
+    const int minn = 2 * npml + 3;
+    if (c.nx < std::max(3, minn) || c.ny < std::max(3, minn) || c.nz < std::max(3, minn))
+        fail("grid too small: each of nx, ny, nz must be >= max(3, 2*npoints_pml+3)");
+    const int64_t cells = int64_t(c.nx) * c.ny * c.nz;
+    if (cells > (int64_t(1) << 30)) fail("grid has more than 2^30 cells (32-bit offsets)");
+    if (!(c.dx > 0 && c.dy > 0 && c.dz > 0)) fail("dx, dy, dz must be > 0");
+    if (!(c.dt > 0)) fail("dt must be > 0");
+    if (c.nstep < 0) fail("nstep must be >= 0");
+    if (npml > 0) {
+        if (!(c.rcoef > 0 && c.rcoef < 1)) fail("rcoef must be in (0,1)");
+        if (!(c.vpml > 0)) fail("vpml must be > 0");
+        if (!(c.npower >= 0)) fail("npower must be >= 0");
+        if (!(c.k_max_pml >= 1)) fail("k_max_pml must be >= 1");
+    }
+    if (!(c.f0 > 0)) fail("f0 must be > 0");
+    if (c.src_kind < 0 || c.src_kind > 3) fail("unknown src_kind");
+    if (c.wavelet < 0 || c.wavelet > 2) fail("unknown wavelet");
+    if (c.storage < 0 || c.storage > 2) fail("unknown storage");
+    if (c.compat_flags & ~ELA3D_COMPAT_REFERENCE) fail("unknown compat flag");
+    // source: strictly inside, so every neighbour the injection reads exists
+    if (c.isrc < 1 || c.isrc > c.nx - 2 || c.jsrc < 1 || c.jsrc > c.ny - 2 || c.ksrc < 1 ||
+        c.ksrc > c.nz - 2)
+        fail("source must lie strictly inside the grid (1..n-2 on each axis)");
+    if (c.sample_every < 1) fail("sample_every must be >= 1");
+    if (c.energy_every < 0) fail("energy_every must be >= 0");
+    if (c.nrec < 0 || (c.nrec > 0 && !in.rec)) fail("nrec > 0 needs a receiver array");
+    if (c.nrec > 65536) fail("at most 65536 receivers");
+    if (c.storage == ELA3D_STORAGE_FP16 && c.fp16_rescale_every < 1)
+        fail("fp16_rescale_every must be >= 1");
+    if (c.block != 0 && c.block != 64 && c.block != 128 && c.block != 256 && c.block != 512)
+        fail("block must be 0, 64, 128, 256 or 512");
+    if (c.zseg < 0) fail("zseg must be >= 0");
+    P.rec.assign(in.rec, in.rec + 3 * size_t(c.nrec));
+    for (int r = 0; r < c.nrec; ++r) {
+        const int i = P.rec[3 * r], j = P.rec[3 * r + 1], k = P.rec[3 * r + 2];
+        if (i < 0 || i >= c.nx || j < 0 || j >= c.ny || k < 0 || k >= c.nz)
+            fail("receiver " + std::to_string(r) + " outside the grid");
+    }
+    c.rec = nullptr;
+    P.t0 = c.t0 > 0 ? c.t0 : 1.2 / c.f0;
+    for (int a = 0; a < 3; ++a) P.ax[a] = make_axis(c, a);
+    P.wavelet.resize(c.nstep);
+    for (int it = 1; it <= c.nstep; ++it) P.wavelet[it - 1] = wavelet_value(c, P.t0, it);
+    P.cells = cells;
+    P.nsamples = c.nstep / c.sample_every;
+    P.nenergy = c.energy_every > 0 ? c.nstep / c.energy_every : 0;
+    P.kmax_shear = (c.compat_flags & ELA3D_COMPAT_GHOST) ? c.nz - 1 : c.nz - 2;
+
+    // Algorithmic bytes of the two-pass schedule: every field streamed once per pass.
+    // stress: read vx,vy,vz + cp,cs,rho, RMW six sigma; velocity: read six sigma + rho,
+    // RMW vx,vy,vz.  Plus 2 accesses per (memory variable, cell) where its a != 0.
+    const double e = storage_bytes(c.storage);
+    const double mb = 4.0;  // model stays FP32
+    const PmlAxis &X = P.ax[0], &Y = P.ax[1], &Z = P.ax[2];
+    const int64_t nxy = int64_t(c.nx) * c.ny;
+    const int64_t int_cells = int64_t(Y.hi - Y.lo) * c.nx * (Z.hi - Z.lo);
+    const int64_t slab_cells = cells - int_cells;
+    const double stress_cell = 3 * e + 3 * mb + 12 * e;
+    const double vel_cell = 6 * e + mb + 6 * e;
+    // C-PML term per memory variable: ranges from CPU:378-587 translated to 0-based
+    const int nx = c.nx, ny = c.ny, nz = c.nz, ks = P.kmax_shear;
+    struct V { const std::vector<double>* a; int axis; int i0, i1, j0, j1, k0, k1; int pass; };
+    const V vars[18] = {
+        {&X.a_h, 0, 0, nx - 2, 1, ny - 1, 1, nz - 1, 0}, {&Y.a, 1, 0, nx - 2, 1, ny - 1, 1, nz - 1, 0},
+        {&Z.a, 2, 0, nx - 2, 1, ny - 1, 1, nz - 1, 0},   {&X.a, 0, 1, nx - 1, 0, ny - 2, 0, nz - 1, 0},
+        {&Y.a_h, 1, 1, nx - 1, 0, ny - 2, 0, nz - 1, 0}, {&X.a, 0, 1, nx - 1, 0, ny - 1, 0, ks, 0},
+        {&Z.a_h, 2, 1, nx - 1, 0, ny - 1, 0, ks, 0},     {&Y.a_h, 1, 0, nx - 1, 0, ny - 2, 0, ks, 0},
+        {&Z.a_h, 2, 0, nx - 1, 0, ny - 2, 0, ks, 0},
+        {&X.a, 0, 1, nx - 1, 1, ny - 1, 1, nz - 1, 1},   {&Y.a, 1, 1, nx - 1, 1, ny - 1, 1, nz - 1, 1},
+        {&Z.a, 2, 1, nx - 1, 1, ny - 1, 1, nz - 1, 1},   {&X.a_h, 0, 0, nx - 2, 0, ny - 2, 1, nz - 1, 1},
+        {&Y.a_h, 1, 0, nx - 2, 0, ny - 2, 1, nz - 1, 1}, {&Z.a, 2, 0, nx - 2, 0, ny - 2, 1, nz - 1, 1},
+        {&X.a_h, 0, 0, nx - 2, 1, ny - 1, 0, nz - 2, 1}, {&Y.a, 1, 0, nx - 2, 1, ny - 1, 0, nz - 2, 1},
+        {&Z.a_h, 2, 0, nx - 2, 1, ny - 1, 0, nz - 2, 1},
+    };
+    double pml_int[2] = {0, 0}, pml_slab[2] = {0, 0};
+    for (const V& v : vars) {
+        // count active cells; split between the interior box (y,z inside [lo,hi)) and slabs
+        const int64_t ni = v.i1 - v.i0 + 1, nj = v.j1 - v.j0 + 1, nk = v.k1 - v.k0 + 1;
+        int64_t tot = 0, inside = 0;
+        if (v.axis == 0) {
+            const int64_t act = active_count(*v.a, v.i0, v.i1);
+            tot = act * nj * nk;
+            const int64_t jj = std::max(0, std::min(v.j1, Y.hi - 1) - std::max(v.j0, Y.lo) + 1);
+            const int64_t kk = std::max(0, std::min(v.k1, Z.hi - 1) - std::max(v.k0, Z.lo) + 1);
+            inside = act * jj * kk;
+        } else if (v.axis == 1) {
+            tot = active_count(*v.a, v.j0, v.j1) * ni * nk;
+        } else {
+            tot = active_count(*v.a, v.k0, v.k1) * ni * nj;
+        }
+        pml_int[v.pass] += 2 * e * inside;
+        pml_slab[v.pass] += 2 * e * (tot - inside);
+    }
+    (void)nxy;
+    P.alg_stress_interior = stress_cell * int_cells + pml_int[0];
+    P.alg_stress_slab = stress_cell * slab_cells + pml_slab[0];
+    P.alg_velocity_interior = vel_cell * int_cells + pml_int[1];
+    P.alg_velocity_slab = vel_cell * slab_cells + pml_slab[1];
+    P.alg_pml = pml_int[0] + pml_int[1] + pml_slab[0] + pml_slab[1];
+    return P;
+}
+
+int64_t device_bytes_estimate(const ela3d_config& c, const ShotPlan* plan)
+{
+    const int64_t cells = int64_t(c.nx) * c.ny * c.nz;
+    const int e = storage_bytes(c.storage);
+    int64_t mem = 0;
+    if (plan) {
+        const PmlAxis &X = plan->ax[0], &Y = plan->ax[1], &Z = plan->ax[2];
+        mem = 6 * (int64_t(c.nz) * c.ny * X.width() + int64_t(c.nz) * Y.width() * c.nx +
+                   int64_t(Z.width()) * c.ny * c.nx);
+    } else {
+        const int w = 2 * c.npoints_pml + 2;
+        mem = 6 * (int64_t(c.nz) * c.ny * w + int64_t(c.nz) * w * c.nx + int64_t(w) * c.ny * c.nx);
+    }
+    const int64_t tables = 6 * 8 * int64_t(c.nx + c.ny + c.nz);
+    const int64_t samples = int64_t(c.nstep / std::max(1, c.sample_every)) * c.nrec * 3 * 8;
+    const int64_t energy = c.energy_every > 0 ? int64_t(c.nstep / c.energy_every) * 16 : 0;
+    return cells * (9 * e + 3 * 4) + mem * e + tables + samples + energy + 8 * int64_t(c.nstep) +
+           (int64_t(4) << 20);
+}
+
+}  // namespace ela3d

@@ -1,0 +1,60 @@
+// core.hpp -- host-side shot description shared by the runtime, the server and the CLI.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ela3d.h"
+
+namespace ela3d {
+
+// Error carried out of internal code; converted to a status code at the C ABI.
+struct Error {
+    int code;
+    std::string msg;
+};
+
+void set_last_error(const std::string& msg);
+
+// One axis of C-PML profile tables (CPU:158-361), FP64.  Index i is the 0-based
+// grid index; "half" tables are evaluated at x = h*i + h/2.
+struct PmlAxis {
+    int n = 0;
+    std::vector<double> d, K, alpha, a, b;
+    std::vector<double> d_h, K_h, alpha_h, a_h, b_h;
+    // compact region holding every index where a or a_h is non-zero: [0,lo) u [hi,n)
+    int lo = 0, hi = 0;
+    int width() const { return lo + (n - hi); }
+    // compact index or -1
+    int compact(int i) const { return i < lo ? i : (i >= hi ? lo + (i - hi) : -1); }
+};
+
+struct ShotPlan {
+    ela3d_config cfg{};             // validated copy (rec pointer cleared)
+    std::vector<int32_t> rec;       // [nrec][3]
+    PmlAxis ax[3];
+    std::vector<double> wavelet;    // s(t) at t=(it-1)dt, it=1..nstep (FP64)
+    int64_t cells = 0;
+    int nsamples = 0, nenergy = 0;
+    int kmax_shear = 0;             // nz-2, or nz-1 with the ghost quirk
+    double t0 = 0.0;
+    // algorithmic traffic (bytes) of the two-pass schedule
+    double alg_stress_interior = 0, alg_stress_slab = 0;
+    double alg_velocity_interior = 0, alg_velocity_slab = 0;
+    double alg_pml = 0;
+    double alg_step() const {
+        return alg_stress_interior + alg_stress_slab + alg_velocity_interior + alg_velocity_slab;
+    }
+};
+
+int storage_bytes(int storage);
+// Validate and expand a config; throws Error.
+ShotPlan make_plan(const ela3d_config& cfg);
+// Tables of one axis (0 x, 1 y, 2 z) for a validated config.
+PmlAxis make_axis(const ela3d_config& cfg, int axis);
+double wavelet_value(const ela3d_config& cfg, double t0, int it);
+// Device bytes a shot of this config holds (used by admission control).
+int64_t device_bytes_estimate(const ela3d_config& cfg, const ShotPlan* plan);
+
+}  // namespace ela3d

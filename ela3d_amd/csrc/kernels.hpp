@@ -1,0 +1,76 @@
+// kernels.hpp -- argument blocks of the gfx950 stencil kernels (host and device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+namespace ela3d {
+
+typedef _Float16 half_t;
+
+// A launch covers up to kMaxBox boxes.  A box is a flat xy range [p0,p1)
+// (p = i + j*nx) times a z range [k0,k1); its blocks march z in segments of
+// zseg planes, one thread per xy position.
+constexpr int kMaxBox = 4;
+struct Box {
+    int p0, p1, k0, k1;
+    int nchunk;  // ceil((p1-p0)/block)
+    int blk0;    // first logical block of this box
+};
+
+// launch modes
+constexpr int kInterior = 0;  // y and z outside the C-PML: only x is checked per lane
+constexpr int kSlab = 1;      // y or z inside the C-PML: every axis checked
+
+template <typename T>
+struct StepArgs {
+    T* f[9];           // vx vy vz sxx syy szz sxy sxz syz, x-fastest nx*ny*nz
+    T* m[18];          // memory variables, compact per axis family (see shot.cpp)
+    const float* cp;
+    const float* cs;
+    const float* rho;
+    const double* tab[3];  // per axis [6][n]: a, b, K (full grid), a, b, K (half grid)
+    int nx, ny, nz, nxy;
+    int lo[3], hi[3], w[3];  // compact C-PML regions [0,lo) u [hi,n), width w
+    int kmax_shear;          // last plane of the sxz/syz update
+    int dirichlet;
+    int emargin;             // energy box [emargin, n-1-emargin) per axis
+    double dt, idx, idy, idz;
+    float sc[27], isc[27];   // FP16 storage scales (stored = value*sc)
+    double* epart;           // energy partials [nblocks][2]
+    Box box[kMaxBox];
+    int nbox, zseg, nblocks, block;
+};
+
+template <typename T>
+struct TailArgs {
+    T* f[9];
+    const float* cp;
+    const float* rho;
+    const int* rec;
+    int nrec;
+    double* seis;
+    int nsamples, sample_every;
+    const double* epart;
+    int nepart;
+    double* energy;
+    int nenergy, energy_every;
+    double evol;             // cell volume
+    const double* wavelet;
+    int nwavelet;
+    int src_kind, src, nx, nxy;
+    double sin_t, cos_t, dt;
+    int* step;               // steps completed (device counter)
+    int* flag;               // set to 1 when a sample or the energy is not finite
+    float sc[9], isc[9];
+};
+
+template <typename T>
+void launch_stress(const StepArgs<T>& a, int mode, hipStream_t s);
+template <typename T>
+void launch_velocity(const StepArgs<T>& a, int mode, bool energy, hipStream_t s);
+template <typename T>
+void launch_tail(const TailArgs<T>& a, hipStream_t s);
+void launch_copy(const float4* src, float4* dst, long n4, hipStream_t s);
+
+}  // namespace ela3d

@@ -1,0 +1,526 @@
+// shot.cpp -- device-resident shot: buffers, launch geometry, step replay, readback.
+//
+// HBM layout of one shot (x-fastest, 0-based, offset = i + j*nx + k*nx*ny):
+//   9 wavefields  [nz][ny][nx]  storage type T
+//   3 model arrays cp, cs, rho  [nz][ny][nx]  FP32
+//   18 C-PML memory variables in compact slabs holding only the absorbing layers:
+//     x family (d/dx: ids 9,12,14,18,21,24)  [nz][ny][wx]
+//     y family (d/dy: ids 10,13,16,19,22,25) [nz][wy][nx]
+//     z family (d/dz: ids 11,15,17,20,23,26) [wz][ny][nx]
+//   where an axis of n points keeps indices [0,lo) u [hi,n) (width w = lo+n-hi).
+//   FP64: C-PML a/b/K tables [3 axes][6][n], wavelet [nstep], seismograms
+//   [nsamples][nrec][3], energy [nenergy][2], energy partials.
+#include "shot.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+namespace ela3d {
+
+// axis family of each memory variable (index 0..17 -> field id 9..26)
+static const int kMemAxis[18] = {0, 1, 2, 0, 1, 0, 2, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2};
+
+Tuning choose_tuning(const ShotPlan& p)
+{
+    Tuning t;
+    const ela3d_config& c = p.cfg;
+    t.block = c.block > 0 ? c.block : 256;
+    if (c.zseg > 0) {
+        t.zseg_interior = c.zseg;
+        t.zseg_slab = c.zseg;
+        return t;
+    }
+    // enough workgroups to fill 256 CUs several times over, while keeping the
+    // z-segment long enough that the extra plane read at its start stays small
+    const int64_t nchunk = (int64_t(c.nx) * (p.ax[1].hi - p.ax[1].lo) + t.block - 1) / t.block;
+    const int nzi = std::max(1, p.ax[2].hi - p.ax[2].lo);
+    int zs = 32;
+    while (zs > 4 && nchunk * ((nzi + zs - 1) / zs) < 4096) zs /= 2;
+    t.zseg_interior = zs;
+    t.zseg_slab = 8;
+    return t;
+}
+
+static void add_box(LaunchGeom& g, int p0, int p1, int k0, int k1)
+{
+    if (p1 <= p0 || k1 <= k0) return;
+    Box& b = g.box[g.nbox++];
+    b.p0 = p0; b.p1 = p1; b.k0 = k0; b.k1 = k1;
+    b.nchunk = (p1 - p0 + g.block - 1) / g.block;
+    b.blk0 = g.nblocks;
+    g.nblocks += b.nchunk * ((k1 - k0 + g.zseg - 1) / g.zseg);
+}
+
+void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& in, LaunchGeom& sl)
+{
+    const ela3d_config& c = p.cfg;
+    const int nx = c.nx, nxy = c.nx * c.ny;
+    const int ylo = p.ax[1].lo, yhi = p.ax[1].hi, zlo = p.ax[2].lo, zhi = p.ax[2].hi;
+    in = LaunchGeom();
+    sl = LaunchGeom();
+    in.block = sl.block = t.block;
+    in.zseg = t.zseg_interior;
+    sl.zseg = t.zseg_slab;
+    add_box(in, ylo * nx, yhi * nx, zlo, zhi);
+    add_box(sl, 0, nxy, 0, zlo);                 // z-min absorbing layer
+    add_box(sl, 0, nxy, zhi, c.nz);              // z-max absorbing layer
+    add_box(sl, 0, ylo * nx, zlo, zhi);          // y-min absorbing layer
+    add_box(sl, yhi * nx, nxy, zlo, zhi);        // y-max absorbing layer
+}
+
+template <typename T>
+class ShotImpl final : public Shot {
+public:
+    ShotImpl(ShotPlan plan, const float* cp, const float* cs, const float* rho, hipStream_t ext)
+    {
+        plan_ = std::move(plan);
+        const ela3d_config& c = plan_.cfg;
+        HIP_CHECK(hipSetDevice(c.device));
+        device_ = c.device;
+        if (ext) {
+            stream_ = ext;
+            own_stream_ = false;
+        } else {
+            HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+            own_stream_ = true;
+        }
+        tune_ = choose_tuning(plan_);
+        geometry(plan_, tune_, gin_, gsl_);
+        allocate();
+        upload(cp, cs, rho);
+        for (int q = 0; q < 27; ++q) {
+            sc_[q] = 1.0f;
+            isc_[q] = 1.0f;
+        }
+    }
+
+    ~ShotImpl() override
+    {
+        hipSetDevice(device_);
+        if (stream_) hipStreamSynchronize(stream_);
+        for (auto& g : graphs_)
+            if (g) hipGraphExecDestroy(g);
+        if (arena_) hipFree(arena_);
+        if (own_stream_ && stream_) hipStreamDestroy(stream_);
+    }
+
+    hipStream_t stream() const override { return stream_; }
+    int steps_done() const override { return host_step_; }
+    int64_t device_bytes() const override { return arena_bytes_; }
+
+    void enqueue_steps(int n) override
+    {
+        HIP_CHECK(hipSetDevice(device_));
+        for (int s = 0; s < n; ++s) {
+            const int it = host_step_ + 1;
+            const bool e = energy_step(it);
+            if (plan_.cfg.use_graph) {
+                hipGraphExec_t& g = graphs_[e ? 1 : 0];
+                if (!g) g = capture(e);
+                HIP_CHECK(hipGraphLaunch(g, stream_));
+            } else {
+                launch_step(e, nullptr, 0);
+            }
+            host_step_ = it;
+        }
+        HIP_CHECK(hipGetLastError());
+    }
+
+    int sync() override
+    {
+        HIP_CHECK(hipSetDevice(device_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        int flag = 0;
+        HIP_CHECK(hipMemcpy(&flag, d_flag_, sizeof(int), hipMemcpyDeviceToHost));
+        return flag ? ELA3D_E_UNSTABLE : ELA3D_OK;
+    }
+
+    void seismograms(double* out) override
+    {
+        HIP_CHECK(hipSetDevice(device_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        const size_t n = size_t(plan_.nsamples) * plan_.cfg.nrec * 3;
+        if (n) HIP_CHECK(hipMemcpy(out, d_seis_, n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+
+    void energy(double* out) override
+    {
+        HIP_CHECK(hipSetDevice(device_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        const size_t n = size_t(plan_.nenergy) * 2;
+        if (n) HIP_CHECK(hipMemcpy(out, d_energy_, n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+
+    void snapshot(int field, double* out) override
+    {
+        HIP_CHECK(hipSetDevice(device_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        const ela3d_config& c = plan_.cfg;
+        const size_t cells = size_t(plan_.cells);
+        if (field < 9) {
+            std::vector<T> h(cells);
+            HIP_CHECK(hipMemcpy(h.data(), d_f_[field], cells * sizeof(T), hipMemcpyDeviceToHost));
+            const double inv = 1.0 / sc_[field];
+            for (size_t q = 0; q < cells; ++q) out[q] = double(h[q]) * inv;
+            return;
+        }
+        const int m = field - 9, axis = kMemAxis[m];
+        const size_t nm = mem_count(axis);
+        std::vector<T> h(nm);
+        if (nm) HIP_CHECK(hipMemcpy(h.data(), d_m_[m], nm * sizeof(T), hipMemcpyDeviceToHost));
+        std::fill(out, out + cells, 0.0);
+        const double inv = 1.0 / sc_[field];
+        for_each_compact(axis, [&](size_t full, size_t comp) { out[full] = double(h[comp]) * inv; });
+        (void)c;
+    }
+
+    void set_field(int field, const double* in) override
+    {
+        HIP_CHECK(hipSetDevice(device_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        const size_t cells = size_t(plan_.cells);
+        if (field < 9) {
+            std::vector<T> h(cells);
+            for (size_t q = 0; q < cells; ++q) h[q] = T(in[q] * sc_[field]);
+            HIP_CHECK(hipMemcpy(d_f_[field], h.data(), cells * sizeof(T), hipMemcpyHostToDevice));
+            return;
+        }
+        const int m = field - 9, axis = kMemAxis[m];
+        const size_t nm = mem_count(axis);
+        std::vector<T> h(nm);
+        std::vector<char> covered(cells, 0);
+        for_each_compact(axis, [&](size_t full, size_t comp) {
+            h[comp] = T(in[full] * sc_[field]);
+            covered[full] = 1;
+        });
+        for (size_t q = 0; q < cells; ++q)
+            if (!covered[q] && in[q] != 0.0)
+                throw Error{ELA3D_E_INVALID,
+                            "memory variables are stored only inside the C-PML layers; "
+                            "non-zero value outside them"};
+        if (nm) HIP_CHECK(hipMemcpy(d_m_[m], h.data(), nm * sizeof(T), hipMemcpyHostToDevice));
+    }
+
+    void profile(int n, std::vector<ela3d_kernel_stats>& st, double& wall_ms) override
+    {
+        HIP_CHECK(hipSetDevice(device_));
+        static const char* names[5] = {"stress_interior", "stress_slab", "velocity_interior",
+                                       "velocity_slab", "tail"};
+        st.assign(5, ela3d_kernel_stats());
+        for (int q = 0; q < 5; ++q) {
+            std::snprintf(st[q].name, sizeof(st[q].name), "%s", names[q]);
+            st[q].launches = 0;
+            st[q].total_ms = 0.0;
+        }
+        st[0].alg_bytes_per_launch = plan_.alg_stress_interior;
+        st[1].alg_bytes_per_launch = plan_.alg_stress_slab;
+        st[2].alg_bytes_per_launch = plan_.alg_velocity_interior;
+        st[3].alg_bytes_per_launch = plan_.alg_velocity_slab;
+        st[4].alg_bytes_per_launch = 0;
+        const int chunk = 64;
+        std::vector<hipEvent_t> ev(size_t(chunk) * 6);
+        for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+        hipEvent_t t0, t1;
+        HIP_CHECK(hipEventCreate(&t0));
+        HIP_CHECK(hipEventCreate(&t1));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        HIP_CHECK(hipEventRecord(t0, stream_));
+        int done = 0;
+        while (done < n) {
+            const int m = std::min(chunk, n - done);
+            for (int s = 0; s < m; ++s) {
+                const int it = host_step_ + 1;
+                launch_step(energy_step(it), &ev[size_t(s) * 6], 1);
+                host_step_ = it;
+            }
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipEventSynchronize(ev[size_t(m - 1) * 6 + 5]));
+            for (int s = 0; s < m; ++s)
+                for (int q = 0; q < 5; ++q) {
+                    float ms = 0.f;
+                    HIP_CHECK(hipEventElapsedTime(&ms, ev[size_t(s) * 6 + q], ev[size_t(s) * 6 + q + 1]));
+                    st[q].total_ms += ms;
+                    st[q].launches += 1;
+                }
+            done += m;
+        }
+        HIP_CHECK(hipEventRecord(t1, stream_));
+        HIP_CHECK(hipEventSynchronize(t1));
+        float w = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&w, t0, t1));
+        wall_ms = w;
+        for (auto& e : ev) hipEventDestroy(e);
+        hipEventDestroy(t0);
+        hipEventDestroy(t1);
+    }
+
+    void info(ela3d_info& o) override
+    {
+        std::memset(&o, 0, sizeof(o));
+        o.cells = plan_.cells;
+        o.device_bytes = arena_bytes_;
+        o.alg_bytes_per_step = plan_.alg_step();
+        o.alg_bytes_stress_interior = plan_.alg_stress_interior;
+        o.alg_bytes_velocity_interior = plan_.alg_velocity_interior;
+        o.pml_bytes_per_step = plan_.alg_pml;
+        o.nsamples = plan_.nsamples;
+        o.nenergy = plan_.nenergy;
+        o.steps_done = host_step_;
+        o.block = tune_.block;
+        o.zseg = tune_.zseg_interior;
+        for (int a = 0; a < 3; ++a) {
+            o.region_lo[a] = plan_.ax[a].lo;
+            o.region_hi[a] = plan_.ax[a].hi;
+        }
+    }
+
+private:
+    bool energy_step(int it) const
+    {
+        const int ee = plan_.cfg.energy_every;
+        return ee > 0 && it % ee == 0 && (it / ee) <= plan_.nenergy;
+    }
+
+    size_t mem_count(int axis) const
+    {
+        const ela3d_config& c = plan_.cfg;
+        const PmlAxis& A = plan_.ax[axis];
+        if (axis == 0) return size_t(c.nz) * c.ny * A.width();
+        if (axis == 1) return size_t(c.nz) * A.width() * c.nx;
+        return size_t(A.width()) * c.ny * c.nx;
+    }
+
+    template <typename F>
+    void for_each_compact(int axis, F fn) const
+    {
+        const ela3d_config& c = plan_.cfg;
+        const PmlAxis& A = plan_.ax[axis];
+        const int w = A.width();
+        for (int k = 0; k < c.nz; ++k)
+            for (int j = 0; j < c.ny; ++j)
+                for (int i = 0; i < c.nx; ++i) {
+                    size_t comp;
+                    if (axis == 0) {
+                        const int q = A.compact(i);
+                        if (q < 0) continue;
+                        comp = (size_t(k) * c.ny + j) * w + q;
+                    } else if (axis == 1) {
+                        const int q = A.compact(j);
+                        if (q < 0) continue;
+                        comp = (size_t(k) * w + q) * c.nx + i;
+                    } else {
+                        const int q = A.compact(k);
+                        if (q < 0) continue;
+                        comp = (size_t(q) * c.ny + j) * c.nx + i;
+                    }
+                    fn(size_t(i) + size_t(j) * c.nx + size_t(k) * c.nx * c.ny, comp);
+                }
+    }
+
+    void allocate()
+    {
+        const ela3d_config& c = plan_.cfg;
+        const size_t cells = size_t(plan_.cells);
+        auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+        size_t off = 0;
+        size_t o_f[9], o_m[18], o_model[3], o_tab[3];
+        for (int q = 0; q < 9; ++q) { o_f[q] = off; off += al(cells * sizeof(T)); }
+        for (int q = 0; q < 18; ++q) { o_m[q] = off; off += al(mem_count(kMemAxis[q]) * sizeof(T)); }
+        for (int q = 0; q < 3; ++q) { o_model[q] = off; off += al(cells * sizeof(float)); }
+        const int ns[3] = {c.nx, c.ny, c.nz};
+        for (int q = 0; q < 3; ++q) { o_tab[q] = off; off += al(6 * size_t(ns[q]) * sizeof(double)); }
+        const size_t o_wav = off; off += al(std::max<size_t>(1, plan_.wavelet.size()) * sizeof(double));
+        const size_t o_rec = off; off += al(std::max<size_t>(1, plan_.rec.size()) * sizeof(int));
+        const size_t nseis = size_t(plan_.nsamples) * c.nrec * 3;
+        const size_t o_seis = off; off += al(std::max<size_t>(1, nseis) * sizeof(double));
+        const size_t o_en = off; off += al(std::max<size_t>(1, size_t(plan_.nenergy) * 2) * sizeof(double));
+        nepart_ = gin_.nblocks + gsl_.nblocks;
+        const size_t o_ep = off; off += al(size_t(std::max(1, nepart_)) * 2 * sizeof(double));
+        const size_t o_ctl = off; off += al(2 * sizeof(int));
+        arena_bytes_ = int64_t(off);
+        HIP_CHECK(hipMalloc(&arena_, off));
+        char* b = static_cast<char*>(arena_);
+        for (int q = 0; q < 9; ++q) d_f_[q] = reinterpret_cast<T*>(b + o_f[q]);
+        for (int q = 0; q < 18; ++q) d_m_[q] = reinterpret_cast<T*>(b + o_m[q]);
+        for (int q = 0; q < 3; ++q) d_model_[q] = reinterpret_cast<float*>(b + o_model[q]);
+        for (int q = 0; q < 3; ++q) d_tab_[q] = reinterpret_cast<double*>(b + o_tab[q]);
+        d_wav_ = reinterpret_cast<double*>(b + o_wav);
+        d_rec_ = reinterpret_cast<int*>(b + o_rec);
+        d_seis_ = reinterpret_cast<double*>(b + o_seis);
+        d_energy_ = reinterpret_cast<double*>(b + o_en);
+        d_epart_ = reinterpret_cast<double*>(b + o_ep);
+        d_step_ = reinterpret_cast<int*>(b + o_ctl);
+        d_flag_ = d_step_ + 1;
+        // zero wavefields, memory variables and outputs (CPU:107-140)
+        HIP_CHECK(hipMemsetAsync(arena_, 0, o_model[0], stream_));
+        HIP_CHECK(hipMemsetAsync(b + o_seis, 0, off - o_seis, stream_));
+    }
+
+    void upload(const float* cp, const float* cs, const float* rho)
+    {
+        const ela3d_config& c = plan_.cfg;
+        const size_t cells = size_t(plan_.cells);
+        const float* src[3] = {cp, cs, rho};
+        for (int q = 0; q < 3; ++q)
+            HIP_CHECK(hipMemcpyAsync(d_model_[q], src[q], cells * sizeof(float), hipMemcpyHostToDevice, stream_));
+        for (int a = 0; a < 3; ++a) {
+            const PmlAxis& A = plan_.ax[a];
+            std::vector<double> t(6 * size_t(A.n));
+            for (int i = 0; i < A.n; ++i) {
+                t[i] = A.a[i]; t[A.n + i] = A.b[i]; t[2 * A.n + i] = A.K[i];
+                t[3 * A.n + i] = A.a_h[i]; t[4 * A.n + i] = A.b_h[i]; t[5 * A.n + i] = A.K_h[i];
+            }
+            HIP_CHECK(hipMemcpyAsync(d_tab_[a], t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+            HIP_CHECK(hipStreamSynchronize(stream_));
+        }
+        if (!plan_.wavelet.empty())
+            HIP_CHECK(hipMemcpyAsync(d_wav_, plan_.wavelet.data(), plan_.wavelet.size() * sizeof(double),
+                                     hipMemcpyHostToDevice, stream_));
+        if (!plan_.rec.empty())
+            HIP_CHECK(hipMemcpyAsync(d_rec_, plan_.rec.data(), plan_.rec.size() * sizeof(int),
+                                     hipMemcpyHostToDevice, stream_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        (void)c;
+    }
+
+    StepArgs<T> step_args(const LaunchGeom& g, double* epart) const
+    {
+        const ela3d_config& c = plan_.cfg;
+        StepArgs<T> a;
+        std::memset(&a, 0, sizeof(a));
+        for (int q = 0; q < 9; ++q) a.f[q] = d_f_[q];
+        for (int q = 0; q < 18; ++q) a.m[q] = d_m_[q];
+        a.cp = d_model_[0];
+        a.cs = d_model_[1];
+        a.rho = d_model_[2];
+        for (int q = 0; q < 3; ++q) a.tab[q] = d_tab_[q];
+        a.nx = c.nx; a.ny = c.ny; a.nz = c.nz; a.nxy = c.nx * c.ny;
+        for (int q = 0; q < 3; ++q) {
+            a.lo[q] = plan_.ax[q].lo;
+            a.hi[q] = plan_.ax[q].hi;
+            a.w[q] = plan_.ax[q].width();
+        }
+        a.kmax_shear = plan_.kmax_shear;
+        a.dirichlet = c.dirichlet ? 1 : 0;
+        a.emargin = std::max(0, c.npoints_pml);
+        a.dt = c.dt;
+        a.idx = 1.0 / c.dx;
+        a.idy = 1.0 / c.dy;
+        a.idz = 1.0 / c.dz;
+        for (int q = 0; q < 27; ++q) {
+            a.sc[q] = sc_[q];
+            a.isc[q] = isc_[q];
+        }
+        a.epart = epart;
+        for (int q = 0; q < g.nbox; ++q) a.box[q] = g.box[q];
+        a.nbox = g.nbox;
+        a.zseg = g.zseg;
+        a.nblocks = g.nblocks;
+        a.block = g.block;
+        return a;
+    }
+
+    TailArgs<T> tail_args() const
+    {
+        const ela3d_config& c = plan_.cfg;
+        TailArgs<T> a;
+        std::memset(&a, 0, sizeof(a));
+        for (int q = 0; q < 9; ++q) a.f[q] = d_f_[q];
+        a.cp = d_model_[0];
+        a.rho = d_model_[2];
+        a.rec = d_rec_;
+        a.nrec = c.nrec;
+        a.seis = d_seis_;
+        a.nsamples = plan_.nsamples;
+        a.sample_every = c.sample_every;
+        a.epart = d_epart_;
+        a.nepart = nepart_;
+        a.energy = d_energy_;
+        a.nenergy = plan_.nenergy;
+        a.energy_every = c.energy_every;
+        a.evol = c.dx * c.dy * c.dz;
+        a.wavelet = d_wav_;
+        a.nwavelet = int(plan_.wavelet.size());
+        a.src_kind = c.src_kind;
+        a.nx = c.nx;
+        a.nxy = c.nx * c.ny;
+        a.src = c.isrc + c.jsrc * c.nx + c.ksrc * c.nx * c.ny;
+        const double th = c.angle_force * 3.141592653589793238462643 / 180.0;
+        a.sin_t = std::sin(th);
+        a.cos_t = std::cos(th);
+        a.dt = c.dt;
+        a.step = d_step_;
+        a.flag = d_flag_;
+        for (int q = 0; q < 9; ++q) {
+            a.sc[q] = sc_[q];
+            a.isc[q] = isc_[q];
+        }
+        return a;
+    }
+
+    // One time step: stress (interior, slabs) -> velocity (interior, slabs) -> tail.
+    // With ev != nullptr, an event is recorded before the first and after every launch.
+    void launch_step(bool energy, hipEvent_t* ev, int)
+    {
+        if (ev) HIP_CHECK(hipEventRecord(ev[0], stream_));
+        launch_stress(step_args(gin_, nullptr), kInterior, stream_);
+        if (ev) HIP_CHECK(hipEventRecord(ev[1], stream_));
+        launch_stress(step_args(gsl_, nullptr), kSlab, stream_);
+        if (ev) HIP_CHECK(hipEventRecord(ev[2], stream_));
+        launch_velocity(step_args(gin_, d_epart_), kInterior, energy, stream_);
+        if (ev) HIP_CHECK(hipEventRecord(ev[3], stream_));
+        launch_velocity(step_args(gsl_, d_epart_ + 2 * size_t(gin_.nblocks)), kSlab, energy, stream_);
+        if (ev) HIP_CHECK(hipEventRecord(ev[4], stream_));
+        launch_tail(tail_args(), stream_);
+        if (ev) HIP_CHECK(hipEventRecord(ev[5], stream_));
+    }
+
+    hipGraphExec_t capture(bool energy)
+    {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        launch_step(energy, nullptr, 0);
+        HIP_CHECK(hipStreamEndCapture(stream_, &graph));
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        HIP_CHECK(e);
+        return exec;
+    }
+
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    bool own_stream_ = true;
+    Tuning tune_;
+    LaunchGeom gin_, gsl_;
+    void* arena_ = nullptr;
+    int64_t arena_bytes_ = 0;
+    T* d_f_[9] = {};
+    T* d_m_[18] = {};
+    float* d_model_[3] = {};
+    double* d_tab_[3] = {};
+    double *d_wav_ = nullptr, *d_seis_ = nullptr, *d_energy_ = nullptr, *d_epart_ = nullptr;
+    int *d_rec_ = nullptr, *d_step_ = nullptr, *d_flag_ = nullptr;
+    int nepart_ = 0;
+    int host_step_ = 0;
+    float sc_[27], isc_[27];
+    hipGraphExec_t graphs_[2] = {nullptr, nullptr};
+};
+
+std::unique_ptr<Shot> make_shot(ShotPlan plan, const float* cp, const float* cs, const float* rho,
+                                hipStream_t ext)
+{
+    switch (plan.cfg.storage) {
+    case ELA3D_STORAGE_FP64:
+        return std::unique_ptr<Shot>(new ShotImpl<double>(std::move(plan), cp, cs, rho, ext));
+    case ELA3D_STORAGE_FP16:
+        throw Error{ELA3D_E_INVALID, "FP16 storage is not enabled in this build"};
+    default:
+        return std::unique_ptr<Shot>(new ShotImpl<float>(std::move(plan), cp, cs, rho, ext));
+    }
+}
+
+}  // namespace ela3d

@@ -1,0 +1,62 @@
+// shot.hpp -- one forward shot resident on one GPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "core.hpp"
+#include "kernels.hpp"
+
+struct ela3d_shot;  // opaque C handle, defined in shot.cpp
+
+namespace ela3d {
+
+#define HIP_CHECK(expr)                                                                   \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::ela3d::Error{e_ == hipErrorOutOfMemory ? ELA3D_E_OOM : ELA3D_E_HIP,   \
+                                 std::string(#expr) + ": " + hipGetErrorString(e_)};      \
+    } while (0)
+
+// Launch shape of one stress/velocity launch.
+struct LaunchGeom {
+    Box box[kMaxBox];
+    int nbox = 0, zseg = 1, nblocks = 0, block = 256;
+};
+
+// Launch tuning chosen for a grid (see tune.cpp).
+struct Tuning {
+    int block = 256;
+    int zseg_interior = 16;
+    int zseg_slab = 8;
+};
+
+class Shot {
+public:
+    virtual ~Shot() = default;
+    virtual void enqueue_steps(int n) = 0;          // asynchronous
+    virtual int sync() = 0;                         // ELA3D_OK or ELA3D_E_UNSTABLE
+    virtual void seismograms(double* out) = 0;      // nsamples*nrec*3
+    virtual void energy(double* out) = 0;           // nenergy*2
+    virtual void snapshot(int field, double* out) = 0;
+    virtual void set_field(int field, const double* in) = 0;
+    virtual void profile(int n, std::vector<ela3d_kernel_stats>& st, double& wall_ms) = 0;
+    virtual void info(ela3d_info& out) = 0;
+    virtual hipStream_t stream() const = 0;
+    virtual int steps_done() const = 0;
+    virtual int64_t device_bytes() const = 0;
+    const ShotPlan& plan() const { return plan_; }
+
+protected:
+    ShotPlan plan_;
+};
+
+std::unique_ptr<Shot> make_shot(ShotPlan plan, const float* cp, const float* cs, const float* rho,
+                                hipStream_t external_stream = nullptr);
+Tuning choose_tuning(const ShotPlan& p);
+void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& interior, LaunchGeom& slab);
+
+}  // namespace ela3d

@@ -1,0 +1,203 @@
+/*
+ * ela3d.h -- C ABI of the MI355X-native SEISMIC_CPML 3-D isotropic elastic
+ * velocity-stress time loop.
+ *
+ * The reference (thejatmik/ela3d) has no library API: its "surface" is the
+ * parameter block and time loop inside main() of
+ * seismicCPML_3d_isotropic_simpelmodel.cpp (tagged CPU below; the CUDA variants
+ * seismicCPML_3d_isotropic_cudazerocopy.cu = ZC, elatic_3d_unified_cuda_memory.cu
+ * = UNI, seismic_CPML_3D_isotropic_homogeneous_withCUDA.cu = HOM).  Each entry
+ * point below names the reference lines it replaces.
+ *
+ * Conventions
+ *   - Arrays are x-fastest, 0-based: offset = i + j*nx + k*nx*ny.  The reference
+ *     uses 1-based (i,j,k) with the same x-fastest order (CPU:381); subtract 1.
+ *   - All functions return ELA3D_OK (0) or a negative ELA3D_E_* code; the library
+ *     never exits the process and no C++ exception crosses this boundary.
+ *     ela3d_last_error() gives a message for the calling thread's last failure.
+ *   - A shot owns all of its device memory; model arrays are copied at create;
+ *     outputs are copied into caller buffers.  No returned pointer aliases
+ *     library memory.  One shot = one HIP stream; a shot is not thread-safe.
+ *     The server (ela3d_server_*) is thread-safe for submit/poll/wait.
+ */
+#ifndef ELA3D_H
+#define ELA3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ELA3D_ABI_VERSION 1
+
+/* ---- status codes (replace HANDLE_ERROR -> exit(EXIT_FAILURE), HOM:153-162) ---- */
+#define ELA3D_OK           0
+#define ELA3D_E_INVALID   -1  /* invalid config or argument (CPU:631-634 is the reference's only check) */
+#define ELA3D_E_OOM       -2  /* device or host allocation failed                        */
+#define ELA3D_E_UNSTABLE  -3  /* non-finite wavefield detected (e.g. dt above Courant)   */
+#define ELA3D_E_HIP       -4  /* HIP runtime error                                        */
+#define ELA3D_E_NODEVICE  -5  /* no usable GPU                                            */
+#define ELA3D_E_RANGE     -6  /* output buffer too small / index out of range            */
+#define ELA3D_E_STATE     -7  /* call not valid in the object's current state            */
+#define ELA3D_E_BUDGET    -8  /* shot can never fit the server's device-memory budget    */
+#define ELA3D_E_NOTFOUND  -9  /* unknown ticket                                           */
+#define ELA3D_E_TIMEOUT  -10  /* wait timed out                                           */
+
+/* ---- source kinds (CPU:611-622, ZC:446-459) ---- */
+#define ELA3D_SRC_REFERENCE_SIGMA 0 /* CPU:620-622: sxx,syy += fx*dt*(l+2m); szz += fy*dt*(l+2m) */
+#define ELA3D_SRC_FORCE           1 /* CPU:614-617: vx += fx*dt/rho_x; vy += fy*dt/rho_y        */
+#define ELA3D_SRC_EXPLOSIVE       2 /* sxx = syy = szz += s*dt*(l+2m)                          */
+#define ELA3D_SRC_FORCE3          3 /* ZC:451-453: vx, vy, vz += s*dt/rho_avg                  */
+
+/* ---- source time functions, t = (it-1)*dt, a = pi^2 f0^2 (CPU:608-610) ---- */
+#define ELA3D_WAVELET_GAUSSIAN  0 /* HOM:464 */
+#define ELA3D_WAVELET_DGAUSSIAN 1 /* CPU:610 */
+#define ELA3D_WAVELET_RICKER    2 /* UNI:441 */
+
+/* ---- wavefield storage ---- */
+#define ELA3D_STORAGE_FP32 0 /* FP32 fields and memory variables (default)                */
+#define ELA3D_STORAGE_FP16 1 /* FP16 storage, per-field power-of-two scale, FP32 math     */
+#define ELA3D_STORAGE_FP64 2 /* FP64 storage and math (the GPU FP64 reference run)        */
+
+/* ---- reproduce the reference CPU program's quirks (all table/loop values only) ---- */
+#define ELA3D_COMPAT_XHALF_A   1u /* CPU:223: ca_x_half is never assigned and stays 0      */
+#define ELA3D_COMPAT_ZHALF_MIN 2u /* CPU:325-336: z tables normalised by the y thickness,
+                                     z-half min side tested with the stale y abscissa     */
+#define ELA3D_COMPAT_GHOST     4u /* CPU:445: sxz/syz also updated on k0=nz-1 against a
+                                     zero ghost plane                                      */
+#define ELA3D_COMPAT_REFERENCE 7u /* all of the above: the reference CPU path as written  */
+
+/* field ids for ela3d_snapshot / ela3d_set_field */
+#define ELA3D_FIELD_VX 0
+#define ELA3D_FIELD_VY 1
+#define ELA3D_FIELD_VZ 2
+#define ELA3D_FIELD_SXX 3
+#define ELA3D_FIELD_SYY 4
+#define ELA3D_FIELD_SZZ 5
+#define ELA3D_FIELD_SXY 6
+#define ELA3D_FIELD_SXZ 7
+#define ELA3D_FIELD_SYZ 8
+/* 9..17: memory_dvx_dx, dvy_dy, dvz_dz, dvy_dx, dvx_dy, dvz_dx, dvx_dz, dvz_dy, dvy_dz
+   18..26: memory_dsigmaxx_dx, dsigmaxy_dy, dsigmaxz_dz, dsigmaxy_dx, dsigmayy_dy,
+           dsigmayz_dz, dsigmaxz_dx, dsigmayz_dy, dsigmazz_dz          (CPU:395-583) */
+#define ELA3D_NUM_FIELDS 27
+
+/* The reference's parameter block (CPU:19-78, 146-153) as one versioned struct.
+ * Fill with ela3d_config_default() first. */
+typedef struct ela3d_config {
+    uint32_t struct_size;   /* sizeof(ela3d_config) */
+    uint32_t abi_version;   /* ELA3D_ABI_VERSION */
+    int32_t nx, ny, nz;     /* NIMX, NIMY, NIMZ (CPU:20-23) */
+    double dx, dy, dz;      /* DELTAX, DELTAY, DELTAZ [m] (CPU:35-37) */
+    double dt;              /* DELTATT [s] (CPU:31) */
+    int32_t nstep;          /* NSTEP (CPU:30) */
+    int32_t npoints_pml;    /* NPOINTS_PML (CPU:66); 0 disables the C-PML */
+    double npower;          /* NPOWER (CPU:75), default 2 */
+    double k_max_pml;       /* K_MAX_PML (CPU:77), default 1 */
+    double alpha_max_pml;   /* ALPHA_MAX_PML; <= 0 means pi*f0 (CPU:78) */
+    double rcoef;           /* Rcoef (CPU:151), default 1e-3 */
+    double vpml;            /* vpml (CPU:153), default 3000 m/s */
+    double f0;              /* dominant frequency [Hz] (CPU:62) */
+    double t0;              /* wavelet delay [s]; <= 0 means 1.2/f0 (CPU:63) */
+    double factor;          /* amplitude (CPU:64), default 1e7 */
+    double angle_force;     /* ANGLE_FORCE [deg] (CPU:72) */
+    int32_t src_kind;       /* ELA3D_SRC_* */
+    int32_t wavelet;        /* ELA3D_WAVELET_* */
+    int32_t isrc, jsrc, ksrc; /* ISOURCE-1, JSOURCE-1, KSOURCE-1 (0-based, CPU:69-71) */
+    int32_t nrec;           /* number of receivers */
+    const int32_t* rec;     /* [nrec][3] 0-based (i,j,k); replaces the plane gather CPU:593-602 */
+    int32_t sample_every;   /* sampgat (CPU:32): seismogram sample every N steps */
+    int32_t energy_every;   /* total energy every N steps; 0 = off (no reference counterpart) */
+    int32_t storage;        /* ELA3D_STORAGE_* */
+    uint32_t compat_flags;  /* ELA3D_COMPAT_* */
+    int32_t dirichlet;      /* 1: vx=vy=vz=0 on the six faces after each velocity update */
+    int32_t fp16_rescale_every; /* FP16 storage: re-choose scales every N steps (default 100) */
+    int32_t device;         /* HIP device ordinal */
+    int32_t use_graph;      /* 1 (default): replay a hipGraph of one step */
+    int32_t block;          /* threads per workgroup, 0 = autotuned */
+    int32_t zseg;           /* z planes per workgroup, 0 = autotuned */
+    int32_t reserved[6];
+} ela3d_config;
+
+typedef struct ela3d_shot ela3d_shot;
+
+typedef struct ela3d_info {
+    int64_t cells;                 /* nx*ny*nz */
+    int64_t device_bytes;          /* device memory held by the shot */
+    double alg_bytes_per_step;     /* algorithmic HBM bytes of one step (two-pass schedule) */
+    double alg_bytes_stress_interior;   /* per launch */
+    double alg_bytes_velocity_interior; /* per launch */
+    double pml_bytes_per_step;     /* the C-PML memory-variable share of alg_bytes_per_step */
+    int32_t nsamples;              /* seismogram samples over nstep */
+    int32_t nenergy;               /* energy samples over nstep */
+    int32_t steps_done;
+    int32_t block, zseg;           /* launch shape in use */
+    int32_t region_lo[3], region_hi[3]; /* C-PML compact regions: [0,lo) u [hi,n) per axis */
+} ela3d_info;
+
+#define ELA3D_MAX_KERNELS 8
+typedef struct ela3d_kernel_stats {
+    char name[48];
+    int64_t launches;
+    double total_ms;               /* summed hipEvent durations */
+    double alg_bytes_per_launch;
+} ela3d_kernel_stats;
+
+/* ---- shot API (replaces main() CPU:17-726 / ZC:487-1506) ---- */
+void ela3d_config_default(ela3d_config* cfg);
+int ela3d_validate(const ela3d_config* cfg);
+/* Create a shot: validate, build FP64 C-PML tables (CPU:142-361) and the FP64
+ * wavelet (CPU:604-610), copy the model (CPU:41-59), zero the 27 fields (CPU:107-140). */
+int ela3d_create(const ela3d_config* cfg, const float* cp, const float* cs, const float* rho,
+                 ela3d_shot** out);
+/* Enqueue n time steps (CPU:376-622) on the shot's stream; asynchronous. */
+int ela3d_step(ela3d_shot* shot, int32_t nsteps);
+/* Wait for the shot's queued work; ELA3D_E_UNSTABLE if a non-finite value reached a receiver. */
+int ela3d_sync(ela3d_shot* shot);
+/* ela3d_step(nstep - done) + ela3d_sync */
+int ela3d_run(ela3d_shot* shot);
+/* Seismograms [nsamples][nrec][3] as FP64 (replaces rechorv*.txt, CPU:624-671). */
+int ela3d_seismograms(ela3d_shot* shot, double* out, size_t cap);
+/* Energy [nenergy][2] = (kinetic, potential) per energy step, FP64. */
+int ela3d_energy(ela3d_shot* shot, double* out, size_t cap);
+/* Full-grid copy of one field (ids above) as FP64, nx*ny*nz values (CPU:685-722 wrote vx,vy,vz). */
+int ela3d_snapshot(ela3d_shot* shot, int32_t field, double* out, size_t cap);
+/* Overwrite one field from FP64 values (rounded to the storage type). */
+int ela3d_set_field(ela3d_shot* shot, int32_t field, const double* in, size_t n);
+int ela3d_info_get(ela3d_shot* shot, ela3d_info* out);
+/* Run n steps eagerly with hipEvents around every launch; per-kernel totals. */
+int ela3d_profile(ela3d_shot* shot, int32_t nsteps, ela3d_kernel_stats* out, int32_t cap,
+                  int32_t* nkernels, double* wall_ms);
+void ela3d_destroy(ela3d_shot* shot);
+
+const char* ela3d_strerror(int code);
+const char* ela3d_last_error(void);
+int ela3d_device_count(int32_t* out);
+/* Stream-copy kernel of n bytes, timed with hipEvents (HBM ceiling calibration). */
+int ela3d_copy_bench(int32_t device, int64_t bytes, int32_t reps, double* ms_per_copy);
+
+/* ---- seeded synthetic models (CPU:41-59 homogeneous; layered per the workload spec) ---- */
+int ela3d_model_homogeneous(int32_t nx, int32_t ny, int32_t nz, float vp, float vs, float rho,
+                            float* cp, float* cs, float* rh);
+typedef struct ela3d_layered_spec {
+    int32_t nx, ny, nz;
+    double h;                 /* grid spacing [m] */
+    int32_t nlayers;          /* 6 */
+    double vp_top, vp_bottom; /* 2000, 4500 m/s, linear in layer index */
+    double noise;             /* 0.03: uniform +-3 % per cell */
+    uint64_t seed;            /* splitmix64 seed */
+    double bump_amp;          /* A = 600 m (anticline uplift) */
+    double bump_width;        /* w = 1200 m */
+    double vp_vs;             /* 1.73 */
+    double gardner_a, gardner_b; /* rho = a * vp^b: 310, 0.25 */
+} ela3d_layered_spec;
+void ela3d_layered_default(ela3d_layered_spec* s);
+int ela3d_model_layered(const ela3d_layered_spec* s, float* cp, float* cs, float* rh,
+                        double* vp_max);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ELA3D_H */
