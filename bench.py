@@ -98,7 +98,17 @@ def main():
     ap.add_argument("--workload", default="classic")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--graph", action="store_true", help="also time the hipGraph replay path")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--calibrate", type=float, default=0.0,
+                    help="only run the stream-copy kernel over this many GiB (HBM ceiling)")
     args = ap.parse_args()
+    if args.calibrate > 0:
+        import ela3d_amd
+        nbytes = int(args.calibrate * (1 << 30))
+        ms = ela3d_amd.copy_bench(nbytes, reps=20)
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                          "ms_per_copy": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the time loop "
               "shards by z-slabs across GPUs, which this build does not run.", file=sys.stderr)
@@ -111,6 +121,13 @@ def main():
     K, W = args.steps, args.warmup
     cfg, model, desc = make_workload(args.workload, K + W)
     log(f"[bench] workload {args.workload}: {desc}; steps={K} warmup={W}")
+    tuned = None
+    if not args.no_tune:
+        t = time.perf_counter()
+        tuned = ela3d_amd.autotune(cfg, model, steps=6)
+        log(f"[bench] autotune: block={tuned['block']} zseg={tuned['zseg_interior']}/"
+            f"{tuned['zseg_slab']} ({tuned['ms_per_step']:.3f} ms/step over "
+            f"{tuned['candidates']} shapes, {time.perf_counter() - t:.1f} s)")
     with Shot(cfg, *model) as shot:
         shot.step(W)
         shot.sync()
@@ -166,6 +183,7 @@ def main():
             "alg_bytes_per_step": info["alg_bytes_per_step"],
             "step_hbm_frac": round(step_bw / PEAK_HBM, 4),
             "graph_ms_per_step": graph_ms,
+            "launch_shape": {"block": info["block"], "zseg": info["zseg"]},
         },
         "roofline": {
             "bound": "hbm", "kernel": dom["name"], "achieved": round(achieved / 1e9, 2),

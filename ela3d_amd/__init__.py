@@ -138,6 +138,25 @@ class Config:
         s, keep = self.to_struct()
         check(lib().ela3d_validate(C.byref(s)))
 
+    def plan(self) -> dict:
+        """Host-side expansion of this config (no GPU): cells, C-PML regions, traffic."""
+        s, keep = self.to_struct()
+        inf = InfoStruct()
+        check(lib().ela3d_plan(C.byref(s), C.byref(inf)))
+        d = {k: getattr(inf, k) for k, _ in InfoStruct._fields_}
+        d["region_lo"] = list(inf.region_lo)
+        d["region_hi"] = list(inf.region_hi)
+        return d
+
+    def pml_tables(self, axis: int) -> dict:
+        """The library's FP64 C-PML tables of one axis (d, K, alpha, a, b; full and half)."""
+        s, keep = self.to_struct()
+        n = (self.nx, self.ny, self.nz)[axis]
+        out = np.zeros(10 * n, dtype=np.float64)
+        check(lib().ela3d_pml_tables(C.byref(s), int(axis), _dptr(out), out.size))
+        names = ["d", "K", "alpha", "a", "b", "d_half", "K_half", "alpha_half", "a_half", "b_half"]
+        return {nm: out[i * n:(i + 1) * n].copy() for i, nm in enumerate(names)}
+
 
 def _fptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_float))
@@ -224,6 +243,18 @@ class Shot:
         ks = [dict(name=st[q].name.decode(), launches=st[q].launches, total_ms=st[q].total_ms,
                    alg_bytes_per_launch=st[q].alg_bytes_per_launch) for q in range(nk.value)]
         return ks, wall.value
+
+
+def autotune(cfg: Config, model, steps: int = 8) -> dict:
+    """Pick the fastest launch shape for this grid (cached for later shots)."""
+    from ._lib import TuningStruct
+    shape = (cfg.nz, cfg.ny, cfg.nx)
+    m = [np.ascontiguousarray(np.asarray(a, dtype=np.float32)).reshape(shape) for a in model]
+    s, keep = cfg.to_struct()
+    t = TuningStruct()
+    check(lib().ela3d_autotune(C.byref(s), _fptr(m[0]), _fptr(m[1]), _fptr(m[2]), int(steps),
+                               C.byref(t)))
+    return {k: getattr(t, k) for k, _ in TuningStruct._fields_}
 
 
 def device_count() -> int:

@@ -40,8 +40,8 @@ class ConfigStruct(C.Structure):
 class InfoStruct(C.Structure):
     _fields_ = [
         ("cells", C.c_int64), ("device_bytes", C.c_int64),
-        ("alg_bytes_per_step", C.c_double), ("alg_bytes_stress_interior", C.c_double),
-        ("alg_bytes_velocity_interior", C.c_double), ("pml_bytes_per_step", C.c_double),
+        ("alg_bytes_per_step", C.c_double), ("alg_bytes_stress", C.c_double),
+        ("alg_bytes_velocity", C.c_double), ("pml_bytes_per_step", C.c_double),
         ("nsamples", C.c_int32), ("nenergy", C.c_int32), ("steps_done", C.c_int32),
         ("block", C.c_int32), ("zseg", C.c_int32),
         ("region_lo", C.c_int32 * 3), ("region_hi", C.c_int32 * 3),
@@ -51,6 +51,11 @@ class InfoStruct(C.Structure):
 class KernelStats(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_int64), ("total_ms", C.c_double),
                 ("alg_bytes_per_launch", C.c_double)]
+
+
+class TuningStruct(C.Structure):
+    _fields_ = [("block", C.c_int32), ("zseg_interior", C.c_int32), ("zseg_slab", C.c_int32),
+                ("candidates", C.c_int32), ("ms_per_step", C.c_double)]
 
 
 class LayeredSpec(C.Structure):
@@ -110,6 +115,11 @@ def lib() -> C.CDLL:
                                 P(C.c_double)]
     L.ela3d_destroy.argtypes = [C.c_void_p]
     L.ela3d_destroy.restype = None
+    L.ela3d_plan.argtypes = [P(ConfigStruct), P(InfoStruct)]
+    L.ela3d_autotune.argtypes = [P(ConfigStruct), P(f), P(f), P(f), C.c_int32, P(TuningStruct)]
+    L.ela3d_tuning_set.argtypes = [P(ConfigStruct), P(TuningStruct)]
+    L.ela3d_tuning_get.argtypes = [P(ConfigStruct), P(TuningStruct)]
+    L.ela3d_pml_tables.argtypes = [P(ConfigStruct), C.c_int32, P(C.c_double), C.c_size_t]
     L.ela3d_strerror.argtypes = [C.c_int]
     L.ela3d_strerror.restype = C.c_char_p
     L.ela3d_last_error.argtypes = []

@@ -235,6 +235,46 @@ int ela3d_profile(ela3d_shot* s, int32_t nsteps, ela3d_kernel_stats* out, int32_
 
 void ela3d_destroy(ela3d_shot* s) { delete s; }
 
+int ela3d_plan(const ela3d_config* cfg, ela3d_info* out)
+{
+    return guarded([&] {
+        if (!cfg || !out) throw Error{ELA3D_E_INVALID, "null argument"};
+        ela3d::ShotPlan p = ela3d::make_plan(*cfg);
+        std::memset(out, 0, sizeof(*out));
+        out->cells = p.cells;
+        out->device_bytes = ela3d::device_bytes_estimate(p.cfg, &p);
+        out->alg_bytes_per_step = p.alg_step();
+        out->alg_bytes_stress = p.alg_stress_interior + p.alg_stress_slab;
+        out->alg_bytes_velocity = p.alg_velocity_interior + p.alg_velocity_slab;
+        out->pml_bytes_per_step = p.alg_pml;
+        out->nsamples = p.nsamples;
+        out->nenergy = p.nenergy;
+        ela3d::Tuning t = ela3d::choose_tuning(p);
+        out->block = t.block;
+        out->zseg = t.zseg_interior;
+        for (int a = 0; a < 3; ++a) {
+            out->region_lo[a] = p.ax[a].lo;
+            out->region_hi[a] = p.ax[a].hi;
+        }
+        return ELA3D_OK;
+    });
+}
+
+int ela3d_pml_tables(const ela3d_config* cfg, int32_t axis, double* out, size_t cap)
+{
+    return guarded([&] {
+        if (!cfg || !out || axis < 0 || axis > 2) throw Error{ELA3D_E_INVALID, "bad argument"};
+        ela3d::ShotPlan p = ela3d::make_plan(*cfg);
+        const ela3d::PmlAxis& A = p.ax[axis];
+        const size_t n = size_t(A.n);
+        if (cap < 10 * n) throw Error{ELA3D_E_RANGE, "table buffer too small"};
+        const std::vector<double>* t[10] = {&A.d, &A.K, &A.alpha, &A.a, &A.b,
+                                            &A.d_h, &A.K_h, &A.alpha_h, &A.a_h, &A.b_h};
+        for (int q = 0; q < 10; ++q) std::memcpy(out + q * n, t[q]->data(), n * sizeof(double));
+        return ELA3D_OK;
+    });
+}
+
 const char* ela3d_strerror(int code)
 {
     switch (code) {

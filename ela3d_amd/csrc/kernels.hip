@@ -67,6 +67,7 @@ __device__ __forceinline__ Box pick_box(const Box* bx, int nbox, int lb)
     if (nbox > 1 && lb >= bx[1].blk0) B = bx[1];
     if (nbox > 2 && lb >= bx[2].blk0) B = bx[2];
     if (nbox > 3 && lb >= bx[3].blk0) B = bx[3];
+    if (nbox > 4 && lb >= bx[4].blk0) B = bx[4];
     return B;
 }
 
@@ -83,17 +84,14 @@ __device__ __forceinline__ R cpml(T* m, int om, R a, R b, R K, R d, float sc, fl
 // Stress pass: sigma_xx/yy/zz, sigma_xy, sigma_xz, sigma_yz + 9 memory variables
 // ---------------------------------------------------------------------------
 template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
+__device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, int local)
 {
     typedef typename Arith<T>::R R;
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const Box B = pick_box(A.box, A.nbox, lb);
-    const int local = lb - B.blk0;
     const int chunk = local % B.nchunk, zs = local / B.nchunk;
     const int p = B.p0 + chunk * A.block + (int)threadIdx.x;
     if (p >= B.p1) return;
-    const int kb = B.k0 + zs * A.zseg;
-    const int ke = min(kb + A.zseg, B.k1);
+    const int kb = B.k0 + zs * B.zseg;
+    const int ke = min(kb + B.zseg, B.k1);
     const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = A.nxy;
     const int j = p / nx, i = p - j * nx;
 
@@ -240,18 +238,16 @@ __device__ __forceinline__ double wave_sum(double v)
 // fused energy partial sums of the updated state.
 // ---------------------------------------------------------------------------
 template <typename T, int MODE, bool ENERGY>
-__global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
+__device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B, int local,
+                                              double& ek, double& ep)
 {
     typedef typename Arith<T>::R R;
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const Box B = pick_box(A.box, A.nbox, lb);
-    const int local = lb - B.blk0;
     const int chunk = local % B.nchunk, zs = local / B.nchunk;
     const int p = B.p0 + chunk * A.block + (int)threadIdx.x;
     const bool active = p < B.p1;
-    if (!ENERGY && !active) return;
-    const int kb = B.k0 + zs * A.zseg;
-    const int ke = active ? min(kb + A.zseg, B.k1) : kb;
+    if (!active) return;
+    const int kb = B.k0 + zs * B.zseg;
+    const int ke = min(kb + B.zseg, B.k1);
     const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = A.nxy;
     const int j = p / nx, i = p - j * nx;
 
@@ -285,7 +281,6 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
     }
     const int em = A.emargin;
     const bool ebox_ij = ENERGY && active && i >= em && i < nx - 1 - em && j >= em && j < ny - 1 - em;
-    double ek = 0.0, ep = 0.0;
 
     int o = kb * nxy + p;
     R sxz_m = 0, syz_m = 0, szz_c = 0, rho_c = 0;
@@ -326,12 +321,7 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
         const bool face = A.dirichlet && (face_ij || k == 0 || top);
         bool ux = false, uy = false, uz = false;
         R vx_n = 0, vy_n = 0, vz_n = 0;
-        if (face) {
-            stf(vxp, o, (R)0, A.sc[0]);
-            stf(vyp, o, (R)0, A.sc[1]);
-            stf(vzp, o, (R)0, A.sc[2]);
-            ux = uy = uz = true;
-        } else {
+        {
             if (rVX && k >= 1) {  // CPU:502-529
                 const R rh = (rho_c + (R)rhop[o - 1]) / (R)2;
                 const R DT_over_rho = dt / rh;
@@ -344,7 +334,6 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
                     if (cz >= 0) d3 = cpml(A.m[11], omz, az, bz, Kz, d3, A.sc[20], A.isc[20]);
                 }
                 vx_n = DT_over_rho * (d1 + d2 + d3) + ldf(vxp, o, A.isc[0]);
-                stf(vxp, o, vx_n, A.sc[0]);
                 ux = true;
             }
             if (rVY && k >= 1) {  // CPU:531-558
@@ -359,7 +348,6 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
                     if (cz >= 0) d3 = cpml(A.m[14], omz, az, bz, Kz, d3, A.sc[23], A.isc[23]);
                 }
                 vy_n = DT_over_rho * (d1 + d2 + d3) + ldf(vyp, o, A.isc[1]);
-                stf(vyp, o, vy_n, A.sc[1]);
                 uy = true;
             }
             if (rVZ && k <= nz - 2) {  // CPU:562-590
@@ -374,10 +362,17 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
                     if (cz >= 0) d3 = cpml(A.m[17], omz, azh, bzh, Kzh, d3, A.sc[26], A.isc[26]);
                 }
                 vz_n = DT_over_rho * (d1 + d2 + d3) + ldf(vzp, o, A.isc[2]);
-                stf(vzp, o, vz_n, A.sc[2]);
                 uz = true;
             }
         }
+        // rigid faces: the memory variables above are still advanced, then v = 0
+        if (face) {
+            vx_n = vy_n = vz_n = (R)0;
+            ux = uy = uz = true;
+        }
+        if (ux) stf(vxp, o, vx_n, A.sc[0]);
+        if (uy) stf(vyp, o, vy_n, A.sc[1]);
+        if (uz) stf(vzp, o, vz_n, A.sc[2]);
         if (ENERGY && ebox_ij && k >= em && k < nz - 1 - em) {
             // state after this velocity update: cell-local values, FP64 terms
             const double vx = ux ? (double)vx_n : (double)ldf(vxp, o, A.isc[0]);
@@ -397,7 +392,34 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
         szz_c = szz_u;
         rho_c = rho_u;
     }
-    if (ENERGY) {
+}
+
+// ---------------------------------------------------------------------------
+// One launch per pass: the C-PML slab boxes come first in logical block order,
+// then the interior box; each block runs the body specialised for its box.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
+{
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const Box B = pick_box(A.box, A.nbox, lb);
+    if (B.mode == kSlab)
+        stress_body<T, kSlab>(A, B, lb - B.blk0);
+    else
+        stress_body<T, kInterior>(A, B, lb - B.blk0);
+}
+
+template <typename T, bool ENERGY>
+__global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
+{
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const Box B = pick_box(A.box, A.nbox, lb);
+    double ek = 0.0, ep = 0.0;
+    if (B.mode == kSlab)
+        velocity_body<T, kSlab, ENERGY>(A, B, lb - B.blk0, ek, ep);
+    else
+        velocity_body<T, kInterior, ENERGY>(A, B, lb - B.blk0, ek, ep);
+    if (ENERGY) {  // per-block FP64 partials, summed in block order by the tail kernel
         __shared__ double red[2][4];
         ek = wave_sum(ek);
         ep = wave_sum(ep);
@@ -531,30 +553,20 @@ __global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ s, floa
 
 // ---------------------------------------------------------------------------
 template <typename T>
-void launch_stress(const StepArgs<T>& a, int mode, hipStream_t s)
+void launch_stress(const StepArgs<T>& a, hipStream_t s)
 {
     if (a.nblocks <= 0) return;
-    if (mode == kInterior)
-        hipLaunchKernelGGL((k_stress<T, kInterior>), dim3(a.nblocks), dim3(a.block), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_stress<T, kSlab>), dim3(a.nblocks), dim3(a.block), 0, s, a);
+    hipLaunchKernelGGL((k_stress<T>), dim3(a.nblocks), dim3(a.block), 0, s, a);
 }
 
 template <typename T>
-void launch_velocity(const StepArgs<T>& a, int mode, bool energy, hipStream_t s)
+void launch_velocity(const StepArgs<T>& a, bool energy, hipStream_t s)
 {
     if (a.nblocks <= 0) return;
-    if (mode == kInterior) {
-        if (energy)
-            hipLaunchKernelGGL((k_velocity<T, kInterior, true>), dim3(a.nblocks), dim3(a.block), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_velocity<T, kInterior, false>), dim3(a.nblocks), dim3(a.block), 0, s, a);
-    } else {
-        if (energy)
-            hipLaunchKernelGGL((k_velocity<T, kSlab, true>), dim3(a.nblocks), dim3(a.block), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_velocity<T, kSlab, false>), dim3(a.nblocks), dim3(a.block), 0, s, a);
-    }
+    if (energy)
+        hipLaunchKernelGGL((k_velocity<T, true>), dim3(a.nblocks), dim3(a.block), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_velocity<T, false>), dim3(a.nblocks), dim3(a.block), 0, s, a);
 }
 
 template <typename T>
@@ -571,12 +583,12 @@ void launch_copy(const float4* src, float4* dst, long n4, hipStream_t s)
     hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n4);
 }
 
-template void launch_stress<float>(const StepArgs<float>&, int, hipStream_t);
-template void launch_stress<double>(const StepArgs<double>&, int, hipStream_t);
-template void launch_stress<half_t>(const StepArgs<half_t>&, int, hipStream_t);
-template void launch_velocity<float>(const StepArgs<float>&, int, bool, hipStream_t);
-template void launch_velocity<double>(const StepArgs<double>&, int, bool, hipStream_t);
-template void launch_velocity<half_t>(const StepArgs<half_t>&, int, bool, hipStream_t);
+template void launch_stress<float>(const StepArgs<float>&, hipStream_t);
+template void launch_stress<double>(const StepArgs<double>&, hipStream_t);
+template void launch_stress<half_t>(const StepArgs<half_t>&, hipStream_t);
+template void launch_velocity<float>(const StepArgs<float>&, bool, hipStream_t);
+template void launch_velocity<double>(const StepArgs<double>&, bool, hipStream_t);
+template void launch_velocity<half_t>(const StepArgs<half_t>&, bool, hipStream_t);
 template void launch_tail<float>(const TailArgs<float>&, hipStream_t);
 template void launch_tail<double>(const TailArgs<double>&, hipStream_t);
 template void launch_tail<half_t>(const TailArgs<half_t>&, hipStream_t);

@@ -8,19 +8,21 @@ namespace ela3d {
 
 typedef _Float16 half_t;
 
+// box modes
+constexpr int kInterior = 0;  // y and z outside the C-PML: only x is checked per lane
+constexpr int kSlab = 1;      // y or z inside the C-PML: every axis checked
+
 // A launch covers up to kMaxBox boxes.  A box is a flat xy range [p0,p1)
 // (p = i + j*nx) times a z range [k0,k1); its blocks march z in segments of
 // zseg planes, one thread per xy position.
-constexpr int kMaxBox = 4;
+constexpr int kMaxBox = 5;
 struct Box {
     int p0, p1, k0, k1;
     int nchunk;  // ceil((p1-p0)/block)
+    int zseg;    // planes per block
     int blk0;    // first logical block of this box
+    int mode;    // kInterior or kSlab
 };
-
-// launch modes
-constexpr int kInterior = 0;  // y and z outside the C-PML: only x is checked per lane
-constexpr int kSlab = 1;      // y or z inside the C-PML: every axis checked
 
 template <typename T>
 struct StepArgs {
@@ -39,7 +41,7 @@ struct StepArgs {
     float sc[27], isc[27];   // FP16 storage scales (stored = value*sc)
     double* epart;           // energy partials [nblocks][2]
     Box box[kMaxBox];
-    int nbox, zseg, nblocks, block;
+    int nbox, nblocks, block;
 };
 
 template <typename T>
@@ -66,9 +68,9 @@ struct TailArgs {
 };
 
 template <typename T>
-void launch_stress(const StepArgs<T>& a, int mode, hipStream_t s);
+void launch_stress(const StepArgs<T>& a, hipStream_t s);
 template <typename T>
-void launch_velocity(const StepArgs<T>& a, int mode, bool energy, hipStream_t s);
+void launch_velocity(const StepArgs<T>& a, bool energy, hipStream_t s);
 template <typename T>
 void launch_tail(const TailArgs<T>& a, hipStream_t s);
 void launch_copy(const float4* src, float4* dst, long n4, hipStream_t s);

@@ -27,6 +27,7 @@ Tuning choose_tuning(const ShotPlan& p)
 {
     Tuning t;
     const ela3d_config& c = p.cfg;
+    if (c.block == 0 && c.zseg == 0 && cached_tuning(c, t)) return t;
     t.block = c.block > 0 ? c.block : 256;
     if (c.zseg > 0) {
         t.zseg_interior = c.zseg;
@@ -44,31 +45,33 @@ Tuning choose_tuning(const ShotPlan& p)
     return t;
 }
 
-static void add_box(LaunchGeom& g, int p0, int p1, int k0, int k1)
+static void add_box(LaunchGeom& g, int p0, int p1, int k0, int k1, int zseg, int mode)
 {
     if (p1 <= p0 || k1 <= k0) return;
     Box& b = g.box[g.nbox++];
     b.p0 = p0; b.p1 = p1; b.k0 = k0; b.k1 = k1;
     b.nchunk = (p1 - p0 + g.block - 1) / g.block;
+    b.zseg = std::max(1, zseg);
     b.blk0 = g.nblocks;
-    g.nblocks += b.nchunk * ((k1 - k0 + g.zseg - 1) / g.zseg);
+    b.mode = mode;
+    g.nblocks += b.nchunk * ((k1 - k0 + b.zseg - 1) / b.zseg);
 }
 
-void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& in, LaunchGeom& sl)
+// One launch per pass.  The four absorbing-layer boxes come first in logical
+// block order so their slower blocks start early; the interior box follows.
+void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& g)
 {
     const ela3d_config& c = p.cfg;
     const int nx = c.nx, nxy = c.nx * c.ny;
     const int ylo = p.ax[1].lo, yhi = p.ax[1].hi, zlo = p.ax[2].lo, zhi = p.ax[2].hi;
-    in = LaunchGeom();
-    sl = LaunchGeom();
-    in.block = sl.block = t.block;
-    in.zseg = t.zseg_interior;
-    sl.zseg = t.zseg_slab;
-    add_box(in, ylo * nx, yhi * nx, zlo, zhi);
-    add_box(sl, 0, nxy, 0, zlo);                 // z-min absorbing layer
-    add_box(sl, 0, nxy, zhi, c.nz);              // z-max absorbing layer
-    add_box(sl, 0, ylo * nx, zlo, zhi);          // y-min absorbing layer
-    add_box(sl, yhi * nx, nxy, zlo, zhi);        // y-max absorbing layer
+    g = LaunchGeom();
+    g.block = t.block;
+    const int zs = t.zseg_slab, zi = t.zseg_interior;
+    add_box(g, 0, nxy, 0, zlo, zs, kSlab);                 // z-min absorbing layer
+    add_box(g, 0, nxy, zhi, c.nz, zs, kSlab);              // z-max absorbing layer
+    add_box(g, 0, ylo * nx, zlo, zhi, zi, kSlab);          // y-min absorbing layer
+    add_box(g, yhi * nx, nxy, zlo, zhi, zi, kSlab);        // y-max absorbing layer
+    add_box(g, ylo * nx, yhi * nx, zlo, zhi, zi, kInterior);
 }
 
 template <typename T>
@@ -88,13 +91,14 @@ public:
             own_stream_ = true;
         }
         tune_ = choose_tuning(plan_);
-        geometry(plan_, tune_, gin_, gsl_);
+        geometry(plan_, tune_, geo_);
         allocate();
         upload(cp, cs, rho);
         for (int q = 0; q < 27; ++q) {
             sc_[q] = 1.0f;
             isc_[q] = 1.0f;
         }
+        args_ = step_args(geo_, d_epart_);
     }
 
     ~ShotImpl() override
@@ -122,7 +126,7 @@ public:
                 if (!g) g = capture(e);
                 HIP_CHECK(hipGraphLaunch(g, stream_));
             } else {
-                launch_step(e, nullptr, 0);
+                launch_step(e, nullptr);
             }
             host_step_ = it;
         }
@@ -207,21 +211,19 @@ public:
     void profile(int n, std::vector<ela3d_kernel_stats>& st, double& wall_ms) override
     {
         HIP_CHECK(hipSetDevice(device_));
-        static const char* names[5] = {"stress_interior", "stress_slab", "velocity_interior",
-                                       "velocity_slab", "tail"};
-        st.assign(5, ela3d_kernel_stats());
-        for (int q = 0; q < 5; ++q) {
+        static const char* names[kLaunches] = {"stress", "velocity", "tail"};
+        st.assign(kLaunches, ela3d_kernel_stats());
+        for (int q = 0; q < kLaunches; ++q) {
             std::snprintf(st[q].name, sizeof(st[q].name), "%s", names[q]);
             st[q].launches = 0;
             st[q].total_ms = 0.0;
         }
-        st[0].alg_bytes_per_launch = plan_.alg_stress_interior;
-        st[1].alg_bytes_per_launch = plan_.alg_stress_slab;
-        st[2].alg_bytes_per_launch = plan_.alg_velocity_interior;
-        st[3].alg_bytes_per_launch = plan_.alg_velocity_slab;
-        st[4].alg_bytes_per_launch = 0;
+        st[0].alg_bytes_per_launch = plan_.alg_stress_interior + plan_.alg_stress_slab;
+        st[1].alg_bytes_per_launch = plan_.alg_velocity_interior + plan_.alg_velocity_slab;
+        st[2].alg_bytes_per_launch = 0;
         const int chunk = 64;
-        std::vector<hipEvent_t> ev(size_t(chunk) * 6);
+        const int E = kLaunches + 1;
+        std::vector<hipEvent_t> ev(size_t(chunk) * E);
         for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
         hipEvent_t t0, t1;
         HIP_CHECK(hipEventCreate(&t0));
@@ -233,15 +235,15 @@ public:
             const int m = std::min(chunk, n - done);
             for (int s = 0; s < m; ++s) {
                 const int it = host_step_ + 1;
-                launch_step(energy_step(it), &ev[size_t(s) * 6], 1);
+                launch_step(energy_step(it), &ev[size_t(s) * E]);
                 host_step_ = it;
             }
             HIP_CHECK(hipGetLastError());
-            HIP_CHECK(hipEventSynchronize(ev[size_t(m - 1) * 6 + 5]));
+            HIP_CHECK(hipEventSynchronize(ev[size_t(m - 1) * E + kLaunches]));
             for (int s = 0; s < m; ++s)
-                for (int q = 0; q < 5; ++q) {
+                for (int q = 0; q < kLaunches; ++q) {
                     float ms = 0.f;
-                    HIP_CHECK(hipEventElapsedTime(&ms, ev[size_t(s) * 6 + q], ev[size_t(s) * 6 + q + 1]));
+                    HIP_CHECK(hipEventElapsedTime(&ms, ev[size_t(s) * E + q], ev[size_t(s) * E + q + 1]));
                     st[q].total_ms += ms;
                     st[q].launches += 1;
                 }
@@ -257,14 +259,32 @@ public:
         hipEventDestroy(t1);
     }
 
+    void retune(const Tuning& t) override
+    {
+        HIP_CHECK(hipSetDevice(device_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        LaunchGeom g;
+        geometry(plan_, t, g);
+        if (g.nblocks > epart_cap_) throw Error{ELA3D_E_INVALID, "launch shape needs too many blocks"};
+        tune_ = t;
+        geo_ = g;
+        nepart_ = g.nblocks;
+        args_ = step_args(geo_, d_epart_);
+        for (auto& x : graphs_)
+            if (x) {
+                hipGraphExecDestroy(x);
+                x = nullptr;
+            }
+    }
+
     void info(ela3d_info& o) override
     {
         std::memset(&o, 0, sizeof(o));
         o.cells = plan_.cells;
         o.device_bytes = arena_bytes_;
         o.alg_bytes_per_step = plan_.alg_step();
-        o.alg_bytes_stress_interior = plan_.alg_stress_interior;
-        o.alg_bytes_velocity_interior = plan_.alg_velocity_interior;
+        o.alg_bytes_stress = plan_.alg_stress_interior + plan_.alg_stress_slab;
+        o.alg_bytes_velocity = plan_.alg_velocity_interior + plan_.alg_velocity_slab;
         o.pml_bytes_per_step = plan_.alg_pml;
         o.nsamples = plan_.nsamples;
         o.nenergy = plan_.nenergy;
@@ -337,8 +357,10 @@ private:
         const size_t nseis = size_t(plan_.nsamples) * c.nrec * 3;
         const size_t o_seis = off; off += al(std::max<size_t>(1, nseis) * sizeof(double));
         const size_t o_en = off; off += al(std::max<size_t>(1, size_t(plan_.nenergy) * 2) * sizeof(double));
-        nepart_ = gin_.nblocks + gsl_.nblocks;
-        const size_t o_ep = off; off += al(size_t(std::max(1, nepart_)) * 2 * sizeof(double));
+        nepart_ = geo_.nblocks;
+        // room for any launch shape the tuner may pick (block >= 64, zseg >= 1)
+        epart_cap_ = int(plan_.cells / 64 + int64_t(kMaxBox) * c.nz + 8);
+        const size_t o_ep = off; off += al(size_t(std::max(epart_cap_, nepart_)) * 2 * sizeof(double));
         const size_t o_ctl = off; off += al(2 * sizeof(int));
         arena_bytes_ = int64_t(off);
         HIP_CHECK(hipMalloc(&arena_, off));
@@ -417,7 +439,6 @@ private:
         a.epart = epart;
         for (int q = 0; q < g.nbox; ++q) a.box[q] = g.box[q];
         a.nbox = g.nbox;
-        a.zseg = g.zseg;
         a.nblocks = g.nblocks;
         a.block = g.block;
         return a;
@@ -461,21 +482,17 @@ private:
         return a;
     }
 
-    // One time step: stress (interior, slabs) -> velocity (interior, slabs) -> tail.
+    // One time step: stress pass -> velocity pass -> tail (receivers, energy, source).
     // With ev != nullptr, an event is recorded before the first and after every launch.
-    void launch_step(bool energy, hipEvent_t* ev, int)
+    void launch_step(bool energy, hipEvent_t* ev)
     {
         if (ev) HIP_CHECK(hipEventRecord(ev[0], stream_));
-        launch_stress(step_args(gin_, nullptr), kInterior, stream_);
+        launch_stress(args_, stream_);
         if (ev) HIP_CHECK(hipEventRecord(ev[1], stream_));
-        launch_stress(step_args(gsl_, nullptr), kSlab, stream_);
+        launch_velocity(args_, energy, stream_);
         if (ev) HIP_CHECK(hipEventRecord(ev[2], stream_));
-        launch_velocity(step_args(gin_, d_epart_), kInterior, energy, stream_);
-        if (ev) HIP_CHECK(hipEventRecord(ev[3], stream_));
-        launch_velocity(step_args(gsl_, d_epart_ + 2 * size_t(gin_.nblocks)), kSlab, energy, stream_);
-        if (ev) HIP_CHECK(hipEventRecord(ev[4], stream_));
         launch_tail(tail_args(), stream_);
-        if (ev) HIP_CHECK(hipEventRecord(ev[5], stream_));
+        if (ev) HIP_CHECK(hipEventRecord(ev[3], stream_));
     }
 
     hipGraphExec_t capture(bool energy)
@@ -483,7 +500,7 @@ private:
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-        launch_step(energy, nullptr, 0);
+        launch_step(energy, nullptr);
         HIP_CHECK(hipStreamEndCapture(stream_, &graph));
         hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         hipGraphDestroy(graph);
@@ -495,7 +512,9 @@ private:
     hipStream_t stream_ = nullptr;
     bool own_stream_ = true;
     Tuning tune_;
-    LaunchGeom gin_, gsl_;
+    static constexpr int kLaunches = 3;  // stress, velocity, tail
+    LaunchGeom geo_;
+    StepArgs<T> args_;
     void* arena_ = nullptr;
     int64_t arena_bytes_ = 0;
     T* d_f_[9] = {};
@@ -504,7 +523,7 @@ private:
     double* d_tab_[3] = {};
     double *d_wav_ = nullptr, *d_seis_ = nullptr, *d_energy_ = nullptr, *d_epart_ = nullptr;
     int *d_rec_ = nullptr, *d_step_ = nullptr, *d_flag_ = nullptr;
-    int nepart_ = 0;
+    int nepart_ = 0, epart_cap_ = 0;
     int host_step_ = 0;
     float sc_[27], isc_[27];
     hipGraphExec_t graphs_[2] = {nullptr, nullptr};

@@ -21,10 +21,10 @@ namespace ela3d {
                                  std::string(#expr) + ": " + hipGetErrorString(e_)};      \
     } while (0)
 
-// Launch shape of one stress/velocity launch.
+// Launch shape shared by the stress and the velocity launch.
 struct LaunchGeom {
     Box box[kMaxBox];
-    int nbox = 0, zseg = 1, nblocks = 0, block = 256;
+    int nbox = 0, nblocks = 0, block = 256;
 };
 
 // Launch tuning chosen for a grid (see tune.cpp).
@@ -45,6 +45,7 @@ public:
     virtual void set_field(int field, const double* in) = 0;
     virtual void profile(int n, std::vector<ela3d_kernel_stats>& st, double& wall_ms) = 0;
     virtual void info(ela3d_info& out) = 0;
+    virtual void retune(const Tuning& t) = 0;       // new launch shape (drops captured graphs)
     virtual hipStream_t stream() const = 0;
     virtual int steps_done() const = 0;
     virtual int64_t device_bytes() const = 0;
@@ -57,6 +58,8 @@ protected:
 std::unique_ptr<Shot> make_shot(ShotPlan plan, const float* cp, const float* cs, const float* rho,
                                 hipStream_t external_stream = nullptr);
 Tuning choose_tuning(const ShotPlan& p);
-void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& interior, LaunchGeom& slab);
+bool cached_tuning(const ela3d_config& c, Tuning& out);
+void store_tuning(const ela3d_config& c, const Tuning& t, double ms);
+void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& g);
 
 }  // namespace ela3d

@@ -127,8 +127,8 @@ typedef struct ela3d_info {
     int64_t cells;                 /* nx*ny*nz */
     int64_t device_bytes;          /* device memory held by the shot */
     double alg_bytes_per_step;     /* algorithmic HBM bytes of one step (two-pass schedule) */
-    double alg_bytes_stress_interior;   /* per launch */
-    double alg_bytes_velocity_interior; /* per launch */
+    double alg_bytes_stress;       /* per stress launch (interior + C-PML slabs) */
+    double alg_bytes_velocity;     /* per velocity launch (interior + C-PML slabs) */
     double pml_bytes_per_step;     /* the C-PML memory-variable share of alg_bytes_per_step */
     int32_t nsamples;              /* seismogram samples over nstep */
     int32_t nenergy;               /* energy samples over nstep */
@@ -171,6 +171,26 @@ int ela3d_info_get(ela3d_shot* shot, ela3d_info* out);
 int ela3d_profile(ela3d_shot* shot, int32_t nsteps, ela3d_kernel_stats* out, int32_t cap,
                   int32_t* nkernels, double* wall_ms);
 void ela3d_destroy(ela3d_shot* shot);
+/* Host-only introspection (no GPU): the plan a config expands to (cells, regions,
+ * algorithmic bytes) and its FP64 C-PML tables of one axis, 10*n doubles:
+ * d, K, alpha, a, b (full grid) then the same on the half grid (CPU:158-361). */
+int ela3d_plan(const ela3d_config* cfg, ela3d_info* out);
+int ela3d_pml_tables(const ela3d_config* cfg, int32_t axis, double* out, size_t cap);
+
+/* ---- launch-shape autotuner (no reference counterpart) ----
+ * Times every candidate (threads per workgroup, z planes per workgroup in the
+ * interior and in the absorbing layers) on a scratch shot of this grid and
+ * caches the fastest for (nx, ny, nz, storage, npoints_pml); shots created with
+ * block = 0 and zseg = 0 use the cached shape. */
+typedef struct ela3d_tuning {
+    int32_t block, zseg_interior, zseg_slab;
+    int32_t candidates;
+    double ms_per_step;   /* stress + velocity launches */
+} ela3d_tuning;
+int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, const float* rho,
+                   int32_t steps, ela3d_tuning* out);
+int ela3d_tuning_set(const ela3d_config* cfg, const ela3d_tuning* t);
+int ela3d_tuning_get(const ela3d_config* cfg, ela3d_tuning* t);
 
 const char* ela3d_strerror(int code);
 const char* ela3d_last_error(void);

@@ -34,7 +34,8 @@ def _one_step(cfg, model, seed):
         seis_g = shot.seismograms()[:1]
         en_g = shot.energy()[:1]
     assert max_rel(seis_g, seis_o[:1]) <= TOL[cfg.storage]
-    assert np.allclose(en_g, en_o[:1], rtol=1e-9, atol=0)
+    # energy sums squares of the state, so it inherits the state's relative error
+    assert np.allclose(en_g, en_o[:1], rtol=10 * TOL[cfg.storage], atol=0)
     return worst
 
 
