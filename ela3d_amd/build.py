@@ -25,8 +25,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{INCLUDE}"]
 
-LIB_SOURCES = ["core.cpp", "models.cpp", "capi.cpp", "shot.cpp", "kernels.hip", "serve.cpp",
-               "tune.cpp"]
+LIB_SOURCES = ["core.cpp", "models.cpp", "capi.cpp", "shot.cpp", "kernels.hip", "tail.hip",
+               "serve.cpp", "tune.cpp"]
 
 
 def _run(cmd):

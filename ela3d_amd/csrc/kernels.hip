@@ -1,55 +1,28 @@
 // kernels.hip -- gfx950 stencil kernels of the SEISMIC_CPML elastic time loop.
 //
-// Bandwidth-bound (about 0.8 FLOP/B), so no MFMA: the design goal is one HBM
-// pass per field per half step.
-//  * Threads own one xy position p = i + j*nx of a flattened plane (so rows of
-//    any nx coalesce with no idle lanes) and march z through a segment of
-//    planes, carrying the z-1 / z+1 neighbours in registers (register queue);
-//    in-plane neighbours come from the same or adjacent wavefronts via L1/L2.
+// Bandwidth-bound (about 0.8 FLOP/B), so no MFMA: the goal is one HBM pass per
+// field per half step with enough loads in flight to cover HBM latency.
+//  * Threads own one xy position p = i + j*nx of a flattened plane (rows of any
+//    nx coalesce with no idle lanes) and march z through a segment of planes,
+//    carrying the z-1 / z+1 neighbours in registers (register queue); in-plane
+//    neighbours are re-read through L1/L2 at clamped, always-valid offsets.
+//  * Each z step is load-all -> compute -> predicated store: no load sits behind
+//    a branch, so one plane costs one memory round trip.  With PIPE the loads of
+//    plane k+1 are issued before plane k is computed (software pipelining).
 //  * Blocks are renumbered so each of the 8 XCDs walks a contiguous run of
 //    (chunk, z-segment) tiles, keeping in-plane halos inside one XCD's L2.
-//  * C-PML memory variables live in compact per-axis slabs.  Interior launches
-//    test x per lane only (y and z are outside the C-PML by construction);
-//    slab launches over the y and z absorbing layers test all three axes.
+//  * C-PML memory variables live in compact per-axis slabs.  Interior boxes
+//    test x per lane only (y and z lie outside the C-PML by construction);
+//    absorbing-layer boxes test all three axes.  One launch covers both.
 //  * Arithmetic is FP32 (FP64 for the FP64 storage instantiation); the C-PML
-//    tables and the wavelet are FP64 in memory; energy and seismograms
-//    accumulate in FP64.
+//    tables and the wavelet are FP64 in memory; energy accumulates in FP64.
 //
 // Updates follow the reference CPU loops (seismicCPML_3d_isotropic_simpelmodel.cpp):
 // normal stress CPU:377-413, sigma_xy CPU:415-442, sigma_xz/yz CPU:444-499,
-// vx/vy CPU:501-560, vz CPU:562-590, receivers CPU:592-602, source CPU:604-622.
-#include <type_traits>
-
-#include "kernels.hpp"
+// vx/vy CPU:501-560, vz CPU:562-590.
+#include "device_util.hpp"
 
 namespace ela3d {
-
-template <typename T>
-struct Arith {
-    typedef float R;
-};
-template <>
-struct Arith<double> {
-    typedef double R;
-};
-
-template <typename T>
-__device__ __forceinline__ typename Arith<T>::R ldf(const T* p, int o, float isc)
-{
-    if constexpr (std::is_same<T, half_t>::value)
-        return (float)p[o] * isc;
-    else
-        return p[o];
-}
-
-template <typename T>
-__device__ __forceinline__ void stf(T* p, int o, typename Arith<T>::R v, float sc)
-{
-    if constexpr (std::is_same<T, half_t>::value)
-        p[o] = (half_t)(v * sc);
-    else
-        p[o] = v;
-}
 
 // Blocks are dispatched round-robin over the 8 XCDs; renumber so XCD x owns
 // the contiguous logical range [x*q + min(x,r), ...).  A bijection for any n.
@@ -72,18 +45,258 @@ __device__ __forceinline__ Box pick_box(const Box* bx, int nbox, int lb)
 }
 
 // memory-variable recurrence psi <- b psi + a D;  D <- D/K + psi  (CPU:395-405)
-template <typename T, typename R>
-__device__ __forceinline__ R cpml(T* m, int om, R a, R b, R K, R d, float sc, float isc)
+template <typename R>
+__device__ __forceinline__ R cpml(R psi_old, R a, R b, R K, R& d)
 {
-    R psi = b * ldf(m, om, isc) + a * d;
-    stf(m, om, psi, sc);
-    return d / K + psi;
+    const R psi = b * psi_old + a * d;
+    d = d / K + psi;
+    return psi;
 }
 
-// ---------------------------------------------------------------------------
+// Per-lane constants: position, clamped neighbour offsets, C-PML coefficients.
+template <typename R>
+struct Lane {
+    int p, i, j;
+    unsigned dE, dW, dN, dS;   // +x, -x, +y, -y offsets, 0 where the neighbour is off-grid
+    int cx, cy;                // compact C-PML column / row, -1 outside the layers
+    R ax, bx, Kx, axh, bxh, Kxh;
+    R ay, by, Ky, ayh, byh, Kyh;
+};
+
+template <typename T, int MODE, typename R>
+__device__ __forceinline__ void make_lane(const StepArgs<T>& A, int p, Lane<R>& L)
+{
+    const int nx = A.nx, ny = A.ny;
+    L.p = p;
+    L.j = p / nx;
+    L.i = p - L.j * nx;
+    L.dE = L.i < nx - 1 ? 1u : 0u;
+    L.dW = L.i > 0 ? 1u : 0u;
+    L.dN = L.j < ny - 1 ? unsigned(nx) : 0u;
+    L.dS = L.j > 0 ? unsigned(nx) : 0u;
+    L.cx = L.i < A.lo[0] ? L.i : (L.i >= A.hi[0] ? A.lo[0] + (L.i - A.hi[0]) : -1);
+    L.ax = 0; L.bx = 1; L.Kx = 1; L.axh = 0; L.bxh = 1; L.Kxh = 1;
+    if (L.cx >= 0) {
+        const double* t = A.tab[0];
+        const int i = L.i;
+        L.ax = (R)t[i]; L.bx = (R)t[nx + i]; L.Kx = (R)t[2 * nx + i];
+        L.axh = (R)t[3 * nx + i]; L.bxh = (R)t[4 * nx + i]; L.Kxh = (R)t[5 * nx + i];
+    }
+    L.cy = -1;
+    L.ay = 0; L.by = 1; L.Ky = 1; L.ayh = 0; L.byh = 1; L.Kyh = 1;
+    if (MODE == kSlab) {
+        const int j = L.j;
+        L.cy = j < A.lo[1] ? j : (j >= A.hi[1] ? A.lo[1] + (j - A.hi[1]) : -1);
+        if (L.cy >= 0) {
+            const double* t = A.tab[1];
+            L.ay = (R)t[j]; L.by = (R)t[ny + j]; L.Ky = (R)t[2 * ny + j];
+            L.ayh = (R)t[3 * ny + j]; L.byh = (R)t[4 * ny + j]; L.Kyh = (R)t[5 * ny + j];
+        }
+    }
+}
+
+// z C-PML coefficients of plane k (uniform across the block)
+template <typename R>
+struct ZTab {
+    int cz;
+    R az, bz, Kz, azh, bzh, Kzh;
+};
+
+template <typename T, int MODE, typename R>
+__device__ __forceinline__ ZTab<R> ztab(const StepArgs<T>& A, int k)
+{
+    ZTab<R> z;
+    z.cz = -1;
+    z.az = 0; z.bz = 1; z.Kz = 1; z.azh = 0; z.bzh = 1; z.Kzh = 1;
+    if (MODE == kSlab) {
+        const int nz = A.nz;
+        z.cz = k < A.lo[2] ? k : (k >= A.hi[2] ? A.lo[2] + (k - A.hi[2]) : -1);
+        if (z.cz >= 0) {
+            const double* t = A.tab[2];
+            z.az = (R)t[k]; z.bz = (R)t[nz + k]; z.Kz = (R)t[2 * nz + k];
+            z.azh = (R)t[3 * nz + k]; z.bzh = (R)t[4 * nz + k]; z.Kzh = (R)t[5 * nz + k];
+        }
+    }
+    return z;
+}
+
+// ===========================================================================
 // Stress pass: sigma_xx/yy/zz, sigma_xy, sigma_xz, sigma_yz + 9 memory variables
-// ---------------------------------------------------------------------------
-template <typename T, int MODE>
+// ===========================================================================
+template <typename R>
+struct SPlane {
+    R vz_c, vx_u, vy_u, cs_u, rho_u, cp_c;            // own column: plane k, k+1
+    R vxE, vyS, vyW, vxN, vzW, vzN, csW, csN, rhoW, rhoN;  // in-plane neighbours
+    R sxx, syy, szz, sxy, sxz, syz;                   // stresses to update
+    R mx[3], my[3], mz[3];                            // memory variables (C-PML lanes)
+};
+
+template <typename T, int MODE, typename R>
+__device__ __forceinline__ void stress_load(const StepArgs<T>& A, const Lane<R>& L, int k,
+                                            SPlane<R>& P)
+{
+    const unsigned nxy = A.nxy;
+    const unsigned o = unsigned(k) * nxy + unsigned(L.p);
+    const bool top = (k == A.nz - 1);
+    const unsigned dU = top ? 0u : nxy;
+    T *vx = A.f[0], *vy = A.f[1], *vz = A.f[2];
+    const float *cp = A.cp, *cs = A.cs, *rho = A.rho;
+    P.vz_c = ldf(vz, o, A.isc[2]);
+    P.cp_c = cp[o];
+    P.vx_u = ldf(vx, o + dU, A.isc[0]);
+    P.vy_u = ldf(vy, o + dU, A.isc[1]);
+    P.cs_u = cs[o + dU];
+    P.rho_u = rho[o + dU];
+    P.vxE = ldf(vx, o + L.dE, A.isc[0]);
+    P.vyS = ldf(vy, o - L.dS, A.isc[1]);
+    P.vyW = ldf(vy, o - L.dW, A.isc[1]);
+    P.vxN = ldf(vx, o + L.dN, A.isc[0]);
+    P.vzW = ldf(vz, o - L.dW, A.isc[2]);
+    P.vzN = ldf(vz, o + L.dN, A.isc[2]);
+    P.csW = cs[o - L.dW];
+    P.csN = cs[o + L.dN];
+    P.rhoW = rho[o - L.dW];
+    P.rhoN = rho[o + L.dN];
+    P.sxx = ldf(A.f[3], o, A.isc[3]);
+    P.syy = ldf(A.f[4], o, A.isc[4]);
+    P.szz = ldf(A.f[5], o, A.isc[5]);
+    P.sxy = ldf(A.f[6], o, A.isc[6]);
+    P.sxz = ldf(A.f[7], o, A.isc[7]);
+    P.syz = ldf(A.f[8], o, A.isc[8]);
+    if (L.cx >= 0) {
+        const unsigned om = unsigned((k * A.ny + L.j) * A.w[0] + L.cx);
+        P.mx[0] = ldf(A.m[0], om, A.isc[9]);
+        P.mx[1] = ldf(A.m[3], om, A.isc[12]);
+        P.mx[2] = ldf(A.m[5], om, A.isc[14]);
+    }
+    if (MODE == kSlab) {
+        if (L.cy >= 0) {
+            const unsigned om = unsigned((k * A.w[1] + L.cy) * A.nx + L.i);
+            P.my[0] = ldf(A.m[1], om, A.isc[10]);
+            P.my[1] = ldf(A.m[4], om, A.isc[13]);
+            P.my[2] = ldf(A.m[7], om, A.isc[16]);
+        }
+        const int cz = k < A.lo[2] ? k : (k >= A.hi[2] ? A.lo[2] + (k - A.hi[2]) : -1);
+        if (cz >= 0) {
+            const unsigned om = unsigned((cz * A.ny + L.j) * A.nx + L.i);
+            P.mz[0] = ldf(A.m[2], om, A.isc[11]);
+            P.mz[1] = ldf(A.m[6], om, A.isc[15]);
+            P.mz[2] = ldf(A.m[8], om, A.isc[17]);
+        }
+    }
+    if (top) {  // zero ghost plane above the top (only read with the ghost quirk)
+        P.vx_u = 0;
+        P.vy_u = 0;
+        P.cs_u = 0;
+        P.rho_u = 0;
+    }
+}
+
+// queue: values of plane k carried from the previous step of the march
+template <typename R>
+struct SQueue {
+    R vx_c, vy_c, vz_m, cs_c, rho_c;
+};
+
+template <typename T, int MODE, typename R>
+__device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<R>& L, int k,
+                                              const SPlane<R>& P, const SQueue<R>& Q)
+{
+    const unsigned o = unsigned(k) * unsigned(A.nxy) + unsigned(L.p);
+    const R dt = (R)A.dt, idx = (R)A.idx, idy = (R)A.idy, idz = (R)A.idz;
+    const ZTab<R> Z = ztab<T, MODE, R>(A, k);
+    const int nx = A.nx, ny = A.ny;
+    const unsigned omx = unsigned((k * ny + L.j) * A.w[0] + L.cx);
+    const unsigned omy = unsigned((k * A.w[1] + L.cy) * nx + L.i);
+    const unsigned omz = unsigned((Z.cz * ny + L.j) * nx + L.i);
+
+    // sigma_xx, sigma_yy, sigma_zz: i in [0,nx-2], j in [1,ny-1], k in [1,nz-1] (CPU:377-413)
+    {
+        const bool on = (L.i <= nx - 2) && (L.j >= 1) && (k >= 1);
+        const R vp = P.cp_c;
+        const R lambda = Q.rho_c * (vp * vp - (R)2 * Q.cs_c * Q.cs_c);
+        const R l2m = Q.rho_c * vp * vp;
+        const R DT_l2m = dt * l2m, DT_l = dt * lambda;
+        R dvx_dx = (P.vxE - Q.vx_c) * idx;
+        R dvy_dy = (Q.vy_c - P.vyS) * idy;
+        R dvz_dz = (P.vz_c - Q.vz_m) * idz;
+        R m0 = 0, m1 = 0, m2 = 0;
+        if (L.cx >= 0) m0 = cpml(P.mx[0], L.axh, L.bxh, L.Kxh, dvx_dx);
+        if (MODE == kSlab) {
+            if (L.cy >= 0) m1 = cpml(P.my[0], L.ay, L.by, L.Ky, dvy_dy);
+            if (Z.cz >= 0) m2 = cpml(P.mz[0], Z.az, Z.bz, Z.Kz, dvz_dz);
+        }
+        if (on) {
+            stf(A.f[3], o, DT_l2m * dvx_dx + DT_l * (dvy_dy + dvz_dz) + P.sxx, A.sc[3]);
+            stf(A.f[4], o, DT_l * (dvx_dx + dvz_dz) + DT_l2m * dvy_dy + P.syy, A.sc[4]);
+            stf(A.f[5], o, DT_l * (dvx_dx + dvy_dy) + DT_l2m * dvz_dz + P.szz, A.sc[5]);
+            if (L.cx >= 0) stf(A.m[0], omx, m0, A.sc[9]);
+            if (MODE == kSlab) {
+                if (L.cy >= 0) stf(A.m[1], omy, m1, A.sc[10]);
+                if (Z.cz >= 0) stf(A.m[2], omz, m2, A.sc[11]);
+            }
+        }
+    }
+    // sigma_xy: i in [1,nx-1], j in [0,ny-2]; mu averaged over (i-1,j),(i,j+1) (CPU:415-442)
+    {
+        const bool on = (L.i >= 1) && (L.j <= ny - 2);
+        const R vs = (P.csW + P.csN) / (R)2;
+        const R rh = (P.rhoW + P.rhoN) / (R)2;
+        const R DT_mu = dt * (rh * vs * vs);
+        R dvy_dx = (Q.vy_c - P.vyW) * idx;
+        R dvx_dy = (P.vxN - Q.vx_c) * idy;
+        R m3 = 0, m4 = 0;
+        if (L.cx >= 0) m3 = cpml(P.mx[1], L.ax, L.bx, L.Kx, dvy_dx);
+        if (MODE == kSlab && L.cy >= 0) m4 = cpml(P.my[1], L.ayh, L.byh, L.Kyh, dvx_dy);
+        if (on) {
+            stf(A.f[6], o, DT_mu * (dvy_dx + dvx_dy) + P.sxy, A.sc[6]);
+            if (L.cx >= 0) stf(A.m[3], omx, m3, A.sc[12]);
+            if (MODE == kSlab && L.cy >= 0) stf(A.m[4], omy, m4, A.sc[13]);
+        }
+    }
+    if (k <= A.kmax_shear) {
+        // sigma_xz: i in [1,nx-1]; mu averaged over (i-1,k),(i,k+1) (CPU:444-470)
+        {
+            const bool on = (L.i >= 1);
+            const R vs = (P.csW + P.cs_u) / (R)2;
+            const R rh = (P.rhoW + P.rho_u) / (R)2;
+            const R DT_mu = dt * (rh * vs * vs);
+            R dvz_dx = (P.vz_c - P.vzW) * idx;
+            R dvx_dz = (P.vx_u - Q.vx_c) * idz;
+            R m5 = 0, m6 = 0;
+            if (L.cx >= 0) m5 = cpml(P.mx[2], L.ax, L.bx, L.Kx, dvz_dx);
+            if (MODE == kSlab && Z.cz >= 0) m6 = cpml(P.mz[1], Z.azh, Z.bzh, Z.Kzh, dvx_dz);
+            if (on) {
+                stf(A.f[7], o, DT_mu * (dvz_dx + dvx_dz) + P.sxz, A.sc[7]);
+                if (L.cx >= 0) stf(A.m[5], omx, m5, A.sc[14]);
+                if (MODE == kSlab && Z.cz >= 0) stf(A.m[6], omz, m6, A.sc[15]);
+            }
+        }
+        // sigma_yz: j in [0,ny-2]; mu averaged over (j+1,k),(j,k+1) (CPU:472-497)
+        {
+            const bool on = (L.j <= ny - 2);
+            const R vs = (P.csN + P.cs_u) / (R)2;
+            const R rh = (P.rhoN + P.rho_u) / (R)2;
+            const R DT_mu = dt * (rh * vs * vs);
+            R dvz_dy = (P.vzN - P.vz_c) * idy;
+            R dvy_dz = (P.vy_u - Q.vy_c) * idz;
+            R m7 = 0, m8 = 0;
+            if (MODE == kSlab) {
+                if (L.cy >= 0) m7 = cpml(P.my[2], L.ayh, L.byh, L.Kyh, dvz_dy);
+                if (Z.cz >= 0) m8 = cpml(P.mz[2], Z.azh, Z.bzh, Z.Kzh, dvy_dz);
+            }
+            if (on) {
+                stf(A.f[8], o, DT_mu * (dvz_dy + dvy_dz) + P.syz, A.sc[8]);
+                if (MODE == kSlab) {
+                    if (L.cy >= 0) stf(A.m[7], omy, m7, A.sc[16]);
+                    if (Z.cz >= 0) stf(A.m[8], omz, m8, A.sc[17]);
+                }
+            }
+        }
+    }
+}
+
+template <typename T, int MODE, bool PIPE>
 __device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, int local)
 {
     typedef typename Arith<T>::R R;
@@ -92,305 +305,260 @@ __device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, 
     if (p >= B.p1) return;
     const int kb = B.k0 + zs * B.zseg;
     const int ke = min(kb + B.zseg, B.k1);
-    const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = A.nxy;
-    const int j = p / nx, i = p - j * nx;
-
-    const bool rN = (i <= nx - 2) && (j >= 1);   // sigma_xx/yy/zz: i in [0,nx-2], j in [1,ny-1]
-    const bool rXY = (i >= 1) && (j <= ny - 2);  // sigma_xy: i in [1,nx-1], j in [0,ny-2]
-    const bool rXZ = (i >= 1);                   // sigma_xz: i in [1,nx-1]
-    const bool rYZ = (j <= ny - 2);              // sigma_yz: j in [0,ny-2]
-    const R dt = (R)A.dt, idx = (R)A.idx, idy = (R)A.idy, idz = (R)A.idz;
-
-    T *vxp = A.f[0], *vyp = A.f[1], *vzp = A.f[2];
-    T *sxxp = A.f[3], *syyp = A.f[4], *szzp = A.f[5], *sxyp = A.f[6], *sxzp = A.f[7],
-      *syzp = A.f[8];
-    const float *cpp = A.cp, *csp = A.cs, *rhop = A.rho;
-
-    // x C-PML coefficients of this lane (constant along z)
-    const int cx = i < A.lo[0] ? i : (i >= A.hi[0] ? A.lo[0] + (i - A.hi[0]) : -1);
-    R ax = 0, bx = 1, Kx = 1, axh = 0, bxh = 1, Kxh = 1;
-    if (cx >= 0) {
-        const double* t = A.tab[0];
-        ax = (R)t[i]; bx = (R)t[nx + i]; Kx = (R)t[2 * nx + i];
-        axh = (R)t[3 * nx + i]; bxh = (R)t[4 * nx + i]; Kxh = (R)t[5 * nx + i];
+    Lane<R> L;
+    make_lane<T, MODE, R>(A, p, L);
+    SQueue<R> Q;
+    {
+        const unsigned o = unsigned(kb) * unsigned(A.nxy) + unsigned(p);
+        Q.vx_c = ldf(A.f[0], o, A.isc[0]);
+        Q.vy_c = ldf(A.f[1], o, A.isc[1]);
+        Q.cs_c = A.cs[o];
+        Q.rho_c = A.rho[o];
+        Q.vz_m = kb >= 1 ? ldf(A.f[2], o - unsigned(A.nxy), A.isc[2]) : (R)0;
     }
-    int cy = -1;
-    R ay = 0, by = 1, Ky = 1, ayh = 0, byh = 1, Kyh = 1;
-    if (MODE == kSlab) {
-        cy = j < A.lo[1] ? j : (j >= A.hi[1] ? A.lo[1] + (j - A.hi[1]) : -1);
-        if (cy >= 0) {
-            const double* t = A.tab[1];
-            ay = (R)t[j]; by = (R)t[ny + j]; Ky = (R)t[2 * ny + j];
-            ayh = (R)t[3 * ny + j]; byh = (R)t[4 * ny + j]; Kyh = (R)t[5 * ny + j];
-        }
-    }
-
-    int o = kb * nxy + p;
-    R vx_c = ldf(vxp, o, A.isc[0]);
-    R vy_c = ldf(vyp, o, A.isc[1]);
-    R cs_c = csp[o], rho_c = rhop[o];
-    R vz_m = kb >= 1 ? ldf(vzp, o - nxy, A.isc[2]) : (R)0;
-
-    for (int k = kb; k < ke; ++k, o += nxy) {
-        const bool top = (k == nz - 1);
-        const R vz_c = ldf(vzp, o, A.isc[2]);
-        R vx_u = 0, vy_u = 0, cs_u = 0, rho_u = 0;  // plane k+1; zero ghost above the top
-        if (!top) {
-            vx_u = ldf(vxp, o + nxy, A.isc[0]);
-            vy_u = ldf(vyp, o + nxy, A.isc[1]);
-            cs_u = csp[o + nxy];
-            rho_u = rhop[o + nxy];
-        }
-        int cz = -1;
-        R az = 0, bz = 1, Kz = 1, azh = 0, bzh = 1, Kzh = 1;
-        if (MODE == kSlab) {
-            cz = k < A.lo[2] ? k : (k >= A.hi[2] ? A.lo[2] + (k - A.hi[2]) : -1);
-            if (cz >= 0) {
-                const double* t = A.tab[2];
-                az = (R)t[k]; bz = (R)t[nz + k]; Kz = (R)t[2 * nz + k];
-                azh = (R)t[3 * nz + k]; bzh = (R)t[4 * nz + k]; Kzh = (R)t[5 * nz + k];
-            }
-        }
-        const int omx = (k * ny + j) * A.w[0] + cx;
-        const int omy = (k * A.w[1] + cy) * nx + i;
-        const int omz = (cz * ny + j) * nx + i;
-
-        if (rN && k >= 1) {  // CPU:377-413
-            const R vp = cpp[o];
-            const R lambda = rho_c * (vp * vp - (R)2 * cs_c * cs_c);
-            const R l2m = rho_c * vp * vp;
-            const R DT_l2m = dt * l2m, DT_l = dt * lambda;
-            R dvx_dx = (ldf(vxp, o + 1, A.isc[0]) - vx_c) * idx;
-            R dvy_dy = (vy_c - ldf(vyp, o - nx, A.isc[1])) * idy;
-            R dvz_dz = (vz_c - vz_m) * idz;
-            if (cx >= 0) dvx_dx = cpml(A.m[0], omx, axh, bxh, Kxh, dvx_dx, A.sc[9], A.isc[9]);
-            if (MODE == kSlab) {
-                if (cy >= 0) dvy_dy = cpml(A.m[1], omy, ay, by, Ky, dvy_dy, A.sc[10], A.isc[10]);
-                if (cz >= 0) dvz_dz = cpml(A.m[2], omz, az, bz, Kz, dvz_dz, A.sc[11], A.isc[11]);
-            }
-            R sxx = ldf(sxxp, o, A.isc[3]), syy = ldf(syyp, o, A.isc[4]), szz = ldf(szzp, o, A.isc[5]);
-            sxx = DT_l2m * dvx_dx + DT_l * (dvy_dy + dvz_dz) + sxx;
-            syy = DT_l * (dvx_dx + dvz_dz) + DT_l2m * dvy_dy + syy;
-            szz = DT_l * (dvx_dx + dvy_dy) + DT_l2m * dvz_dz + szz;
-            stf(sxxp, o, sxx, A.sc[3]);
-            stf(syyp, o, syy, A.sc[4]);
-            stf(szzp, o, szz, A.sc[5]);
-        }
-        if (rXY) {  // CPU:415-442, mu averaged over (i-1,j) and (i,j+1)
-            const R vs = ((R)csp[o - 1] + (R)csp[o + nx]) / (R)2;
-            const R rh = ((R)rhop[o - 1] + (R)rhop[o + nx]) / (R)2;
-            const R DT_mu = dt * (rh * vs * vs);
-            R dvy_dx = (vy_c - ldf(vyp, o - 1, A.isc[1])) * idx;
-            R dvx_dy = (ldf(vxp, o + nx, A.isc[0]) - vx_c) * idy;
-            if (cx >= 0) dvy_dx = cpml(A.m[3], omx, ax, bx, Kx, dvy_dx, A.sc[12], A.isc[12]);
-            if (MODE == kSlab && cy >= 0)
-                dvx_dy = cpml(A.m[4], omy, ayh, byh, Kyh, dvx_dy, A.sc[13], A.isc[13]);
-            R sxy = ldf(sxyp, o, A.isc[6]);
-            sxy = DT_mu * (dvy_dx + dvx_dy) + sxy;
-            stf(sxyp, o, sxy, A.sc[6]);
-        }
-        if (k <= A.kmax_shear) {
-            if (rXZ) {  // CPU:444-470, mu averaged over (i-1,k) and (i,k+1)
-                const R vs = ((R)csp[o - 1] + cs_u) / (R)2;
-                const R rh = ((R)rhop[o - 1] + rho_u) / (R)2;
-                const R DT_mu = dt * (rh * vs * vs);
-                R dvz_dx = (vz_c - ldf(vzp, o - 1, A.isc[2])) * idx;
-                R dvx_dz = (vx_u - vx_c) * idz;
-                if (cx >= 0) dvz_dx = cpml(A.m[5], omx, ax, bx, Kx, dvz_dx, A.sc[14], A.isc[14]);
-                if (MODE == kSlab && cz >= 0)
-                    dvx_dz = cpml(A.m[6], omz, azh, bzh, Kzh, dvx_dz, A.sc[15], A.isc[15]);
-                R sxz = ldf(sxzp, o, A.isc[7]);
-                sxz = DT_mu * (dvz_dx + dvx_dz) + sxz;
-                stf(sxzp, o, sxz, A.sc[7]);
-            }
-            if (rYZ) {  // CPU:472-497, mu averaged over (j+1,k) and (j,k+1)
-                const R vs = ((R)csp[o + nx] + cs_u) / (R)2;
-                const R rh = ((R)rhop[o + nx] + rho_u) / (R)2;
-                const R DT_mu = dt * (rh * vs * vs);
-                R dvz_dy = (ldf(vzp, o + nx, A.isc[2]) - vz_c) * idy;
-                R dvy_dz = (vy_u - vy_c) * idz;
-                if (MODE == kSlab) {
-                    if (cy >= 0) dvz_dy = cpml(A.m[7], omy, ayh, byh, Kyh, dvz_dy, A.sc[16], A.isc[16]);
-                    if (cz >= 0) dvy_dz = cpml(A.m[8], omz, azh, bzh, Kzh, dvy_dz, A.sc[17], A.isc[17]);
-                }
-                R syz = ldf(syzp, o, A.isc[8]);
-                syz = DT_mu * (dvz_dy + dvy_dz) + syz;
-                stf(syzp, o, syz, A.sc[8]);
-            }
-        }
-        vx_c = vx_u;
-        vy_c = vy_u;
-        cs_c = cs_u;
-        rho_c = rho_u;
-        vz_m = vz_c;
+    SPlane<R> P;
+    if (PIPE) stress_load<T, MODE, R>(A, L, kb, P);
+    for (int k = kb; k < ke; ++k) {
+        SPlane<R> N;
+        if (PIPE)
+            stress_load<T, MODE, R>(A, L, min(k + 1, ke - 1), N);  // next plane in flight
+        else
+            stress_load<T, MODE, R>(A, L, k, P);
+        stress_update<T, MODE, R>(A, L, k, P, Q);
+        Q.vx_c = P.vx_u;
+        Q.vy_c = P.vy_u;
+        Q.cs_c = P.cs_u;
+        Q.rho_c = P.rho_u;
+        Q.vz_m = P.vz_c;
+        if (PIPE) P = N;
     }
 }
 
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// ---------------------------------------------------------------------------
+// ===========================================================================
 // Velocity pass: vx, vy, vz + 9 memory variables; Dirichlet faces; optional
 // fused energy partial sums of the updated state.
-// ---------------------------------------------------------------------------
-template <typename T, int MODE, bool ENERGY>
+// ===========================================================================
+template <typename R>
+struct VPlane {
+    R sxx, syy, sxy, sxz, syz, szz_u, rho_u;   // own column: plane k, k+1
+    R sxxW, sxyS, rhoW, sxyE, syyN, rhoN, sxzE, syzS;  // in-plane neighbours
+    R vx, vy, vz;                              // velocities to update
+    R cp, cs;                                  // energy steps only
+    R mx[3], my[3], mz[3];
+};
+
+template <typename T, int MODE, bool ENERGY, typename R>
+__device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<R>& L, int k,
+                                              VPlane<R>& P)
+{
+    const unsigned nxy = A.nxy;
+    const unsigned o = unsigned(k) * nxy + unsigned(L.p);
+    const bool top = (k == A.nz - 1);
+    const unsigned dU = top ? 0u : nxy;
+    T *sxx = A.f[3], *syy = A.f[4], *szz = A.f[5], *sxy = A.f[6], *sxz = A.f[7], *syz = A.f[8];
+    const float* rho = A.rho;
+    P.sxx = ldf(sxx, o, A.isc[3]);
+    P.syy = ldf(syy, o, A.isc[4]);
+    P.sxy = ldf(sxy, o, A.isc[6]);
+    P.sxz = ldf(sxz, o, A.isc[7]);
+    P.syz = ldf(syz, o, A.isc[8]);
+    P.szz_u = ldf(szz, o + dU, A.isc[5]);
+    P.rho_u = rho[o + dU];
+    P.sxxW = ldf(sxx, o - L.dW, A.isc[3]);
+    P.sxyS = ldf(sxy, o - L.dS, A.isc[6]);
+    P.rhoW = rho[o - L.dW];
+    P.sxyE = ldf(sxy, o + L.dE, A.isc[6]);
+    P.syyN = ldf(syy, o + L.dN, A.isc[4]);
+    P.rhoN = rho[o + L.dN];
+    P.sxzE = ldf(sxz, o + L.dE, A.isc[7]);
+    P.syzS = ldf(syz, o - L.dS, A.isc[8]);
+    P.vx = ldf(A.f[0], o, A.isc[0]);
+    P.vy = ldf(A.f[1], o, A.isc[1]);
+    P.vz = ldf(A.f[2], o, A.isc[2]);
+    if (ENERGY) {
+        P.cp = A.cp[o];
+        P.cs = A.cs[o];
+    }
+    if (L.cx >= 0) {
+        const unsigned om = unsigned((k * A.ny + L.j) * A.w[0] + L.cx);
+        P.mx[0] = ldf(A.m[9], om, A.isc[18]);
+        P.mx[1] = ldf(A.m[12], om, A.isc[21]);
+        P.mx[2] = ldf(A.m[15], om, A.isc[24]);
+    }
+    if (MODE == kSlab) {
+        if (L.cy >= 0) {
+            const unsigned om = unsigned((k * A.w[1] + L.cy) * A.nx + L.i);
+            P.my[0] = ldf(A.m[10], om, A.isc[19]);
+            P.my[1] = ldf(A.m[13], om, A.isc[22]);
+            P.my[2] = ldf(A.m[16], om, A.isc[25]);
+        }
+        const int cz = k < A.lo[2] ? k : (k >= A.hi[2] ? A.lo[2] + (k - A.hi[2]) : -1);
+        if (cz >= 0) {
+            const unsigned om = unsigned((cz * A.ny + L.j) * A.nx + L.i);
+            P.mz[0] = ldf(A.m[11], om, A.isc[20]);
+            P.mz[1] = ldf(A.m[14], om, A.isc[23]);
+            P.mz[2] = ldf(A.m[17], om, A.isc[26]);
+        }
+    }
+    if (top) {
+        P.szz_u = 0;
+        P.rho_u = 0;
+    }
+}
+
+template <typename R>
+struct VQueue {
+    R sxz_m, syz_m, szz_c, rho_c;
+};
+
+template <typename T, int MODE, bool ENERGY, typename R>
+__device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane<R>& L, int k,
+                                                const VPlane<R>& P, const VQueue<R>& Q,
+                                                double& ek, double& ep)
+{
+    const unsigned o = unsigned(k) * unsigned(A.nxy) + unsigned(L.p);
+    const R dt = (R)A.dt, idx = (R)A.idx, idy = (R)A.idy, idz = (R)A.idz;
+    const ZTab<R> Z = ztab<T, MODE, R>(A, k);
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const unsigned omx = unsigned((k * ny + L.j) * A.w[0] + L.cx);
+    const unsigned omy = unsigned((k * A.w[1] + L.cy) * nx + L.i);
+    const unsigned omz = unsigned((Z.cz * ny + L.j) * nx + L.i);
+    const bool face = A.dirichlet &&
+                      (L.i == 0 || L.i == nx - 1 || L.j == 0 || L.j == ny - 1 || k == 0 || k == nz - 1);
+
+    // vx: i in [1,nx-1], j in [1,ny-1], k in [1,nz-1] (CPU:502-529)
+    const bool onx = (L.i >= 1) && (L.j >= 1) && (k >= 1);
+    R vx = P.vx;
+    {
+        const R rh = (Q.rho_c + P.rhoW) / (R)2;
+        const R DT_over_rho = dt / rh;
+        R d1 = (P.sxx - P.sxxW) * idx;
+        R d2 = (P.sxy - P.sxyS) * idy;
+        R d3 = (P.sxz - Q.sxz_m) * idz;
+        R m9 = 0, m10 = 0, m11 = 0;
+        if (L.cx >= 0) m9 = cpml(P.mx[0], L.ax, L.bx, L.Kx, d1);
+        if (MODE == kSlab) {
+            if (L.cy >= 0) m10 = cpml(P.my[0], L.ay, L.by, L.Ky, d2);
+            if (Z.cz >= 0) m11 = cpml(P.mz[0], Z.az, Z.bz, Z.Kz, d3);
+        }
+        if (onx) {
+            vx = DT_over_rho * (d1 + d2 + d3) + P.vx;
+            if (L.cx >= 0) stf(A.m[9], omx, m9, A.sc[18]);
+            if (MODE == kSlab) {
+                if (L.cy >= 0) stf(A.m[10], omy, m10, A.sc[19]);
+                if (Z.cz >= 0) stf(A.m[11], omz, m11, A.sc[20]);
+            }
+        }
+    }
+    // vy: i in [0,nx-2], j in [0,ny-2], k in [1,nz-1] (CPU:531-558)
+    const bool ony = (L.i <= nx - 2) && (L.j <= ny - 2) && (k >= 1);
+    R vy = P.vy;
+    {
+        const R rh = (Q.rho_c + P.rhoN) / (R)2;
+        const R DT_over_rho = dt / rh;
+        R d1 = (P.sxyE - P.sxy) * idx;
+        R d2 = (P.syyN - P.syy) * idy;
+        R d3 = (P.syz - Q.syz_m) * idz;
+        R m12 = 0, m13 = 0, m14 = 0;
+        if (L.cx >= 0) m12 = cpml(P.mx[1], L.axh, L.bxh, L.Kxh, d1);
+        if (MODE == kSlab) {
+            if (L.cy >= 0) m13 = cpml(P.my[1], L.ayh, L.byh, L.Kyh, d2);
+            if (Z.cz >= 0) m14 = cpml(P.mz[1], Z.az, Z.bz, Z.Kz, d3);
+        }
+        if (ony) {
+            vy = DT_over_rho * (d1 + d2 + d3) + P.vy;
+            if (L.cx >= 0) stf(A.m[12], omx, m12, A.sc[21]);
+            if (MODE == kSlab) {
+                if (L.cy >= 0) stf(A.m[13], omy, m13, A.sc[22]);
+                if (Z.cz >= 0) stf(A.m[14], omz, m14, A.sc[23]);
+            }
+        }
+    }
+    // vz: i in [0,nx-2], j in [1,ny-1], k in [0,nz-2] (CPU:562-590)
+    const bool onz = (L.i <= nx - 2) && (L.j >= 1) && (k <= nz - 2);
+    R vz = P.vz;
+    {
+        const R rh = (Q.rho_c + P.rho_u) / (R)2;
+        const R DT_over_rho = dt / rh;
+        R d1 = (P.sxzE - P.sxz) * idx;
+        R d2 = (P.syz - P.syzS) * idy;
+        R d3 = (P.szz_u - Q.szz_c) * idz;
+        R m15 = 0, m16 = 0, m17 = 0;
+        if (L.cx >= 0) m15 = cpml(P.mx[2], L.axh, L.bxh, L.Kxh, d1);
+        if (MODE == kSlab) {
+            if (L.cy >= 0) m16 = cpml(P.my[2], L.ay, L.by, L.Ky, d2);
+            if (Z.cz >= 0) m17 = cpml(P.mz[2], Z.azh, Z.bzh, Z.Kzh, d3);
+        }
+        if (onz) {
+            vz = DT_over_rho * (d1 + d2 + d3) + P.vz;
+            if (L.cx >= 0) stf(A.m[15], omx, m15, A.sc[24]);
+            if (MODE == kSlab) {
+                if (L.cy >= 0) stf(A.m[16], omy, m16, A.sc[25]);
+                if (Z.cz >= 0) stf(A.m[17], omz, m17, A.sc[26]);
+            }
+        }
+    }
+    // rigid faces: the memory variables above are still advanced, then v = 0
+    if (face) {
+        vx = 0;
+        vy = 0;
+        vz = 0;
+    }
+    if (onx || face) stf(A.f[0], o, vx, A.sc[0]);
+    if (ony || face) stf(A.f[1], o, vy, A.sc[1]);
+    if (onz || face) stf(A.f[2], o, vz, A.sc[2]);
+    if (ENERGY) {
+        const int em = A.emargin;
+        if (L.i >= em && L.i < nx - 1 - em && L.j >= em && L.j < ny - 1 - em && k >= em &&
+            k < nz - 1 - em) {
+            // state after this velocity update: cell-local values, FP64 terms
+            const double r = Q.rho_c, vp = P.cp, vs = P.cs;
+            const double mu = r * vs * vs, lam = r * (vp * vp - 2.0 * vs * vs);
+            const double sxx = P.sxx, syy = P.syy, szz = Q.szz_c;
+            const double sxy = P.sxy, sxz = P.sxz, syz = P.syz;
+            const double tr = sxx + syy + szz;
+            const double ss = sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + sxz * sxz + syz * syz);
+            const double dvx = vx, dvy = vy, dvz = vz;
+            ek += 0.5 * r * (dvx * dvx + dvy * dvy + dvz * dvz);
+            ep += 0.5 * (ss / (2.0 * mu) - lam * tr * tr / (2.0 * mu * (3.0 * lam + 2.0 * mu)));
+        }
+    }
+}
+
+template <typename T, int MODE, bool ENERGY, bool PIPE>
 __device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B, int local,
                                               double& ek, double& ep)
 {
     typedef typename Arith<T>::R R;
     const int chunk = local % B.nchunk, zs = local / B.nchunk;
     const int p = B.p0 + chunk * A.block + (int)threadIdx.x;
-    const bool active = p < B.p1;
-    if (!active) return;
+    if (p >= B.p1) return;
     const int kb = B.k0 + zs * B.zseg;
     const int ke = min(kb + B.zseg, B.k1);
-    const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = A.nxy;
-    const int j = p / nx, i = p - j * nx;
-
-    const bool rVX = (i >= 1) && (j >= 1);            // vx: i in [1,nx-1], j in [1,ny-1]
-    const bool rVY = (i <= nx - 2) && (j <= ny - 2);  // vy: i in [0,nx-2], j in [0,ny-2]
-    const bool rVZ = (i <= nx - 2) && (j >= 1);       // vz: i in [0,nx-2], j in [1,ny-1]
-    const bool face_ij = (i == 0) || (i == nx - 1) || (j == 0) || (j == ny - 1);
-    const R dt = (R)A.dt, idx = (R)A.idx, idy = (R)A.idy, idz = (R)A.idz;
-
-    T *vxp = A.f[0], *vyp = A.f[1], *vzp = A.f[2];
-    T *sxxp = A.f[3], *syyp = A.f[4], *szzp = A.f[5], *sxyp = A.f[6], *sxzp = A.f[7],
-      *syzp = A.f[8];
-    const float* rhop = A.rho;
-
-    const int cx = i < A.lo[0] ? i : (i >= A.hi[0] ? A.lo[0] + (i - A.hi[0]) : -1);
-    R ax = 0, bx = 1, Kx = 1, axh = 0, bxh = 1, Kxh = 1;
-    if (cx >= 0 && active) {
-        const double* t = A.tab[0];
-        ax = (R)t[i]; bx = (R)t[nx + i]; Kx = (R)t[2 * nx + i];
-        axh = (R)t[3 * nx + i]; bxh = (R)t[4 * nx + i]; Kxh = (R)t[5 * nx + i];
+    Lane<R> L;
+    make_lane<T, MODE, R>(A, p, L);
+    VQueue<R> Q;
+    {
+        const unsigned o = unsigned(kb) * unsigned(A.nxy) + unsigned(p);
+        Q.sxz_m = kb >= 1 ? ldf(A.f[7], o - unsigned(A.nxy), A.isc[7]) : (R)0;
+        Q.syz_m = kb >= 1 ? ldf(A.f[8], o - unsigned(A.nxy), A.isc[8]) : (R)0;
+        Q.szz_c = ldf(A.f[5], o, A.isc[5]);
+        Q.rho_c = A.rho[o];
     }
-    int cy = -1;
-    R ay = 0, by = 1, Ky = 1, ayh = 0, byh = 1, Kyh = 1;
-    if (MODE == kSlab && active) {
-        cy = j < A.lo[1] ? j : (j >= A.hi[1] ? A.lo[1] + (j - A.hi[1]) : -1);
-        if (cy >= 0) {
-            const double* t = A.tab[1];
-            ay = (R)t[j]; by = (R)t[ny + j]; Ky = (R)t[2 * ny + j];
-            ayh = (R)t[3 * ny + j]; byh = (R)t[4 * ny + j]; Kyh = (R)t[5 * ny + j];
-        }
-    }
-    const int em = A.emargin;
-    const bool ebox_ij = ENERGY && active && i >= em && i < nx - 1 - em && j >= em && j < ny - 1 - em;
-
-    int o = kb * nxy + p;
-    R sxz_m = 0, syz_m = 0, szz_c = 0, rho_c = 0;
-    if (active) {
-        if (kb >= 1) {
-            sxz_m = ldf(sxzp, o - nxy, A.isc[7]);
-            syz_m = ldf(syzp, o - nxy, A.isc[8]);
-        }
-        szz_c = ldf(szzp, o, A.isc[5]);
-        rho_c = rhop[o];
-    }
-
-    for (int k = kb; k < ke; ++k, o += nxy) {
-        const bool top = (k == nz - 1);
-        const R sxx_c = ldf(sxxp, o, A.isc[3]);
-        const R syy_c = ldf(syyp, o, A.isc[4]);
-        const R sxy_c = ldf(sxyp, o, A.isc[6]);
-        const R sxz_c = ldf(sxzp, o, A.isc[7]);
-        const R syz_c = ldf(syzp, o, A.isc[8]);
-        R szz_u = 0, rho_u = 0;
-        if (!top) {
-            szz_u = ldf(szzp, o + nxy, A.isc[5]);
-            rho_u = rhop[o + nxy];
-        }
-        int cz = -1;
-        R az = 0, bz = 1, Kz = 1, azh = 0, bzh = 1, Kzh = 1;
-        if (MODE == kSlab) {
-            cz = k < A.lo[2] ? k : (k >= A.hi[2] ? A.lo[2] + (k - A.hi[2]) : -1);
-            if (cz >= 0) {
-                const double* t = A.tab[2];
-                az = (R)t[k]; bz = (R)t[nz + k]; Kz = (R)t[2 * nz + k];
-                azh = (R)t[3 * nz + k]; bzh = (R)t[4 * nz + k]; Kzh = (R)t[5 * nz + k];
-            }
-        }
-        const int omx = (k * ny + j) * A.w[0] + cx;
-        const int omy = (k * A.w[1] + cy) * nx + i;
-        const int omz = (cz * ny + j) * nx + i;
-        const bool face = A.dirichlet && (face_ij || k == 0 || top);
-        bool ux = false, uy = false, uz = false;
-        R vx_n = 0, vy_n = 0, vz_n = 0;
-        {
-            if (rVX && k >= 1) {  // CPU:502-529
-                const R rh = (rho_c + (R)rhop[o - 1]) / (R)2;
-                const R DT_over_rho = dt / rh;
-                R d1 = (sxx_c - ldf(sxxp, o - 1, A.isc[3])) * idx;
-                R d2 = (sxy_c - ldf(sxyp, o - nx, A.isc[6])) * idy;
-                R d3 = (sxz_c - sxz_m) * idz;
-                if (cx >= 0) d1 = cpml(A.m[9], omx, ax, bx, Kx, d1, A.sc[18], A.isc[18]);
-                if (MODE == kSlab) {
-                    if (cy >= 0) d2 = cpml(A.m[10], omy, ay, by, Ky, d2, A.sc[19], A.isc[19]);
-                    if (cz >= 0) d3 = cpml(A.m[11], omz, az, bz, Kz, d3, A.sc[20], A.isc[20]);
-                }
-                vx_n = DT_over_rho * (d1 + d2 + d3) + ldf(vxp, o, A.isc[0]);
-                ux = true;
-            }
-            if (rVY && k >= 1) {  // CPU:531-558
-                const R rh = (rho_c + (R)rhop[o + nx]) / (R)2;
-                const R DT_over_rho = dt / rh;
-                R d1 = (ldf(sxyp, o + 1, A.isc[6]) - sxy_c) * idx;
-                R d2 = (ldf(syyp, o + nx, A.isc[4]) - syy_c) * idy;
-                R d3 = (syz_c - syz_m) * idz;
-                if (cx >= 0) d1 = cpml(A.m[12], omx, axh, bxh, Kxh, d1, A.sc[21], A.isc[21]);
-                if (MODE == kSlab) {
-                    if (cy >= 0) d2 = cpml(A.m[13], omy, ayh, byh, Kyh, d2, A.sc[22], A.isc[22]);
-                    if (cz >= 0) d3 = cpml(A.m[14], omz, az, bz, Kz, d3, A.sc[23], A.isc[23]);
-                }
-                vy_n = DT_over_rho * (d1 + d2 + d3) + ldf(vyp, o, A.isc[1]);
-                uy = true;
-            }
-            if (rVZ && k <= nz - 2) {  // CPU:562-590
-                const R rh = (rho_c + rho_u) / (R)2;
-                const R DT_over_rho = dt / rh;
-                R d1 = (ldf(sxzp, o + 1, A.isc[7]) - sxz_c) * idx;
-                R d2 = (syz_c - ldf(syzp, o - nx, A.isc[8])) * idy;
-                R d3 = (szz_u - szz_c) * idz;
-                if (cx >= 0) d1 = cpml(A.m[15], omx, axh, bxh, Kxh, d1, A.sc[24], A.isc[24]);
-                if (MODE == kSlab) {
-                    if (cy >= 0) d2 = cpml(A.m[16], omy, ay, by, Ky, d2, A.sc[25], A.isc[25]);
-                    if (cz >= 0) d3 = cpml(A.m[17], omz, azh, bzh, Kzh, d3, A.sc[26], A.isc[26]);
-                }
-                vz_n = DT_over_rho * (d1 + d2 + d3) + ldf(vzp, o, A.isc[2]);
-                uz = true;
-            }
-        }
-        // rigid faces: the memory variables above are still advanced, then v = 0
-        if (face) {
-            vx_n = vy_n = vz_n = (R)0;
-            ux = uy = uz = true;
-        }
-        if (ux) stf(vxp, o, vx_n, A.sc[0]);
-        if (uy) stf(vyp, o, vy_n, A.sc[1]);
-        if (uz) stf(vzp, o, vz_n, A.sc[2]);
-        if (ENERGY && ebox_ij && k >= em && k < nz - 1 - em) {
-            // state after this velocity update: cell-local values, FP64 terms
-            const double vx = ux ? (double)vx_n : (double)ldf(vxp, o, A.isc[0]);
-            const double vy = uy ? (double)vy_n : (double)ldf(vyp, o, A.isc[1]);
-            const double vz = uz ? (double)vz_n : (double)ldf(vzp, o, A.isc[2]);
-            const double r = rho_c, vp = A.cp[o], vs = A.cs[o];
-            const double mu = r * vs * vs, lam = r * (vp * vp - 2.0 * vs * vs);
-            const double sxx = sxx_c, syy = syy_c, szz = szz_c;
-            const double sxy = sxy_c, sxz = sxz_c, syz = syz_c;
-            const double tr = sxx + syy + szz;
-            const double ss = sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + sxz * sxz + syz * syz);
-            ek += 0.5 * r * (vx * vx + vy * vy + vz * vz);
-            ep += 0.5 * (ss / (2.0 * mu) - lam * tr * tr / (2.0 * mu * (3.0 * lam + 2.0 * mu)));
-        }
-        sxz_m = sxz_c;
-        syz_m = syz_c;
-        szz_c = szz_u;
-        rho_c = rho_u;
+    VPlane<R> P;
+    if (PIPE) velocity_load<T, MODE, ENERGY, R>(A, L, kb, P);
+    for (int k = kb; k < ke; ++k) {
+        VPlane<R> N;
+        if (PIPE)
+            velocity_load<T, MODE, ENERGY, R>(A, L, min(k + 1, ke - 1), N);
+        else
+            velocity_load<T, MODE, ENERGY, R>(A, L, k, P);
+        velocity_update<T, MODE, ENERGY, R>(A, L, k, P, Q, ek, ep);
+        Q.sxz_m = P.sxz;
+        Q.syz_m = P.syz;
+        Q.szz_c = P.szz_u;
+        Q.rho_c = P.rho_u;
+        if (PIPE) P = N;
     }
 }
 
@@ -398,27 +566,27 @@ __device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B
 // One launch per pass: the C-PML slab boxes come first in logical block order,
 // then the interior box; each block runs the body specialised for its box.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool PIPE>
 __global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
 {
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const Box B = pick_box(A.box, A.nbox, lb);
     if (B.mode == kSlab)
-        stress_body<T, kSlab>(A, B, lb - B.blk0);
+        stress_body<T, kSlab, PIPE>(A, B, lb - B.blk0);
     else
-        stress_body<T, kInterior>(A, B, lb - B.blk0);
+        stress_body<T, kInterior, PIPE>(A, B, lb - B.blk0);
 }
 
-template <typename T, bool ENERGY>
+template <typename T, bool ENERGY, bool PIPE>
 __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
 {
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const Box B = pick_box(A.box, A.nbox, lb);
     double ek = 0.0, ep = 0.0;
     if (B.mode == kSlab)
-        velocity_body<T, kSlab, ENERGY>(A, B, lb - B.blk0, ek, ep);
+        velocity_body<T, kSlab, ENERGY, PIPE>(A, B, lb - B.blk0, ek, ep);
     else
-        velocity_body<T, kInterior, ENERGY>(A, B, lb - B.blk0, ek, ep);
+        velocity_body<T, kInterior, ENERGY, PIPE>(A, B, lb - B.blk0, ek, ep);
     if (ENERGY) {  // per-block FP64 partials, summed in block order by the tail kernel
         __shared__ double red[2][4];
         ek = wave_sum(ek);
@@ -442,145 +610,32 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
     }
 }
 
-// ---------------------------------------------------------------------------
-// Per-step tail (one workgroup): receivers (before the source, as CPU:593-602
-// precedes CPU:604-622), energy reduction, source injection, step counter.
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(256) k_tail(const TailArgs<T> A)
-{
-    typedef typename Arith<T>::R R;
-    __shared__ double red[2][4];
-    __shared__ int bad;
-    const int tid = threadIdx.x;
-    const int it = *A.step + 1;
-    if (tid == 0) bad = 0;
-    __syncthreads();
-    if (A.nrec > 0 && it % A.sample_every == 0) {
-        const int s = it / A.sample_every - 1;
-        if (s < A.nsamples) {
-            for (int r = tid; r < A.nrec; r += blockDim.x) {
-                const int o = A.rec[3 * r] + A.rec[3 * r + 1] * A.nx + A.rec[3 * r + 2] * A.nxy;
-                const double vx = (double)ldf(A.f[0], o, A.isc[0]);
-                const double vy = (double)ldf(A.f[1], o, A.isc[1]);
-                const double vz = (double)ldf(A.f[2], o, A.isc[2]);
-                double* dst = A.seis + (size_t(s) * A.nrec + r) * 3;
-                dst[0] = vx;
-                dst[1] = vy;
-                dst[2] = vz;
-                if (!isfinite(vx) || !isfinite(vy) || !isfinite(vz)) bad = 1;
-            }
-        }
-    }
-    if (A.energy_every > 0 && it % A.energy_every == 0) {
-        const int s = it / A.energy_every - 1;
-        double a = 0.0, b = 0.0;
-        for (int q = tid; q < A.nepart; q += blockDim.x) {
-            a += A.epart[2 * q];
-            b += A.epart[2 * q + 1];
-        }
-        a = wave_sum(a);
-        b = wave_sum(b);
-        if ((tid & 63) == 0) {
-            red[0][tid >> 6] = a;
-            red[1][tid >> 6] = b;
-        }
-        __syncthreads();
-        if (tid == 0 && s < A.nenergy) {
-            double ea = 0.0, eb = 0.0;
-            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-                ea += red[0][w];
-                eb += red[1][w];
-            }
-            A.energy[2 * s] = ea * A.evol;
-            A.energy[2 * s + 1] = eb * A.evol;
-            if (!isfinite(ea) || !isfinite(eb)) bad = 1;
-        }
-    }
-    __syncthreads();
-    if (tid == 0) {
-        if (it - 1 < A.nwavelet) {
-            const double st = A.wavelet[it - 1];
-            const int o = A.src;
-            const R DT = (R)A.dt;
-            const R l2m = (R)A.rho[o] * (R)A.cp[o] * (R)A.cp[o];
-            switch (A.src_kind) {
-            case 0: {  // CPU:620-622
-                const R fx = (R)(A.sin_t * st), fy = (R)(A.cos_t * st);
-                stf(A.f[3], o, ldf(A.f[3], o, A.isc[3]) + fx * DT * l2m, A.sc[3]);
-                stf(A.f[4], o, ldf(A.f[4], o, A.isc[4]) + fx * DT * l2m, A.sc[4]);
-                stf(A.f[5], o, ldf(A.f[5], o, A.isc[5]) + fy * DT * l2m, A.sc[5]);
-                break;
-            }
-            case 1: {  // CPU:614-617
-                const R fx = (R)(A.sin_t * st), fy = (R)(A.cos_t * st);
-                const R rx = ((R)A.rho[o] + (R)A.rho[o - 1]) / (R)2;
-                const R ry = ((R)A.rho[o] + (R)A.rho[o + A.nx]) / (R)2;
-                stf(A.f[0], o, ldf(A.f[0], o, A.isc[0]) + fx * DT / rx, A.sc[0]);
-                stf(A.f[1], o, ldf(A.f[1], o, A.isc[1]) + fy * DT / ry, A.sc[1]);
-                break;
-            }
-            case 2: {
-                const R s = (R)st;
-                stf(A.f[3], o, ldf(A.f[3], o, A.isc[3]) + s * DT * l2m, A.sc[3]);
-                stf(A.f[4], o, ldf(A.f[4], o, A.isc[4]) + s * DT * l2m, A.sc[4]);
-                stf(A.f[5], o, ldf(A.f[5], o, A.isc[5]) + s * DT * l2m, A.sc[5]);
-                break;
-            }
-            default: {  // ZC:451-453
-                const R s = (R)st;
-                const R rx = ((R)A.rho[o] + (R)A.rho[o - 1]) / (R)2;
-                const R ry = ((R)A.rho[o] + (R)A.rho[o + A.nx]) / (R)2;
-                const R rz = ((R)A.rho[o] + (R)A.rho[o + A.nxy]) / (R)2;
-                stf(A.f[0], o, ldf(A.f[0], o, A.isc[0]) + s * DT / rx, A.sc[0]);
-                stf(A.f[1], o, ldf(A.f[1], o, A.isc[1]) + s * DT / ry, A.sc[1]);
-                stf(A.f[2], o, ldf(A.f[2], o, A.isc[2]) + s * DT / rz, A.sc[2]);
-                break;
-            }
-            }
-        }
-        if (bad) *A.flag = 1;
-        *A.step = it;
-    }
-}
-
-__global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ s, float4* __restrict__ d, long n)
-{
-    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) d[i] = s[i];
-}
-
-// ---------------------------------------------------------------------------
 template <typename T>
 void launch_stress(const StepArgs<T>& a, hipStream_t s)
 {
     if (a.nblocks <= 0) return;
-    hipLaunchKernelGGL((k_stress<T>), dim3(a.nblocks), dim3(a.block), 0, s, a);
+    if (a.pipe)
+        hipLaunchKernelGGL((k_stress<T, true>), dim3(a.nblocks), dim3(a.block), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_stress<T, false>), dim3(a.nblocks), dim3(a.block), 0, s, a);
 }
 
 template <typename T>
 void launch_velocity(const StepArgs<T>& a, bool energy, hipStream_t s)
 {
     if (a.nblocks <= 0) return;
-    if (energy)
-        hipLaunchKernelGGL((k_velocity<T, true>), dim3(a.nblocks), dim3(a.block), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_velocity<T, false>), dim3(a.nblocks), dim3(a.block), 0, s, a);
-}
-
-template <typename T>
-void launch_tail(const TailArgs<T>& a, hipStream_t s)
-{
-    hipLaunchKernelGGL((k_tail<T>), dim3(1), dim3(256), 0, s, a);
-}
-
-void launch_copy(const float4* src, float4* dst, long n4, hipStream_t s)
-{
-    long blocks = (n4 + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n4);
+    const dim3 g(a.nblocks), b(a.block);
+    if (energy) {
+        if (a.pipe)
+            hipLaunchKernelGGL((k_velocity<T, true, true>), g, b, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_velocity<T, true, false>), g, b, 0, s, a);
+    } else {
+        if (a.pipe)
+            hipLaunchKernelGGL((k_velocity<T, false, true>), g, b, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_velocity<T, false, false>), g, b, 0, s, a);
+    }
 }
 
 template void launch_stress<float>(const StepArgs<float>&, hipStream_t);
@@ -589,8 +644,5 @@ template void launch_stress<half_t>(const StepArgs<half_t>&, hipStream_t);
 template void launch_velocity<float>(const StepArgs<float>&, bool, hipStream_t);
 template void launch_velocity<double>(const StepArgs<double>&, bool, hipStream_t);
 template void launch_velocity<half_t>(const StepArgs<half_t>&, bool, hipStream_t);
-template void launch_tail<float>(const TailArgs<float>&, hipStream_t);
-template void launch_tail<double>(const TailArgs<double>&, hipStream_t);
-template void launch_tail<half_t>(const TailArgs<half_t>&, hipStream_t);
 
 }  // namespace ela3d

@@ -42,6 +42,7 @@ struct StepArgs {
     double* epart;           // energy partials [nblocks][2]
     Box box[kMaxBox];
     int nbox, nblocks, block;
+    int pipe;                // 1: issue plane k+1's loads before computing plane k
 };
 
 template <typename T>

@@ -440,6 +440,7 @@ private:
         for (int q = 0; q < g.nbox; ++q) a.box[q] = g.box[q];
         a.nbox = g.nbox;
         a.nblocks = g.nblocks;
+        a.pipe = tune_.pipe;
         a.block = g.block;
         return a;
     }

@@ -32,6 +32,7 @@ struct Tuning {
     int block = 256;
     int zseg_interior = 16;
     int zseg_slab = 8;
+    int pipe = 1;
 };
 
 class Shot {

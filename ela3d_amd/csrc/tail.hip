@@ -1,0 +1,135 @@
+// tail.hip -- per-step tail kernel (receivers, energy reduction, source, step
+// counter) and the stream-copy kernel used to calibrate the HBM ceiling.
+#include "device_util.hpp"
+
+namespace ela3d {
+
+// ---------------------------------------------------------------------------
+// Per-step tail (one workgroup): receivers (before the source, as CPU:593-602
+// precedes CPU:604-622), energy reduction, source injection, step counter.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_tail(const TailArgs<T> A)
+{
+    typedef typename Arith<T>::R R;
+    __shared__ double red[2][4];
+    __shared__ int bad;
+    const int tid = threadIdx.x;
+    const int it = *A.step + 1;
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    if (A.nrec > 0 && it % A.sample_every == 0) {
+        const int s = it / A.sample_every - 1;
+        if (s < A.nsamples) {
+            for (int r = tid; r < A.nrec; r += blockDim.x) {
+                const int o = A.rec[3 * r] + A.rec[3 * r + 1] * A.nx + A.rec[3 * r + 2] * A.nxy;
+                const double vx = (double)ldf(A.f[0], o, A.isc[0]);
+                const double vy = (double)ldf(A.f[1], o, A.isc[1]);
+                const double vz = (double)ldf(A.f[2], o, A.isc[2]);
+                double* dst = A.seis + (size_t(s) * A.nrec + r) * 3;
+                dst[0] = vx;
+                dst[1] = vy;
+                dst[2] = vz;
+                if (!isfinite(vx) || !isfinite(vy) || !isfinite(vz)) bad = 1;
+            }
+        }
+    }
+    if (A.energy_every > 0 && it % A.energy_every == 0) {
+        const int s = it / A.energy_every - 1;
+        double a = 0.0, b = 0.0;
+        for (int q = tid; q < A.nepart; q += blockDim.x) {
+            a += A.epart[2 * q];
+            b += A.epart[2 * q + 1];
+        }
+        a = wave_sum(a);
+        b = wave_sum(b);
+        if ((tid & 63) == 0) {
+            red[0][tid >> 6] = a;
+            red[1][tid >> 6] = b;
+        }
+        __syncthreads();
+        if (tid == 0 && s < A.nenergy) {
+            double ea = 0.0, eb = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+                ea += red[0][w];
+                eb += red[1][w];
+            }
+            A.energy[2 * s] = ea * A.evol;
+            A.energy[2 * s + 1] = eb * A.evol;
+            if (!isfinite(ea) || !isfinite(eb)) bad = 1;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        if (it - 1 < A.nwavelet) {
+            const double st = A.wavelet[it - 1];
+            const int o = A.src;
+            const R DT = (R)A.dt;
+            const R l2m = (R)A.rho[o] * (R)A.cp[o] * (R)A.cp[o];
+            switch (A.src_kind) {
+            case 0: {  // CPU:620-622
+                const R fx = (R)(A.sin_t * st), fy = (R)(A.cos_t * st);
+                stf(A.f[3], o, ldf(A.f[3], o, A.isc[3]) + fx * DT * l2m, A.sc[3]);
+                stf(A.f[4], o, ldf(A.f[4], o, A.isc[4]) + fx * DT * l2m, A.sc[4]);
+                stf(A.f[5], o, ldf(A.f[5], o, A.isc[5]) + fy * DT * l2m, A.sc[5]);
+                break;
+            }
+            case 1: {  // CPU:614-617
+                const R fx = (R)(A.sin_t * st), fy = (R)(A.cos_t * st);
+                const R rx = ((R)A.rho[o] + (R)A.rho[o - 1]) / (R)2;
+                const R ry = ((R)A.rho[o] + (R)A.rho[o + A.nx]) / (R)2;
+                stf(A.f[0], o, ldf(A.f[0], o, A.isc[0]) + fx * DT / rx, A.sc[0]);
+                stf(A.f[1], o, ldf(A.f[1], o, A.isc[1]) + fy * DT / ry, A.sc[1]);
+                break;
+            }
+            case 2: {
+                const R s = (R)st;
+                stf(A.f[3], o, ldf(A.f[3], o, A.isc[3]) + s * DT * l2m, A.sc[3]);
+                stf(A.f[4], o, ldf(A.f[4], o, A.isc[4]) + s * DT * l2m, A.sc[4]);
+                stf(A.f[5], o, ldf(A.f[5], o, A.isc[5]) + s * DT * l2m, A.sc[5]);
+                break;
+            }
+            default: {  // ZC:451-453
+                const R s = (R)st;
+                const R rx = ((R)A.rho[o] + (R)A.rho[o - 1]) / (R)2;
+                const R ry = ((R)A.rho[o] + (R)A.rho[o + A.nx]) / (R)2;
+                const R rz = ((R)A.rho[o] + (R)A.rho[o + A.nxy]) / (R)2;
+                stf(A.f[0], o, ldf(A.f[0], o, A.isc[0]) + s * DT / rx, A.sc[0]);
+                stf(A.f[1], o, ldf(A.f[1], o, A.isc[1]) + s * DT / ry, A.sc[1]);
+                stf(A.f[2], o, ldf(A.f[2], o, A.isc[2]) + s * DT / rz, A.sc[2]);
+                break;
+            }
+            }
+        }
+        if (bad) *A.flag = 1;
+        *A.step = it;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ s, float4* __restrict__ d, long n)
+{
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) d[i] = s[i];
+}
+
+// ---------------------------------------------------------------------------
+template <typename T>
+void launch_tail(const TailArgs<T>& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_tail<T>), dim3(1), dim3(256), 0, s, a);
+}
+
+void launch_copy(const float4* src, float4* dst, long n4, hipStream_t s)
+{
+    long blocks = (n4 + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n4);
+}
+
+template void launch_tail<float>(const TailArgs<float>&, hipStream_t);
+template void launch_tail<double>(const TailArgs<double>&, hipStream_t);
+template void launch_tail<half_t>(const TailArgs<half_t>&, hipStream_t);
+
+}  // namespace ela3d
