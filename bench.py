@@ -101,13 +101,20 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--calibrate", type=float, default=0.0,
                     help="only run the stream-copy kernel over this many GiB (HBM ceiling)")
+    ap.add_argument("--copy-variant", type=int, default=-1,
+                    help="copy kernel for --calibrate (0-3); default: all four")
+    ap.add_argument("--shape", default="",
+                    help="fixed launch shape block,zseg_interior,zseg_slab,pipe (skips the tuner)")
     args = ap.parse_args()
     if args.calibrate > 0:
         import ela3d_amd
         nbytes = int(args.calibrate * (1 << 30))
-        ms = ela3d_amd.copy_bench(nbytes, reps=20)
-        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
-                          "ms_per_copy": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        variants = [args.copy_variant] if args.copy_variant >= 0 else [0, 1, 2, 3]
+        for v in variants:
+            ms = ela3d_amd.copy_bench(nbytes, reps=20, variant=v)
+            print(json.dumps({"copy_variant": v, "copy_bytes_read": nbytes,
+                              "copy_bytes_written": nbytes, "ms_per_copy": ms,
+                              "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the time loop "
@@ -122,7 +129,11 @@ def main():
     cfg, model, desc = make_workload(args.workload, K + W)
     log(f"[bench] workload {args.workload}: {desc}; steps={K} warmup={W}")
     tuned = None
-    if not args.no_tune:
+    if args.shape:
+        b, zi, zs, pp = (int(x) for x in args.shape.split(","))
+        ela3d_amd.set_tuning(cfg, block=b, zseg_interior=zi, zseg_slab=zs, pipe=pp)
+        log(f"[bench] fixed launch shape block={b} zseg={zi}/{zs} pipe={pp}")
+    elif not args.no_tune:
         t = time.perf_counter()
         tuned = ela3d_amd.autotune(cfg, model, steps=6)
         log(f"[bench] autotune: block={tuned['block']} zseg={tuned['zseg_interior']}/"

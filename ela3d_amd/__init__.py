@@ -257,16 +257,25 @@ def autotune(cfg: Config, model, steps: int = 8) -> dict:
     return {k: getattr(t, k) for k, _ in TuningStruct._fields_}
 
 
+def set_tuning(cfg: Config, block: int, zseg_interior: int, zseg_slab: int, pipe: int = 1):
+    """Pin the launch shape later shots of this grid use (as if the tuner had chosen it)."""
+    from ._lib import TuningStruct
+    s, keep = cfg.to_struct()
+    t = TuningStruct(block=block, zseg_interior=zseg_interior, zseg_slab=zseg_slab, pipe=pipe,
+                     candidates=0, ms_per_step=0.0)
+    check(lib().ela3d_tuning_set(C.byref(s), C.byref(t)))
+
+
 def device_count() -> int:
     n = C.c_int32()
     lib().ela3d_device_count(C.byref(n))
     return n.value
 
 
-def copy_bench(nbytes: int, reps: int = 20, device: int = 0) -> float:
+def copy_bench(nbytes: int, reps: int = 20, device: int = 0, variant: int = 0) -> float:
     """ms per device-to-device copy of nbytes with the library's stream-copy kernel."""
     ms = C.c_double()
-    check(lib().ela3d_copy_bench(int(device), int(nbytes), int(reps), C.byref(ms)))
+    check(lib().ela3d_copy_bench(int(device), int(nbytes), int(reps), int(variant), C.byref(ms)))
     return ms.value
 
 

@@ -125,7 +125,7 @@ def lib() -> C.CDLL:
     L.ela3d_last_error.argtypes = []
     L.ela3d_last_error.restype = C.c_char_p
     L.ela3d_device_count.argtypes = [P(C.c_int32)]
-    L.ela3d_copy_bench.argtypes = [C.c_int32, C.c_int64, C.c_int32, P(C.c_double)]
+    L.ela3d_copy_bench.argtypes = [C.c_int32, C.c_int64, C.c_int32, C.c_int32, P(C.c_double)]
     L.ela3d_model_homogeneous.argtypes = [C.c_int32, C.c_int32, C.c_int32, f, f, f, P(f), P(f), P(f)]
     L.ela3d_layered_default.argtypes = [P(LayeredSpec)]
     L.ela3d_layered_default.restype = None

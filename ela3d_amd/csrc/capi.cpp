@@ -303,12 +303,14 @@ int ela3d_device_count(int32_t* out)
     return ELA3D_OK;
 }
 
-int ela3d_copy_bench(int32_t device, int64_t bytes, int32_t reps, double* ms_per_copy)
+int ela3d_copy_bench(int32_t device, int64_t bytes, int32_t reps, int32_t variant, double* ms_per_copy)
 {
     return guarded([&] {
         int rc = need_device(device);
         if (rc) return rc;
-        if (bytes < 16 || reps < 1 || !ms_per_copy) throw Error{ELA3D_E_INVALID, "bad argument"};
+        if (bytes < 16 || reps < 1 || !ms_per_copy || variant < 0 || variant > 3 ||
+            bytes > (int64_t(1) << 36))
+            throw Error{ELA3D_E_INVALID, "bad argument"};
         HIP_CHECK(hipSetDevice(device));
         const long n4 = long(bytes / 16);
         void *a = nullptr, *b = nullptr;
@@ -324,9 +326,9 @@ int ela3d_copy_bench(int32_t device, int64_t bytes, int32_t reps, double* ms_per
         HIP_CHECK(hipEventCreate(&t0));
         HIP_CHECK(hipEventCreate(&t1));
         HIP_CHECK(hipMemsetAsync(a, 0, size_t(n4) * 16, st));
-        ela3d::launch_copy((const float4*)a, (float4*)b, n4, st);  // warm-up
+        ela3d::launch_copy((const float4*)a, (float4*)b, n4, variant, st);  // warm-up
         HIP_CHECK(hipEventRecord(t0, st));
-        for (int r = 0; r < reps; ++r) ela3d::launch_copy((const float4*)a, (float4*)b, n4, st);
+        for (int r = 0; r < reps; ++r) ela3d::launch_copy((const float4*)a, (float4*)b, n4, variant, st);
         HIP_CHECK(hipEventRecord(t1, st));
         HIP_CHECK(hipEventSynchronize(t1));
         float ms = 0.f;

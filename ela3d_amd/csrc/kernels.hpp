@@ -74,6 +74,6 @@ template <typename T>
 void launch_velocity(const StepArgs<T>& a, bool energy, hipStream_t s);
 template <typename T>
 void launch_tail(const TailArgs<T>& a, hipStream_t s);
-void launch_copy(const float4* src, float4* dst, long n4, hipStream_t s);
+void launch_copy(const float4* src, float4* dst, long n4, int variant, hipStream_t s);
 
 }  // namespace ela3d

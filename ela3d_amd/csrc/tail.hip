@@ -106,11 +106,44 @@ __global__ void __launch_bounds__(256) k_tail(const TailArgs<T> A)
     }
 }
 
+// HBM-ceiling calibration copies.  variant 0: grid-stride float4; 1: one
+// float4 per thread, one pass; 2: four float4 per thread with nontemporal
+// loads and stores; 3: one float per thread (the stencil kernels' access width).
 __global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ s, float4* __restrict__ d, long n)
 {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const long stride = (long)gridDim.x * blockDim.x;
     for (; i < n; i += stride) d[i] = s[i];
+}
+
+__global__ void __launch_bounds__(256) k_copy1(const float4* __restrict__ s, float4* __restrict__ d, long n)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) k_copy4nt(const f4v* __restrict__ s, f4v* __restrict__ d, long n)
+{
+    const long base = blockIdx.x * 1024L + threadIdx.x;
+    f4v v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const long i = base + 256L * u;
+        if (i < n) v[u] = __builtin_nontemporal_load(s + i);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const long i = base + 256L * u;
+        if (i < n) __builtin_nontemporal_store(v[u], d + i);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_copyf(const float* __restrict__ s, float* __restrict__ d, long n)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) d[i] = s[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -120,12 +153,23 @@ void launch_tail(const TailArgs<T>& a, hipStream_t s)
     hipLaunchKernelGGL((k_tail<T>), dim3(1), dim3(256), 0, s, a);
 }
 
-void launch_copy(const float4* src, float4* dst, long n4, hipStream_t s)
+void launch_copy(const float4* src, float4* dst, long n4, int variant, hipStream_t s)
 {
-    long blocks = (n4 + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n4);
+    if (variant == 1) {
+        hipLaunchKernelGGL(k_copy1, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, s, src, dst, n4);
+    } else if (variant == 2) {
+        hipLaunchKernelGGL(k_copy4nt, dim3((unsigned)((n4 + 1023) / 1024)), dim3(256), 0, s,
+                           (const f4v*)src, (f4v*)dst, n4);
+    } else if (variant == 3) {
+        const long nf = 4 * n4;
+        hipLaunchKernelGGL(k_copyf, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, (const float*)src,
+                           (float*)dst, nf);
+    } else {
+        long blocks = (n4 + 255) / 256;
+        if (blocks > 65536) blocks = 65536;
+        if (blocks < 1) blocks = 1;
+        hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n4);
+    }
 }
 
 template void launch_tail<float>(const TailArgs<float>&, hipStream_t);

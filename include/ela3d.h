@@ -196,8 +196,10 @@ int ela3d_tuning_get(const ela3d_config* cfg, ela3d_tuning* t);
 const char* ela3d_strerror(int code);
 const char* ela3d_last_error(void);
 int ela3d_device_count(int32_t* out);
-/* Stream-copy kernel of n bytes, timed with hipEvents (HBM ceiling calibration). */
-int ela3d_copy_bench(int32_t device, int64_t bytes, int32_t reps, double* ms_per_copy);
+/* Stream-copy kernel of n bytes, timed with hipEvents (HBM ceiling calibration).
+ * variant 0: grid-stride float4; 1: one float4 per thread; 2: four float4 per
+ * thread, nontemporal; 3: one float per thread. */
+int ela3d_copy_bench(int32_t device, int64_t bytes, int32_t reps, int32_t variant, double* ms_per_copy);
 
 /* ---- seeded synthetic models (CPU:41-59 homogeneous; layered per the workload spec) ---- */
 int ela3d_model_homogeneous(int32_t nx, int32_t ny, int32_t nz, float vp, float vs, float rho,
