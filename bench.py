@@ -104,7 +104,7 @@ def main():
     ap.add_argument("--copy-variant", type=int, default=-1,
                     help="copy kernel for --calibrate (0-3); default: all four")
     ap.add_argument("--shape", default="",
-                    help="fixed launch shape block,zseg_interior,zseg_slab,pipe (skips the tuner)")
+                    help="fixed launch shape block,zseg_interior,zseg_slab,vec (skips the tuner)")
     args = ap.parse_args()
     if args.calibrate > 0:
         import ela3d_amd
@@ -131,8 +131,8 @@ def main():
     tuned = None
     if args.shape:
         b, zi, zs, pp = (int(x) for x in args.shape.split(","))
-        ela3d_amd.set_tuning(cfg, block=b, zseg_interior=zi, zseg_slab=zs, pipe=pp)
-        log(f"[bench] fixed launch shape block={b} zseg={zi}/{zs} pipe={pp}")
+        ela3d_amd.set_tuning(cfg, block=b, zseg_interior=zi, zseg_slab=zs, vec=pp)
+        log(f"[bench] fixed launch shape block={b} zseg={zi}/{zs} vec={pp}")
     elif not args.no_tune:
         t = time.perf_counter()
         tuned = ela3d_amd.autotune(cfg, model, steps=6)

@@ -257,11 +257,11 @@ def autotune(cfg: Config, model, steps: int = 8) -> dict:
     return {k: getattr(t, k) for k, _ in TuningStruct._fields_}
 
 
-def set_tuning(cfg: Config, block: int, zseg_interior: int, zseg_slab: int, pipe: int = 1):
+def set_tuning(cfg: Config, block: int, zseg_interior: int, zseg_slab: int, vec: int = 2):
     """Pin the launch shape later shots of this grid use (as if the tuner had chosen it)."""
     from ._lib import TuningStruct
     s, keep = cfg.to_struct()
-    t = TuningStruct(block=block, zseg_interior=zseg_interior, zseg_slab=zseg_slab, pipe=pipe,
+    t = TuningStruct(block=block, zseg_interior=zseg_interior, zseg_slab=zseg_slab, vec=vec,
                      candidates=0, ms_per_step=0.0)
     check(lib().ela3d_tuning_set(C.byref(s), C.byref(t)))
 

@@ -55,7 +55,7 @@ class KernelStats(C.Structure):
 
 class TuningStruct(C.Structure):
     _fields_ = [("block", C.c_int32), ("zseg_interior", C.c_int32), ("zseg_slab", C.c_int32),
-                ("pipe", C.c_int32), ("candidates", C.c_int32), ("ms_per_step", C.c_double)]
+                ("vec", C.c_int32), ("candidates", C.c_int32), ("ms_per_step", C.c_double)]
 
 
 class LayeredSpec(C.Structure):

@@ -121,6 +121,7 @@ ShotPlan make_plan(const ela3d_config& in)
         fail("grid too small: each of nx, ny, nz must be >= max(3, 2*npoints_pml+3)");
     const int64_t cells = int64_t(c.nx) * c.ny * c.nz;
     if (cells > (int64_t(1) << 30)) fail("grid has more than 2^30 cells (32-bit offsets)");
+    if (c.nx + c.ny > 2700) fail("nx + ny must be <= 2700 (x and y C-PML tables are staged in LDS)");
     if (!(c.dx > 0 && c.dy > 0 && c.dz > 0)) fail("dx, dy, dz must be > 0");
     if (!(c.dt > 0)) fail("dt must be > 0");
     if (c.nstep < 0) fail("nstep must be >= 0");

@@ -1,21 +1,25 @@
 // kernels.hip -- gfx950 stencil kernels of the SEISMIC_CPML elastic time loop.
 //
 // Bandwidth-bound (about 0.8 FLOP/B), so no MFMA: the goal is one HBM pass per
-// field per half step with enough loads in flight to cover HBM latency.
-//  * Threads own one xy position p = i + j*nx of a flattened plane (rows of any
-//    nx coalesce with no idle lanes) and march z through a segment of planes,
-//    carrying the z-1 / z+1 neighbours in registers (register queue); in-plane
-//    neighbours are re-read through L1/L2 at clamped, always-valid offsets.
-//  * Each z step is load-all -> compute -> predicated store: no load sits behind
-//    a branch, so one plane costs one memory round trip.  With PIPE the loads of
-//    plane k+1 are issued before plane k is computed (software pipelining).
-//  * Blocks are renumbered so each of the 8 XCDs walks a contiguous run of
+// field per half step, with wide per-lane accesses and enough loads in flight.
+//  * Planes are stored with a padded stride nxyp (a multiple of 64 elements, so
+//    every plane and every workgroup tile starts on a 256-byte boundary) between
+//    zeroed guard regions, so that clamping is never needed for a load.
+//  * Each lane owns VEC consecutive xy positions p..p+VEC-1 of the flattened
+//    plane (p = i + j*nx) and marches z through a segment of planes, carrying
+//    the z-1 / z+1 values in registers.  Own-column data moves as aligned
+//    VEC-wide vectors; in-plane neighbours (p+-1, p+-nx) as unaligned vectors
+//    that hit L1/L2.
+//  * Each z step is load-all -> compute -> predicated store: one memory round
+//    trip per plane.
+//  * Workgroups are renumbered so each of the 8 XCDs walks a contiguous run of
 //    (chunk, z-segment) tiles, keeping in-plane halos inside one XCD's L2.
-//  * C-PML memory variables live in compact per-axis slabs.  Interior boxes
-//    test x per lane only (y and z lie outside the C-PML by construction);
-//    absorbing-layer boxes test all three axes.  One launch covers both.
-//  * Arithmetic is FP32 (FP64 for the FP64 storage instantiation); the C-PML
-//    tables and the wavelet are FP64 in memory; energy accumulates in FP64.
+//  * C-PML memory variables live in compact per-axis slabs.  Interior boxes test
+//    x per element only (y and z lie outside the C-PML by construction); the
+//    absorbing-layer boxes test all three axes.  One launch covers both.  The
+//    FP64 a/b/K tables of x and y are staged once per workgroup into LDS as FP32.
+//  * Arithmetic is FP32 (FP64 for the FP64 storage instantiation); energy
+//    accumulates in FP64.
 //
 // Updates follow the reference CPU loops (seismicCPML_3d_isotropic_simpelmodel.cpp):
 // normal stress CPU:377-413, sigma_xy CPU:415-442, sigma_xz/yz CPU:444-499,
@@ -44,6 +48,69 @@ __device__ __forceinline__ Box pick_box(const Box* bx, int nbox, int lb)
     return B;
 }
 
+// VEC-wide load/store of storage T at element offset o (element-aligned only)
+template <typename T, int V>
+__device__ __forceinline__ void ldv(const T* p, unsigned o, float isc, typename Arith<T>::R (&out)[V])
+{
+    if constexpr (V == 1) {
+        out[0] = ldf(p, o, isc);
+    } else {
+        typedef T vt __attribute__((ext_vector_type(V)));
+        vt v;
+        __builtin_memcpy(&v, p + o, sizeof(vt));
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            if constexpr (std::is_same<T, half_t>::value)
+                out[e] = (float)v[e] * isc;
+            else
+                out[e] = v[e];
+        }
+    }
+}
+
+// model arrays are FP32 in every storage mode; converted to the arithmetic type R
+template <int V, typename R>
+__device__ __forceinline__ void ldm(const float* p, unsigned o, R (&out)[V])
+{
+    if constexpr (V == 1) {
+        out[0] = (R)p[o];
+    } else {
+        typedef float vt __attribute__((ext_vector_type(V)));
+        vt v;
+        __builtin_memcpy(&v, p + o, sizeof(vt));
+#pragma unroll
+        for (int e = 0; e < V; ++e) out[e] = (R)v[e];
+    }
+}
+
+// store the enabled elements; one vector store when all are enabled
+template <typename T, int V>
+__device__ __forceinline__ void stv(T* p, unsigned o, const typename Arith<T>::R (&val)[V], const bool (&en)[V],
+                                    float sc)
+{
+    bool all = true;
+#pragma unroll
+    for (int e = 0; e < V; ++e) all = all && en[e];
+    if constexpr (V > 1) {
+        if (all) {
+            typedef T vt __attribute__((ext_vector_type(V)));
+            vt v;
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                if constexpr (std::is_same<T, half_t>::value)
+                    v[e] = (half_t)(val[e] * sc);
+                else
+                    v[e] = val[e];
+            }
+            __builtin_memcpy(p + o, &v, sizeof(vt));
+            return;
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < V; ++e)
+        if (en[e]) stf(p, o + e, val[e], sc);
+}
+
 // memory-variable recurrence psi <- b psi + a D;  D <- D/K + psi  (CPU:395-405)
 template <typename R>
 __device__ __forceinline__ R cpml(R psi_old, R a, R b, R K, R& d)
@@ -53,49 +120,50 @@ __device__ __forceinline__ R cpml(R psi_old, R a, R b, R K, R& d)
     return psi;
 }
 
-// Per-lane constants: position, clamped neighbour offsets, C-PML coefficients.
-template <typename R>
+// FP32 copies of the x and y tables, staged per workgroup: [6][nx] then [6][ny]
+// (a, b, K on the full grid, a, b, K on the half grid).
+template <typename T>
+__device__ __forceinline__ void stage_tables(const StepArgs<T>& A, float* s)
+{
+    const int nx6 = 6 * A.nx, ny6 = 6 * A.ny;
+    for (int q = threadIdx.x; q < nx6 + ny6; q += blockDim.x)
+        s[q] = q < nx6 ? (float)A.tab[0][q] : (float)A.tab[1][q - nx6];
+    __syncthreads();
+}
+
+// Per-lane geometry of the VEC elements p0..p0+VEC-1.
+template <int V>
 struct Lane {
-    int p, i, j;
-    unsigned dE, dW, dN, dS;   // +x, -x, +y, -y offsets, 0 where the neighbour is off-grid
-    int cx, cy;                // compact C-PML column / row, -1 outside the layers
-    R ax, bx, Kx, axh, bxh, Kxh;
-    R ay, by, Ky, ayh, byh, Kyh;
+    int p0;
+    int i[V], j[V];
+    bool act[V];   // inside this box
+    int cx[V], cy[V];
 };
 
-template <typename T, int MODE, typename R>
-__device__ __forceinline__ void make_lane(const StepArgs<T>& A, int p, Lane<R>& L)
+template <typename T, int MODE, int V>
+__device__ __forceinline__ void make_lane(const StepArgs<T>& A, int p0, int p1, Lane<V>& L)
 {
-    const int nx = A.nx, ny = A.ny;
-    L.p = p;
-    L.j = p / nx;
-    L.i = p - L.j * nx;
-    L.dE = L.i < nx - 1 ? 1u : 0u;
-    L.dW = L.i > 0 ? 1u : 0u;
-    L.dN = L.j < ny - 1 ? unsigned(nx) : 0u;
-    L.dS = L.j > 0 ? unsigned(nx) : 0u;
-    L.cx = L.i < A.lo[0] ? L.i : (L.i >= A.hi[0] ? A.lo[0] + (L.i - A.hi[0]) : -1);
-    L.ax = 0; L.bx = 1; L.Kx = 1; L.axh = 0; L.bxh = 1; L.Kxh = 1;
-    if (L.cx >= 0) {
-        const double* t = A.tab[0];
-        const int i = L.i;
-        L.ax = (R)t[i]; L.bx = (R)t[nx + i]; L.Kx = (R)t[2 * nx + i];
-        L.axh = (R)t[3 * nx + i]; L.bxh = (R)t[4 * nx + i]; L.Kxh = (R)t[5 * nx + i];
-    }
-    L.cy = -1;
-    L.ay = 0; L.by = 1; L.Ky = 1; L.ayh = 0; L.byh = 1; L.Kyh = 1;
-    if (MODE == kSlab) {
-        const int j = L.j;
-        L.cy = j < A.lo[1] ? j : (j >= A.hi[1] ? A.lo[1] + (j - A.hi[1]) : -1);
-        if (L.cy >= 0) {
-            const double* t = A.tab[1];
-            L.ay = (R)t[j]; L.by = (R)t[ny + j]; L.Ky = (R)t[2 * ny + j];
-            L.ayh = (R)t[3 * ny + j]; L.byh = (R)t[4 * ny + j]; L.Kyh = (R)t[5 * ny + j];
+    const int nx = A.nx;
+    L.p0 = p0;
+    int j = p0 / nx, i = p0 - j * nx;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        L.i[e] = i;
+        L.j[e] = j;
+        L.act[e] = (p0 + e) < p1;
+        // elements past the box end own no memory variables (their j may exceed ny-1)
+        L.cx[e] = !L.act[e] ? -1 : (i < A.lo[0] ? i : (i >= A.hi[0] ? A.lo[0] + (i - A.hi[0]) : -1));
+        L.cy[e] = -1;
+        if (MODE == kSlab && L.act[e])
+            L.cy[e] = j < A.lo[1] ? j : (j >= A.hi[1] ? A.lo[1] + (j - A.hi[1]) : -1);
+        if (++i == nx) {
+            i = 0;
+            ++j;
         }
     }
 }
 
-// z C-PML coefficients of plane k (uniform across the block)
+// z C-PML coefficients of plane k (uniform across the workgroup)
 template <typename R>
 struct ZTab {
     int cz;
@@ -123,214 +191,232 @@ __device__ __forceinline__ ZTab<R> ztab(const StepArgs<T>& A, int k)
 // ===========================================================================
 // Stress pass: sigma_xx/yy/zz, sigma_xy, sigma_xz, sigma_yz + 9 memory variables
 // ===========================================================================
-template <typename R>
+template <typename R, int V>
 struct SPlane {
-    R vz_c, vx_u, vy_u, cs_u, rho_u, cp_c;            // own column: plane k, k+1
-    R vxE, vyS, vyW, vxN, vzW, vzN, csW, csN, rhoW, rhoN;  // in-plane neighbours
-    R sxx, syy, szz, sxy, sxz, syz;                   // stresses to update
-    R mx[3], my[3], mz[3];                            // memory variables (C-PML lanes)
+    R vz_c[V], vx_u[V], vy_u[V], cs_u[V], rho_u[V], cp_c[V];        // own column, planes k, k+1
+    R vxE[V], vyS[V], vyW[V], vxN[V], vzW[V], vzN[V], csW[V], csN[V], rhoW[V], rhoN[V];
+    R sxx[V], syy[V], szz[V], sxy[V], sxz[V], syz[V];
+    R mx[3][V], my[3][V], mz[3][V];
 };
 
-template <typename T, int MODE, typename R>
-__device__ __forceinline__ void stress_load(const StepArgs<T>& A, const Lane<R>& L, int k,
-                                            SPlane<R>& P)
+template <typename R, int V>
+struct SQueue {
+    R vx_c[V], vy_c[V], vz_m[V], cs_c[V], rho_c[V];
+};
+
+template <typename T, int MODE, int V, typename R>
+__device__ __forceinline__ void stress_load(const StepArgs<T>& A, const Lane<V>& L, int k, SPlane<R, V>& P)
 {
-    const unsigned nxy = A.nxy;
-    const unsigned o = unsigned(k) * nxy + unsigned(L.p);
+    const unsigned nxyp = A.nxyp, nx = A.nx;
+    const unsigned o = unsigned(k) * nxyp + unsigned(L.p0);
     const bool top = (k == A.nz - 1);
-    const unsigned dU = top ? 0u : nxy;
-    T *vx = A.f[0], *vy = A.f[1], *vz = A.f[2];
-    const float *cp = A.cp, *cs = A.cs, *rho = A.rho;
-    P.vz_c = ldf(vz, o, A.isc[2]);
-    P.cp_c = cp[o];
-    P.vx_u = ldf(vx, o + dU, A.isc[0]);
-    P.vy_u = ldf(vy, o + dU, A.isc[1]);
-    P.cs_u = cs[o + dU];
-    P.rho_u = rho[o + dU];
-    P.vxE = ldf(vx, o + L.dE, A.isc[0]);
-    P.vyS = ldf(vy, o - L.dS, A.isc[1]);
-    P.vyW = ldf(vy, o - L.dW, A.isc[1]);
-    P.vxN = ldf(vx, o + L.dN, A.isc[0]);
-    P.vzW = ldf(vz, o - L.dW, A.isc[2]);
-    P.vzN = ldf(vz, o + L.dN, A.isc[2]);
-    P.csW = cs[o - L.dW];
-    P.csN = cs[o + L.dN];
-    P.rhoW = rho[o - L.dW];
-    P.rhoN = rho[o + L.dN];
-    P.sxx = ldf(A.f[3], o, A.isc[3]);
-    P.syy = ldf(A.f[4], o, A.isc[4]);
-    P.szz = ldf(A.f[5], o, A.isc[5]);
-    P.sxy = ldf(A.f[6], o, A.isc[6]);
-    P.sxz = ldf(A.f[7], o, A.isc[7]);
-    P.syz = ldf(A.f[8], o, A.isc[8]);
-    if (L.cx >= 0) {
-        const unsigned om = unsigned((k * A.ny + L.j) * A.w[0] + L.cx);
-        P.mx[0] = ldf(A.m[0], om, A.isc[9]);
-        P.mx[1] = ldf(A.m[3], om, A.isc[12]);
-        P.mx[2] = ldf(A.m[5], om, A.isc[14]);
-    }
-    if (MODE == kSlab) {
-        if (L.cy >= 0) {
-            const unsigned om = unsigned((k * A.w[1] + L.cy) * A.nx + L.i);
-            P.my[0] = ldf(A.m[1], om, A.isc[10]);
-            P.my[1] = ldf(A.m[4], om, A.isc[13]);
-            P.my[2] = ldf(A.m[7], om, A.isc[16]);
+    const unsigned dU = top ? 0u : nxyp;
+    const T *vx = A.f[0], *vy = A.f[1], *vz = A.f[2];
+    ldv<T, V>(vz, o, A.isc[2], P.vz_c);
+    ldm<V>(A.cp, o, P.cp_c);
+    ldv<T, V>(vx, o + dU, A.isc[0], P.vx_u);
+    ldv<T, V>(vy, o + dU, A.isc[1], P.vy_u);
+    ldm<V>(A.cs, o + dU, P.cs_u);
+    ldm<V>(A.rho, o + dU, P.rho_u);
+    ldv<T, V>(vx, o + 1, A.isc[0], P.vxE);
+    ldv<T, V>(vy, o - nx, A.isc[1], P.vyS);
+    ldv<T, V>(vy, o - 1, A.isc[1], P.vyW);
+    ldv<T, V>(vx, o + nx, A.isc[0], P.vxN);
+    ldv<T, V>(vz, o - 1, A.isc[2], P.vzW);
+    ldv<T, V>(vz, o + nx, A.isc[2], P.vzN);
+    ldm<V>(A.cs, o - 1, P.csW);
+    ldm<V>(A.cs, o + nx, P.csN);
+    ldm<V>(A.rho, o - 1, P.rhoW);
+    ldm<V>(A.rho, o + nx, P.rhoN);
+    ldv<T, V>(A.f[3], o, A.isc[3], P.sxx);
+    ldv<T, V>(A.f[4], o, A.isc[4], P.syy);
+    ldv<T, V>(A.f[5], o, A.isc[5], P.szz);
+    ldv<T, V>(A.f[6], o, A.isc[6], P.sxy);
+    ldv<T, V>(A.f[7], o, A.isc[7], P.sxz);
+    ldv<T, V>(A.f[8], o, A.isc[8], P.syz);
+    const int cz = MODE == kSlab ? (k < A.lo[2] ? k : (k >= A.hi[2] ? A.lo[2] + (k - A.hi[2]) : -1)) : -1;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        if (L.cx[e] >= 0) {
+            const unsigned om = unsigned((k * A.ny + L.j[e]) * A.w[0] + L.cx[e]);
+            P.mx[0][e] = ldf(A.m[0], om, A.isc[9]);
+            P.mx[1][e] = ldf(A.m[3], om, A.isc[12]);
+            P.mx[2][e] = ldf(A.m[5], om, A.isc[14]);
         }
-        const int cz = k < A.lo[2] ? k : (k >= A.hi[2] ? A.lo[2] + (k - A.hi[2]) : -1);
-        if (cz >= 0) {
-            const unsigned om = unsigned((cz * A.ny + L.j) * A.nx + L.i);
-            P.mz[0] = ldf(A.m[2], om, A.isc[11]);
-            P.mz[1] = ldf(A.m[6], om, A.isc[15]);
-            P.mz[2] = ldf(A.m[8], om, A.isc[17]);
+        if (MODE == kSlab) {
+            if (L.cy[e] >= 0) {
+                const unsigned om = unsigned((k * A.w[1] + L.cy[e]) * A.nx + L.i[e]);
+                P.my[0][e] = ldf(A.m[1], om, A.isc[10]);
+                P.my[1][e] = ldf(A.m[4], om, A.isc[13]);
+                P.my[2][e] = ldf(A.m[7], om, A.isc[16]);
+            }
+            if (cz >= 0) {
+                const unsigned om = unsigned((cz * A.ny + L.j[e]) * A.nx + L.i[e]);
+                P.mz[0][e] = ldf(A.m[2], om, A.isc[11]);
+                P.mz[1][e] = ldf(A.m[6], om, A.isc[15]);
+                P.mz[2][e] = ldf(A.m[8], om, A.isc[17]);
+            }
         }
     }
     if (top) {  // zero ghost plane above the top (only read with the ghost quirk)
-        P.vx_u = 0;
-        P.vy_u = 0;
-        P.cs_u = 0;
-        P.rho_u = 0;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            P.vx_u[e] = 0;
+            P.vy_u[e] = 0;
+            P.cs_u[e] = 0;
+            P.rho_u[e] = 0;
+        }
     }
 }
 
-// queue: values of plane k carried from the previous step of the march
-template <typename R>
-struct SQueue {
-    R vx_c, vy_c, vz_m, cs_c, rho_c;
-};
-
-template <typename T, int MODE, typename R>
-__device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<R>& L, int k,
-                                              const SPlane<R>& P, const SQueue<R>& Q)
+template <typename T, int MODE, int V, typename R>
+__device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V>& L, const float* tx,
+                                              const float* ty, int k, const SPlane<R, V>& P,
+                                              const SQueue<R, V>& Q)
 {
-    const unsigned o = unsigned(k) * unsigned(A.nxy) + unsigned(L.p);
+    const unsigned o = unsigned(k) * unsigned(A.nxyp) + unsigned(L.p0);
     const R dt = (R)A.dt, idx = (R)A.idx, idy = (R)A.idy, idz = (R)A.idz;
     const ZTab<R> Z = ztab<T, MODE, R>(A, k);
     const int nx = A.nx, ny = A.ny;
-    const unsigned omx = unsigned((k * ny + L.j) * A.w[0] + L.cx);
-    const unsigned omy = unsigned((k * A.w[1] + L.cy) * nx + L.i);
-    const unsigned omz = unsigned((Z.cz * ny + L.j) * nx + L.i);
-
-    // sigma_xx, sigma_yy, sigma_zz: i in [0,nx-2], j in [1,ny-1], k in [1,nz-1] (CPU:377-413)
-    {
-        const bool on = (L.i <= nx - 2) && (L.j >= 1) && (k >= 1);
-        const R vp = P.cp_c;
-        const R lambda = Q.rho_c * (vp * vp - (R)2 * Q.cs_c * Q.cs_c);
-        const R l2m = Q.rho_c * vp * vp;
-        const R DT_l2m = dt * l2m, DT_l = dt * lambda;
-        R dvx_dx = (P.vxE - Q.vx_c) * idx;
-        R dvy_dy = (Q.vy_c - P.vyS) * idy;
-        R dvz_dz = (P.vz_c - Q.vz_m) * idz;
-        R m0 = 0, m1 = 0, m2 = 0;
-        if (L.cx >= 0) m0 = cpml(P.mx[0], L.axh, L.bxh, L.Kxh, dvx_dx);
-        if (MODE == kSlab) {
-            if (L.cy >= 0) m1 = cpml(P.my[0], L.ay, L.by, L.Ky, dvy_dy);
-            if (Z.cz >= 0) m2 = cpml(P.mz[0], Z.az, Z.bz, Z.Kz, dvz_dz);
-        }
-        if (on) {
-            stf(A.f[3], o, DT_l2m * dvx_dx + DT_l * (dvy_dy + dvz_dz) + P.sxx, A.sc[3]);
-            stf(A.f[4], o, DT_l * (dvx_dx + dvz_dz) + DT_l2m * dvy_dy + P.syy, A.sc[4]);
-            stf(A.f[5], o, DT_l * (dvx_dx + dvy_dy) + DT_l2m * dvz_dz + P.szz, A.sc[5]);
-            if (L.cx >= 0) stf(A.m[0], omx, m0, A.sc[9]);
-            if (MODE == kSlab) {
-                if (L.cy >= 0) stf(A.m[1], omy, m1, A.sc[10]);
-                if (Z.cz >= 0) stf(A.m[2], omz, m2, A.sc[11]);
+    R nsxx[V], nsyy[V], nszz[V], nsxy[V], nsxz[V], nsyz[V];
+    bool onN[V], onXY[V], onXZ[V], onYZ[V];
+    const bool shear_k = (k <= A.kmax_shear);
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        const int i = L.i[e], j = L.j[e], cx = L.cx[e], cy = L.cy[e];
+        const unsigned omx = unsigned((k * ny + j) * A.w[0] + cx);
+        const unsigned omy = unsigned((k * A.w[1] + cy) * nx + i);
+        const unsigned omz = unsigned((Z.cz * ny + j) * nx + i);
+        const R vx_c = Q.vx_c[e], vy_c = Q.vy_c[e];
+        // sigma_xx, sigma_yy, sigma_zz: i in [0,nx-2], j in [1,ny-1], k in [1,nz-1] (CPU:377-413)
+        {
+            onN[e] = L.act[e] && (i <= nx - 2) && (j >= 1) && (k >= 1);
+            const R vp = P.cp_c[e];
+            const R lambda = Q.rho_c[e] * (vp * vp - (R)2 * Q.cs_c[e] * Q.cs_c[e]);
+            const R l2m = Q.rho_c[e] * vp * vp;
+            const R DT_l2m = dt * l2m, DT_l = dt * lambda;
+            R dvx_dx = (P.vxE[e] - vx_c) * idx;
+            R dvy_dy = (vy_c - P.vyS[e]) * idy;
+            R dvz_dz = (P.vz_c[e] - Q.vz_m[e]) * idz;
+            if (cx >= 0) {
+                const R m = cpml(P.mx[0][e], (R)tx[3 * nx + i], (R)tx[4 * nx + i], (R)tx[5 * nx + i], dvx_dx);
+                if (onN[e]) stf(A.m[0], omx, m, A.sc[9]);
             }
+            if (MODE == kSlab) {
+                if (cy >= 0) {
+                    const R m = cpml(P.my[0][e], (R)ty[j], (R)ty[ny + j], (R)ty[2 * ny + j], dvy_dy);
+                    if (onN[e]) stf(A.m[1], omy, m, A.sc[10]);
+                }
+                if (Z.cz >= 0) {
+                    const R m = cpml(P.mz[0][e], Z.az, Z.bz, Z.Kz, dvz_dz);
+                    if (onN[e]) stf(A.m[2], omz, m, A.sc[11]);
+                }
+            }
+            nsxx[e] = DT_l2m * dvx_dx + DT_l * (dvy_dy + dvz_dz) + P.sxx[e];
+            nsyy[e] = DT_l * (dvx_dx + dvz_dz) + DT_l2m * dvy_dy + P.syy[e];
+            nszz[e] = DT_l * (dvx_dx + dvy_dy) + DT_l2m * dvz_dz + P.szz[e];
         }
-    }
-    // sigma_xy: i in [1,nx-1], j in [0,ny-2]; mu averaged over (i-1,j),(i,j+1) (CPU:415-442)
-    {
-        const bool on = (L.i >= 1) && (L.j <= ny - 2);
-        const R vs = (P.csW + P.csN) / (R)2;
-        const R rh = (P.rhoW + P.rhoN) / (R)2;
-        const R DT_mu = dt * (rh * vs * vs);
-        R dvy_dx = (Q.vy_c - P.vyW) * idx;
-        R dvx_dy = (P.vxN - Q.vx_c) * idy;
-        R m3 = 0, m4 = 0;
-        if (L.cx >= 0) m3 = cpml(P.mx[1], L.ax, L.bx, L.Kx, dvy_dx);
-        if (MODE == kSlab && L.cy >= 0) m4 = cpml(P.my[1], L.ayh, L.byh, L.Kyh, dvx_dy);
-        if (on) {
-            stf(A.f[6], o, DT_mu * (dvy_dx + dvx_dy) + P.sxy, A.sc[6]);
-            if (L.cx >= 0) stf(A.m[3], omx, m3, A.sc[12]);
-            if (MODE == kSlab && L.cy >= 0) stf(A.m[4], omy, m4, A.sc[13]);
+        // sigma_xy: i in [1,nx-1], j in [0,ny-2]; mu averaged over (i-1,j),(i,j+1) (CPU:415-442)
+        {
+            onXY[e] = L.act[e] && (i >= 1) && (j <= ny - 2);
+            const R vs = (P.csW[e] + P.csN[e]) / (R)2;
+            const R rh = (P.rhoW[e] + P.rhoN[e]) / (R)2;
+            const R DT_mu = dt * (rh * vs * vs);
+            R dvy_dx = (vy_c - P.vyW[e]) * idx;
+            R dvx_dy = (P.vxN[e] - vx_c) * idy;
+            if (cx >= 0) {
+                const R m = cpml(P.mx[1][e], (R)tx[i], (R)tx[nx + i], (R)tx[2 * nx + i], dvy_dx);
+                if (onXY[e]) stf(A.m[3], omx, m, A.sc[12]);
+            }
+            if (MODE == kSlab && cy >= 0) {
+                const R m = cpml(P.my[1][e], (R)ty[3 * ny + j], (R)ty[4 * ny + j], (R)ty[5 * ny + j], dvx_dy);
+                if (onXY[e]) stf(A.m[4], omy, m, A.sc[13]);
+            }
+            nsxy[e] = DT_mu * (dvy_dx + dvx_dy) + P.sxy[e];
         }
-    }
-    if (k <= A.kmax_shear) {
         // sigma_xz: i in [1,nx-1]; mu averaged over (i-1,k),(i,k+1) (CPU:444-470)
         {
-            const bool on = (L.i >= 1);
-            const R vs = (P.csW + P.cs_u) / (R)2;
-            const R rh = (P.rhoW + P.rho_u) / (R)2;
+            onXZ[e] = L.act[e] && shear_k && (i >= 1);
+            const R vs = (P.csW[e] + P.cs_u[e]) / (R)2;
+            const R rh = (P.rhoW[e] + P.rho_u[e]) / (R)2;
             const R DT_mu = dt * (rh * vs * vs);
-            R dvz_dx = (P.vz_c - P.vzW) * idx;
-            R dvx_dz = (P.vx_u - Q.vx_c) * idz;
-            R m5 = 0, m6 = 0;
-            if (L.cx >= 0) m5 = cpml(P.mx[2], L.ax, L.bx, L.Kx, dvz_dx);
-            if (MODE == kSlab && Z.cz >= 0) m6 = cpml(P.mz[1], Z.azh, Z.bzh, Z.Kzh, dvx_dz);
-            if (on) {
-                stf(A.f[7], o, DT_mu * (dvz_dx + dvx_dz) + P.sxz, A.sc[7]);
-                if (L.cx >= 0) stf(A.m[5], omx, m5, A.sc[14]);
-                if (MODE == kSlab && Z.cz >= 0) stf(A.m[6], omz, m6, A.sc[15]);
+            R dvz_dx = (P.vz_c[e] - P.vzW[e]) * idx;
+            R dvx_dz = (P.vx_u[e] - vx_c) * idz;
+            if (cx >= 0) {
+                const R m = cpml(P.mx[2][e], (R)tx[i], (R)tx[nx + i], (R)tx[2 * nx + i], dvz_dx);
+                if (onXZ[e]) stf(A.m[5], omx, m, A.sc[14]);
             }
+            if (MODE == kSlab && Z.cz >= 0) {
+                const R m = cpml(P.mz[1][e], Z.azh, Z.bzh, Z.Kzh, dvx_dz);
+                if (onXZ[e]) stf(A.m[6], omz, m, A.sc[15]);
+            }
+            nsxz[e] = DT_mu * (dvz_dx + dvx_dz) + P.sxz[e];
         }
         // sigma_yz: j in [0,ny-2]; mu averaged over (j+1,k),(j,k+1) (CPU:472-497)
         {
-            const bool on = (L.j <= ny - 2);
-            const R vs = (P.csN + P.cs_u) / (R)2;
-            const R rh = (P.rhoN + P.rho_u) / (R)2;
+            onYZ[e] = L.act[e] && shear_k && (j <= ny - 2);
+            const R vs = (P.csN[e] + P.cs_u[e]) / (R)2;
+            const R rh = (P.rhoN[e] + P.rho_u[e]) / (R)2;
             const R DT_mu = dt * (rh * vs * vs);
-            R dvz_dy = (P.vzN - P.vz_c) * idy;
-            R dvy_dz = (P.vy_u - Q.vy_c) * idz;
-            R m7 = 0, m8 = 0;
+            R dvz_dy = (P.vzN[e] - P.vz_c[e]) * idy;
+            R dvy_dz = (P.vy_u[e] - vy_c) * idz;
             if (MODE == kSlab) {
-                if (L.cy >= 0) m7 = cpml(P.my[2], L.ayh, L.byh, L.Kyh, dvz_dy);
-                if (Z.cz >= 0) m8 = cpml(P.mz[2], Z.azh, Z.bzh, Z.Kzh, dvy_dz);
-            }
-            if (on) {
-                stf(A.f[8], o, DT_mu * (dvz_dy + dvy_dz) + P.syz, A.sc[8]);
-                if (MODE == kSlab) {
-                    if (L.cy >= 0) stf(A.m[7], omy, m7, A.sc[16]);
-                    if (Z.cz >= 0) stf(A.m[8], omz, m8, A.sc[17]);
+                if (cy >= 0) {
+                    const R m = cpml(P.my[2][e], (R)ty[3 * ny + j], (R)ty[4 * ny + j], (R)ty[5 * ny + j], dvz_dy);
+                    if (onYZ[e]) stf(A.m[7], omy, m, A.sc[16]);
+                }
+                if (Z.cz >= 0) {
+                    const R m = cpml(P.mz[2][e], Z.azh, Z.bzh, Z.Kzh, dvy_dz);
+                    if (onYZ[e]) stf(A.m[8], omz, m, A.sc[17]);
                 }
             }
+            nsyz[e] = DT_mu * (dvz_dy + dvy_dz) + P.syz[e];
         }
     }
+    stv<T, V>(A.f[3], o, nsxx, onN, A.sc[3]);
+    stv<T, V>(A.f[4], o, nsyy, onN, A.sc[4]);
+    stv<T, V>(A.f[5], o, nszz, onN, A.sc[5]);
+    stv<T, V>(A.f[6], o, nsxy, onXY, A.sc[6]);
+    stv<T, V>(A.f[7], o, nsxz, onXZ, A.sc[7]);
+    stv<T, V>(A.f[8], o, nsyz, onYZ, A.sc[8]);
 }
 
-template <typename T, int MODE, bool PIPE>
-__device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, int local)
+template <typename T, int MODE, int V>
+__device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, int local, const float* tx,
+                                            const float* ty)
 {
     typedef typename Arith<T>::R R;
     const int chunk = local % B.nchunk, zs = local / B.nchunk;
-    const int p = B.p0 + chunk * A.block + (int)threadIdx.x;
-    if (p >= B.p1) return;
+    const int p0 = B.p0 + (chunk * A.block + (int)threadIdx.x) * V;
+    if (p0 >= B.p1) return;
     const int kb = B.k0 + zs * B.zseg;
     const int ke = min(kb + B.zseg, B.k1);
-    Lane<R> L;
-    make_lane<T, MODE, R>(A, p, L);
-    SQueue<R> Q;
+    Lane<V> L;
+    make_lane<T, MODE, V>(A, p0, B.p1, L);
+    SQueue<R, V> Q;
     {
-        const unsigned o = unsigned(kb) * unsigned(A.nxy) + unsigned(p);
-        Q.vx_c = ldf(A.f[0], o, A.isc[0]);
-        Q.vy_c = ldf(A.f[1], o, A.isc[1]);
-        Q.cs_c = A.cs[o];
-        Q.rho_c = A.rho[o];
-        Q.vz_m = kb >= 1 ? ldf(A.f[2], o - unsigned(A.nxy), A.isc[2]) : (R)0;
-    }
-    SPlane<R> P;
-    if (PIPE) stress_load<T, MODE, R>(A, L, kb, P);
-    for (int k = kb; k < ke; ++k) {
-        SPlane<R> N;
-        if (PIPE)
-            stress_load<T, MODE, R>(A, L, min(k + 1, ke - 1), N);  // next plane in flight
+        const unsigned o = unsigned(kb) * unsigned(A.nxyp) + unsigned(p0);
+        ldv<T, V>(A.f[0], o, A.isc[0], Q.vx_c);
+        ldv<T, V>(A.f[1], o, A.isc[1], Q.vy_c);
+        ldm<V>(A.cs, o, Q.cs_c);
+        ldm<V>(A.rho, o, Q.rho_c);
+        if (kb >= 1)
+            ldv<T, V>(A.f[2], o - unsigned(A.nxyp), A.isc[2], Q.vz_m);
         else
-            stress_load<T, MODE, R>(A, L, k, P);
-        stress_update<T, MODE, R>(A, L, k, P, Q);
-        Q.vx_c = P.vx_u;
-        Q.vy_c = P.vy_u;
-        Q.cs_c = P.cs_u;
-        Q.rho_c = P.rho_u;
-        Q.vz_m = P.vz_c;
-        if (PIPE) P = N;
+            for (int e = 0; e < V; ++e) Q.vz_m[e] = 0;
+    }
+    for (int k = kb; k < ke; ++k) {
+        SPlane<R, V> P;
+        stress_load<T, MODE, V, R>(A, L, k, P);
+        stress_update<T, MODE, V, R>(A, L, tx, ty, k, P, Q);
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            Q.vx_c[e] = P.vx_u[e];
+            Q.vy_c[e] = P.vy_u[e];
+            Q.cs_c[e] = P.cs_u[e];
+            Q.rho_c[e] = P.rho_u[e];
+            Q.vz_m[e] = P.vz_c[e];
+        }
     }
 }
 
@@ -338,227 +424,246 @@ __device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, 
 // Velocity pass: vx, vy, vz + 9 memory variables; Dirichlet faces; optional
 // fused energy partial sums of the updated state.
 // ===========================================================================
-template <typename R>
+template <typename R, int V>
 struct VPlane {
-    R sxx, syy, sxy, sxz, syz, szz_u, rho_u;   // own column: plane k, k+1
-    R sxxW, sxyS, rhoW, sxyE, syyN, rhoN, sxzE, syzS;  // in-plane neighbours
-    R vx, vy, vz;                              // velocities to update
-    R cp, cs;                                  // energy steps only
-    R mx[3], my[3], mz[3];
+    R sxx[V], syy[V], sxy[V], sxz[V], syz[V], szz_u[V], rho_u[V];
+    R sxxW[V], sxyS[V], rhoW[V], sxyE[V], syyN[V], rhoN[V], sxzE[V], syzS[V];
+    R vx[V], vy[V], vz[V];
+    R cp[V], cs[V];
+    R mx[3][V], my[3][V], mz[3][V];
 };
 
-template <typename T, int MODE, bool ENERGY, typename R>
-__device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<R>& L, int k,
-                                              VPlane<R>& P)
+template <typename R, int V>
+struct VQueue {
+    R sxz_m[V], syz_m[V], szz_c[V], rho_c[V];
+};
+
+template <typename T, int MODE, bool ENERGY, int V, typename R>
+__device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V>& L, int k, VPlane<R, V>& P)
 {
-    const unsigned nxy = A.nxy;
-    const unsigned o = unsigned(k) * nxy + unsigned(L.p);
+    const unsigned nxyp = A.nxyp, nx = A.nx;
+    const unsigned o = unsigned(k) * nxyp + unsigned(L.p0);
     const bool top = (k == A.nz - 1);
-    const unsigned dU = top ? 0u : nxy;
-    T *sxx = A.f[3], *syy = A.f[4], *szz = A.f[5], *sxy = A.f[6], *sxz = A.f[7], *syz = A.f[8];
-    const float* rho = A.rho;
-    P.sxx = ldf(sxx, o, A.isc[3]);
-    P.syy = ldf(syy, o, A.isc[4]);
-    P.sxy = ldf(sxy, o, A.isc[6]);
-    P.sxz = ldf(sxz, o, A.isc[7]);
-    P.syz = ldf(syz, o, A.isc[8]);
-    P.szz_u = ldf(szz, o + dU, A.isc[5]);
-    P.rho_u = rho[o + dU];
-    P.sxxW = ldf(sxx, o - L.dW, A.isc[3]);
-    P.sxyS = ldf(sxy, o - L.dS, A.isc[6]);
-    P.rhoW = rho[o - L.dW];
-    P.sxyE = ldf(sxy, o + L.dE, A.isc[6]);
-    P.syyN = ldf(syy, o + L.dN, A.isc[4]);
-    P.rhoN = rho[o + L.dN];
-    P.sxzE = ldf(sxz, o + L.dE, A.isc[7]);
-    P.syzS = ldf(syz, o - L.dS, A.isc[8]);
-    P.vx = ldf(A.f[0], o, A.isc[0]);
-    P.vy = ldf(A.f[1], o, A.isc[1]);
-    P.vz = ldf(A.f[2], o, A.isc[2]);
+    const unsigned dU = top ? 0u : nxyp;
+    const T *sxx = A.f[3], *syy = A.f[4], *szz = A.f[5], *sxy = A.f[6], *sxz = A.f[7], *syz = A.f[8];
+    ldv<T, V>(sxx, o, A.isc[3], P.sxx);
+    ldv<T, V>(syy, o, A.isc[4], P.syy);
+    ldv<T, V>(sxy, o, A.isc[6], P.sxy);
+    ldv<T, V>(sxz, o, A.isc[7], P.sxz);
+    ldv<T, V>(syz, o, A.isc[8], P.syz);
+    ldv<T, V>(szz, o + dU, A.isc[5], P.szz_u);
+    ldm<V>(A.rho, o + dU, P.rho_u);
+    ldv<T, V>(sxx, o - 1, A.isc[3], P.sxxW);
+    ldv<T, V>(sxy, o - nx, A.isc[6], P.sxyS);
+    ldm<V>(A.rho, o - 1, P.rhoW);
+    ldv<T, V>(sxy, o + 1, A.isc[6], P.sxyE);
+    ldv<T, V>(syy, o + nx, A.isc[4], P.syyN);
+    ldm<V>(A.rho, o + nx, P.rhoN);
+    ldv<T, V>(sxz, o + 1, A.isc[7], P.sxzE);
+    ldv<T, V>(syz, o - nx, A.isc[8], P.syzS);
+    ldv<T, V>(A.f[0], o, A.isc[0], P.vx);
+    ldv<T, V>(A.f[1], o, A.isc[1], P.vy);
+    ldv<T, V>(A.f[2], o, A.isc[2], P.vz);
     if (ENERGY) {
-        P.cp = A.cp[o];
-        P.cs = A.cs[o];
+        ldm<V>(A.cp, o, P.cp);
+        ldm<V>(A.cs, o, P.cs);
     }
-    if (L.cx >= 0) {
-        const unsigned om = unsigned((k * A.ny + L.j) * A.w[0] + L.cx);
-        P.mx[0] = ldf(A.m[9], om, A.isc[18]);
-        P.mx[1] = ldf(A.m[12], om, A.isc[21]);
-        P.mx[2] = ldf(A.m[15], om, A.isc[24]);
-    }
-    if (MODE == kSlab) {
-        if (L.cy >= 0) {
-            const unsigned om = unsigned((k * A.w[1] + L.cy) * A.nx + L.i);
-            P.my[0] = ldf(A.m[10], om, A.isc[19]);
-            P.my[1] = ldf(A.m[13], om, A.isc[22]);
-            P.my[2] = ldf(A.m[16], om, A.isc[25]);
+    const int cz = MODE == kSlab ? (k < A.lo[2] ? k : (k >= A.hi[2] ? A.lo[2] + (k - A.hi[2]) : -1)) : -1;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        if (L.cx[e] >= 0) {
+            const unsigned om = unsigned((k * A.ny + L.j[e]) * A.w[0] + L.cx[e]);
+            P.mx[0][e] = ldf(A.m[9], om, A.isc[18]);
+            P.mx[1][e] = ldf(A.m[12], om, A.isc[21]);
+            P.mx[2][e] = ldf(A.m[15], om, A.isc[24]);
         }
-        const int cz = k < A.lo[2] ? k : (k >= A.hi[2] ? A.lo[2] + (k - A.hi[2]) : -1);
-        if (cz >= 0) {
-            const unsigned om = unsigned((cz * A.ny + L.j) * A.nx + L.i);
-            P.mz[0] = ldf(A.m[11], om, A.isc[20]);
-            P.mz[1] = ldf(A.m[14], om, A.isc[23]);
-            P.mz[2] = ldf(A.m[17], om, A.isc[26]);
+        if (MODE == kSlab) {
+            if (L.cy[e] >= 0) {
+                const unsigned om = unsigned((k * A.w[1] + L.cy[e]) * A.nx + L.i[e]);
+                P.my[0][e] = ldf(A.m[10], om, A.isc[19]);
+                P.my[1][e] = ldf(A.m[13], om, A.isc[22]);
+                P.my[2][e] = ldf(A.m[16], om, A.isc[25]);
+            }
+            if (cz >= 0) {
+                const unsigned om = unsigned((cz * A.ny + L.j[e]) * A.nx + L.i[e]);
+                P.mz[0][e] = ldf(A.m[11], om, A.isc[20]);
+                P.mz[1][e] = ldf(A.m[14], om, A.isc[23]);
+                P.mz[2][e] = ldf(A.m[17], om, A.isc[26]);
+            }
         }
     }
     if (top) {
-        P.szz_u = 0;
-        P.rho_u = 0;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            P.szz_u[e] = 0;
+            P.rho_u[e] = 0;
+        }
     }
 }
 
-template <typename R>
-struct VQueue {
-    R sxz_m, syz_m, szz_c, rho_c;
-};
-
-template <typename T, int MODE, bool ENERGY, typename R>
-__device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane<R>& L, int k,
-                                                const VPlane<R>& P, const VQueue<R>& Q,
-                                                double& ek, double& ep)
+template <typename T, int MODE, bool ENERGY, int V, typename R>
+__device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane<V>& L, const float* tx,
+                                                const float* ty, int k, const VPlane<R, V>& P,
+                                                const VQueue<R, V>& Q, double& ek, double& ep)
 {
-    const unsigned o = unsigned(k) * unsigned(A.nxy) + unsigned(L.p);
+    const unsigned o = unsigned(k) * unsigned(A.nxyp) + unsigned(L.p0);
     const R dt = (R)A.dt, idx = (R)A.idx, idy = (R)A.idy, idz = (R)A.idz;
     const ZTab<R> Z = ztab<T, MODE, R>(A, k);
     const int nx = A.nx, ny = A.ny, nz = A.nz;
-    const unsigned omx = unsigned((k * ny + L.j) * A.w[0] + L.cx);
-    const unsigned omy = unsigned((k * A.w[1] + L.cy) * nx + L.i);
-    const unsigned omz = unsigned((Z.cz * ny + L.j) * nx + L.i);
-    const bool face = A.dirichlet &&
-                      (L.i == 0 || L.i == nx - 1 || L.j == 0 || L.j == ny - 1 || k == 0 || k == nz - 1);
-
-    // vx: i in [1,nx-1], j in [1,ny-1], k in [1,nz-1] (CPU:502-529)
-    const bool onx = (L.i >= 1) && (L.j >= 1) && (k >= 1);
-    R vx = P.vx;
-    {
-        const R rh = (Q.rho_c + P.rhoW) / (R)2;
-        const R DT_over_rho = dt / rh;
-        R d1 = (P.sxx - P.sxxW) * idx;
-        R d2 = (P.sxy - P.sxyS) * idy;
-        R d3 = (P.sxz - Q.sxz_m) * idz;
-        R m9 = 0, m10 = 0, m11 = 0;
-        if (L.cx >= 0) m9 = cpml(P.mx[0], L.ax, L.bx, L.Kx, d1);
-        if (MODE == kSlab) {
-            if (L.cy >= 0) m10 = cpml(P.my[0], L.ay, L.by, L.Ky, d2);
-            if (Z.cz >= 0) m11 = cpml(P.mz[0], Z.az, Z.bz, Z.Kz, d3);
-        }
-        if (onx) {
-            vx = DT_over_rho * (d1 + d2 + d3) + P.vx;
-            if (L.cx >= 0) stf(A.m[9], omx, m9, A.sc[18]);
+    R nvx[V], nvy[V], nvz[V];
+    bool wx[V], wy[V], wz[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        const int i = L.i[e], j = L.j[e], cx = L.cx[e], cy = L.cy[e];
+        const unsigned omx = unsigned((k * ny + j) * A.w[0] + cx);
+        const unsigned omy = unsigned((k * A.w[1] + cy) * nx + i);
+        const unsigned omz = unsigned((Z.cz * ny + j) * nx + i);
+        const bool face = A.dirichlet && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1 || k == 0 || k == nz - 1);
+        // vx: i in [1,nx-1], j in [1,ny-1], k in [1,nz-1] (CPU:502-529)
+        const bool onx = L.act[e] && (i >= 1) && (j >= 1) && (k >= 1);
+        nvx[e] = P.vx[e];
+        {
+            const R rh = (Q.rho_c[e] + P.rhoW[e]) / (R)2;
+            const R DT_over_rho = dt / rh;
+            R d1 = (P.sxx[e] - P.sxxW[e]) * idx;
+            R d2 = (P.sxy[e] - P.sxyS[e]) * idy;
+            R d3 = (P.sxz[e] - Q.sxz_m[e]) * idz;
+            if (cx >= 0) {
+                const R m = cpml(P.mx[0][e], (R)tx[i], (R)tx[nx + i], (R)tx[2 * nx + i], d1);
+                if (onx) stf(A.m[9], omx, m, A.sc[18]);
+            }
             if (MODE == kSlab) {
-                if (L.cy >= 0) stf(A.m[10], omy, m10, A.sc[19]);
-                if (Z.cz >= 0) stf(A.m[11], omz, m11, A.sc[20]);
+                if (cy >= 0) {
+                    const R m = cpml(P.my[0][e], (R)ty[j], (R)ty[ny + j], (R)ty[2 * ny + j], d2);
+                    if (onx) stf(A.m[10], omy, m, A.sc[19]);
+                }
+                if (Z.cz >= 0) {
+                    const R m = cpml(P.mz[0][e], Z.az, Z.bz, Z.Kz, d3);
+                    if (onx) stf(A.m[11], omz, m, A.sc[20]);
+                }
+            }
+            if (onx) nvx[e] = DT_over_rho * (d1 + d2 + d3) + P.vx[e];
+        }
+        // vy: i in [0,nx-2], j in [0,ny-2], k in [1,nz-1] (CPU:531-558)
+        const bool ony = L.act[e] && (i <= nx - 2) && (j <= ny - 2) && (k >= 1);
+        nvy[e] = P.vy[e];
+        {
+            const R rh = (Q.rho_c[e] + P.rhoN[e]) / (R)2;
+            const R DT_over_rho = dt / rh;
+            R d1 = (P.sxyE[e] - P.sxy[e]) * idx;
+            R d2 = (P.syyN[e] - P.syy[e]) * idy;
+            R d3 = (P.syz[e] - Q.syz_m[e]) * idz;
+            if (cx >= 0) {
+                const R m = cpml(P.mx[1][e], (R)tx[3 * nx + i], (R)tx[4 * nx + i], (R)tx[5 * nx + i], d1);
+                if (ony) stf(A.m[12], omx, m, A.sc[21]);
+            }
+            if (MODE == kSlab) {
+                if (cy >= 0) {
+                    const R m = cpml(P.my[1][e], (R)ty[3 * ny + j], (R)ty[4 * ny + j], (R)ty[5 * ny + j], d2);
+                    if (ony) stf(A.m[13], omy, m, A.sc[22]);
+                }
+                if (Z.cz >= 0) {
+                    const R m = cpml(P.mz[1][e], Z.az, Z.bz, Z.Kz, d3);
+                    if (ony) stf(A.m[14], omz, m, A.sc[23]);
+                }
+            }
+            if (ony) nvy[e] = DT_over_rho * (d1 + d2 + d3) + P.vy[e];
+        }
+        // vz: i in [0,nx-2], j in [1,ny-1], k in [0,nz-2] (CPU:562-590)
+        const bool onz = L.act[e] && (i <= nx - 2) && (j >= 1) && (k <= nz - 2);
+        nvz[e] = P.vz[e];
+        {
+            const R rh = (Q.rho_c[e] + P.rho_u[e]) / (R)2;
+            const R DT_over_rho = dt / rh;
+            R d1 = (P.sxzE[e] - P.sxz[e]) * idx;
+            R d2 = (P.syz[e] - P.syzS[e]) * idy;
+            R d3 = (P.szz_u[e] - Q.szz_c[e]) * idz;
+            if (cx >= 0) {
+                const R m = cpml(P.mx[2][e], (R)tx[3 * nx + i], (R)tx[4 * nx + i], (R)tx[5 * nx + i], d1);
+                if (onz) stf(A.m[15], omx, m, A.sc[24]);
+            }
+            if (MODE == kSlab) {
+                if (cy >= 0) {
+                    const R m = cpml(P.my[2][e], (R)ty[j], (R)ty[ny + j], (R)ty[2 * ny + j], d2);
+                    if (onz) stf(A.m[16], omy, m, A.sc[25]);
+                }
+                if (Z.cz >= 0) {
+                    const R m = cpml(P.mz[2][e], Z.azh, Z.bzh, Z.Kzh, d3);
+                    if (onz) stf(A.m[17], omz, m, A.sc[26]);
+                }
+            }
+            if (onz) nvz[e] = DT_over_rho * (d1 + d2 + d3) + P.vz[e];
+        }
+        // rigid faces: the memory variables above are still advanced, then v = 0
+        const bool f = face && L.act[e];
+        if (f) {
+            nvx[e] = 0;
+            nvy[e] = 0;
+            nvz[e] = 0;
+        }
+        wx[e] = onx || f;
+        wy[e] = ony || f;
+        wz[e] = onz || f;
+        if (ENERGY) {
+            const int em = A.emargin;
+            if (L.act[e] && i >= em && i < nx - 1 - em && j >= em && j < ny - 1 - em && k >= em &&
+                k < nz - 1 - em) {
+                // state after this velocity update: cell-local values, FP64 terms
+                const double r = Q.rho_c[e], vp = P.cp[e], vs = P.cs[e];
+                const double mu = r * vs * vs, lam = r * (vp * vp - 2.0 * vs * vs);
+                const double sxx = P.sxx[e], syy = P.syy[e], szz = Q.szz_c[e];
+                const double sxy = P.sxy[e], sxz = P.sxz[e], syz = P.syz[e];
+                const double tr = sxx + syy + szz;
+                const double ss = sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + sxz * sxz + syz * syz);
+                const double dvx = nvx[e], dvy = nvy[e], dvz = nvz[e];
+                ek += 0.5 * r * (dvx * dvx + dvy * dvy + dvz * dvz);
+                ep += 0.5 * (ss / (2.0 * mu) - lam * tr * tr / (2.0 * mu * (3.0 * lam + 2.0 * mu)));
             }
         }
     }
-    // vy: i in [0,nx-2], j in [0,ny-2], k in [1,nz-1] (CPU:531-558)
-    const bool ony = (L.i <= nx - 2) && (L.j <= ny - 2) && (k >= 1);
-    R vy = P.vy;
-    {
-        const R rh = (Q.rho_c + P.rhoN) / (R)2;
-        const R DT_over_rho = dt / rh;
-        R d1 = (P.sxyE - P.sxy) * idx;
-        R d2 = (P.syyN - P.syy) * idy;
-        R d3 = (P.syz - Q.syz_m) * idz;
-        R m12 = 0, m13 = 0, m14 = 0;
-        if (L.cx >= 0) m12 = cpml(P.mx[1], L.axh, L.bxh, L.Kxh, d1);
-        if (MODE == kSlab) {
-            if (L.cy >= 0) m13 = cpml(P.my[1], L.ayh, L.byh, L.Kyh, d2);
-            if (Z.cz >= 0) m14 = cpml(P.mz[1], Z.az, Z.bz, Z.Kz, d3);
-        }
-        if (ony) {
-            vy = DT_over_rho * (d1 + d2 + d3) + P.vy;
-            if (L.cx >= 0) stf(A.m[12], omx, m12, A.sc[21]);
-            if (MODE == kSlab) {
-                if (L.cy >= 0) stf(A.m[13], omy, m13, A.sc[22]);
-                if (Z.cz >= 0) stf(A.m[14], omz, m14, A.sc[23]);
-            }
-        }
-    }
-    // vz: i in [0,nx-2], j in [1,ny-1], k in [0,nz-2] (CPU:562-590)
-    const bool onz = (L.i <= nx - 2) && (L.j >= 1) && (k <= nz - 2);
-    R vz = P.vz;
-    {
-        const R rh = (Q.rho_c + P.rho_u) / (R)2;
-        const R DT_over_rho = dt / rh;
-        R d1 = (P.sxzE - P.sxz) * idx;
-        R d2 = (P.syz - P.syzS) * idy;
-        R d3 = (P.szz_u - Q.szz_c) * idz;
-        R m15 = 0, m16 = 0, m17 = 0;
-        if (L.cx >= 0) m15 = cpml(P.mx[2], L.axh, L.bxh, L.Kxh, d1);
-        if (MODE == kSlab) {
-            if (L.cy >= 0) m16 = cpml(P.my[2], L.ay, L.by, L.Ky, d2);
-            if (Z.cz >= 0) m17 = cpml(P.mz[2], Z.azh, Z.bzh, Z.Kzh, d3);
-        }
-        if (onz) {
-            vz = DT_over_rho * (d1 + d2 + d3) + P.vz;
-            if (L.cx >= 0) stf(A.m[15], omx, m15, A.sc[24]);
-            if (MODE == kSlab) {
-                if (L.cy >= 0) stf(A.m[16], omy, m16, A.sc[25]);
-                if (Z.cz >= 0) stf(A.m[17], omz, m17, A.sc[26]);
-            }
-        }
-    }
-    // rigid faces: the memory variables above are still advanced, then v = 0
-    if (face) {
-        vx = 0;
-        vy = 0;
-        vz = 0;
-    }
-    if (onx || face) stf(A.f[0], o, vx, A.sc[0]);
-    if (ony || face) stf(A.f[1], o, vy, A.sc[1]);
-    if (onz || face) stf(A.f[2], o, vz, A.sc[2]);
-    if (ENERGY) {
-        const int em = A.emargin;
-        if (L.i >= em && L.i < nx - 1 - em && L.j >= em && L.j < ny - 1 - em && k >= em &&
-            k < nz - 1 - em) {
-            // state after this velocity update: cell-local values, FP64 terms
-            const double r = Q.rho_c, vp = P.cp, vs = P.cs;
-            const double mu = r * vs * vs, lam = r * (vp * vp - 2.0 * vs * vs);
-            const double sxx = P.sxx, syy = P.syy, szz = Q.szz_c;
-            const double sxy = P.sxy, sxz = P.sxz, syz = P.syz;
-            const double tr = sxx + syy + szz;
-            const double ss = sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + sxz * sxz + syz * syz);
-            const double dvx = vx, dvy = vy, dvz = vz;
-            ek += 0.5 * r * (dvx * dvx + dvy * dvy + dvz * dvz);
-            ep += 0.5 * (ss / (2.0 * mu) - lam * tr * tr / (2.0 * mu * (3.0 * lam + 2.0 * mu)));
-        }
-    }
+    stv<T, V>(A.f[0], o, nvx, wx, A.sc[0]);
+    stv<T, V>(A.f[1], o, nvy, wy, A.sc[1]);
+    stv<T, V>(A.f[2], o, nvz, wz, A.sc[2]);
 }
 
-template <typename T, int MODE, bool ENERGY, bool PIPE>
-__device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B, int local,
-                                              double& ek, double& ep)
+template <typename T, int MODE, bool ENERGY, int V>
+__device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B, int local, const float* tx,
+                                              const float* ty, double& ek, double& ep)
 {
     typedef typename Arith<T>::R R;
     const int chunk = local % B.nchunk, zs = local / B.nchunk;
-    const int p = B.p0 + chunk * A.block + (int)threadIdx.x;
-    if (p >= B.p1) return;
+    const int p0 = B.p0 + (chunk * A.block + (int)threadIdx.x) * V;
+    if (p0 >= B.p1) return;
     const int kb = B.k0 + zs * B.zseg;
     const int ke = min(kb + B.zseg, B.k1);
-    Lane<R> L;
-    make_lane<T, MODE, R>(A, p, L);
-    VQueue<R> Q;
+    Lane<V> L;
+    make_lane<T, MODE, V>(A, p0, B.p1, L);
+    VQueue<R, V> Q;
     {
-        const unsigned o = unsigned(kb) * unsigned(A.nxy) + unsigned(p);
-        Q.sxz_m = kb >= 1 ? ldf(A.f[7], o - unsigned(A.nxy), A.isc[7]) : (R)0;
-        Q.syz_m = kb >= 1 ? ldf(A.f[8], o - unsigned(A.nxy), A.isc[8]) : (R)0;
-        Q.szz_c = ldf(A.f[5], o, A.isc[5]);
-        Q.rho_c = A.rho[o];
+        const unsigned o = unsigned(kb) * unsigned(A.nxyp) + unsigned(p0);
+        if (kb >= 1) {
+            ldv<T, V>(A.f[7], o - unsigned(A.nxyp), A.isc[7], Q.sxz_m);
+            ldv<T, V>(A.f[8], o - unsigned(A.nxyp), A.isc[8], Q.syz_m);
+        } else {
+            for (int e = 0; e < V; ++e) {
+                Q.sxz_m[e] = 0;
+                Q.syz_m[e] = 0;
+            }
+        }
+        ldv<T, V>(A.f[5], o, A.isc[5], Q.szz_c);
+        ldm<V>(A.rho, o, Q.rho_c);
     }
-    VPlane<R> P;
-    if (PIPE) velocity_load<T, MODE, ENERGY, R>(A, L, kb, P);
     for (int k = kb; k < ke; ++k) {
-        VPlane<R> N;
-        if (PIPE)
-            velocity_load<T, MODE, ENERGY, R>(A, L, min(k + 1, ke - 1), N);
-        else
-            velocity_load<T, MODE, ENERGY, R>(A, L, k, P);
-        velocity_update<T, MODE, ENERGY, R>(A, L, k, P, Q, ek, ep);
-        Q.sxz_m = P.sxz;
-        Q.syz_m = P.syz;
-        Q.szz_c = P.szz_u;
-        Q.rho_c = P.rho_u;
-        if (PIPE) P = N;
+        VPlane<R, V> P;
+        velocity_load<T, MODE, ENERGY, V, R>(A, L, k, P);
+        velocity_update<T, MODE, ENERGY, V, R>(A, L, tx, ty, k, P, Q, ek, ep);
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            Q.sxz_m[e] = P.sxz[e];
+            Q.syz_m[e] = P.syz[e];
+            Q.szz_c[e] = P.szz_u[e];
+            Q.rho_c[e] = P.rho_u[e];
+        }
     }
 }
 
@@ -566,27 +671,33 @@ __device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B
 // One launch per pass: the C-PML slab boxes come first in logical block order,
 // then the interior box; each block runs the body specialised for its box.
 // ---------------------------------------------------------------------------
-template <typename T, bool PIPE>
+template <typename T, int V>
 __global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
 {
+    extern __shared__ float tabs[];
+    stage_tables(A, tabs);
+    const float *tx = tabs, *ty = tabs + 6 * A.nx;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const Box B = pick_box(A.box, A.nbox, lb);
     if (B.mode == kSlab)
-        stress_body<T, kSlab, PIPE>(A, B, lb - B.blk0);
+        stress_body<T, kSlab, V>(A, B, lb - B.blk0, tx, ty);
     else
-        stress_body<T, kInterior, PIPE>(A, B, lb - B.blk0);
+        stress_body<T, kInterior, V>(A, B, lb - B.blk0, tx, ty);
 }
 
-template <typename T, bool ENERGY, bool PIPE>
+template <typename T, bool ENERGY, int V>
 __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
 {
+    extern __shared__ float tabs[];
+    stage_tables(A, tabs);
+    const float *tx = tabs, *ty = tabs + 6 * A.nx;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const Box B = pick_box(A.box, A.nbox, lb);
     double ek = 0.0, ep = 0.0;
     if (B.mode == kSlab)
-        velocity_body<T, kSlab, ENERGY, PIPE>(A, B, lb - B.blk0, ek, ep);
+        velocity_body<T, kSlab, ENERGY, V>(A, B, lb - B.blk0, tx, ty, ek, ep);
     else
-        velocity_body<T, kInterior, ENERGY, PIPE>(A, B, lb - B.blk0, ek, ep);
+        velocity_body<T, kInterior, ENERGY, V>(A, B, lb - B.blk0, tx, ty, ek, ep);
     if (ENERGY) {  // per-block FP64 partials, summed in block order by the tail kernel
         __shared__ double red[2][4];
         ek = wave_sum(ek);
@@ -610,31 +721,50 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
     }
 }
 
+static size_t table_lds(int nx, int ny) { return size_t(6) * (nx + ny) * sizeof(float); }
+
+template <typename T, int V>
+static void launch_stress_v(const StepArgs<T>& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_stress<T, V>), dim3(a.nblocks), dim3(a.block), table_lds(a.nx, a.ny), s, a);
+}
+
+template <typename T, bool E, int V>
+static void launch_velocity_v(const StepArgs<T>& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_velocity<T, E, V>), dim3(a.nblocks), dim3(a.block), table_lds(a.nx, a.ny), s, a);
+}
+
 template <typename T>
 void launch_stress(const StepArgs<T>& a, hipStream_t s)
 {
     if (a.nblocks <= 0) return;
-    if (a.pipe)
-        hipLaunchKernelGGL((k_stress<T, true>), dim3(a.nblocks), dim3(a.block), 0, s, a);
+    if (a.vec == 4)
+        launch_stress_v<T, 4>(a, s);
+    else if (a.vec == 2)
+        launch_stress_v<T, 2>(a, s);
     else
-        hipLaunchKernelGGL((k_stress<T, false>), dim3(a.nblocks), dim3(a.block), 0, s, a);
+        launch_stress_v<T, 1>(a, s);
 }
 
 template <typename T>
 void launch_velocity(const StepArgs<T>& a, bool energy, hipStream_t s)
 {
     if (a.nblocks <= 0) return;
-    const dim3 g(a.nblocks), b(a.block);
     if (energy) {
-        if (a.pipe)
-            hipLaunchKernelGGL((k_velocity<T, true, true>), g, b, 0, s, a);
+        if (a.vec == 4)
+            launch_velocity_v<T, true, 4>(a, s);
+        else if (a.vec == 2)
+            launch_velocity_v<T, true, 2>(a, s);
         else
-            hipLaunchKernelGGL((k_velocity<T, true, false>), g, b, 0, s, a);
+            launch_velocity_v<T, true, 1>(a, s);
     } else {
-        if (a.pipe)
-            hipLaunchKernelGGL((k_velocity<T, false, true>), g, b, 0, s, a);
+        if (a.vec == 4)
+            launch_velocity_v<T, false, 4>(a, s);
+        else if (a.vec == 2)
+            launch_velocity_v<T, false, 2>(a, s);
         else
-            hipLaunchKernelGGL((k_velocity<T, false, false>), g, b, 0, s, a);
+            launch_velocity_v<T, false, 1>(a, s);
     }
 }
 

@@ -33,6 +33,7 @@ struct StepArgs {
     const float* rho;
     const double* tab[3];  // per axis [6][n]: a, b, K (full grid), a, b, K (half grid)
     int nx, ny, nz, nxy;
+    int nxyp;                // padded plane stride of the 9 fields and the model (elements)
     int lo[3], hi[3], w[3];  // compact C-PML regions [0,lo) u [hi,n), width w
     int kmax_shear;          // last plane of the sxz/syz update
     int dirichlet;
@@ -42,7 +43,7 @@ struct StepArgs {
     double* epart;           // energy partials [nblocks][2]
     Box box[kMaxBox];
     int nbox, nblocks, block;
-    int pipe;                // 1: issue plane k+1's loads before computing plane k
+    int vec;                 // consecutive xy positions per lane: 1, 2 or 4
 };
 
 template <typename T>
@@ -61,7 +62,7 @@ struct TailArgs {
     double evol;             // cell volume
     const double* wavelet;
     int nwavelet;
-    int src_kind, src, nx, nxy;
+    int src_kind, src, nx, nxy;  // nxy: plane stride of the device layout
     double sin_t, cos_t, dt;
     int* step;               // steps completed (device counter)
     int* flag;               // set to 1 when a sample or the energy is not finite

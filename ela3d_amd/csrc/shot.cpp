@@ -29,28 +29,33 @@ Tuning choose_tuning(const ShotPlan& p)
     const ela3d_config& c = p.cfg;
     if (c.block == 0 && c.zseg == 0 && cached_tuning(c, t)) return t;
     t.block = c.block > 0 ? c.block : 256;
+    t.vec = 2;
     if (c.zseg > 0) {
         t.zseg_interior = c.zseg;
         t.zseg_slab = c.zseg;
         return t;
     }
-    // enough workgroups to fill 256 CUs several times over, while keeping the
-    // z-segment long enough that the extra plane read at its start stays small
-    const int64_t nchunk = (int64_t(c.nx) * (p.ax[1].hi - p.ax[1].lo) + t.block - 1) / t.block;
+    // enough workgroups to fill 256 CUs several times over; short z-segments
+    // measured best on MI355X (more waves in flight beats longer register queues)
+    const int64_t nchunk = (int64_t(c.nx) * (p.ax[1].hi - p.ax[1].lo) + t.block * t.vec - 1) / (t.block * t.vec);
     const int nzi = std::max(1, p.ax[2].hi - p.ax[2].lo);
-    int zs = 32;
-    while (zs > 4 && nchunk * ((nzi + zs - 1) / zs) < 4096) zs /= 2;
+    int zs = 8;
+    while (zs > 2 && nchunk * ((nzi + zs - 1) / zs) < 8192) zs /= 2;
     t.zseg_interior = zs;
-    t.zseg_slab = 8;
+    t.zseg_slab = 4;
     return t;
 }
+
+int64_t plane_stride(int nx, int ny) { return (int64_t(nx) * ny + 63) / 64 * 64; }
+int64_t guard_elems(int nx) { return (int64_t(nx) + 64 + 63) / 64 * 64; }
 
 static void add_box(LaunchGeom& g, int p0, int p1, int k0, int k1, int zseg, int mode)
 {
     if (p1 <= p0 || k1 <= k0) return;
     Box& b = g.box[g.nbox++];
+    const int per = g.block * g.vec;
     b.p0 = p0; b.p1 = p1; b.k0 = k0; b.k1 = k1;
-    b.nchunk = (p1 - p0 + g.block - 1) / g.block;
+    b.nchunk = (p1 - p0 + per - 1) / per;
     b.zseg = std::max(1, zseg);
     b.blk0 = g.nblocks;
     b.mode = mode;
@@ -59,19 +64,25 @@ static void add_box(LaunchGeom& g, int p0, int p1, int k0, int k1, int zseg, int
 
 // One launch per pass.  The four absorbing-layer boxes come first in logical
 // block order so their slower blocks start early; the interior box follows.
+// Box starts are multiples of 64 elements, so with the padded plane stride
+// every workgroup tile begins on a 256-byte line boundary and VEC groups are
+// aligned; the rows between the y layers and those boundaries go to the slabs.
 void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& g)
 {
     const ela3d_config& c = p.cfg;
     const int nx = c.nx, nxy = c.nx * c.ny;
     const int ylo = p.ax[1].lo, yhi = p.ax[1].hi, zlo = p.ax[2].lo, zhi = p.ax[2].hi;
+    int pa = (ylo * nx + 63) / 64 * 64, pb = (yhi * nx) / 64 * 64;
+    if (pb <= pa) pa = pb = nxy;  // too small for an interior box: all slab
     g = LaunchGeom();
     g.block = t.block;
+    g.vec = t.vec;
     const int zs = t.zseg_slab, zi = t.zseg_interior;
     add_box(g, 0, nxy, 0, zlo, zs, kSlab);                 // z-min absorbing layer
     add_box(g, 0, nxy, zhi, c.nz, zs, kSlab);              // z-max absorbing layer
-    add_box(g, 0, ylo * nx, zlo, zhi, zi, kSlab);          // y-min absorbing layer
-    add_box(g, yhi * nx, nxy, zlo, zhi, zi, kSlab);        // y-max absorbing layer
-    add_box(g, ylo * nx, yhi * nx, zlo, zhi, zi, kInterior);
+    add_box(g, 0, pa, zlo, zhi, zi, kSlab);                // y-min absorbing layer
+    add_box(g, pb, nxy, zlo, zhi, zi, kSlab);              // y-max absorbing layer
+    add_box(g, pa, pb, zlo, zhi, zi, kInterior);
 }
 
 template <typename T>
@@ -166,7 +177,7 @@ public:
         const size_t cells = size_t(plan_.cells);
         if (field < 9) {
             std::vector<T> h(cells);
-            HIP_CHECK(hipMemcpy(h.data(), d_f_[field], cells * sizeof(T), hipMemcpyDeviceToHost));
+            copy_planes(h.data(), d_f_[field], sizeof(T), hipMemcpyDeviceToHost);
             const double inv = 1.0 / sc_[field];
             for (size_t q = 0; q < cells; ++q) out[q] = double(h[q]) * inv;
             return;
@@ -189,7 +200,7 @@ public:
         if (field < 9) {
             std::vector<T> h(cells);
             for (size_t q = 0; q < cells; ++q) h[q] = T(in[q] * sc_[field]);
-            HIP_CHECK(hipMemcpy(d_f_[field], h.data(), cells * sizeof(T), hipMemcpyHostToDevice));
+            copy_planes(d_f_[field], h.data(), sizeof(T), hipMemcpyHostToDevice);
             return;
         }
         const int m = field - 9, axis = kMemAxis[m];
@@ -345,11 +356,15 @@ private:
         const ela3d_config& c = plan_.cfg;
         const size_t cells = size_t(plan_.cells);
         auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+        nxyp_ = plane_stride(c.nx, c.ny);
+        guard_ = guard_elems(c.nx);
+        const size_t padded = size_t(nxyp_) * c.nz + 2 * size_t(guard_);  // guard | planes | guard
         size_t off = 0;
         size_t o_f[9], o_m[18], o_model[3], o_tab[3];
-        for (int q = 0; q < 9; ++q) { o_f[q] = off; off += al(cells * sizeof(T)); }
+        for (int q = 0; q < 9; ++q) { o_f[q] = off; off += al(padded * sizeof(T)); }
         for (int q = 0; q < 18; ++q) { o_m[q] = off; off += al(mem_count(kMemAxis[q]) * sizeof(T)); }
-        for (int q = 0; q < 3; ++q) { o_model[q] = off; off += al(cells * sizeof(float)); }
+        for (int q = 0; q < 3; ++q) { o_model[q] = off; off += al(padded * sizeof(float)); }
+        (void)cells;
         const int ns[3] = {c.nx, c.ny, c.nz};
         for (int q = 0; q < 3; ++q) { o_tab[q] = off; off += al(6 * size_t(ns[q]) * sizeof(double)); }
         const size_t o_wav = off; off += al(std::max<size_t>(1, plan_.wavelet.size()) * sizeof(double));
@@ -358,16 +373,16 @@ private:
         const size_t o_seis = off; off += al(std::max<size_t>(1, nseis) * sizeof(double));
         const size_t o_en = off; off += al(std::max<size_t>(1, size_t(plan_.nenergy) * 2) * sizeof(double));
         nepart_ = geo_.nblocks;
-        // room for any launch shape the tuner may pick (block >= 64, zseg >= 1)
-        epart_cap_ = int(plan_.cells / 64 + int64_t(kMaxBox) * c.nz + 8);
+        // room for any launch shape the tuner may pick (block >= 64, vec >= 1, zseg >= 1)
+        epart_cap_ = int((nxyp_ / 64 + kMaxBox + 1) * int64_t(c.nz) + 8);
         const size_t o_ep = off; off += al(size_t(std::max(epart_cap_, nepart_)) * 2 * sizeof(double));
         const size_t o_ctl = off; off += al(2 * sizeof(int));
         arena_bytes_ = int64_t(off);
         HIP_CHECK(hipMalloc(&arena_, off));
         char* b = static_cast<char*>(arena_);
-        for (int q = 0; q < 9; ++q) d_f_[q] = reinterpret_cast<T*>(b + o_f[q]);
+        for (int q = 0; q < 9; ++q) d_f_[q] = reinterpret_cast<T*>(b + o_f[q]) + guard_;
         for (int q = 0; q < 18; ++q) d_m_[q] = reinterpret_cast<T*>(b + o_m[q]);
-        for (int q = 0; q < 3; ++q) d_model_[q] = reinterpret_cast<float*>(b + o_model[q]);
+        for (int q = 0; q < 3; ++q) d_model_[q] = reinterpret_cast<float*>(b + o_model[q]) + guard_;
         for (int q = 0; q < 3; ++q) d_tab_[q] = reinterpret_cast<double*>(b + o_tab[q]);
         d_wav_ = reinterpret_cast<double*>(b + o_wav);
         d_rec_ = reinterpret_cast<int*>(b + o_rec);
@@ -376,18 +391,27 @@ private:
         d_epart_ = reinterpret_cast<double*>(b + o_ep);
         d_step_ = reinterpret_cast<int*>(b + o_ctl);
         d_flag_ = d_step_ + 1;
-        // zero wavefields, memory variables and outputs (CPU:107-140)
-        HIP_CHECK(hipMemsetAsync(arena_, 0, o_model[0], stream_));
-        HIP_CHECK(hipMemsetAsync(b + o_seis, 0, off - o_seis, stream_));
+        // zero everything: wavefields and memory variables (CPU:107-140), the guards
+        // and plane padding (read as neighbours, never written), outputs
+        HIP_CHECK(hipMemsetAsync(arena_, 0, off, stream_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        (void)o_seis;
+    }
+
+    // host x-fastest nx*ny*nz  <->  device planes of stride nxyp_
+    void copy_planes(void* dst, const void* src, size_t elem, hipMemcpyKind kind)
+    {
+        const ela3d_config& c = plan_.cfg;
+        const size_t row = size_t(c.nx) * c.ny * elem, dev = size_t(nxyp_) * elem;
+        const bool to_dev = kind == hipMemcpyHostToDevice;
+        HIP_CHECK(hipMemcpy2D(dst, to_dev ? dev : row, src, to_dev ? row : dev, row, size_t(c.nz), kind));
     }
 
     void upload(const float* cp, const float* cs, const float* rho)
     {
         const ela3d_config& c = plan_.cfg;
-        const size_t cells = size_t(plan_.cells);
         const float* src[3] = {cp, cs, rho};
-        for (int q = 0; q < 3; ++q)
-            HIP_CHECK(hipMemcpyAsync(d_model_[q], src[q], cells * sizeof(float), hipMemcpyHostToDevice, stream_));
+        for (int q = 0; q < 3; ++q) copy_planes(d_model_[q], src[q], sizeof(float), hipMemcpyHostToDevice);
         for (int a = 0; a < 3; ++a) {
             const PmlAxis& A = plan_.ax[a];
             std::vector<double> t(6 * size_t(A.n));
@@ -420,6 +444,7 @@ private:
         a.rho = d_model_[2];
         for (int q = 0; q < 3; ++q) a.tab[q] = d_tab_[q];
         a.nx = c.nx; a.ny = c.ny; a.nz = c.nz; a.nxy = c.nx * c.ny;
+        a.nxyp = int(nxyp_);
         for (int q = 0; q < 3; ++q) {
             a.lo[q] = plan_.ax[q].lo;
             a.hi[q] = plan_.ax[q].hi;
@@ -440,7 +465,7 @@ private:
         for (int q = 0; q < g.nbox; ++q) a.box[q] = g.box[q];
         a.nbox = g.nbox;
         a.nblocks = g.nblocks;
-        a.pipe = tune_.pipe;
+        a.vec = g.vec;
         a.block = g.block;
         return a;
     }
@@ -468,8 +493,8 @@ private:
         a.nwavelet = int(plan_.wavelet.size());
         a.src_kind = c.src_kind;
         a.nx = c.nx;
-        a.nxy = c.nx * c.ny;
-        a.src = c.isrc + c.jsrc * c.nx + c.ksrc * c.nx * c.ny;
+        a.nxy = int(nxyp_);  // plane stride of the device layout
+        a.src = c.isrc + c.jsrc * c.nx + c.ksrc * int(nxyp_);
         const double th = c.angle_force * 3.141592653589793238462643 / 180.0;
         a.sin_t = std::sin(th);
         a.cos_t = std::cos(th);
@@ -525,6 +550,7 @@ private:
     double *d_wav_ = nullptr, *d_seis_ = nullptr, *d_energy_ = nullptr, *d_epart_ = nullptr;
     int *d_rec_ = nullptr, *d_step_ = nullptr, *d_flag_ = nullptr;
     int nepart_ = 0, epart_cap_ = 0;
+    int64_t nxyp_ = 0, guard_ = 0;
     int host_step_ = 0;
     float sc_[27], isc_[27];
     hipGraphExec_t graphs_[2] = {nullptr, nullptr};

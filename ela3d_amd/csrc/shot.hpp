@@ -24,7 +24,7 @@ namespace ela3d {
 // Launch shape shared by the stress and the velocity launch.
 struct LaunchGeom {
     Box box[kMaxBox];
-    int nbox = 0, nblocks = 0, block = 256;
+    int nbox = 0, nblocks = 0, block = 256, vec = 1;
 };
 
 // Launch tuning chosen for a grid (see tune.cpp).
@@ -32,8 +32,13 @@ struct Tuning {
     int block = 256;
     int zseg_interior = 16;
     int zseg_slab = 8;
-    int pipe = 1;
+    int vec = 2;  // consecutive xy positions per lane
 };
+
+// Device layout of the 9 fields and the 3 model arrays: planes of plane_stride
+// elements (nx*ny rounded up to 64) between zeroed guards of guard_elems.
+int64_t plane_stride(int nx, int ny);
+int64_t guard_elems(int nx);
 
 class Shot {
 public:

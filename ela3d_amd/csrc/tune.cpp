@@ -75,14 +75,14 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
         if (hipGetDeviceCount(&n) != hipSuccess || n <= c.device) throw Error{ELA3D_E_NODEVICE, "no HIP device"};
         std::unique_ptr<ela3d::Shot> shot = ela3d::make_shot(plan, cp, cs, rho);
         static const int blocks[] = {64, 128, 256};
-        static const int zsi[] = {4, 8, 16, 32, 64};
+        static const int zsi[] = {2, 4, 8, 16, 32};
         static const int zss[] = {2, 4, 8};
-        static const int pipes[] = {0, 1};
+        static const int vecs[] = {1, 2, 4};
         Tuning best;
         double best_ms = 1e300;
         int tried = 0;
         std::vector<ela3d_kernel_stats> st;
-        for (int pp : pipes)
+        for (int vv : vecs)
         for (int b : blocks)
             for (int zi : zsi)
                 for (int zs : zss) {
@@ -90,7 +90,7 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
                     t.block = b;
                     t.zseg_interior = zi;
                     t.zseg_slab = zs;
-                    t.pipe = pp;
+                    t.vec = vv;
                     shot->retune(t);
                     double wall = 0.0;
                     shot->profile(1, st, wall);  // warm
@@ -98,8 +98,8 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
                     const double ms = (st[0].total_ms + st[1].total_ms) / steps;
                     ++tried;
                     if (std::getenv("ELA3D_TUNE_VERBOSE"))
-                        std::fprintf(stderr, "[tune] pipe %d block %3d zseg %3d/%d: stress %.4f velocity %.4f ms\n",
-                                     pp, b, zi, zs, st[0].total_ms / steps, st[1].total_ms / steps);
+                        std::fprintf(stderr, "[tune] vec %d block %3d zseg %3d/%d: stress %.4f velocity %.4f ms\n",
+                                     vv, b, zi, zs, st[0].total_ms / steps, st[1].total_ms / steps);
                     if (ms < best_ms) {
                         best_ms = ms;
                         best = t;
@@ -110,7 +110,7 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
             out->block = best.block;
             out->zseg_interior = best.zseg_interior;
             out->zseg_slab = best.zseg_slab;
-            out->pipe = best.pipe;
+            out->vec = best.vec;
             out->ms_per_step = best_ms;
             out->candidates = tried;
         }
@@ -132,7 +132,8 @@ int ela3d_tuning_set(const ela3d_config* cfg, const ela3d_tuning* t)
     u.block = t->block;
     u.zseg_interior = t->zseg_interior;
     u.zseg_slab = t->zseg_slab;
-    u.pipe = t->pipe ? 1 : 0;
+    if (t->vec != 1 && t->vec != 2 && t->vec != 4) return ELA3D_E_INVALID;
+    u.vec = t->vec;
     ela3d::store_tuning(*cfg, u, t->ms_per_step);
     return ELA3D_OK;
 }
@@ -145,7 +146,7 @@ int ela3d_tuning_get(const ela3d_config* cfg, ela3d_tuning* t)
     t->block = u.block;
     t->zseg_interior = u.zseg_interior;
     t->zseg_slab = u.zseg_slab;
-    t->pipe = u.pipe;
+    t->vec = u.vec;
     t->ms_per_step = 0.0;
     t->candidates = 0;
     return ELA3D_OK;

@@ -184,7 +184,7 @@ int ela3d_pml_tables(const ela3d_config* cfg, int32_t axis, double* out, size_t 
  * block = 0 and zseg = 0 use the cached shape. */
 typedef struct ela3d_tuning {
     int32_t block, zseg_interior, zseg_slab;
-    int32_t pipe;         /* 1: software-pipelined z march (next plane's loads in flight) */
+    int32_t vec;          /* consecutive xy positions per lane: 1, 2 or 4 */
     int32_t candidates;
     double ms_per_step;   /* stress + velocity launches */
 } ela3d_tuning;
