@@ -48,6 +48,16 @@ __device__ __forceinline__ Box pick_box(const Box* bx, int nbox, int lb)
     return B;
 }
 
+// Element offset of cell (p, k) from the start of a field or model allocation.
+// Field and model pointers handed to the stencil kernels point at the start of
+// the allocation, g0 guard elements before plane 0, so every neighbour offset
+// (o - 1, o - nx, o + nx + VEC - 1, ...) is a non-negative in-bounds 32-bit value.
+template <typename T>
+__device__ __forceinline__ unsigned cell(const StepArgs<T>& A, int k, int p)
+{
+    return unsigned(A.g0) + unsigned(k) * unsigned(A.nxyp) + unsigned(p);
+}
+
 // VEC-wide load/store of storage T at element offset o (element-aligned only)
 template <typename T, int V>
 __device__ __forceinline__ void ldv(const T* p, unsigned o, float isc, typename Arith<T>::R (&out)[V])
@@ -119,6 +129,11 @@ __device__ __forceinline__ R cpml(R psi_old, R a, R b, R K, R& d)
     d = d / K + psi;
     return psi;
 }
+
+// Element type the x/y C-PML coefficients are read in: FP32 copies staged in
+// LDS for FP32 arithmetic, the FP64 tables themselves for the FP64 instantiation.
+template <typename T>
+using TabP = typename std::conditional<std::is_same<T, double>::value, double, float>::type;
 
 // FP32 copies of the x and y tables, staged per workgroup: [6][nx] then [6][ny]
 // (a, b, K on the full grid, a, b, K on the half grid).
@@ -208,7 +223,7 @@ template <typename T, int MODE, int V, typename R>
 __device__ __forceinline__ void stress_load(const StepArgs<T>& A, const Lane<V>& L, int k, SPlane<R, V>& P)
 {
     const unsigned nxyp = A.nxyp, nx = A.nx;
-    const unsigned o = unsigned(k) * nxyp + unsigned(L.p0);
+    const unsigned o = cell(A, k, L.p0);
     const bool top = (k == A.nz - 1);
     const unsigned dU = top ? 0u : nxyp;
     const T *vx = A.f[0], *vy = A.f[1], *vz = A.f[2];
@@ -250,7 +265,7 @@ __device__ __forceinline__ void stress_load(const StepArgs<T>& A, const Lane<V>&
                 P.my[1][e] = ldf(A.m[4], om, A.isc[13]);
                 P.my[2][e] = ldf(A.m[7], om, A.isc[16]);
             }
-            if (cz >= 0) {
+            if (cz >= 0 && L.act[e]) {  // elements past the box end may sit past the last row
                 const unsigned om = unsigned((cz * A.ny + L.j[e]) * A.nx + L.i[e]);
                 P.mz[0][e] = ldf(A.m[2], om, A.isc[11]);
                 P.mz[1][e] = ldf(A.m[6], om, A.isc[15]);
@@ -270,11 +285,11 @@ __device__ __forceinline__ void stress_load(const StepArgs<T>& A, const Lane<V>&
 }
 
 template <typename T, int MODE, int V, typename R>
-__device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V>& L, const float* tx,
-                                              const float* ty, int k, const SPlane<R, V>& P,
+__device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V>& L, const TabP<T>* tx,
+                                              const TabP<T>* ty, int k, const SPlane<R, V>& P,
                                               const SQueue<R, V>& Q)
 {
-    const unsigned o = unsigned(k) * unsigned(A.nxyp) + unsigned(L.p0);
+    const unsigned o = cell(A, k, L.p0);
     const R dt = (R)A.dt, idx = (R)A.idx, idy = (R)A.idy, idz = (R)A.idz;
     const ZTab<R> Z = ztab<T, MODE, R>(A, k);
     const int nx = A.nx, ny = A.ny;
@@ -382,8 +397,8 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
 }
 
 template <typename T, int MODE, int V>
-__device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, int local, const float* tx,
-                                            const float* ty)
+__device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, int local, const TabP<T>* tx,
+                                            const TabP<T>* ty)
 {
     typedef typename Arith<T>::R R;
     const int chunk = local % B.nchunk, zs = local / B.nchunk;
@@ -395,7 +410,7 @@ __device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, 
     make_lane<T, MODE, V>(A, p0, B.p1, L);
     SQueue<R, V> Q;
     {
-        const unsigned o = unsigned(kb) * unsigned(A.nxyp) + unsigned(p0);
+        const unsigned o = cell(A, kb, p0);
         ldv<T, V>(A.f[0], o, A.isc[0], Q.vx_c);
         ldv<T, V>(A.f[1], o, A.isc[1], Q.vy_c);
         ldm<V>(A.cs, o, Q.cs_c);
@@ -442,7 +457,7 @@ template <typename T, int MODE, bool ENERGY, int V, typename R>
 __device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V>& L, int k, VPlane<R, V>& P)
 {
     const unsigned nxyp = A.nxyp, nx = A.nx;
-    const unsigned o = unsigned(k) * nxyp + unsigned(L.p0);
+    const unsigned o = cell(A, k, L.p0);
     const bool top = (k == A.nz - 1);
     const unsigned dU = top ? 0u : nxyp;
     const T *sxx = A.f[3], *syy = A.f[4], *szz = A.f[5], *sxy = A.f[6], *sxz = A.f[7], *syz = A.f[8];
@@ -484,7 +499,7 @@ __device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V
                 P.my[1][e] = ldf(A.m[13], om, A.isc[22]);
                 P.my[2][e] = ldf(A.m[16], om, A.isc[25]);
             }
-            if (cz >= 0) {
+            if (cz >= 0 && L.act[e]) {  // elements past the box end may sit past the last row
                 const unsigned om = unsigned((cz * A.ny + L.j[e]) * A.nx + L.i[e]);
                 P.mz[0][e] = ldf(A.m[11], om, A.isc[20]);
                 P.mz[1][e] = ldf(A.m[14], om, A.isc[23]);
@@ -502,11 +517,11 @@ __device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V
 }
 
 template <typename T, int MODE, bool ENERGY, int V, typename R>
-__device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane<V>& L, const float* tx,
-                                                const float* ty, int k, const VPlane<R, V>& P,
+__device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane<V>& L, const TabP<T>* tx,
+                                                const TabP<T>* ty, int k, const VPlane<R, V>& P,
                                                 const VQueue<R, V>& Q, double& ek, double& ep)
 {
-    const unsigned o = unsigned(k) * unsigned(A.nxyp) + unsigned(L.p0);
+    const unsigned o = cell(A, k, L.p0);
     const R dt = (R)A.dt, idx = (R)A.idx, idy = (R)A.idy, idz = (R)A.idz;
     const ZTab<R> Z = ztab<T, MODE, R>(A, k);
     const int nx = A.nx, ny = A.ny, nz = A.nz;
@@ -627,8 +642,8 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
 }
 
 template <typename T, int MODE, bool ENERGY, int V>
-__device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B, int local, const float* tx,
-                                              const float* ty, double& ek, double& ep)
+__device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B, int local, const TabP<T>* tx,
+                                              const TabP<T>* ty, double& ek, double& ep)
 {
     typedef typename Arith<T>::R R;
     const int chunk = local % B.nchunk, zs = local / B.nchunk;
@@ -640,7 +655,7 @@ __device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B
     make_lane<T, MODE, V>(A, p0, B.p1, L);
     VQueue<R, V> Q;
     {
-        const unsigned o = unsigned(kb) * unsigned(A.nxyp) + unsigned(p0);
+        const unsigned o = cell(A, kb, p0);
         if (kb >= 1) {
             ldv<T, V>(A.f[7], o - unsigned(A.nxyp), A.isc[7], Q.sxz_m);
             ldv<T, V>(A.f[8], o - unsigned(A.nxyp), A.isc[8], Q.syz_m);
@@ -675,8 +690,15 @@ template <typename T, int V>
 __global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
 {
     extern __shared__ float tabs[];
-    stage_tables(A, tabs);
-    const float *tx = tabs, *ty = tabs + 6 * A.nx;
+    const TabP<T>*tx, *ty;
+    if constexpr (std::is_same<T, double>::value) {
+        tx = A.tab[0];
+        ty = A.tab[1];
+    } else {
+        stage_tables(A, tabs);
+        tx = tabs;
+        ty = tabs + 6 * A.nx;
+    }
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const Box B = pick_box(A.box, A.nbox, lb);
     if (B.mode == kSlab)
@@ -689,8 +711,15 @@ template <typename T, bool ENERGY, int V>
 __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
 {
     extern __shared__ float tabs[];
-    stage_tables(A, tabs);
-    const float *tx = tabs, *ty = tabs + 6 * A.nx;
+    const TabP<T>*tx, *ty;
+    if constexpr (std::is_same<T, double>::value) {
+        tx = A.tab[0];
+        ty = A.tab[1];
+    } else {
+        stage_tables(A, tabs);
+        tx = tabs;
+        ty = tabs + 6 * A.nx;
+    }
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const Box B = pick_box(A.box, A.nbox, lb);
     double ek = 0.0, ep = 0.0;
@@ -721,18 +750,22 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
     }
 }
 
-static size_t table_lds(int nx, int ny) { return size_t(6) * (nx + ny) * sizeof(float); }
+template <typename T>
+static size_t table_lds(int nx, int ny)
+{
+    return std::is_same<T, double>::value ? 0 : size_t(6) * (nx + ny) * sizeof(float);
+}
 
 template <typename T, int V>
 static void launch_stress_v(const StepArgs<T>& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_stress<T, V>), dim3(a.nblocks), dim3(a.block), table_lds(a.nx, a.ny), s, a);
+    hipLaunchKernelGGL((k_stress<T, V>), dim3(a.nblocks), dim3(a.block), table_lds<T>(a.nx, a.ny), s, a);
 }
 
 template <typename T, bool E, int V>
 static void launch_velocity_v(const StepArgs<T>& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_velocity<T, E, V>), dim3(a.nblocks), dim3(a.block), table_lds(a.nx, a.ny), s, a);
+    hipLaunchKernelGGL((k_velocity<T, E, V>), dim3(a.nblocks), dim3(a.block), table_lds<T>(a.nx, a.ny), s, a);
 }
 
 template <typename T>

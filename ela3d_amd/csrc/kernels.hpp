@@ -34,6 +34,7 @@ struct StepArgs {
     const double* tab[3];  // per axis [6][n]: a, b, K (full grid), a, b, K (half grid)
     int nx, ny, nz, nxy;
     int nxyp;                // padded plane stride of the 9 fields and the model (elements)
+    int g0;                  // guard elements before plane 0; f[] and cp/cs/rho point at the guard
     int lo[3], hi[3], w[3];  // compact C-PML regions [0,lo) u [hi,n), width w
     int kmax_shear;          // last plane of the sxz/syz update
     int dirichlet;

@@ -437,14 +437,17 @@ private:
         const ela3d_config& c = plan_.cfg;
         StepArgs<T> a;
         std::memset(&a, 0, sizeof(a));
-        for (int q = 0; q < 9; ++q) a.f[q] = d_f_[q];
+        // stencil kernels address fields and model from the allocation start
+        // (guard_ elements before plane 0) so neighbour offsets never go negative
+        for (int q = 0; q < 9; ++q) a.f[q] = d_f_[q] - guard_;
         for (int q = 0; q < 18; ++q) a.m[q] = d_m_[q];
-        a.cp = d_model_[0];
-        a.cs = d_model_[1];
-        a.rho = d_model_[2];
+        a.cp = d_model_[0] - guard_;
+        a.cs = d_model_[1] - guard_;
+        a.rho = d_model_[2] - guard_;
         for (int q = 0; q < 3; ++q) a.tab[q] = d_tab_[q];
         a.nx = c.nx; a.ny = c.ny; a.nz = c.nz; a.nxy = c.nx * c.ny;
         a.nxyp = int(nxyp_);
+        a.g0 = int(guard_);
         for (int q = 0; q < 3; ++q) {
             a.lo[q] = plan_.ax[q].lo;
             a.hi[q] = plan_.ax[q].hi;

@@ -1,0 +1,118 @@
+"""CPU model of every device address the stencil kernels form, checked against
+the allocation sizes (regression test for the index arithmetic of kernels.hip).
+
+The kernels use unsigned 32-bit element offsets from the start of each field or
+model allocation: o = g0 + k*nxyp + p (g0 guard elements, nxyp the padded plane
+stride), neighbours at o +- 1, o +- nx, o + nxyp and up to VEC-1 beyond.  An
+offset formed below zero wraps to ~2^32 and addresses far outside the buffer,
+so this model reproduces the wrap exactly.  Memory variables use compact per-axis
+offsets and are touched only for elements inside the workgroup's box.
+"""
+import numpy as np
+import pytest
+
+from ela3d_amd import workloads
+from helpers import small_cfg
+
+M32 = 1 << 32
+
+
+def plane_stride(nx, ny):
+    return (nx * ny + 63) // 64 * 64
+
+
+def guard(nx):
+    return (nx + 64 + 63) // 64 * 64
+
+
+def violations(cfg, block, vec, zi, zs, base_at_guard=False):
+    p = cfg.plan()
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    nxy, nxyp, G = nx * ny, plane_stride(nx, ny), guard(nx)
+    alloc = nz * nxyp + 2 * G
+    lo, hi = p["region_lo"], p["region_hi"]
+    w = [lo[a] + ((nx, ny, nz)[a] - hi[a]) for a in range(3)]
+    memsize = [nz * ny * w[0], nz * w[1] * nx, w[2] * ny * nx]
+    pa, pb = (lo[1] * nx + 63) // 64 * 64, (hi[1] * nx) // 64 * 64
+    if pb <= pa:
+        pa = pb = nxy
+    boxes = []
+    for (p0, p1, k0, k1, z, mode) in ((0, nxy, 0, lo[2], zs, 1), (0, nxy, hi[2], nz, zs, 1),
+                                      (0, pa, lo[2], hi[2], zi, 1), (pb, nxy, lo[2], hi[2], zi, 1),
+                                      (pa, pb, lo[2], hi[2], zi, 0)):
+        if p1 > p0 and k1 > k0:
+            boxes.append((p0, p1, k0, k1, z, mode))
+    bad = []
+    # base_at_guard models pointers placed at plane 0 (the addressing that wrapped)
+    cell = (lambda k, q: (k * nxyp + q) % M32) if base_at_guard else (lambda k, q: (G + k * nxyp + q) % M32)
+    shift = G if base_at_guard else 0
+
+    def fld(o, tag):
+        for e in range(vec):
+            if not (shift + (o + e) % M32 < alloc):
+                bad.append(tag)
+
+    per = block * vec
+    for (p0b, p1b, k0, k1, z, mode) in boxes:
+        for chunk in range((p1b - p0b + per - 1) // per):
+            for zsg in range((k1 - k0 + z - 1) // z):
+                kb = k0 + zsg * z
+                ke = min(kb + z, k1)
+                for t in range(block):
+                    p0 = p0b + (chunk * block + t) * vec
+                    if p0 >= p1b:
+                        continue
+                    assert p0 % vec == 0
+                    els = []
+                    for e in range(vec):
+                        j, i = divmod(p0 + e, nx)
+                        act = p0 + e < p1b
+                        cx = -1 if not act else (i if i < lo[0] else (lo[0] + i - hi[0] if i >= hi[0] else -1))
+                        cy = -1
+                        if mode == 1 and act:
+                            cy = j if j < lo[1] else (lo[1] + j - hi[1] if j >= hi[1] else -1)
+                        els.append((i, j, act, cx, cy))
+                    fld(cell(kb, p0), "queue")
+                    if kb >= 1:
+                        fld((cell(kb, p0) - nxyp) % M32, "queue-1")
+                    for k in range(kb, ke):
+                        o = cell(k, p0)
+                        dU = 0 if k == nz - 1 else nxyp
+                        for d in (0, dU, 1, -1, nx, -nx):
+                            fld((o + d) % M32, ("neighbour", k, p0, d))
+                        cz = -1
+                        if mode == 1:
+                            cz = k if k < lo[2] else (lo[2] + k - hi[2] if k >= hi[2] else -1)
+                        for (i, j, act, cx, cy) in els:
+                            if cx >= 0 and ((k * ny + j) * w[0] + cx) % M32 >= memsize[0]:
+                                bad.append(("mx", k, i, j))
+                            if cy >= 0 and ((k * w[1] + cy) * nx + i) % M32 >= memsize[1]:
+                                bad.append(("my", k, i, j))
+                            if cz >= 0 and act and ((cz * ny + j) * nx + i) % M32 >= memsize[2]:
+                                bad.append(("mz", k, i, j))
+    return bad
+
+
+CASES = [(29, 23, 26, 4), (17, 19, 18, 3), (40, 13, 15, 4), (11, 12, 13, 0), (64, 64, 9, 3)]
+
+
+@pytest.mark.parametrize("shape", CASES)
+@pytest.mark.parametrize("vec", [1, 2, 4])
+def test_every_access_in_bounds(shape, vec):
+    nx, ny, nz, npml = shape
+    for compat in (0, 7):
+        cfg = small_cfg(nx=nx, ny=ny, nz=nz, npml=npml, compat=compat)
+        for block, zi, zs in ((64, 2, 2), (256, 4, 4), (128, 32, 8)):
+            assert violations(cfg, block, vec, zi, zs) == []
+
+
+def test_cube_shape_in_bounds():
+    cfg, _ = workloads.homogeneous_cube(n=45, nstep=1)
+    for vec in (1, 2, 4):
+        assert violations(cfg, 256, vec, 4, 4) == []
+
+
+def test_model_catches_wrapped_offsets():
+    """Indexing from plane 0 with unsigned offsets wraps at the first row (o - 1, o - nx)."""
+    bad = violations(small_cfg(), 256, 2, 4, 4, base_at_guard=True)
+    assert ("neighbour", 0, 0, -1) in bad and ("neighbour", 0, 0, -29) in bad
