@@ -137,11 +137,13 @@ using TabP = typename std::conditional<std::is_same<T, double>::value, double, f
 
 // FP32 copies of the x and y tables, staged per workgroup: [6][nx] then [6][ny]
 // (a, b, K on the full grid, a, b, K on the half grid).
+// Interior workgroups read only the x coefficients, so only absorbing-layer
+// workgroups stage the y tables.
 template <typename T>
-__device__ __forceinline__ void stage_tables(const StepArgs<T>& A, float* s)
+__device__ __forceinline__ void stage_tables(const StepArgs<T>& A, float* s, bool with_y)
 {
-    const int nx6 = 6 * A.nx, ny6 = 6 * A.ny;
-    for (int q = threadIdx.x; q < nx6 + ny6; q += blockDim.x)
+    const int nx6 = 6 * A.nx, n = nx6 + (with_y ? 6 * A.ny : 0);
+    for (int q = threadIdx.x; q < n; q += blockDim.x)
         s[q] = q < nx6 ? (float)A.tab[0][q] : (float)A.tab[1][q - nx6];
     __syncthreads();
 }
@@ -690,17 +692,17 @@ template <typename T, int V>
 __global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
 {
     extern __shared__ float tabs[];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const Box B = pick_box(A.box, A.nbox, lb);
     const TabP<T>*tx, *ty;
     if constexpr (std::is_same<T, double>::value) {
         tx = A.tab[0];
         ty = A.tab[1];
     } else {
-        stage_tables(A, tabs);
+        stage_tables(A, tabs, B.mode == kSlab);
         tx = tabs;
         ty = tabs + 6 * A.nx;
     }
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const Box B = pick_box(A.box, A.nbox, lb);
     if (B.mode == kSlab)
         stress_body<T, kSlab, V>(A, B, lb - B.blk0, tx, ty);
     else
@@ -711,17 +713,17 @@ template <typename T, bool ENERGY, int V>
 __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
 {
     extern __shared__ float tabs[];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const Box B = pick_box(A.box, A.nbox, lb);
     const TabP<T>*tx, *ty;
     if constexpr (std::is_same<T, double>::value) {
         tx = A.tab[0];
         ty = A.tab[1];
     } else {
-        stage_tables(A, tabs);
+        stage_tables(A, tabs, B.mode == kSlab);
         tx = tabs;
         ty = tabs + 6 * A.nx;
     }
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const Box B = pick_box(A.box, A.nbox, lb);
     double ek = 0.0, ep = 0.0;
     if (B.mode == kSlab)
         velocity_body<T, kSlab, ENERGY, V>(A, B, lb - B.blk0, tx, ty, ek, ep);
