@@ -93,14 +93,71 @@ __device__ __forceinline__ void ldm(const float* p, unsigned o, R (&out)[V])
     }
 }
 
+// Per-pass view of the storage scales (FP16 storage only; 1 otherwise):
+// r[q] multiplies a stored value of field q on load, w[q] multiplies on store;
+// mx[q] is the largest |value| this thread wrote to field q.
+template <typename T>
+struct Scales {
+    static constexpr bool on = std::is_same<T, half_t>::value;
+    float r[27], w[27], mx[27];
+    __device__ __forceinline__ void track(int q, float v)
+    {
+        if constexpr (on) mx[q] = fmaxf(mx[q], fabsf(v));
+    }
+};
+
+// Fill S for one pass from the device scale table (see kernels.hpp): fields
+// rewritten earlier in this step (`fresh`) are read at their next scale.
+template <typename T>
+__device__ __forceinline__ void load_scales(const StepArgs<T>& A, Scales<T>& S, bool stress_pass)
+{
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        S.mx[q] = 0.f;
+        if constexpr (Scales<T>::on) {
+            const bool fresh = !stress_pass && q >= 3 && q < 9;  // sigma, read by the velocity pass
+            S.r[q] = fresh ? A.scl[3 * 27 + q] : A.scl[27 + q];
+            S.w[q] = A.scl[2 * 27 + q];
+        } else {
+            S.r[q] = 1.f;
+            S.w[q] = 1.f;
+        }
+    }
+}
+
+// Fold this workgroup's maxima into the spread slots (FP16 storage only).
+template <typename T>
+__device__ __forceinline__ void flush_maxima(const StepArgs<T>& A, Scales<T>& S, int q0, int q1)
+{
+    if constexpr (Scales<T>::on) {
+        unsigned* slot = A.maxslot + (blockIdx.x & (kMaxSlots - 1)) * 27;
+        for (int q = q0; q < q1; ++q) {
+            const float m = wave_max(S.mx[q]);
+            if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(slot + q, __float_as_uint(m));
+        }
+    }
+}
+
+// store of one memory-variable element, tracked
+template <typename T>
+__device__ __forceinline__ void stm(T* p, unsigned o, typename Arith<T>::R v, Scales<T>& S, int q)
+{
+    stf(p, o, v, S.w[q]);
+    S.track(q, (float)v);
+}
+
 // store the enabled elements; one vector store when all are enabled
 template <typename T, int V>
 __device__ __forceinline__ void stv(T* p, unsigned o, const typename Arith<T>::R (&val)[V], const bool (&en)[V],
-                                    float sc)
+                                    Scales<T>& S, int q)
 {
+    const float sc = S.w[q];
     bool all = true;
 #pragma unroll
-    for (int e = 0; e < V; ++e) all = all && en[e];
+    for (int e = 0; e < V; ++e) {
+        all = all && en[e];
+        if (en[e]) S.track(q, (float)val[e]);
+    }
     if constexpr (V > 1) {
         if (all) {
             typedef T vt __attribute__((ext_vector_type(V)));
@@ -222,56 +279,57 @@ struct SQueue {
 };
 
 template <typename T, int MODE, int V, typename R>
-__device__ __forceinline__ void stress_load(const StepArgs<T>& A, const Lane<V>& L, int k, SPlane<R, V>& P)
+__device__ __forceinline__ void stress_load(const StepArgs<T>& A, const Lane<V>& L, int k, SPlane<R, V>& P,
+                                            Scales<T>& S)
 {
     const unsigned nxyp = A.nxyp, nx = A.nx;
     const unsigned o = cell(A, k, L.p0);
     const bool top = (k == A.nz - 1);
     const unsigned dU = top ? 0u : nxyp;
     const T *vx = A.f[0], *vy = A.f[1], *vz = A.f[2];
-    ldv<T, V>(vz, o, A.isc[2], P.vz_c);
+    ldv<T, V>(vz, o, S.r[2], P.vz_c);
     ldm<V>(A.cp, o, P.cp_c);
-    ldv<T, V>(vx, o + dU, A.isc[0], P.vx_u);
-    ldv<T, V>(vy, o + dU, A.isc[1], P.vy_u);
+    ldv<T, V>(vx, o + dU, S.r[0], P.vx_u);
+    ldv<T, V>(vy, o + dU, S.r[1], P.vy_u);
     ldm<V>(A.cs, o + dU, P.cs_u);
     ldm<V>(A.rho, o + dU, P.rho_u);
-    ldv<T, V>(vx, o + 1, A.isc[0], P.vxE);
-    ldv<T, V>(vy, o - nx, A.isc[1], P.vyS);
-    ldv<T, V>(vy, o - 1, A.isc[1], P.vyW);
-    ldv<T, V>(vx, o + nx, A.isc[0], P.vxN);
-    ldv<T, V>(vz, o - 1, A.isc[2], P.vzW);
-    ldv<T, V>(vz, o + nx, A.isc[2], P.vzN);
+    ldv<T, V>(vx, o + 1, S.r[0], P.vxE);
+    ldv<T, V>(vy, o - nx, S.r[1], P.vyS);
+    ldv<T, V>(vy, o - 1, S.r[1], P.vyW);
+    ldv<T, V>(vx, o + nx, S.r[0], P.vxN);
+    ldv<T, V>(vz, o - 1, S.r[2], P.vzW);
+    ldv<T, V>(vz, o + nx, S.r[2], P.vzN);
     ldm<V>(A.cs, o - 1, P.csW);
     ldm<V>(A.cs, o + nx, P.csN);
     ldm<V>(A.rho, o - 1, P.rhoW);
     ldm<V>(A.rho, o + nx, P.rhoN);
-    ldv<T, V>(A.f[3], o, A.isc[3], P.sxx);
-    ldv<T, V>(A.f[4], o, A.isc[4], P.syy);
-    ldv<T, V>(A.f[5], o, A.isc[5], P.szz);
-    ldv<T, V>(A.f[6], o, A.isc[6], P.sxy);
-    ldv<T, V>(A.f[7], o, A.isc[7], P.sxz);
-    ldv<T, V>(A.f[8], o, A.isc[8], P.syz);
+    ldv<T, V>(A.f[3], o, S.r[3], P.sxx);
+    ldv<T, V>(A.f[4], o, S.r[4], P.syy);
+    ldv<T, V>(A.f[5], o, S.r[5], P.szz);
+    ldv<T, V>(A.f[6], o, S.r[6], P.sxy);
+    ldv<T, V>(A.f[7], o, S.r[7], P.sxz);
+    ldv<T, V>(A.f[8], o, S.r[8], P.syz);
     const int cz = MODE == kSlab ? (k < A.lo[2] ? k : (k >= A.hi[2] ? A.lo[2] + (k - A.hi[2]) : -1)) : -1;
 #pragma unroll
     for (int e = 0; e < V; ++e) {
         if (L.cx[e] >= 0) {
             const unsigned om = unsigned((k * A.ny + L.j[e]) * A.w[0] + L.cx[e]);
-            P.mx[0][e] = ldf(A.m[0], om, A.isc[9]);
-            P.mx[1][e] = ldf(A.m[3], om, A.isc[12]);
-            P.mx[2][e] = ldf(A.m[5], om, A.isc[14]);
+            P.mx[0][e] = ldf(A.m[0], om, S.r[9]);
+            P.mx[1][e] = ldf(A.m[3], om, S.r[12]);
+            P.mx[2][e] = ldf(A.m[5], om, S.r[14]);
         }
         if (MODE == kSlab) {
             if (L.cy[e] >= 0) {
                 const unsigned om = unsigned((k * A.w[1] + L.cy[e]) * A.nx + L.i[e]);
-                P.my[0][e] = ldf(A.m[1], om, A.isc[10]);
-                P.my[1][e] = ldf(A.m[4], om, A.isc[13]);
-                P.my[2][e] = ldf(A.m[7], om, A.isc[16]);
+                P.my[0][e] = ldf(A.m[1], om, S.r[10]);
+                P.my[1][e] = ldf(A.m[4], om, S.r[13]);
+                P.my[2][e] = ldf(A.m[7], om, S.r[16]);
             }
             if (cz >= 0 && L.act[e]) {  // elements past the box end may sit past the last row
                 const unsigned om = unsigned((cz * A.ny + L.j[e]) * A.nx + L.i[e]);
-                P.mz[0][e] = ldf(A.m[2], om, A.isc[11]);
-                P.mz[1][e] = ldf(A.m[6], om, A.isc[15]);
-                P.mz[2][e] = ldf(A.m[8], om, A.isc[17]);
+                P.mz[0][e] = ldf(A.m[2], om, S.r[11]);
+                P.mz[1][e] = ldf(A.m[6], om, S.r[15]);
+                P.mz[2][e] = ldf(A.m[8], om, S.r[17]);
             }
         }
     }
@@ -289,7 +347,7 @@ __device__ __forceinline__ void stress_load(const StepArgs<T>& A, const Lane<V>&
 template <typename T, int MODE, int V, typename R>
 __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V>& L, const TabP<T>* tx,
                                               const TabP<T>* ty, int k, const SPlane<R, V>& P,
-                                              const SQueue<R, V>& Q)
+                                              const SQueue<R, V>& Q, Scales<T>& S)
 {
     const unsigned o = cell(A, k, L.p0);
     const R dt = (R)A.dt, idx = (R)A.idx, idy = (R)A.idy, idz = (R)A.idz;
@@ -317,16 +375,16 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvz_dz = (P.vz_c[e] - Q.vz_m[e]) * idz;
             if (cx >= 0) {
                 const R m = cpml(P.mx[0][e], (R)tx[3 * nx + i], (R)tx[4 * nx + i], (R)tx[5 * nx + i], dvx_dx);
-                if (onN[e]) stf(A.m[0], omx, m, A.sc[9]);
+                if (onN[e]) stm(A.m[0], omx, m, S, 9);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[0][e], (R)ty[j], (R)ty[ny + j], (R)ty[2 * ny + j], dvy_dy);
-                    if (onN[e]) stf(A.m[1], omy, m, A.sc[10]);
+                    if (onN[e]) stm(A.m[1], omy, m, S, 10);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[0][e], Z.az, Z.bz, Z.Kz, dvz_dz);
-                    if (onN[e]) stf(A.m[2], omz, m, A.sc[11]);
+                    if (onN[e]) stm(A.m[2], omz, m, S, 11);
                 }
             }
             nsxx[e] = DT_l2m * dvx_dx + DT_l * (dvy_dy + dvz_dz) + P.sxx[e];
@@ -343,11 +401,11 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvx_dy = (P.vxN[e] - vx_c) * idy;
             if (cx >= 0) {
                 const R m = cpml(P.mx[1][e], (R)tx[i], (R)tx[nx + i], (R)tx[2 * nx + i], dvy_dx);
-                if (onXY[e]) stf(A.m[3], omx, m, A.sc[12]);
+                if (onXY[e]) stm(A.m[3], omx, m, S, 12);
             }
             if (MODE == kSlab && cy >= 0) {
                 const R m = cpml(P.my[1][e], (R)ty[3 * ny + j], (R)ty[4 * ny + j], (R)ty[5 * ny + j], dvx_dy);
-                if (onXY[e]) stf(A.m[4], omy, m, A.sc[13]);
+                if (onXY[e]) stm(A.m[4], omy, m, S, 13);
             }
             nsxy[e] = DT_mu * (dvy_dx + dvx_dy) + P.sxy[e];
         }
@@ -361,11 +419,11 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvx_dz = (P.vx_u[e] - vx_c) * idz;
             if (cx >= 0) {
                 const R m = cpml(P.mx[2][e], (R)tx[i], (R)tx[nx + i], (R)tx[2 * nx + i], dvz_dx);
-                if (onXZ[e]) stf(A.m[5], omx, m, A.sc[14]);
+                if (onXZ[e]) stm(A.m[5], omx, m, S, 14);
             }
             if (MODE == kSlab && Z.cz >= 0) {
                 const R m = cpml(P.mz[1][e], Z.azh, Z.bzh, Z.Kzh, dvx_dz);
-                if (onXZ[e]) stf(A.m[6], omz, m, A.sc[15]);
+                if (onXZ[e]) stm(A.m[6], omz, m, S, 15);
             }
             nsxz[e] = DT_mu * (dvz_dx + dvx_dz) + P.sxz[e];
         }
@@ -380,27 +438,27 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[2][e], (R)ty[3 * ny + j], (R)ty[4 * ny + j], (R)ty[5 * ny + j], dvz_dy);
-                    if (onYZ[e]) stf(A.m[7], omy, m, A.sc[16]);
+                    if (onYZ[e]) stm(A.m[7], omy, m, S, 16);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[2][e], Z.azh, Z.bzh, Z.Kzh, dvy_dz);
-                    if (onYZ[e]) stf(A.m[8], omz, m, A.sc[17]);
+                    if (onYZ[e]) stm(A.m[8], omz, m, S, 17);
                 }
             }
             nsyz[e] = DT_mu * (dvz_dy + dvy_dz) + P.syz[e];
         }
     }
-    stv<T, V>(A.f[3], o, nsxx, onN, A.sc[3]);
-    stv<T, V>(A.f[4], o, nsyy, onN, A.sc[4]);
-    stv<T, V>(A.f[5], o, nszz, onN, A.sc[5]);
-    stv<T, V>(A.f[6], o, nsxy, onXY, A.sc[6]);
-    stv<T, V>(A.f[7], o, nsxz, onXZ, A.sc[7]);
-    stv<T, V>(A.f[8], o, nsyz, onYZ, A.sc[8]);
+    stv<T, V>(A.f[3], o, nsxx, onN, S, 3);
+    stv<T, V>(A.f[4], o, nsyy, onN, S, 4);
+    stv<T, V>(A.f[5], o, nszz, onN, S, 5);
+    stv<T, V>(A.f[6], o, nsxy, onXY, S, 6);
+    stv<T, V>(A.f[7], o, nsxz, onXZ, S, 7);
+    stv<T, V>(A.f[8], o, nsyz, onYZ, S, 8);
 }
 
 template <typename T, int MODE, int V>
 __device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, int local, const TabP<T>* tx,
-                                            const TabP<T>* ty)
+                                            const TabP<T>* ty, Scales<T>& S)
 {
     typedef typename Arith<T>::R R;
     const int chunk = local % B.nchunk, zs = local / B.nchunk;
@@ -413,19 +471,19 @@ __device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, 
     SQueue<R, V> Q;
     {
         const unsigned o = cell(A, kb, p0);
-        ldv<T, V>(A.f[0], o, A.isc[0], Q.vx_c);
-        ldv<T, V>(A.f[1], o, A.isc[1], Q.vy_c);
+        ldv<T, V>(A.f[0], o, S.r[0], Q.vx_c);
+        ldv<T, V>(A.f[1], o, S.r[1], Q.vy_c);
         ldm<V>(A.cs, o, Q.cs_c);
         ldm<V>(A.rho, o, Q.rho_c);
         if (kb >= 1)
-            ldv<T, V>(A.f[2], o - unsigned(A.nxyp), A.isc[2], Q.vz_m);
+            ldv<T, V>(A.f[2], o - unsigned(A.nxyp), S.r[2], Q.vz_m);
         else
             for (int e = 0; e < V; ++e) Q.vz_m[e] = 0;
     }
     for (int k = kb; k < ke; ++k) {
         SPlane<R, V> P;
-        stress_load<T, MODE, V, R>(A, L, k, P);
-        stress_update<T, MODE, V, R>(A, L, tx, ty, k, P, Q);
+        stress_load<T, MODE, V, R>(A, L, k, P, S);
+        stress_update<T, MODE, V, R>(A, L, tx, ty, k, P, Q, S);
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             Q.vx_c[e] = P.vx_u[e];
@@ -456,31 +514,32 @@ struct VQueue {
 };
 
 template <typename T, int MODE, bool ENERGY, int V, typename R>
-__device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V>& L, int k, VPlane<R, V>& P)
+__device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V>& L, int k, VPlane<R, V>& P,
+                                              Scales<T>& S)
 {
     const unsigned nxyp = A.nxyp, nx = A.nx;
     const unsigned o = cell(A, k, L.p0);
     const bool top = (k == A.nz - 1);
     const unsigned dU = top ? 0u : nxyp;
     const T *sxx = A.f[3], *syy = A.f[4], *szz = A.f[5], *sxy = A.f[6], *sxz = A.f[7], *syz = A.f[8];
-    ldv<T, V>(sxx, o, A.isc[3], P.sxx);
-    ldv<T, V>(syy, o, A.isc[4], P.syy);
-    ldv<T, V>(sxy, o, A.isc[6], P.sxy);
-    ldv<T, V>(sxz, o, A.isc[7], P.sxz);
-    ldv<T, V>(syz, o, A.isc[8], P.syz);
-    ldv<T, V>(szz, o + dU, A.isc[5], P.szz_u);
+    ldv<T, V>(sxx, o, S.r[3], P.sxx);
+    ldv<T, V>(syy, o, S.r[4], P.syy);
+    ldv<T, V>(sxy, o, S.r[6], P.sxy);
+    ldv<T, V>(sxz, o, S.r[7], P.sxz);
+    ldv<T, V>(syz, o, S.r[8], P.syz);
+    ldv<T, V>(szz, o + dU, S.r[5], P.szz_u);
     ldm<V>(A.rho, o + dU, P.rho_u);
-    ldv<T, V>(sxx, o - 1, A.isc[3], P.sxxW);
-    ldv<T, V>(sxy, o - nx, A.isc[6], P.sxyS);
+    ldv<T, V>(sxx, o - 1, S.r[3], P.sxxW);
+    ldv<T, V>(sxy, o - nx, S.r[6], P.sxyS);
     ldm<V>(A.rho, o - 1, P.rhoW);
-    ldv<T, V>(sxy, o + 1, A.isc[6], P.sxyE);
-    ldv<T, V>(syy, o + nx, A.isc[4], P.syyN);
+    ldv<T, V>(sxy, o + 1, S.r[6], P.sxyE);
+    ldv<T, V>(syy, o + nx, S.r[4], P.syyN);
     ldm<V>(A.rho, o + nx, P.rhoN);
-    ldv<T, V>(sxz, o + 1, A.isc[7], P.sxzE);
-    ldv<T, V>(syz, o - nx, A.isc[8], P.syzS);
-    ldv<T, V>(A.f[0], o, A.isc[0], P.vx);
-    ldv<T, V>(A.f[1], o, A.isc[1], P.vy);
-    ldv<T, V>(A.f[2], o, A.isc[2], P.vz);
+    ldv<T, V>(sxz, o + 1, S.r[7], P.sxzE);
+    ldv<T, V>(syz, o - nx, S.r[8], P.syzS);
+    ldv<T, V>(A.f[0], o, S.r[0], P.vx);
+    ldv<T, V>(A.f[1], o, S.r[1], P.vy);
+    ldv<T, V>(A.f[2], o, S.r[2], P.vz);
     if (ENERGY) {
         ldm<V>(A.cp, o, P.cp);
         ldm<V>(A.cs, o, P.cs);
@@ -490,22 +549,22 @@ __device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V
     for (int e = 0; e < V; ++e) {
         if (L.cx[e] >= 0) {
             const unsigned om = unsigned((k * A.ny + L.j[e]) * A.w[0] + L.cx[e]);
-            P.mx[0][e] = ldf(A.m[9], om, A.isc[18]);
-            P.mx[1][e] = ldf(A.m[12], om, A.isc[21]);
-            P.mx[2][e] = ldf(A.m[15], om, A.isc[24]);
+            P.mx[0][e] = ldf(A.m[9], om, S.r[18]);
+            P.mx[1][e] = ldf(A.m[12], om, S.r[21]);
+            P.mx[2][e] = ldf(A.m[15], om, S.r[24]);
         }
         if (MODE == kSlab) {
             if (L.cy[e] >= 0) {
                 const unsigned om = unsigned((k * A.w[1] + L.cy[e]) * A.nx + L.i[e]);
-                P.my[0][e] = ldf(A.m[10], om, A.isc[19]);
-                P.my[1][e] = ldf(A.m[13], om, A.isc[22]);
-                P.my[2][e] = ldf(A.m[16], om, A.isc[25]);
+                P.my[0][e] = ldf(A.m[10], om, S.r[19]);
+                P.my[1][e] = ldf(A.m[13], om, S.r[22]);
+                P.my[2][e] = ldf(A.m[16], om, S.r[25]);
             }
             if (cz >= 0 && L.act[e]) {  // elements past the box end may sit past the last row
                 const unsigned om = unsigned((cz * A.ny + L.j[e]) * A.nx + L.i[e]);
-                P.mz[0][e] = ldf(A.m[11], om, A.isc[20]);
-                P.mz[1][e] = ldf(A.m[14], om, A.isc[23]);
-                P.mz[2][e] = ldf(A.m[17], om, A.isc[26]);
+                P.mz[0][e] = ldf(A.m[11], om, S.r[20]);
+                P.mz[1][e] = ldf(A.m[14], om, S.r[23]);
+                P.mz[2][e] = ldf(A.m[17], om, S.r[26]);
             }
         }
     }
@@ -521,7 +580,8 @@ __device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V
 template <typename T, int MODE, bool ENERGY, int V, typename R>
 __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane<V>& L, const TabP<T>* tx,
                                                 const TabP<T>* ty, int k, const VPlane<R, V>& P,
-                                                const VQueue<R, V>& Q, double& ek, double& ep)
+                                                const VQueue<R, V>& Q, double& ek, double& ep,
+                                                Scales<T>& S)
 {
     const unsigned o = cell(A, k, L.p0);
     const R dt = (R)A.dt, idx = (R)A.idx, idy = (R)A.idy, idz = (R)A.idz;
@@ -547,16 +607,16 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d3 = (P.sxz[e] - Q.sxz_m[e]) * idz;
             if (cx >= 0) {
                 const R m = cpml(P.mx[0][e], (R)tx[i], (R)tx[nx + i], (R)tx[2 * nx + i], d1);
-                if (onx) stf(A.m[9], omx, m, A.sc[18]);
+                if (onx) stm(A.m[9], omx, m, S, 18);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[0][e], (R)ty[j], (R)ty[ny + j], (R)ty[2 * ny + j], d2);
-                    if (onx) stf(A.m[10], omy, m, A.sc[19]);
+                    if (onx) stm(A.m[10], omy, m, S, 19);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[0][e], Z.az, Z.bz, Z.Kz, d3);
-                    if (onx) stf(A.m[11], omz, m, A.sc[20]);
+                    if (onx) stm(A.m[11], omz, m, S, 20);
                 }
             }
             if (onx) nvx[e] = DT_over_rho * (d1 + d2 + d3) + P.vx[e];
@@ -572,16 +632,16 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d3 = (P.syz[e] - Q.syz_m[e]) * idz;
             if (cx >= 0) {
                 const R m = cpml(P.mx[1][e], (R)tx[3 * nx + i], (R)tx[4 * nx + i], (R)tx[5 * nx + i], d1);
-                if (ony) stf(A.m[12], omx, m, A.sc[21]);
+                if (ony) stm(A.m[12], omx, m, S, 21);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[1][e], (R)ty[3 * ny + j], (R)ty[4 * ny + j], (R)ty[5 * ny + j], d2);
-                    if (ony) stf(A.m[13], omy, m, A.sc[22]);
+                    if (ony) stm(A.m[13], omy, m, S, 22);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[1][e], Z.az, Z.bz, Z.Kz, d3);
-                    if (ony) stf(A.m[14], omz, m, A.sc[23]);
+                    if (ony) stm(A.m[14], omz, m, S, 23);
                 }
             }
             if (ony) nvy[e] = DT_over_rho * (d1 + d2 + d3) + P.vy[e];
@@ -597,16 +657,16 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d3 = (P.szz_u[e] - Q.szz_c[e]) * idz;
             if (cx >= 0) {
                 const R m = cpml(P.mx[2][e], (R)tx[3 * nx + i], (R)tx[4 * nx + i], (R)tx[5 * nx + i], d1);
-                if (onz) stf(A.m[15], omx, m, A.sc[24]);
+                if (onz) stm(A.m[15], omx, m, S, 24);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[2][e], (R)ty[j], (R)ty[ny + j], (R)ty[2 * ny + j], d2);
-                    if (onz) stf(A.m[16], omy, m, A.sc[25]);
+                    if (onz) stm(A.m[16], omy, m, S, 25);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[2][e], Z.azh, Z.bzh, Z.Kzh, d3);
-                    if (onz) stf(A.m[17], omz, m, A.sc[26]);
+                    if (onz) stm(A.m[17], omz, m, S, 26);
                 }
             }
             if (onz) nvz[e] = DT_over_rho * (d1 + d2 + d3) + P.vz[e];
@@ -638,14 +698,14 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             }
         }
     }
-    stv<T, V>(A.f[0], o, nvx, wx, A.sc[0]);
-    stv<T, V>(A.f[1], o, nvy, wy, A.sc[1]);
-    stv<T, V>(A.f[2], o, nvz, wz, A.sc[2]);
+    stv<T, V>(A.f[0], o, nvx, wx, S, 0);
+    stv<T, V>(A.f[1], o, nvy, wy, S, 1);
+    stv<T, V>(A.f[2], o, nvz, wz, S, 2);
 }
 
 template <typename T, int MODE, bool ENERGY, int V>
 __device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B, int local, const TabP<T>* tx,
-                                              const TabP<T>* ty, double& ek, double& ep)
+                                              const TabP<T>* ty, double& ek, double& ep, Scales<T>& S)
 {
     typedef typename Arith<T>::R R;
     const int chunk = local % B.nchunk, zs = local / B.nchunk;
@@ -659,21 +719,21 @@ __device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B
     {
         const unsigned o = cell(A, kb, p0);
         if (kb >= 1) {
-            ldv<T, V>(A.f[7], o - unsigned(A.nxyp), A.isc[7], Q.sxz_m);
-            ldv<T, V>(A.f[8], o - unsigned(A.nxyp), A.isc[8], Q.syz_m);
+            ldv<T, V>(A.f[7], o - unsigned(A.nxyp), S.r[7], Q.sxz_m);
+            ldv<T, V>(A.f[8], o - unsigned(A.nxyp), S.r[8], Q.syz_m);
         } else {
             for (int e = 0; e < V; ++e) {
                 Q.sxz_m[e] = 0;
                 Q.syz_m[e] = 0;
             }
         }
-        ldv<T, V>(A.f[5], o, A.isc[5], Q.szz_c);
+        ldv<T, V>(A.f[5], o, S.r[5], Q.szz_c);
         ldm<V>(A.rho, o, Q.rho_c);
     }
     for (int k = kb; k < ke; ++k) {
         VPlane<R, V> P;
-        velocity_load<T, MODE, ENERGY, V, R>(A, L, k, P);
-        velocity_update<T, MODE, ENERGY, V, R>(A, L, tx, ty, k, P, Q, ek, ep);
+        velocity_load<T, MODE, ENERGY, V, R>(A, L, k, P, S);
+        velocity_update<T, MODE, ENERGY, V, R>(A, L, tx, ty, k, P, Q, ek, ep, S);
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             Q.sxz_m[e] = P.sxz[e];
@@ -694,6 +754,8 @@ __global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
     extern __shared__ float tabs[];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const Box B = pick_box(A.box, A.nbox, lb);
+    Scales<T> S;
+    load_scales(A, S, true);
     const TabP<T>*tx, *ty;
     if constexpr (std::is_same<T, double>::value) {
         tx = A.tab[0];
@@ -704,9 +766,10 @@ __global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
         ty = tabs + 6 * A.nx;
     }
     if (B.mode == kSlab)
-        stress_body<T, kSlab, V>(A, B, lb - B.blk0, tx, ty);
+        stress_body<T, kSlab, V>(A, B, lb - B.blk0, tx, ty, S);
     else
-        stress_body<T, kInterior, V>(A, B, lb - B.blk0, tx, ty);
+        stress_body<T, kInterior, V>(A, B, lb - B.blk0, tx, ty, S);
+    flush_maxima(A, S, 3, 18);
 }
 
 template <typename T, bool ENERGY, int V>
@@ -715,6 +778,8 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
     extern __shared__ float tabs[];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const Box B = pick_box(A.box, A.nbox, lb);
+    Scales<T> S;
+    load_scales(A, S, false);
     const TabP<T>*tx, *ty;
     if constexpr (std::is_same<T, double>::value) {
         tx = A.tab[0];
@@ -726,9 +791,11 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
     }
     double ek = 0.0, ep = 0.0;
     if (B.mode == kSlab)
-        velocity_body<T, kSlab, ENERGY, V>(A, B, lb - B.blk0, tx, ty, ek, ep);
+        velocity_body<T, kSlab, ENERGY, V>(A, B, lb - B.blk0, tx, ty, ek, ep, S);
     else
-        velocity_body<T, kInterior, ENERGY, V>(A, B, lb - B.blk0, tx, ty, ek, ep);
+        velocity_body<T, kInterior, ENERGY, V>(A, B, lb - B.blk0, tx, ty, ek, ep, S);
+    flush_maxima(A, S, 0, 3);
+    flush_maxima(A, S, 18, 27);
     if (ENERGY) {  // per-block FP64 partials, summed in block order by the tail kernel
         __shared__ double red[2][4];
         ek = wave_sum(ek);

@@ -12,6 +12,15 @@ typedef _Float16 half_t;
 constexpr int kInterior = 0;  // y and z outside the C-PML: only x is checked per lane
 constexpr int kSlab = 1;      // y or z inside the C-PML: every axis checked
 
+// FP16 storage keeps one power-of-two scale per field (stored = value * scale)
+// in a device table of kScaleTable floats, so replayed graphs see live values:
+//   [0,27) current scale of the data in memory, [27,54) its inverse,
+//   [54,81) scale of the data being written this step, [81,108) its inverse.
+// The tail kernel makes next current at the end of a step and picks new next
+// scales from the maxima the stencil kernels fold into kMaxSlots spread slots.
+constexpr int kScaleTable = 4 * 27;
+constexpr int kMaxSlots = 64;
+
 // A launch covers up to kMaxBox boxes.  A box is a flat xy range [p0,p1)
 // (p = i + j*nx) times a z range [k0,k1); its blocks march z in segments of
 // zseg planes, one thread per xy position.
@@ -40,7 +49,8 @@ struct StepArgs {
     int dirichlet;
     int emargin;             // energy box [emargin, n-1-emargin) per axis
     double dt, idx, idy, idz;
-    float sc[27], isc[27];   // FP16 storage scales (stored = value*sc)
+    const float* scl;        // FP16 storage scales, kScaleTable floats (layout below)
+    unsigned* maxslot;       // [kMaxSlots][27] |value| maxima as float bits (FP16 storage)
     double* epart;           // energy partials [nblocks][2]
     Box box[kMaxBox];
     int nbox, nblocks, block;
@@ -67,7 +77,9 @@ struct TailArgs {
     double sin_t, cos_t, dt;
     int* step;               // steps completed (device counter)
     int* flag;               // set to 1 when a sample or the energy is not finite
-    float sc[9], isc[9];
+    float* scl;              // FP16 storage scale table (see kScaleTable)
+    unsigned* maxslot;       // [kMaxSlots][27] maxima folded by this step's stencil kernels
+    int rescale_every;       // FP16 storage: re-choose every scale each N steps
 };
 
 template <typename T>

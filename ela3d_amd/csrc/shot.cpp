@@ -105,10 +105,6 @@ public:
         geometry(plan_, tune_, geo_);
         allocate();
         upload(cp, cs, rho);
-        for (int q = 0; q < 27; ++q) {
-            sc_[q] = 1.0f;
-            isc_[q] = 1.0f;
-        }
         args_ = step_args(geo_, d_epart_);
     }
 
@@ -178,7 +174,7 @@ public:
         if (field < 9) {
             std::vector<T> h(cells);
             copy_planes(h.data(), d_f_[field], sizeof(T), hipMemcpyDeviceToHost);
-            const double inv = 1.0 / sc_[field];
+            const double inv = 1.0 / current_scale(field);
             for (size_t q = 0; q < cells; ++q) out[q] = double(h[q]) * inv;
             return;
         }
@@ -187,7 +183,7 @@ public:
         std::vector<T> h(nm);
         if (nm) HIP_CHECK(hipMemcpy(h.data(), d_m_[m], nm * sizeof(T), hipMemcpyDeviceToHost));
         std::fill(out, out + cells, 0.0);
-        const double inv = 1.0 / sc_[field];
+        const double inv = 1.0 / current_scale(field);
         for_each_compact(axis, [&](size_t full, size_t comp) { out[full] = double(h[comp]) * inv; });
         (void)c;
     }
@@ -199,7 +195,8 @@ public:
         const size_t cells = size_t(plan_.cells);
         if (field < 9) {
             std::vector<T> h(cells);
-            for (size_t q = 0; q < cells; ++q) h[q] = T(in[q] * sc_[field]);
+            const double sc = current_scale(field);
+            for (size_t q = 0; q < cells; ++q) h[q] = T(in[q] * sc);
             copy_planes(d_f_[field], h.data(), sizeof(T), hipMemcpyHostToDevice);
             return;
         }
@@ -207,8 +204,9 @@ public:
         const size_t nm = mem_count(axis);
         std::vector<T> h(nm);
         std::vector<char> covered(cells, 0);
+        const double sc = current_scale(field);
         for_each_compact(axis, [&](size_t full, size_t comp) {
-            h[comp] = T(in[full] * sc_[field]);
+            h[comp] = T(in[full] * sc);
             covered[full] = 1;
         });
         for (size_t q = 0; q < cells; ++q)
@@ -377,6 +375,8 @@ private:
         epart_cap_ = int((nxyp_ / 64 + kMaxBox + 1) * int64_t(c.nz) + 8);
         const size_t o_ep = off; off += al(size_t(std::max(epart_cap_, nepart_)) * 2 * sizeof(double));
         const size_t o_ctl = off; off += al(2 * sizeof(int));
+        const size_t o_scl = off; off += al(kScaleTable * sizeof(float));
+        const size_t o_slot = off; off += al(size_t(kMaxSlots) * 27 * sizeof(unsigned));
         arena_bytes_ = int64_t(off);
         HIP_CHECK(hipMalloc(&arena_, off));
         char* b = static_cast<char*>(arena_);
@@ -391,10 +391,15 @@ private:
         d_epart_ = reinterpret_cast<double*>(b + o_ep);
         d_step_ = reinterpret_cast<int*>(b + o_ctl);
         d_flag_ = d_step_ + 1;
+        d_scl_ = reinterpret_cast<float*>(b + o_scl);
+        d_maxslot_ = reinterpret_cast<unsigned*>(b + o_slot);
         // zero everything: wavefields and memory variables (CPU:107-140), the guards
         // and plane padding (read as neighbours, never written), outputs
         HIP_CHECK(hipMemsetAsync(arena_, 0, off, stream_));
         HIP_CHECK(hipStreamSynchronize(stream_));
+        // every storage scale starts at 1 (the FP16 tail re-chooses them from the data)
+        const std::vector<float> ones(kScaleTable, 1.0f);
+        HIP_CHECK(hipMemcpy(d_scl_, ones.data(), ones.size() * sizeof(float), hipMemcpyHostToDevice));
         (void)o_seis;
     }
 
@@ -460,10 +465,8 @@ private:
         a.idx = 1.0 / c.dx;
         a.idy = 1.0 / c.dy;
         a.idz = 1.0 / c.dz;
-        for (int q = 0; q < 27; ++q) {
-            a.sc[q] = sc_[q];
-            a.isc[q] = isc_[q];
-        }
+        a.scl = d_scl_;
+        a.maxslot = d_maxslot_;
         a.epart = epart;
         for (int q = 0; q < g.nbox; ++q) a.box[q] = g.box[q];
         a.nbox = g.nbox;
@@ -504,10 +507,9 @@ private:
         a.dt = c.dt;
         a.step = d_step_;
         a.flag = d_flag_;
-        for (int q = 0; q < 9; ++q) {
-            a.sc[q] = sc_[q];
-            a.isc[q] = isc_[q];
-        }
+        a.scl = d_scl_;
+        a.maxslot = d_maxslot_;
+        a.rescale_every = std::max(1, c.fp16_rescale_every);
         return a;
     }
 
@@ -555,7 +557,17 @@ private:
     int nepart_ = 0, epart_cap_ = 0;
     int64_t nxyp_ = 0, guard_ = 0;
     int host_step_ = 0;
-    float sc_[27], isc_[27];
+    float* d_scl_ = nullptr;         // FP16 scale table (kScaleTable floats)
+    unsigned* d_maxslot_ = nullptr;  // [kMaxSlots][27]
+
+    // scale of the data currently stored for field q (1 unless FP16 storage)
+    double current_scale(int q)
+    {
+        if (!std::is_same<T, half_t>::value) return 1.0;
+        float s = 1.f;
+        HIP_CHECK(hipMemcpy(&s, d_scl_ + q, sizeof(float), hipMemcpyDeviceToHost));
+        return s;
+    }
     hipGraphExec_t graphs_[2] = {nullptr, nullptr};
 };
 
@@ -566,7 +578,7 @@ std::unique_ptr<Shot> make_shot(ShotPlan plan, const float* cp, const float* cs,
     case ELA3D_STORAGE_FP64:
         return std::unique_ptr<Shot>(new ShotImpl<double>(std::move(plan), cp, cs, rho, ext));
     case ELA3D_STORAGE_FP16:
-        throw Error{ELA3D_E_INVALID, "FP16 storage is not enabled in this build"};
+        return std::unique_ptr<Shot>(new ShotImpl<half_t>(std::move(plan), cp, cs, rho, ext));
     default:
         return std::unique_ptr<Shot>(new ShotImpl<float>(std::move(plan), cp, cs, rho, ext));
     }

@@ -4,6 +4,20 @@
 
 namespace ela3d {
 
+// scale of the data written this step (FP16 storage; 1 otherwise)
+template <typename T>
+__device__ __forceinline__ float nsc(const TailArgs<T>& A, int q)
+{
+    if constexpr (std::is_same<T, half_t>::value) return A.scl[54 + q];
+    return 1.f;
+}
+template <typename T>
+__device__ __forceinline__ float nisc(const TailArgs<T>& A, int q)
+{
+    if constexpr (std::is_same<T, half_t>::value) return A.scl[81 + q];
+    return 1.f;
+}
+
 // ---------------------------------------------------------------------------
 // Per-step tail (one workgroup): receivers (before the source, as CPU:593-602
 // precedes CPU:604-622), energy reduction, source injection, step counter.
@@ -23,9 +37,9 @@ __global__ void __launch_bounds__(256) k_tail(const TailArgs<T> A)
         if (s < A.nsamples) {
             for (int r = tid; r < A.nrec; r += blockDim.x) {
                 const int o = A.rec[3 * r] + A.rec[3 * r + 1] * A.nx + A.rec[3 * r + 2] * A.nxy;
-                const double vx = (double)ldf(A.f[0], o, A.isc[0]);
-                const double vy = (double)ldf(A.f[1], o, A.isc[1]);
-                const double vz = (double)ldf(A.f[2], o, A.isc[2]);
+                const double vx = (double)ldf(A.f[0], o, nisc<T>(A, 0));
+                const double vy = (double)ldf(A.f[1], o, nisc<T>(A, 1));
+                const double vz = (double)ldf(A.f[2], o, nisc<T>(A, 2));
                 double* dst = A.seis + (size_t(s) * A.nrec + r) * 3;
                 dst[0] = vx;
                 dst[1] = vy;
@@ -69,24 +83,24 @@ __global__ void __launch_bounds__(256) k_tail(const TailArgs<T> A)
             switch (A.src_kind) {
             case 0: {  // CPU:620-622
                 const R fx = (R)(A.sin_t * st), fy = (R)(A.cos_t * st);
-                stf(A.f[3], o, ldf(A.f[3], o, A.isc[3]) + fx * DT * l2m, A.sc[3]);
-                stf(A.f[4], o, ldf(A.f[4], o, A.isc[4]) + fx * DT * l2m, A.sc[4]);
-                stf(A.f[5], o, ldf(A.f[5], o, A.isc[5]) + fy * DT * l2m, A.sc[5]);
+                stf(A.f[3], o, ldf(A.f[3], o, nisc<T>(A, 3)) + fx * DT * l2m, nsc<T>(A, 3));
+                stf(A.f[4], o, ldf(A.f[4], o, nisc<T>(A, 4)) + fx * DT * l2m, nsc<T>(A, 4));
+                stf(A.f[5], o, ldf(A.f[5], o, nisc<T>(A, 5)) + fy * DT * l2m, nsc<T>(A, 5));
                 break;
             }
             case 1: {  // CPU:614-617
                 const R fx = (R)(A.sin_t * st), fy = (R)(A.cos_t * st);
                 const R rx = ((R)A.rho[o] + (R)A.rho[o - 1]) / (R)2;
                 const R ry = ((R)A.rho[o] + (R)A.rho[o + A.nx]) / (R)2;
-                stf(A.f[0], o, ldf(A.f[0], o, A.isc[0]) + fx * DT / rx, A.sc[0]);
-                stf(A.f[1], o, ldf(A.f[1], o, A.isc[1]) + fy * DT / ry, A.sc[1]);
+                stf(A.f[0], o, ldf(A.f[0], o, nisc<T>(A, 0)) + fx * DT / rx, nsc<T>(A, 0));
+                stf(A.f[1], o, ldf(A.f[1], o, nisc<T>(A, 1)) + fy * DT / ry, nsc<T>(A, 1));
                 break;
             }
             case 2: {
                 const R s = (R)st;
-                stf(A.f[3], o, ldf(A.f[3], o, A.isc[3]) + s * DT * l2m, A.sc[3]);
-                stf(A.f[4], o, ldf(A.f[4], o, A.isc[4]) + s * DT * l2m, A.sc[4]);
-                stf(A.f[5], o, ldf(A.f[5], o, A.isc[5]) + s * DT * l2m, A.sc[5]);
+                stf(A.f[3], o, ldf(A.f[3], o, nisc<T>(A, 3)) + s * DT * l2m, nsc<T>(A, 3));
+                stf(A.f[4], o, ldf(A.f[4], o, nisc<T>(A, 4)) + s * DT * l2m, nsc<T>(A, 4));
+                stf(A.f[5], o, ldf(A.f[5], o, nisc<T>(A, 5)) + s * DT * l2m, nsc<T>(A, 5));
                 break;
             }
             default: {  // ZC:451-453
@@ -94,15 +108,44 @@ __global__ void __launch_bounds__(256) k_tail(const TailArgs<T> A)
                 const R rx = ((R)A.rho[o] + (R)A.rho[o - 1]) / (R)2;
                 const R ry = ((R)A.rho[o] + (R)A.rho[o + A.nx]) / (R)2;
                 const R rz = ((R)A.rho[o] + (R)A.rho[o + A.nxy]) / (R)2;
-                stf(A.f[0], o, ldf(A.f[0], o, A.isc[0]) + s * DT / rx, A.sc[0]);
-                stf(A.f[1], o, ldf(A.f[1], o, A.isc[1]) + s * DT / ry, A.sc[1]);
-                stf(A.f[2], o, ldf(A.f[2], o, A.isc[2]) + s * DT / rz, A.sc[2]);
+                stf(A.f[0], o, ldf(A.f[0], o, nisc<T>(A, 0)) + s * DT / rx, nsc<T>(A, 0));
+                stf(A.f[1], o, ldf(A.f[1], o, nisc<T>(A, 1)) + s * DT / ry, nsc<T>(A, 1));
+                stf(A.f[2], o, ldf(A.f[2], o, nisc<T>(A, 2)) + s * DT / rz, nsc<T>(A, 2));
                 break;
             }
             }
         }
         if (bad) *A.flag = 1;
         *A.step = it;
+    }
+    if constexpr (std::is_same<T, half_t>::value) {
+        // FP16 storage: the data now sits at the "next" scales; make them current
+        // and choose the scales of the next step from this step's maxima, every
+        // rescale_every steps, or early when a stored maximum leaves [2^8, 2^13]
+        // (headroom below 65504, precision above the subnormals).
+        __syncthreads();
+        if (tid < 27) {
+            const int q = tid;
+            unsigned mb = 0;
+            for (int s = 0; s < kMaxSlots; ++s) {
+                mb = max(mb, A.maxslot[s * 27 + q]);
+                A.maxslot[s * 27 + q] = 0u;
+            }
+            const float m = __uint_as_float(mb);
+            float sc = A.scl[54 + q];
+            A.scl[q] = sc;
+            A.scl[27 + q] = A.scl[81 + q];
+            const float stored = m * sc;
+            const bool due = (it % A.rescale_every == 0) || stored > 8192.f || stored < 256.f;
+            if (due && m > 0.f && isfinite(m)) {
+                int e;
+                frexpf(m, &e);  // m <= 2^e
+                const int x = min(100, max(-100, 12 - e));
+                sc = ldexpf(1.f, x);  // stored maximum in (2^11, 2^12]
+            }
+            A.scl[54 + q] = sc;
+            A.scl[81 + q] = 1.f / sc;
+        }
     }
 }
 
