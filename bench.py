@@ -137,7 +137,7 @@ def main():
         t = time.perf_counter()
         tuned = ela3d_amd.autotune(cfg, model, steps=6)
         log(f"[bench] autotune: block={tuned['block']} zseg={tuned['zseg_interior']}/"
-            f"{tuned['zseg_slab']} ({tuned['ms_per_step']:.3f} ms/step over "
+            f"{tuned['zseg_slab']} vec={tuned['vec']} ({tuned['ms_per_step']:.3f} ms/step over "
             f"{tuned['candidates']} shapes, {time.perf_counter() - t:.1f} s)")
     with Shot(cfg, *model) as shot:
         shot.step(W)
@@ -194,7 +194,8 @@ def main():
             "alg_bytes_per_step": info["alg_bytes_per_step"],
             "step_hbm_frac": round(step_bw / PEAK_HBM, 4),
             "graph_ms_per_step": graph_ms,
-            "launch_shape": {"block": info["block"], "zseg": info["zseg"]},
+            "launch_shape": {"block": info["block"], "zseg": info["zseg"],
+                             "zseg_slab": info["zseg_slab"], "vec": info["vec"]},
         },
         "roofline": {
             "bound": "hbm", "kernel": dom["name"], "achieved": round(achieved / 1e9, 2),

@@ -43,7 +43,7 @@ class InfoStruct(C.Structure):
         ("alg_bytes_per_step", C.c_double), ("alg_bytes_stress", C.c_double),
         ("alg_bytes_velocity", C.c_double), ("pml_bytes_per_step", C.c_double),
         ("nsamples", C.c_int32), ("nenergy", C.c_int32), ("steps_done", C.c_int32),
-        ("block", C.c_int32), ("zseg", C.c_int32),
+        ("block", C.c_int32), ("zseg", C.c_int32), ("zseg_slab", C.c_int32), ("vec", C.c_int32),
         ("region_lo", C.c_int32 * 3), ("region_hi", C.c_int32 * 3),
     ]
 

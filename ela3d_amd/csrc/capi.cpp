@@ -252,6 +252,8 @@ int ela3d_plan(const ela3d_config* cfg, ela3d_info* out)
         ela3d::Tuning t = ela3d::choose_tuning(p);
         out->block = t.block;
         out->zseg = t.zseg_interior;
+        out->zseg_slab = t.zseg_slab;
+        out->vec = t.vec;
         for (int a = 0; a < 3; ++a) {
             out->region_lo[a] = p.ax[a].lo;
             out->region_hi[a] = p.ax[a].hi;

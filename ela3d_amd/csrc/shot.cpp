@@ -105,6 +105,7 @@ public:
         geometry(plan_, tune_, geo_);
         allocate();
         upload(cp, cs, rho);
+        initial_scales(cp, rho);
         args_ = step_args(geo_, d_epart_);
     }
 
@@ -300,6 +301,8 @@ public:
         o.steps_done = host_step_;
         o.block = tune_.block;
         o.zseg = tune_.zseg_interior;
+        o.zseg_slab = tune_.zseg_slab;
+        o.vec = tune_.vec;
         for (int a = 0; a < 3; ++a) {
             o.region_lo[a] = plan_.ax[a].lo;
             o.region_hi[a] = plan_.ax[a].hi;
@@ -397,10 +400,46 @@ private:
         // and plane padding (read as neighbours, never written), outputs
         HIP_CHECK(hipMemsetAsync(arena_, 0, off, stream_));
         HIP_CHECK(hipStreamSynchronize(stream_));
-        // every storage scale starts at 1 (the FP16 tail re-chooses them from the data)
         const std::vector<float> ones(kScaleTable, 1.0f);
         HIP_CHECK(hipMemcpy(d_scl_, ones.data(), ones.size() * sizeof(float), hipMemcpyHostToDevice));
         (void)o_seis;
+    }
+
+    // FP16 storage: scales before the first write, from the size each field can
+    // reach near the source (one period of the peak injection; v and sigma related
+    // by the impedance rho*cp, memory variables by 1/h), so that the first writes
+    // fit the FP16 range.  The tail kernel then tracks the data every step.
+    void initial_scales(const float* cp, const float* rho)
+    {
+        if (!std::is_same<T, half_t>::value) return;
+        const ela3d_config& c = plan_.cfg;
+        const size_t s = size_t(c.isrc) + size_t(c.jsrc) * c.nx + size_t(c.ksrc) * c.nx * c.ny;
+        double peak = 0.0;
+        for (double w : plan_.wavelet) peak = std::max(peak, std::fabs(w));
+        const double r = rho[s], v = cp[s], imp = r * v;
+        const double lobe = std::max(1.0, 1.0 / (c.f0 * c.dt));
+        double sig, vel;
+        if (c.src_kind == ELA3D_SRC_REFERENCE_SIGMA || c.src_kind == ELA3D_SRC_EXPLOSIVE) {
+            sig = peak * c.dt * r * v * v * lobe;
+            vel = sig / imp;
+        } else {
+            vel = peak * c.dt / r * lobe;
+            sig = vel * imp;
+        }
+        const double h = std::min(c.dx, std::min(c.dy, c.dz));
+        std::vector<float> tab(kScaleTable, 1.0f);
+        for (int q = 0; q < 27; ++q) {
+            const double est = q < 3 ? vel : (q < 9 ? sig : (q < 18 ? vel / h : sig / h));
+            float sc = 1.0f;
+            if (est > 0.0 && std::isfinite(est)) {
+                int e = 0;
+                std::frexp(est, &e);
+                sc = std::ldexp(1.0f, std::min(100, std::max(-100, 12 - e)));
+            }
+            tab[q] = tab[54 + q] = sc;
+            tab[27 + q] = tab[81 + q] = 1.0f / sc;
+        }
+        HIP_CHECK(hipMemcpy(d_scl_, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
     }
 
     // host x-fastest nx*ny*nz  <->  device planes of stride nxyp_

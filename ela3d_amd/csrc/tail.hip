@@ -18,6 +18,14 @@ __device__ __forceinline__ float nisc(const TailArgs<T>& A, int q)
     return 1.f;
 }
 
+// source injection into field q at cell o, recording the magnitude written
+template <typename T>
+__device__ __forceinline__ void inject(const TailArgs<T>& A, int q, int o, typename Arith<T>::R v, float* inj)
+{
+    stf(A.f[q], o, v, nsc<T>(A, q));
+    inj[q] = fabsf((float)v);
+}
+
 // ---------------------------------------------------------------------------
 // Per-step tail (one workgroup): receivers (before the source, as CPU:593-602
 // precedes CPU:604-622), energy reduction, source injection, step counter.
@@ -28,9 +36,11 @@ __global__ void __launch_bounds__(256) k_tail(const TailArgs<T> A)
     typedef typename Arith<T>::R R;
     __shared__ double red[2][4];
     __shared__ int bad;
+    __shared__ float inj[9];  // |value| written by the source injection, per field
     const int tid = threadIdx.x;
     const int it = *A.step + 1;
     if (tid == 0) bad = 0;
+    if (tid < 9) inj[tid] = 0.f;
     __syncthreads();
     if (A.nrec > 0 && it % A.sample_every == 0) {
         const int s = it / A.sample_every - 1;
@@ -83,24 +93,24 @@ __global__ void __launch_bounds__(256) k_tail(const TailArgs<T> A)
             switch (A.src_kind) {
             case 0: {  // CPU:620-622
                 const R fx = (R)(A.sin_t * st), fy = (R)(A.cos_t * st);
-                stf(A.f[3], o, ldf(A.f[3], o, nisc<T>(A, 3)) + fx * DT * l2m, nsc<T>(A, 3));
-                stf(A.f[4], o, ldf(A.f[4], o, nisc<T>(A, 4)) + fx * DT * l2m, nsc<T>(A, 4));
-                stf(A.f[5], o, ldf(A.f[5], o, nisc<T>(A, 5)) + fy * DT * l2m, nsc<T>(A, 5));
+                inject(A, 3, o, ldf(A.f[3], o, nisc<T>(A, 3)) + fx * DT * l2m, inj);
+                inject(A, 4, o, ldf(A.f[4], o, nisc<T>(A, 4)) + fx * DT * l2m, inj);
+                inject(A, 5, o, ldf(A.f[5], o, nisc<T>(A, 5)) + fy * DT * l2m, inj);
                 break;
             }
             case 1: {  // CPU:614-617
                 const R fx = (R)(A.sin_t * st), fy = (R)(A.cos_t * st);
                 const R rx = ((R)A.rho[o] + (R)A.rho[o - 1]) / (R)2;
                 const R ry = ((R)A.rho[o] + (R)A.rho[o + A.nx]) / (R)2;
-                stf(A.f[0], o, ldf(A.f[0], o, nisc<T>(A, 0)) + fx * DT / rx, nsc<T>(A, 0));
-                stf(A.f[1], o, ldf(A.f[1], o, nisc<T>(A, 1)) + fy * DT / ry, nsc<T>(A, 1));
+                inject(A, 0, o, ldf(A.f[0], o, nisc<T>(A, 0)) + fx * DT / rx, inj);
+                inject(A, 1, o, ldf(A.f[1], o, nisc<T>(A, 1)) + fy * DT / ry, inj);
                 break;
             }
             case 2: {
                 const R s = (R)st;
-                stf(A.f[3], o, ldf(A.f[3], o, nisc<T>(A, 3)) + s * DT * l2m, nsc<T>(A, 3));
-                stf(A.f[4], o, ldf(A.f[4], o, nisc<T>(A, 4)) + s * DT * l2m, nsc<T>(A, 4));
-                stf(A.f[5], o, ldf(A.f[5], o, nisc<T>(A, 5)) + s * DT * l2m, nsc<T>(A, 5));
+                inject(A, 3, o, ldf(A.f[3], o, nisc<T>(A, 3)) + s * DT * l2m, inj);
+                inject(A, 4, o, ldf(A.f[4], o, nisc<T>(A, 4)) + s * DT * l2m, inj);
+                inject(A, 5, o, ldf(A.f[5], o, nisc<T>(A, 5)) + s * DT * l2m, inj);
                 break;
             }
             default: {  // ZC:451-453
@@ -108,9 +118,9 @@ __global__ void __launch_bounds__(256) k_tail(const TailArgs<T> A)
                 const R rx = ((R)A.rho[o] + (R)A.rho[o - 1]) / (R)2;
                 const R ry = ((R)A.rho[o] + (R)A.rho[o + A.nx]) / (R)2;
                 const R rz = ((R)A.rho[o] + (R)A.rho[o + A.nxy]) / (R)2;
-                stf(A.f[0], o, ldf(A.f[0], o, nisc<T>(A, 0)) + s * DT / rx, nsc<T>(A, 0));
-                stf(A.f[1], o, ldf(A.f[1], o, nisc<T>(A, 1)) + s * DT / ry, nsc<T>(A, 1));
-                stf(A.f[2], o, ldf(A.f[2], o, nisc<T>(A, 2)) + s * DT / rz, nsc<T>(A, 2));
+                inject(A, 0, o, ldf(A.f[0], o, nisc<T>(A, 0)) + s * DT / rx, inj);
+                inject(A, 1, o, ldf(A.f[1], o, nisc<T>(A, 1)) + s * DT / ry, inj);
+                inject(A, 2, o, ldf(A.f[2], o, nisc<T>(A, 2)) + s * DT / rz, inj);
                 break;
             }
             }
@@ -131,7 +141,7 @@ __global__ void __launch_bounds__(256) k_tail(const TailArgs<T> A)
                 mb = max(mb, A.maxslot[s * 27 + q]);
                 A.maxslot[s * 27 + q] = 0u;
             }
-            const float m = __uint_as_float(mb);
+            const float m = fmaxf(__uint_as_float(mb), q < 9 ? inj[q] : 0.f);
             float sc = A.scl[54 + q];
             A.scl[q] = sc;
             A.scl[27 + q] = A.scl[81 + q];

@@ -133,7 +133,8 @@ typedef struct ela3d_info {
     int32_t nsamples;              /* seismogram samples over nstep */
     int32_t nenergy;               /* energy samples over nstep */
     int32_t steps_done;
-    int32_t block, zseg;           /* launch shape in use */
+    int32_t block, zseg;           /* launch shape in use: threads, interior z planes per workgroup */
+    int32_t zseg_slab, vec;        /* absorbing-layer z planes per workgroup, xy positions per lane */
     int32_t region_lo[3], region_hi[3]; /* C-PML compact regions: [0,lo) u [hi,n) per axis */
 } ela3d_info;
 
