@@ -80,14 +80,21 @@ def cpu_baseline(cfg, model, budget_s=15.0):
     }
 
 
+PMC_FILES = {"classic": "pmc_classic_101x641x640_fp32.json"}
+
+
 def load_traffic(workload, kernel):
-    path = os.path.join(ROOT, "profiles", f"pmc_{workload}.json")
-    if not os.path.exists(path):
-        return None
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel of known size), with the
+    launch shape it was measured at."""
+    name = PMC_FILES.get(workload)
+    path = os.path.join(ROOT, "profiles", name) if name else ""
+    if not name or not os.path.exists(path):
+        return None, None
     with open(path) as f:
         d = json.load(f)
     k = d.get("kernels", {}).get(kernel)
-    return None if k is None else k.get("hbm_bytes_per_dispatch")
+    return (None if k is None else k.get("hbm_bytes_per_dispatch")), d.get("launch_shape")
 
 
 def main():
@@ -171,6 +178,7 @@ def main():
     avg_ms = dom["total_ms"] / dom["launches"]
     achieved = dom["alg_bytes_per_launch"] / (avg_ms * 1e-3)
     step_bw = info["alg_bytes_per_step"] / (ms_per_step * 1e-3)
+    traffic, traffic_shape = load_traffic(args.workload, dom["name"])
     log(f"[bench] step: {ms_per_step:.4f} ms, {mcells:.1f} Mcell-updates/s, algorithmic "
         f"{info['alg_bytes_per_step'] / 1e6:.1f} MB/step -> {step_bw / 1e9:.1f} GB/s "
         f"({100 * step_bw / PEAK_HBM:.1f}% of peak)")
@@ -200,7 +208,8 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": dom["name"], "achieved": round(achieved / 1e9, 2),
             "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4),
-            "traffic": load_traffic(args.workload, dom["name"]),
+            "traffic": traffic,
+            "traffic_launch_shape": traffic_shape,
             "alg_bytes_per_launch": dom["alg_bytes_per_launch"],
             "avg_launch_ms": round(avg_ms, 5),
         },
