@@ -11,6 +11,12 @@ kernel is bracketed by hipEvents on the shot's own stream, which gives the
 dominant kernel's average duration for the roofline.  Prints ONE JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload classic]
+
+``--workload serving`` instead measures the shot server on the seeded stream of
+256 mixed-size shots (edges 64..256, NSTEP 500..2000, Poisson arrivals at
+4 shots/s): closed-loop shots/s of the batched server against one-at-a-time
+execution (the plain per-step-synchronous loop and graph replay), and p50/p99
+latency under the Poisson arrivals.
 """
 from __future__ import annotations
 
@@ -97,6 +103,136 @@ def load_traffic(workload, kernel):
     return (None if k is None else k.get("hbm_bytes_per_dispatch")), d.get("launch_shape")
 
 
+def percentile(a, q):
+    return float(np.percentile(np.asarray(a, np.float64), q)) if len(a) else None
+
+
+def run_server(shots, arrivals=None, label="", **opts):
+    """Submit every shot (at its arrival time if given) and wait for all.
+
+    Returns (makespan_s, results) with per-shot status, submit/admit/done times
+    relative to the start of the run."""
+    import ela3d_amd
+    res = [None] * len(shots)
+    with ela3d_amd.Server(**opts) as srv:
+        t_start = time.perf_counter()
+        tickets = []
+        for q, (cfg, model) in enumerate(shots):
+            if arrivals is not None:
+                dt = arrivals[q] - (time.perf_counter() - t_start)
+                if dt > 0:
+                    time.sleep(dt)
+            tickets.append(srv.submit(cfg, *model))
+        base = None
+        for q, t in enumerate(tickets):
+            info = srv.wait(t)
+            srv.release(t)
+            res[q] = info
+            if (q + 1) % 32 == 0 or q + 1 == len(tickets):
+                log(f"[serving] {label}: {q + 1}/{len(tickets)} shots done, "
+                    f"{time.perf_counter() - t_start:.1f} s")
+        makespan = time.perf_counter() - t_start
+        stats = srv.stats()
+    base = min(r["t_submit"] for r in res)
+    for r in res:
+        for k in ("t_submit", "t_admit", "t_done"):
+            r[k] -= base
+    return makespan, res, stats
+
+
+def serving_main(args):
+    import ela3d_amd
+    from ela3d_amd import workloads
+    if ela3d_amd.device_count() < 1:
+        raise SystemExit("no GPU visible")
+    specs = workloads.serving_stream(nshots=args.shots, seed=args.seed)
+    arrivals = [s.arrival for s in specs]
+    t = time.perf_counter()
+    shots = [s.materialize() for s in specs]
+    log(f"[serving] {len(shots)} shots materialised in {time.perf_counter() - t:.1f} s; "
+        f"{sum(s.cell_updates for s in specs) / 1e9:.1f} Gcell-updates in total")
+    if not args.no_tune:  # one launch shape per grid size, chosen before the timed runs
+        for n in sorted({s.edge for s in specs}):
+            cfg, model = [sh for sh, sp in zip(shots, specs) if sp.edge == n][0]
+            tt = ela3d_amd.autotune(cfg.replace(nstep=64), model, steps=4)
+            log(f"[serving] tuned {n}^3: block={tt['block']} zseg={tt['zseg_interior']}/"
+                f"{tt['zseg_slab']} vec={tt['vec']} {tt['ms_per_step']:.3f} ms/step")
+    cells = sum(s.cell_updates for s in specs)
+    alg = 0.0
+    for cfg, _ in shots:
+        alg += cfg.plan()["alg_bytes_per_step"] * cfg.nstep
+    common = dict(hbm_budget_bytes=int(args.budget_gb * 1e9), steps_per_round=args.round)
+    modes = {}
+    # batched server, closed loop (everything submitted at once)
+    mk, res, st = run_server(shots, label="batched", max_active=args.max_active, **common)
+    modes["batched"] = (mk, res, st)
+    # one shot at a time: graph replay, then the plain per-step-synchronous loop
+    mk, res, st = run_server(shots, label="one-at-a-time graph", max_active=1, **common)
+    modes["graph_one_at_a_time"] = (mk, res, st)
+    if not args.no_plain:
+        mk, res, st = run_server(shots, label="plain loop", plain_loop=True, **common)
+        modes["plain_loop"] = (mk, res, st)
+    # batched server under the Poisson arrivals (open loop)
+    mk_open, res, st = run_server(shots, arrivals=arrivals, label="batched Poisson",
+                                  max_active=args.max_active, **common)
+    lat_open = [r["t_done"] - r["t_submit"] for r in res]
+    failed_open = sum(1 for r in res if r["status"] != 0)
+    report = {}
+    for name, (mk, res, st) in modes.items():
+        failed = sum(1 for r in res if r["status"] != 0)
+        service = [r["t_done"] - r["t_admit"] for r in res]
+        d = {"shots_per_s": round(len(res) / mk, 4), "makespan_s": round(mk, 3),
+             "gcell_updates_per_s": round(cells / mk / 1e9, 3), "failed": failed,
+             "peak_device_bytes": st["peak_bytes"]}
+        if name != "batched":  # one-at-a-time: Poisson latency from the measured service times
+            lat = workloads.fifo_latencies(arrivals, service)
+            d["poisson_p50_s"] = round(percentile(lat, 50), 4)
+            d["poisson_p99_s"] = round(percentile(lat, 99), 4)
+            d["latency_model"] = "FIFO (Lindley) over measured per-shot service times"
+        report[name] = d
+    report["batched"]["poisson_p50_s"] = round(percentile(lat_open, 50), 4)
+    report["batched"]["poisson_p99_s"] = round(percentile(lat_open, 99), 4)
+    report["batched"]["poisson_makespan_s"] = round(mk_open, 3)
+    report["batched"]["poisson_failed"] = failed_open
+    report["batched"]["latency_model"] = "measured, open loop (submitted at the Poisson arrival times)"
+    b = modes["batched"][0]
+    ref = "plain_loop" if "plain_loop" in report else "graph_one_at_a_time"
+    ratio = modes[ref][0] / b
+    for k, v in report.items():
+        log(f"[serving] {k:<22} {json.dumps(v)}")
+    agg = alg / b
+    out = {
+        "metric": METRIC,
+        "value": report["batched"]["shots_per_s"],
+        "unit": "shots/s",
+        "n_gpus": 1, "steps": len(shots), "warmup": 0,
+        "ms_per_step": round(b * 1e3 / len(shots), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": (f"serving stream: {len(shots)} seeded shots, cubic edges "
+                                f"{list(workloads.SERVING_EDGES)}, NSTEP 500-2000, Poisson 4 shots/s"),
+                   "max_active": args.max_active, "steps_per_round": args.round,
+                   "hbm_budget_bytes": int(args.budget_gb * 1e9), "cell_updates": cells,
+                   "speedup_vs_" + ref: round(ratio, 3), "modes": report},
+        "roofline": {"bound": "hbm", "kernel": "all (aggregate over the stream)",
+                     "achieved": round(agg / 1e9, 2), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": round(agg / PEAK_HBM, 4), "traffic": None},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        log("[serving] CPU baseline (oracle restatement) on a sample shot ...")
+        spec = [s for s in specs if s.edge == 128][0]
+        cfg, model = shots[specs.index(spec)]
+        cb = cpu_baseline(cfg, model, budget_s=10.0)
+        mean_cells = cells / len(specs)
+        cb["sample"] = (f"{cb['sample']}; a {spec.edge}^3 shot of the stream, shots/s = "
+                        f"Mcell-updates/s / mean {mean_cells / 1e6:.0f} M cell-updates per shot")
+        cb["value"] = round(cb["value"] * 1e6 / mean_cells, 5)
+        cb["unit"] = "shots/s"
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +248,12 @@ def main():
                     help="copy kernel for --calibrate (0-3); default: all four")
     ap.add_argument("--shape", default="",
                     help="fixed launch shape block,zseg_interior,zseg_slab,vec (skips the tuner)")
+    ap.add_argument("--shots", type=int, default=256, help="serving: shots in the stream")
+    ap.add_argument("--seed", type=int, default=7, help="serving: stream seed")
+    ap.add_argument("--max-active", type=int, default=8, help="serving: concurrent shots")
+    ap.add_argument("--round", type=int, default=50, help="serving: steps per round")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serving: HBM budget (GB)")
+    ap.add_argument("--no-plain", action="store_true", help="serving: skip the plain-loop leg")
     args = ap.parse_args()
     if args.calibrate > 0:
         import ela3d_amd
@@ -128,6 +270,8 @@ def main():
               "shards by z-slabs across GPUs, which this build does not run.", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "serving":
+        return serving_main(args)
     import ela3d_amd
     from ela3d_amd import Shot
     if ela3d_amd.device_count() < 1:

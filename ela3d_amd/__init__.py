@@ -24,7 +24,7 @@ __all__ = [
     "copy_bench", "SRC_REFERENCE_SIGMA", "SRC_FORCE", "SRC_EXPLOSIVE", "SRC_FORCE3",
     "WAVELET_GAUSSIAN", "WAVELET_DGAUSSIAN", "WAVELET_RICKER", "STORAGE_FP32", "STORAGE_FP16",
     "STORAGE_FP64", "COMPAT_XHALF_A", "COMPAT_ZHALF_MIN", "COMPAT_GHOST", "COMPAT_REFERENCE",
-    "FIELD_NAMES",
+    "FIELD_NAMES", "Server", "SHOT_QUEUED", "SHOT_RUNNING", "SHOT_DONE",
 ]
 
 SRC_REFERENCE_SIGMA, SRC_FORCE, SRC_EXPLOSIVE, SRC_FORCE3 = 0, 1, 2, 3
@@ -243,6 +243,102 @@ class Shot:
         ks = [dict(name=st[q].name.decode(), launches=st[q].launches, total_ms=st[q].total_ms,
                    alg_bytes_per_launch=st[q].alg_bytes_per_launch) for q in range(nk.value)]
         return ks, wall.value
+
+
+SHOT_QUEUED, SHOT_RUNNING, SHOT_DONE = 0, 1, 2
+
+
+class Server:
+    """Continuous-batching shot server on one GPU (ela3d_server_create ... destroy).
+
+    ``submit`` copies the configuration and model and returns a ticket; a device
+    thread admits shots in arrival order within ``hbm_budget_bytes`` and
+    ``max_active``, steps them concurrently (one stream per shot, graph replays in
+    rounds of ``steps_per_round``) and retires them as they finish.
+    ``plain_loop=True`` runs one shot at a time with per-step synchronous host
+    traffic instead (the reference's way of driving its kernels).
+    """
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: Optional[int] = None,
+                 max_active: Optional[int] = None, steps_per_round: Optional[int] = None,
+                 use_graph: bool = True, plain_loop: bool = False):
+        from ._lib import ServerOpts
+        L = lib()
+        o = ServerOpts()
+        L.ela3d_server_opts_default(C.byref(o))
+        o.device = device
+        if hbm_budget_bytes is not None:
+            o.hbm_budget_bytes = int(hbm_budget_bytes)
+        if max_active is not None:
+            o.max_active = int(max_active)
+        if steps_per_round is not None:
+            o.steps_per_round = int(steps_per_round)
+        o.use_graph = 1 if use_graph else 0
+        o.plain_loop = 1 if plain_loop else 0
+        self.opts = o
+        h = C.c_void_p()
+        check(L.ela3d_server_create(C.byref(o), C.byref(h)))
+        self._h = h
+        self._cfg = {}
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().ela3d_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, cfg: Config, cp, cs, rho) -> int:
+        shape = (cfg.nz, cfg.ny, cfg.nx)
+        m = [np.ascontiguousarray(np.asarray(a, dtype=np.float32)).reshape(shape) for a in (cp, cs, rho)]
+        s, keep = cfg.to_struct()
+        t = C.c_uint64()
+        check(lib().ela3d_submit(self._h, C.byref(s), _fptr(m[0]), _fptr(m[1]), _fptr(m[2]), C.byref(t)))
+        self._cfg[t.value] = cfg
+        return t.value
+
+    def poll(self, ticket: int) -> int:
+        st = C.c_int32()
+        check(lib().ela3d_poll(self._h, C.c_uint64(ticket), C.byref(st)))
+        return st.value
+
+    def wait(self, ticket: int, timeout: float = -1.0) -> dict:
+        """Block until the shot is done; returns its timing and status (0 = OK)."""
+        from ._lib import ResultStruct
+        r = ResultStruct()
+        check(lib().ela3d_wait(self._h, C.c_uint64(ticket), float(timeout), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in ResultStruct._fields_}
+
+    def result(self, ticket: int, release: bool = True):
+        """(seismograms [nsamples][nrec][3], energy [nenergy][2], info) of a finished shot."""
+        info = self.wait(ticket)
+        seis = np.zeros((info["nsamples"], info["nrec"], 3), dtype=np.float64)
+        en = np.zeros((info["nenergy"], 2), dtype=np.float64)
+        check(lib().ela3d_result_seismograms(self._h, C.c_uint64(ticket), _dptr(seis), seis.size))
+        check(lib().ela3d_result_energy(self._h, C.c_uint64(ticket), _dptr(en), en.size))
+        if release:
+            self.release(ticket)
+        return seis, en, info
+
+    def release(self, ticket: int):
+        check(lib().ela3d_result_release(self._h, C.c_uint64(ticket)))
+        self._cfg.pop(ticket, None)
+
+    def stats(self) -> dict:
+        out = (C.c_int64 * 8)()
+        check(lib().ela3d_server_stats(self._h, out))
+        keys = ["submitted", "done", "failed", "active", "queued", "bytes_in_use", "peak_bytes", "rounds"]
+        return dict(zip(keys, list(out)))
 
 
 def autotune(cfg: Config, model, steps: int = 8) -> dict:

@@ -221,17 +221,11 @@ ShotPlan make_plan(const ela3d_config& in)
 
 int64_t device_bytes_estimate(const ela3d_config& c, const ShotPlan* plan)
 {
+    if (plan) return shot_device_bytes(*plan);
     const int64_t cells = int64_t(c.nx) * c.ny * c.nz;
     const int e = storage_bytes(c.storage);
-    int64_t mem = 0;
-    if (plan) {
-        const PmlAxis &X = plan->ax[0], &Y = plan->ax[1], &Z = plan->ax[2];
-        mem = 6 * (int64_t(c.nz) * c.ny * X.width() + int64_t(c.nz) * Y.width() * c.nx +
-                   int64_t(Z.width()) * c.ny * c.nx);
-    } else {
-        const int w = 2 * c.npoints_pml + 2;
-        mem = 6 * (int64_t(c.nz) * c.ny * w + int64_t(c.nz) * w * c.nx + int64_t(w) * c.ny * c.nx);
-    }
+    const int w = 2 * c.npoints_pml + 2;
+    const int64_t mem = 6 * (int64_t(c.nz) * c.ny * w + int64_t(c.nz) * w * c.nx + int64_t(w) * c.ny * c.nx);
     const int64_t tables = 6 * 8 * int64_t(c.nx + c.ny + c.nz);
     const int64_t samples = int64_t(c.nstep / std::max(1, c.sample_every)) * c.nrec * 3 * 8;
     const int64_t energy = c.energy_every > 0 ? int64_t(c.nstep / c.energy_every) * 16 : 0;

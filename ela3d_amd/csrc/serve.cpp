@@ -1,0 +1,464 @@
+// serve.cpp -- continuous batching of forward shots on one GPU.
+//
+// One device thread owns every admitted shot.  Each shot has its own HIP stream
+// and keeps up to two rounds of steps_per_round graph replays in flight; after
+// each round it records an event.  The thread polls those events: a shot whose
+// last round completed is retired (results copied to host, device memory freed)
+// and the admission step then starts queued shots in arrival order while the
+// device-memory budget and the active-shot cap allow.  Shots of any grid size
+// and step count therefore overlap on the device, and arrivals join while
+// earlier shots are still running.
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+
+#include "shot.hpp"
+
+namespace ela3d {
+
+struct Request {
+    uint64_t ticket = 0;
+    ela3d_config cfg{};
+    std::vector<int32_t> rec;
+    std::vector<float> cp, cs, rho;
+    int64_t bytes = 0;           // device bytes charged to the budget
+    std::atomic<int> state{ELA3D_SHOT_QUEUED};
+    std::atomic<int> status{ELA3D_OK};
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    int nsamples = 0, nenergy = 0;
+    std::vector<double> seis, energy;
+    // device-thread only
+    std::unique_ptr<Shot> shot;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int ev_head = 0, ev_count = 0;
+    int enqueued = 0;
+};
+
+class Server {
+public:
+    explicit Server(const ela3d_server_opts& o) : opts_(o)
+    {
+        if (opts_.plain_loop) opts_.max_active = 1;
+        t0_ = std::chrono::steady_clock::now();
+        th_ = std::thread([this] { loop(); });
+    }
+
+    ~Server()
+    {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (th_.joinable()) th_.join();
+    }
+
+    double now() const
+    {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+    }
+
+    uint64_t submit(const ela3d_config& cfg, const float* cp, const float* cs, const float* rho)
+    {
+        ShotPlan plan = make_plan(cfg);  // validates; throws Error
+        auto r = std::unique_ptr<Request>(new Request);
+        r->cfg = plan.cfg;
+        r->rec = plan.rec;
+        const size_t n = size_t(plan.cells);
+        r->cp.assign(cp, cp + n);
+        r->cs.assign(cs, cs + n);
+        r->rho.assign(rho, rho + n);
+        r->bytes = device_bytes_estimate(plan.cfg, &plan);
+        r->nsamples = plan.nsamples;
+        r->nenergy = plan.nenergy;
+        if (r->bytes > opts_.hbm_budget_bytes)
+            throw Error{ELA3D_E_BUDGET, "shot needs " + std::to_string(r->bytes) +
+                                            " device bytes, more than the server budget"};
+        std::lock_guard<std::mutex> g(mu_);
+        r->ticket = ++next_ticket_;
+        r->t_submit = now();
+        const uint64_t t = r->ticket;
+        queue_.push_back(r.get());
+        all_[t] = std::move(r);
+        ++submitted_;
+        cv_.notify_all();
+        return t;
+    }
+
+    Request* find(uint64_t t)
+    {
+        auto it = all_.find(t);
+        if (it == all_.end()) throw Error{ELA3D_E_NOTFOUND, "unknown ticket"};
+        return it->second.get();
+    }
+
+    int poll(uint64_t t)
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        Request* r = find(t);
+        const int st = r->status.load();
+        return st != ELA3D_OK ? st : r->state.load();
+    }
+
+    int wait(uint64_t t, double timeout_s, ela3d_result& res)
+    {
+        std::unique_lock<std::mutex> g(mu_);
+        Request* r = find(t);
+        auto done = [&] { return r->state == ELA3D_SHOT_DONE; };
+        if (timeout_s < 0)
+            cv_done_.wait(g, done);
+        else if (!cv_done_.wait_for(g, std::chrono::duration<double>(timeout_s), done))
+            return ELA3D_E_TIMEOUT;
+        std::memset(&res, 0, sizeof(res));
+        res.ticket = t;
+        res.status = r->status;
+        res.nsamples = r->nsamples;
+        res.nrec = r->cfg.nrec;
+        res.nenergy = r->nenergy;
+        res.t_submit = r->t_submit;
+        res.t_admit = r->t_admit;
+        res.t_done = r->t_done;
+        res.device_bytes = r->bytes;
+        return ELA3D_OK;
+    }
+
+    void results(uint64_t t, double* out, size_t cap, bool seis)
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        Request* r = find(t);
+        if (r->state != ELA3D_SHOT_DONE) throw Error{ELA3D_E_STATE, "shot not finished"};
+        const std::vector<double>& v = seis ? r->seis : r->energy;
+        if (cap < v.size() || (!v.empty() && !out)) throw Error{ELA3D_E_RANGE, "buffer too small"};
+        if (!v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(double));
+    }
+
+    void release(uint64_t t)
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        Request* r = find(t);
+        if (r->state != ELA3D_SHOT_DONE) throw Error{ELA3D_E_STATE, "shot not finished"};
+        all_.erase(t);
+    }
+
+    void stats(int64_t* out)
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        out[0] = submitted_;
+        out[1] = done_;
+        out[2] = failed_;
+        out[3] = int64_t(active_.size());
+        out[4] = int64_t(queue_.size());
+        out[5] = in_use_;
+        out[6] = peak_;
+        out[7] = rounds_.load();
+    }
+
+private:
+    // ---- device thread ----
+    void loop()
+    {
+        hipSetDevice(opts_.device);
+        for (;;) {
+            std::vector<Request*> admitted;
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                if (stop_) {  // shutting down: cancel queued shots, stop running ones
+                    for (Request* r : queue_) {
+                        r->status = ELA3D_E_STATE;
+                        r->state = ELA3D_SHOT_DONE;
+                        r->t_done = now();
+                        ++done_;
+                        ++failed_;
+                    }
+                    queue_.clear();
+                    cv_done_.notify_all();
+                    for (Request* r : active_)
+                        if (r->enqueued < r->cfg.nstep) {
+                            r->enqueued = r->cfg.nstep;
+                            if (r->status == ELA3D_OK) r->status = ELA3D_E_STATE;
+                        }
+                    if (active_.empty()) break;
+                }
+                if (active_.empty() && queue_.empty()) {
+                    cv_.wait_for(g, std::chrono::milliseconds(20));
+                    continue;
+                }
+                // admission in arrival order while the budget and the cap allow
+                while (!queue_.empty() && int(active_.size() + admitted.size()) < opts_.max_active &&
+                       in_use_ + queue_.front()->bytes <= opts_.hbm_budget_bytes) {
+                    Request* r = queue_.front();
+                    queue_.pop_front();
+                    r->state = ELA3D_SHOT_RUNNING;
+                    r->t_admit = now();
+                    in_use_ += r->bytes;
+                    peak_ = std::max(peak_, in_use_);
+                    admitted.push_back(r);
+                }
+            }
+            for (Request* r : admitted) start(r);
+            bool progressed = false;
+            for (size_t q = 0; q < active_.size(); ++q) progressed |= advance(active_[q]);
+            // retire finished shots
+            for (size_t q = 0; q < active_.size();) {
+                Request* r = active_[q];
+                if (r->shot == nullptr || (r->enqueued >= r->cfg.nstep && r->ev_count == 0)) {
+                    finish(r);
+                    std::lock_guard<std::mutex> g(mu_);
+                    active_.erase(active_.begin() + q);
+                    progressed = true;
+                } else {
+                    ++q;
+                }
+            }
+            if (progressed)
+                ++rounds_;
+            else
+                std::this_thread::sleep_for(std::chrono::microseconds(50));
+        }
+    }
+
+    void start(Request* r)
+    {
+        try {
+            ShotPlan plan = make_plan([&] {
+                ela3d_config c = r->cfg;
+                c.rec = r->rec.data();
+                c.use_graph = opts_.use_graph;
+                c.device = opts_.device;
+                return c;
+            }());
+            r->shot = make_shot(std::move(plan), r->cp.data(), r->cs.data(), r->rho.data());
+            const int64_t actual = r->shot->device_bytes();
+            {
+                std::lock_guard<std::mutex> g(mu_);
+                in_use_ += actual - r->bytes;
+                r->bytes = actual;
+                peak_ = std::max(peak_, in_use_);
+            }
+            for (auto& e : r->ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            std::vector<float>().swap(r->cp);  // the model now lives on the device
+            std::vector<float>().swap(r->cs);
+            std::vector<float>().swap(r->rho);
+        } catch (const Error& e) {
+            r->status = e.code;
+            r->shot.reset();
+        } catch (...) {
+            r->status = ELA3D_E_INVALID;
+            r->shot.reset();
+        }
+        std::lock_guard<std::mutex> g(mu_);
+        active_.push_back(r);
+    }
+
+    // keep up to two rounds in flight; pop completed rounds.  true if anything moved
+    bool advance(Request* r)
+    {
+        if (!r->shot) return false;
+        bool moved = false;
+        try {
+            while (r->ev_count > 0) {
+                hipEvent_t e = r->ev[r->ev_head];
+                const hipError_t q = hipEventQuery(e);
+                if (q == hipErrorNotReady) break;
+                HIP_CHECK(q);
+                r->ev_head ^= 1;
+                --r->ev_count;
+                moved = true;
+                if (r->shot->unstable_seen()) {  // blown-up shot: stop it early
+                    r->status = ELA3D_E_UNSTABLE;
+                    r->enqueued = r->cfg.nstep;
+                    r->ev_count = 0;
+                    return true;
+                }
+            }
+            if (opts_.plain_loop) {
+                if (r->enqueued < r->cfg.nstep) {
+                    r->shot->step_plain(r->cfg.nstep - r->enqueued);
+                    r->enqueued = r->cfg.nstep;
+                    moved = true;
+                }
+                return moved;
+            }
+            while (r->ev_count < 2 && r->enqueued < r->cfg.nstep) {
+                const int n = std::min(opts_.steps_per_round, r->cfg.nstep - r->enqueued);
+                r->shot->enqueue_steps(n);
+                r->shot->enqueue_flag_readback();
+                r->enqueued += n;
+                const int slot = (r->ev_head + r->ev_count) & 1;
+                HIP_CHECK(hipEventRecord(r->ev[slot], r->shot->stream()));
+                ++r->ev_count;
+                moved = true;
+            }
+        } catch (const Error& e) {
+            r->status = e.code;
+            r->enqueued = r->cfg.nstep;
+            r->ev_count = 0;
+        }
+        return moved;
+    }
+
+    void finish(Request* r)
+    {
+        if (r->shot) {
+            try {
+                const int rc = r->shot->sync();
+                if (rc != ELA3D_OK && r->status == ELA3D_OK) r->status = rc;
+                r->seis.resize(size_t(r->nsamples) * r->cfg.nrec * 3);
+                r->energy.resize(size_t(r->nenergy) * 2);
+                if (!r->seis.empty()) r->shot->seismograms(r->seis.data());
+                if (!r->energy.empty()) r->shot->energy(r->energy.data());
+            } catch (const Error& e) {
+                if (r->status == ELA3D_OK) r->status = e.code;
+            }
+            for (auto& e : r->ev)
+                if (e) hipEventDestroy(e);
+            r->shot.reset();
+        }
+        std::lock_guard<std::mutex> g(mu_);
+        in_use_ -= r->bytes;
+        r->state = ELA3D_SHOT_DONE;
+        r->t_done = now();
+        ++done_;
+        if (r->status != ELA3D_OK) ++failed_;
+        cv_done_.notify_all();
+    }
+
+    ela3d_server_opts opts_;
+    std::chrono::steady_clock::time_point t0_;
+    std::thread th_;
+    std::mutex mu_;
+    std::condition_variable cv_, cv_done_;
+    std::deque<Request*> queue_;
+    std::vector<Request*> active_;  // changed by the device thread under mu_
+    std::map<uint64_t, std::unique_ptr<Request>> all_;
+    uint64_t next_ticket_ = 0;
+    int64_t in_use_ = 0, peak_ = 0;
+    int64_t submitted_ = 0, done_ = 0, failed_ = 0;
+    std::atomic<int64_t> rounds_{0};  // device-thread passes that moved any shot
+    bool stop_ = false;
+};
+
+}  // namespace ela3d
+
+struct ela3d_server {
+    std::unique_ptr<ela3d::Server> impl;
+};
+
+using ela3d::Error;
+
+template <typename F>
+static int sguard(F fn)
+{
+    try {
+        return fn();
+    } catch (const Error& e) {
+        ela3d::set_last_error(e.msg);
+        return e.code;
+    } catch (const std::exception& e) {
+        ela3d::set_last_error(e.what());
+        return ELA3D_E_INVALID;
+    }
+}
+
+extern "C" {
+
+void ela3d_server_opts_default(ela3d_server_opts* o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->struct_size = sizeof(ela3d_server_opts);
+    o->device = 0;
+    o->hbm_budget_bytes = int64_t(200) * 1000 * 1000 * 1000;
+    o->max_active = 16;
+    o->steps_per_round = 50;
+    o->use_graph = 1;
+    o->plain_loop = 0;
+}
+
+int ela3d_server_create(const ela3d_server_opts* o, ela3d_server** out)
+{
+    return sguard([&] {
+        if (!o || !out || o->struct_size != sizeof(ela3d_server_opts)) throw Error{ELA3D_E_INVALID, "bad options"};
+        if (o->max_active < 1 || o->steps_per_round < 1 || o->hbm_budget_bytes <= 0)
+            throw Error{ELA3D_E_INVALID, "max_active, steps_per_round and the budget must be positive"};
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= o->device || o->device < 0)
+            throw Error{ELA3D_E_NODEVICE, "no HIP device"};
+        auto s = new ela3d_server;
+        s->impl.reset(new ela3d::Server(*o));
+        *out = s;
+        return ELA3D_OK;
+    });
+}
+
+int ela3d_submit(ela3d_server* s, const ela3d_config* cfg, const float* cp, const float* cs, const float* rho,
+                 uint64_t* ticket)
+{
+    return sguard([&] {
+        if (!s || !cfg || !cp || !cs || !rho || !ticket) throw Error{ELA3D_E_INVALID, "null argument"};
+        *ticket = s->impl->submit(*cfg, cp, cs, rho);
+        return ELA3D_OK;
+    });
+}
+
+int ela3d_poll(ela3d_server* s, uint64_t ticket, int32_t* state)
+{
+    return sguard([&] {
+        if (!s || !state) throw Error{ELA3D_E_INVALID, "null argument"};
+        *state = s->impl->poll(ticket);
+        return ELA3D_OK;
+    });
+}
+
+int ela3d_wait(ela3d_server* s, uint64_t ticket, double timeout_s, ela3d_result* res)
+{
+    return sguard([&] {
+        if (!s || !res) throw Error{ELA3D_E_INVALID, "null argument"};
+        return s->impl->wait(ticket, timeout_s, *res);
+    });
+}
+
+int ela3d_result_seismograms(ela3d_server* s, uint64_t ticket, double* out, size_t cap)
+{
+    return sguard([&] {
+        if (!s) throw Error{ELA3D_E_INVALID, "null server"};
+        s->impl->results(ticket, out, cap, true);
+        return ELA3D_OK;
+    });
+}
+
+int ela3d_result_energy(ela3d_server* s, uint64_t ticket, double* out, size_t cap)
+{
+    return sguard([&] {
+        if (!s) throw Error{ELA3D_E_INVALID, "null server"};
+        s->impl->results(ticket, out, cap, false);
+        return ELA3D_OK;
+    });
+}
+
+int ela3d_result_release(ela3d_server* s, uint64_t ticket)
+{
+    return sguard([&] {
+        if (!s) throw Error{ELA3D_E_INVALID, "null server"};
+        s->impl->release(ticket);
+        return ELA3D_OK;
+    });
+}
+
+int ela3d_server_stats(ela3d_server* s, int64_t* out)
+{
+    return sguard([&] {
+        if (!s || !out) throw Error{ELA3D_E_INVALID, "null argument"};
+        s->impl->stats(out);
+        return ELA3D_OK;
+    });
+}
+
+void ela3d_server_destroy(ela3d_server* s) { delete s; }
+
+}  // extern "C"

@@ -85,6 +85,50 @@ void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& g)
     add_box(g, pa, pb, zlo, zhi, zi, kInterior);
 }
 
+static size_t mem_elems(const ShotPlan& p, int axis)
+{
+    const ela3d_config& c = p.cfg;
+    const PmlAxis& A = p.ax[axis];
+    if (axis == 0) return size_t(c.nz) * c.ny * A.width();
+    if (axis == 1) return size_t(c.nz) * A.width() * c.nx;
+    return size_t(A.width()) * c.ny * c.nx;
+}
+
+// Byte offsets of every buffer in a shot's single device allocation.  Each
+// buffer starts on a 256-byte boundary; the energy-partial buffer has room for
+// any launch shape the tuner may pick (block >= 64, vec >= 1, zseg >= 1).
+ArenaLayout arena_layout(const ShotPlan& p, int elem, int nepart)
+{
+    const ela3d_config& c = p.cfg;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    ArenaLayout L;
+    const int64_t nxyp = plane_stride(c.nx, c.ny);
+    const size_t padded = size_t(nxyp) * c.nz + 2 * size_t(guard_elems(c.nx));  // guard | planes | guard
+    size_t off = 0;
+    for (int q = 0; q < 9; ++q) { L.f[q] = off; off += al(padded * elem); }
+    for (int q = 0; q < 18; ++q) { L.m[q] = off; off += al(mem_elems(p, kMemAxis[q]) * elem); }
+    for (int q = 0; q < 3; ++q) { L.model[q] = off; off += al(padded * sizeof(float)); }
+    const int ns[3] = {c.nx, c.ny, c.nz};
+    for (int q = 0; q < 3; ++q) { L.tab[q] = off; off += al(6 * size_t(ns[q]) * sizeof(double)); }
+    L.wav = off; off += al(std::max<size_t>(1, p.wavelet.size()) * sizeof(double));
+    L.rec = off; off += al(std::max<size_t>(1, p.rec.size()) * sizeof(int));
+    const size_t nseis = size_t(p.nsamples) * c.nrec * 3;
+    L.seis = off; off += al(std::max<size_t>(1, nseis) * sizeof(double));
+    L.energy = off; off += al(std::max<size_t>(1, size_t(p.nenergy) * 2) * sizeof(double));
+    L.epart_cap = int((nxyp / 64 + kMaxBox + 1) * int64_t(c.nz) + 8);
+    L.epart = off; off += al(size_t(std::max(L.epart_cap, nepart)) * 2 * sizeof(double));
+    L.ctl = off; off += al(2 * sizeof(int));
+    L.scl = off; off += al(kScaleTable * sizeof(float));
+    L.slot = off; off += al(size_t(kMaxSlots) * 27 * sizeof(unsigned));
+    L.total = off;
+    return L;
+}
+
+int64_t shot_device_bytes(const ShotPlan& p)
+{
+    return int64_t(arena_layout(p, storage_bytes(p.cfg.storage), 0).total);
+}
+
 template <typename T>
 class ShotImpl final : public Shot {
 public:
@@ -116,6 +160,7 @@ public:
         for (auto& g : graphs_)
             if (g) hipGraphExecDestroy(g);
         if (arena_) hipFree(arena_);
+        if (h_flag_) hipHostFree(h_flag_);
         if (own_stream_ && stream_) hipStreamDestroy(stream_);
     }
 
@@ -140,6 +185,40 @@ public:
         }
         HIP_CHECK(hipGetLastError());
     }
+
+    void step_plain(int n) override
+    {
+        const ela3d_config& c = plan_.cfg;
+        HIP_CHECK(hipSetDevice(device_));
+        std::vector<double> row(size_t(std::max(1, c.nrec)) * 3);
+        for (int s = 0; s < n; ++s) {
+            const int done = host_step_;
+            const int it = done + 1;
+            HIP_CHECK(hipMemcpyAsync(d_step_, &done, sizeof(int), hipMemcpyHostToDevice, stream_));
+            HIP_CHECK(hipStreamSynchronize(stream_));
+            launch_step(energy_step(it), nullptr);
+            HIP_CHECK(hipGetLastError());
+            if (c.nrec > 0 && it % c.sample_every == 0 && it / c.sample_every - 1 < plan_.nsamples) {
+                const size_t r = size_t(it / c.sample_every - 1) * c.nrec * 3;
+                HIP_CHECK(hipMemcpyAsync(row.data(), d_seis_ + r, size_t(c.nrec) * 3 * sizeof(double),
+                                         hipMemcpyDeviceToHost, stream_));
+                HIP_CHECK(hipStreamSynchronize(stream_));
+            }
+            host_step_ = it;
+        }
+        HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    void enqueue_flag_readback() override
+    {
+        if (!h_flag_) {
+            HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_flag_), sizeof(int), hipHostMallocDefault));
+            *h_flag_ = 0;
+        }
+        HIP_CHECK(hipMemcpyAsync(h_flag_, d_flag_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    }
+
+    bool unstable_seen() const override { return h_flag_ && *static_cast<volatile int*>(h_flag_) != 0; }
 
     int sync() override
     {
@@ -355,31 +434,18 @@ private:
     void allocate()
     {
         const ela3d_config& c = plan_.cfg;
-        const size_t cells = size_t(plan_.cells);
-        auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
         nxyp_ = plane_stride(c.nx, c.ny);
         guard_ = guard_elems(c.nx);
-        const size_t padded = size_t(nxyp_) * c.nz + 2 * size_t(guard_);  // guard | planes | guard
-        size_t off = 0;
-        size_t o_f[9], o_m[18], o_model[3], o_tab[3];
-        for (int q = 0; q < 9; ++q) { o_f[q] = off; off += al(padded * sizeof(T)); }
-        for (int q = 0; q < 18; ++q) { o_m[q] = off; off += al(mem_count(kMemAxis[q]) * sizeof(T)); }
-        for (int q = 0; q < 3; ++q) { o_model[q] = off; off += al(padded * sizeof(float)); }
-        (void)cells;
-        const int ns[3] = {c.nx, c.ny, c.nz};
-        for (int q = 0; q < 3; ++q) { o_tab[q] = off; off += al(6 * size_t(ns[q]) * sizeof(double)); }
-        const size_t o_wav = off; off += al(std::max<size_t>(1, plan_.wavelet.size()) * sizeof(double));
-        const size_t o_rec = off; off += al(std::max<size_t>(1, plan_.rec.size()) * sizeof(int));
-        const size_t nseis = size_t(plan_.nsamples) * c.nrec * 3;
-        const size_t o_seis = off; off += al(std::max<size_t>(1, nseis) * sizeof(double));
-        const size_t o_en = off; off += al(std::max<size_t>(1, size_t(plan_.nenergy) * 2) * sizeof(double));
         nepart_ = geo_.nblocks;
-        // room for any launch shape the tuner may pick (block >= 64, vec >= 1, zseg >= 1)
-        epart_cap_ = int((nxyp_ / 64 + kMaxBox + 1) * int64_t(c.nz) + 8);
-        const size_t o_ep = off; off += al(size_t(std::max(epart_cap_, nepart_)) * 2 * sizeof(double));
-        const size_t o_ctl = off; off += al(2 * sizeof(int));
-        const size_t o_scl = off; off += al(kScaleTable * sizeof(float));
-        const size_t o_slot = off; off += al(size_t(kMaxSlots) * 27 * sizeof(unsigned));
+        const ArenaLayout L = arena_layout(plan_, int(sizeof(T)), nepart_);
+        epart_cap_ = L.epart_cap;
+        const size_t off = L.total;
+        const size_t* o_f = L.f;
+        const size_t* o_m = L.m;
+        const size_t* o_model = L.model;
+        const size_t* o_tab = L.tab;
+        const size_t o_wav = L.wav, o_rec = L.rec, o_seis = L.seis, o_en = L.energy, o_ep = L.epart;
+        const size_t o_ctl = L.ctl, o_scl = L.scl, o_slot = L.slot;
         arena_bytes_ = int64_t(off);
         HIP_CHECK(hipMalloc(&arena_, off));
         char* b = static_cast<char*>(arena_);
@@ -402,7 +468,6 @@ private:
         HIP_CHECK(hipStreamSynchronize(stream_));
         const std::vector<float> ones(kScaleTable, 1.0f);
         HIP_CHECK(hipMemcpy(d_scl_, ones.data(), ones.size() * sizeof(float), hipMemcpyHostToDevice));
-        (void)o_seis;
     }
 
     // FP16 storage: scales before the first write, from the size each field can
@@ -596,6 +661,7 @@ private:
     int nepart_ = 0, epart_cap_ = 0;
     int64_t nxyp_ = 0, guard_ = 0;
     int host_step_ = 0;
+    int* h_flag_ = nullptr;  // pinned mirror of d_flag_ (serving)
     float* d_scl_ = nullptr;         // FP16 scale table (kScaleTable floats)
     unsigned* d_maxslot_ = nullptr;  // [kMaxSlots][27]
 

@@ -40,11 +40,26 @@ struct Tuning {
 int64_t plane_stride(int nx, int ny);
 int64_t guard_elems(int nx);
 
+struct ArenaLayout {
+    size_t f[9], m[18], model[3], tab[3];
+    size_t wav, rec, seis, energy, epart, ctl, scl, slot, total;
+    int epart_cap;
+};
+ArenaLayout arena_layout(const ShotPlan& p, int elem_bytes, int nepart);
+
 class Shot {
 public:
     virtual ~Shot() = default;
     virtual void enqueue_steps(int n) = 0;          // asynchronous
+    // n steps driven the way a plain host loop drives them: per step a blocking
+    // host-to-device copy of the step index, eager launches, and a blocking
+    // readback of that step's receiver samples (synchronous).
+    virtual void step_plain(int n) = 0;
     virtual int sync() = 0;                         // ELA3D_OK or ELA3D_E_UNSTABLE
+    // queue a copy of the instability flag into pinned host memory; after the
+    // stream has passed that point, unstable_seen() reports it without a sync
+    virtual void enqueue_flag_readback() = 0;
+    virtual bool unstable_seen() const = 0;
     virtual void seismograms(double* out) = 0;      // nsamples*nrec*3
     virtual void energy(double* out) = 0;           // nenergy*2
     virtual void snapshot(int field, double* out) = 0;

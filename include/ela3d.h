@@ -178,6 +178,53 @@ void ela3d_destroy(ela3d_shot* shot);
 int ela3d_plan(const ela3d_config* cfg, ela3d_info* out);
 int ela3d_pml_tables(const ela3d_config* cfg, int32_t axis, double* out, size_t cap);
 
+/* ---- serving: continuous batching of forward shots on one GPU (no reference
+ * counterpart; the reference runs one hard-coded shot per process) ----
+ * A device thread admits queued shots in arrival order while the device-memory
+ * budget and the active-shot cap allow, steps every admitted shot on its own
+ * stream in rounds of steps_per_round graph replays, and retires finished shots
+ * (results copied to host) so that later arrivals are admitted while earlier
+ * ones still run.  plain_loop = 1 instead runs one shot at a time the way the
+ * reference drives its kernels (ZC:1374-1494): eager launches, a blocking
+ * host-to-device copy of the step index and a blocking receiver readback every
+ * step.  submit/poll/wait are thread-safe. */
+typedef struct ela3d_server ela3d_server;
+typedef struct ela3d_server_opts {
+    uint32_t struct_size;      /* sizeof(ela3d_server_opts) */
+    int32_t device;
+    int64_t hbm_budget_bytes;  /* admission budget for shot device memory (default 200 GB) */
+    int32_t max_active;        /* shots resident and stepping at once (default 16) */
+    int32_t steps_per_round;   /* steps enqueued per shot between completion checks (default 50) */
+    int32_t use_graph;         /* replay a hipGraph per step (default 1) */
+    int32_t plain_loop;        /* 1: reference-pattern one-shot-at-a-time loop */
+    int32_t reserved[8];
+} ela3d_server_opts;
+typedef struct ela3d_result {
+    uint64_t ticket;
+    int32_t status;            /* ELA3D_OK or the failing ELA3D_E_* code */
+    int32_t nsamples, nrec, nenergy;
+    double t_submit, t_admit, t_done; /* seconds since the server started */
+    int64_t device_bytes;
+} ela3d_result;
+#define ELA3D_SHOT_QUEUED  0
+#define ELA3D_SHOT_RUNNING 1
+#define ELA3D_SHOT_DONE    2
+void ela3d_server_opts_default(ela3d_server_opts* o);
+int ela3d_server_create(const ela3d_server_opts* o, ela3d_server** out);
+/* Queue one shot; config and model are copied.  ELA3D_E_BUDGET if it can never fit. */
+int ela3d_submit(ela3d_server* s, const ela3d_config* cfg, const float* cp, const float* cs,
+                 const float* rho, uint64_t* ticket);
+/* state: ELA3D_SHOT_QUEUED / RUNNING / DONE, or a negative ELA3D_E_* code if it failed */
+int ela3d_poll(ela3d_server* s, uint64_t ticket, int32_t* state);
+/* Block until the shot is done (timeout_s < 0: forever); fills *res. */
+int ela3d_wait(ela3d_server* s, uint64_t ticket, double timeout_s, ela3d_result* res);
+int ela3d_result_seismograms(ela3d_server* s, uint64_t ticket, double* out, size_t cap);
+int ela3d_result_energy(ela3d_server* s, uint64_t ticket, double* out, size_t cap);
+int ela3d_result_release(ela3d_server* s, uint64_t ticket);
+/* out[8]: submitted, done, failed, active, queued, device bytes in use, peak device bytes, rounds */
+int ela3d_server_stats(ela3d_server* s, int64_t* out);
+void ela3d_server_destroy(ela3d_server* s);
+
 /* ---- launch-shape autotuner (no reference counterpart) ----
  * Times every candidate (threads per workgroup, z planes per workgroup in the
  * interior and in the absorbing layers) on a scratch shot of this grid and
