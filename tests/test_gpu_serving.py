@@ -116,7 +116,9 @@ def test_budget_rejection_and_tickets(gpu):
 def test_served_shot_vs_fp64_oracle(gpu):
     spec = [s for s in mixed_specs(12, seed=8) if s.layered][0]
     cfg, model = materialize(spec)
-    (seis, en, info), _ = serve([(cfg, model)])
+    out, _ = serve([(cfg, model)])
+    seis, en, info = out[0]
+    assert info["status"] == 0
     s64, e64, _ = oracle.run(cfg, model, dtype=np.float64, threads=16)
     err = rel_l2(seis, s64)
     print(f"served {cfg.nx}^3 layered shot: seismogram rel L2 vs FP64 {err:.3e}")
