@@ -107,6 +107,71 @@ static int64_t active_count(const std::vector<double>& a, int lo_i, int hi_i)
     return n;
 }
 
+// Bytes per cell of each pass with every field streamed once: stress reads
+// vx, vy, vz and cp, cs, rho and read-modify-writes the six stresses; velocity
+// reads the six stresses and rho and read-modify-writes vx, vy, vz.  The model
+// stays FP32 in every storage mode.
+static double stress_cell_bytes(const ela3d_config& c) { return 15.0 * storage_bytes(c.storage) + 12.0; }
+static double velocity_cell_bytes(const ela3d_config& c) { return 12.0 * storage_bytes(c.storage) + 4.0; }
+
+int split_boxes(const ShotPlan& p, BoxRange out[5])
+{
+    const ela3d_config& c = p.cfg;
+    const int nx = c.nx, nxy = c.nx * c.ny;
+    const int ylo = p.ax[1].lo, yhi = p.ax[1].hi, zlo = p.ax[2].lo, zhi = p.ax[2].hi;
+    int pa = (ylo * nx + 63) / 64 * 64, pb = (yhi * nx) / 64 * 64;
+    if (pb <= pa) pa = pb = nxy;  // too small for an interior box: all absorbing layer
+    const BoxRange all[5] = {{0, nxy, 0, zlo, 1}, {0, nxy, zhi, c.nz, 1}, {0, pa, zlo, zhi, 1},
+                             {pb, nxy, zlo, zhi, 1}, {pa, pb, zlo, zhi, 0}};
+    int n = 0;
+    for (const BoxRange& b : all)
+        if (b.p1 > b.p0 && b.k1 > b.k0) out[n++] = b;
+    return n;
+}
+
+// Memory variables with the table they use and the inclusive 0-based i/j/k
+// ranges of their update (CPU:378-587), per pass.
+double box_alg_bytes(const ShotPlan& P, const BoxRange& b, int pass)
+{
+    const ela3d_config& c = P.cfg;
+    const int nx = c.nx, ny = c.ny, nz = c.nz, ks = P.kmax_shear;
+    const double e = storage_bytes(c.storage);
+    const double cells = double(b.p1 - b.p0) * double(b.k1 - b.k0);
+    double bytes = (pass == 0 ? stress_cell_bytes(c) : velocity_cell_bytes(c)) * cells;
+    const PmlAxis &X = P.ax[0], &Y = P.ax[1], &Z = P.ax[2];
+    struct V { const std::vector<double>* a; int axis; int i0, i1, j0, j1, k0, k1; int pass; };
+    const V vars[18] = {
+        {&X.a_h, 0, 0, nx - 2, 1, ny - 1, 1, nz - 1, 0}, {&Y.a, 1, 0, nx - 2, 1, ny - 1, 1, nz - 1, 0},
+        {&Z.a, 2, 0, nx - 2, 1, ny - 1, 1, nz - 1, 0},   {&X.a, 0, 1, nx - 1, 0, ny - 2, 0, nz - 1, 0},
+        {&Y.a_h, 1, 1, nx - 1, 0, ny - 2, 0, nz - 1, 0}, {&X.a, 0, 1, nx - 1, 0, ny - 1, 0, ks, 0},
+        {&Z.a_h, 2, 1, nx - 1, 0, ny - 1, 0, ks, 0},     {&Y.a_h, 1, 0, nx - 1, 0, ny - 2, 0, ks, 0},
+        {&Z.a_h, 2, 0, nx - 1, 0, ny - 2, 0, ks, 0},
+        {&X.a, 0, 1, nx - 1, 1, ny - 1, 1, nz - 1, 1},   {&Y.a, 1, 1, nx - 1, 1, ny - 1, 1, nz - 1, 1},
+        {&Z.a, 2, 1, nx - 1, 1, ny - 1, 1, nz - 1, 1},   {&X.a_h, 0, 0, nx - 2, 0, ny - 2, 1, nz - 1, 1},
+        {&Y.a_h, 1, 0, nx - 2, 0, ny - 2, 1, nz - 1, 1}, {&Z.a, 2, 0, nx - 2, 0, ny - 2, 1, nz - 1, 1},
+        {&X.a_h, 0, 0, nx - 2, 1, ny - 1, 0, nz - 2, 1}, {&Y.a, 1, 0, nx - 2, 1, ny - 1, 0, nz - 2, 1},
+        {&Z.a_h, 2, 0, nx - 2, 1, ny - 1, 0, nz - 2, 1},
+    };
+    // 2 accesses (read + write) per (memory variable, cell) where its table a != 0
+    int64_t count = 0;
+    for (const V& v : vars) {
+        if (v.pass != pass) continue;
+        const int k0 = std::max(v.k0, b.k0), k1 = std::min(v.k1, b.k1 - 1);
+        if (k1 < k0) continue;
+        int64_t kact = k1 - k0 + 1;
+        if (v.axis == 2) kact = active_count(*v.a, k0, k1);
+        if (kact == 0) continue;
+        for (int p = b.p0; p < b.p1; ++p) {
+            const int j = p / nx, i = p - j * nx;
+            if (i < v.i0 || i > v.i1 || j < v.j0 || j > v.j1) continue;
+            if (v.axis == 0 && (*v.a)[i] == 0.0) continue;
+            if (v.axis == 1 && (*v.a)[j] == 0.0) continue;
+            count += kact;
+        }
+    }
+    return bytes + 2.0 * e * double(count);
+}
+
 ShotPlan make_plan(const ela3d_config& in)
 {
     if (in.struct_size != sizeof(ela3d_config)) fail("config.struct_size does not match this library");
@@ -165,57 +230,19 @@ ShotPlan make_plan(const ela3d_config& in)
     P.nenergy = c.energy_every > 0 ? c.nstep / c.energy_every : 0;
     P.kmax_shear = (c.compat_flags & ELA3D_COMPAT_GHOST) ? c.nz - 1 : c.nz - 2;
 
-    // Algorithmic bytes of the two-pass schedule: every field streamed once per pass.
-    // stress: read vx,vy,vz + cp,cs,rho, RMW six sigma; velocity: read six sigma + rho,
-    // RMW vx,vy,vz.  Plus 2 accesses per (memory variable, cell) where its a != 0.
-    const double e = storage_bytes(c.storage);
-    const double mb = 4.0;  // model stays FP32
-    const PmlAxis &X = P.ax[0], &Y = P.ax[1], &Z = P.ax[2];
-    const int64_t nxy = int64_t(c.nx) * c.ny;
-    const int64_t int_cells = int64_t(Y.hi - Y.lo) * c.nx * (Z.hi - Z.lo);
-    const int64_t slab_cells = cells - int_cells;
-    const double stress_cell = 3 * e + 3 * mb + 12 * e;
-    const double vel_cell = 6 * e + mb + 6 * e;
-    // C-PML term per memory variable: ranges from CPU:378-587 translated to 0-based
-    const int nx = c.nx, ny = c.ny, nz = c.nz, ks = P.kmax_shear;
-    struct V { const std::vector<double>* a; int axis; int i0, i1, j0, j1, k0, k1; int pass; };
-    const V vars[18] = {
-        {&X.a_h, 0, 0, nx - 2, 1, ny - 1, 1, nz - 1, 0}, {&Y.a, 1, 0, nx - 2, 1, ny - 1, 1, nz - 1, 0},
-        {&Z.a, 2, 0, nx - 2, 1, ny - 1, 1, nz - 1, 0},   {&X.a, 0, 1, nx - 1, 0, ny - 2, 0, nz - 1, 0},
-        {&Y.a_h, 1, 1, nx - 1, 0, ny - 2, 0, nz - 1, 0}, {&X.a, 0, 1, nx - 1, 0, ny - 1, 0, ks, 0},
-        {&Z.a_h, 2, 1, nx - 1, 0, ny - 1, 0, ks, 0},     {&Y.a_h, 1, 0, nx - 1, 0, ny - 2, 0, ks, 0},
-        {&Z.a_h, 2, 0, nx - 1, 0, ny - 2, 0, ks, 0},
-        {&X.a, 0, 1, nx - 1, 1, ny - 1, 1, nz - 1, 1},   {&Y.a, 1, 1, nx - 1, 1, ny - 1, 1, nz - 1, 1},
-        {&Z.a, 2, 1, nx - 1, 1, ny - 1, 1, nz - 1, 1},   {&X.a_h, 0, 0, nx - 2, 0, ny - 2, 1, nz - 1, 1},
-        {&Y.a_h, 1, 0, nx - 2, 0, ny - 2, 1, nz - 1, 1}, {&Z.a, 2, 0, nx - 2, 0, ny - 2, 1, nz - 1, 1},
-        {&X.a_h, 0, 0, nx - 2, 1, ny - 1, 0, nz - 2, 1}, {&Y.a, 1, 0, nx - 2, 1, ny - 1, 0, nz - 2, 1},
-        {&Z.a_h, 2, 0, nx - 2, 1, ny - 1, 0, nz - 2, 1},
-    };
-    double pml_int[2] = {0, 0}, pml_slab[2] = {0, 0};
-    for (const V& v : vars) {
-        // count active cells; split between the interior box (y,z inside [lo,hi)) and slabs
-        const int64_t ni = v.i1 - v.i0 + 1, nj = v.j1 - v.j0 + 1, nk = v.k1 - v.k0 + 1;
-        int64_t tot = 0, inside = 0;
-        if (v.axis == 0) {
-            const int64_t act = active_count(*v.a, v.i0, v.i1);
-            tot = act * nj * nk;
-            const int64_t jj = std::max(0, std::min(v.j1, Y.hi - 1) - std::max(v.j0, Y.lo) + 1);
-            const int64_t kk = std::max(0, std::min(v.k1, Z.hi - 1) - std::max(v.k0, Z.lo) + 1);
-            inside = act * jj * kk;
-        } else if (v.axis == 1) {
-            tot = active_count(*v.a, v.j0, v.j1) * ni * nk;
+    BoxRange boxes[5];
+    const int nb = split_boxes(P, boxes);
+    for (int q = 0; q < nb; ++q) {
+        const double bs = box_alg_bytes(P, boxes[q], 0), bv = box_alg_bytes(P, boxes[q], 1);
+        if (boxes[q].mode == 0) {
+            P.alg_stress_interior += bs;
+            P.alg_velocity_interior += bv;
         } else {
-            tot = active_count(*v.a, v.k0, v.k1) * ni * nj;
+            P.alg_stress_slab += bs;
+            P.alg_velocity_slab += bv;
         }
-        pml_int[v.pass] += 2 * e * inside;
-        pml_slab[v.pass] += 2 * e * (tot - inside);
     }
-    (void)nxy;
-    P.alg_stress_interior = stress_cell * int_cells + pml_int[0];
-    P.alg_stress_slab = stress_cell * slab_cells + pml_slab[0];
-    P.alg_velocity_interior = vel_cell * int_cells + pml_int[1];
-    P.alg_velocity_slab = vel_cell * slab_cells + pml_slab[1];
-    P.alg_pml = pml_int[0] + pml_int[1] + pml_slab[0] + pml_slab[1];
+    P.alg_pml = P.alg_step() - (stress_cell_bytes(c) + velocity_cell_bytes(c)) * double(cells);
     return P;
 }
 

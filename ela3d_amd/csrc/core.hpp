@@ -39,7 +39,8 @@ struct ShotPlan {
     int nsamples = 0, nenergy = 0;
     int kmax_shear = 0;             // nz-2, or nz-1 with the ghost quirk
     double t0 = 0.0;
-    // algorithmic traffic (bytes) of the two-pass schedule
+    // algorithmic traffic (bytes) of the two-pass schedule, split by launch
+    // (interior box; absorbing-layer boxes)
     double alg_stress_interior = 0, alg_stress_slab = 0;
     double alg_velocity_interior = 0, alg_velocity_slab = 0;
     double alg_pml = 0;
@@ -47,6 +48,19 @@ struct ShotPlan {
         return alg_stress_interior + alg_stress_slab + alg_velocity_interior + alg_velocity_slab;
     }
 };
+
+// An xy range [p0,p1) of the flattened plane (p = i + j*nx) times planes
+// [k0,k1); mode 0 = interior (y and z outside the C-PML), 1 = absorbing layer.
+struct BoxRange {
+    int p0, p1, k0, k1, mode;
+};
+// The boxes of one pass: z-min and z-max absorbing layers over whole planes,
+// then the y-min and y-max absorbing layers between them, then the interior,
+// whose xy range is cut at multiples of 64 elements (rows between the y layer
+// edge and that cut go to the y-layer boxes).  Empty boxes are dropped.
+int split_boxes(const ShotPlan& p, BoxRange out[5]);
+// Algorithmic bytes of one pass (0 stress, 1 velocity) over one box.
+double box_alg_bytes(const ShotPlan& p, const BoxRange& b, int pass);
 
 int storage_bytes(int storage);
 // Validate and expand a config; throws Error.

@@ -16,8 +16,9 @@
 //    (chunk, z-segment) tiles, keeping in-plane halos inside one XCD's L2.
 //  * C-PML memory variables live in compact per-axis slabs.  Interior boxes test
 //    x per element only (y and z lie outside the C-PML by construction); the
-//    absorbing-layer boxes test all three axes.  One launch covers both.  The
-//    FP64 a/b/K tables of x and y are staged once per workgroup into LDS as FP32.
+//    absorbing-layer boxes test all three axes.  Each pass is two launches, one
+//    per box mode, each compiled for its mode.  The FP64 a/b/K tables of x (and
+//    of y in absorbing-layer launches) are staged per workgroup into LDS as FP32.
 //  * Arithmetic is FP32 (FP64 for the FP64 storage instantiation); energy
 //    accumulates in FP64.
 //
@@ -744,16 +745,16 @@ __device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B
     }
 }
 
+// Two launches per pass: one over the absorbing-layer boxes, one over the
+// interior box.  Each is compiled for its own mode, so the interior kernel
+// carries none of the y/z C-PML state and keeps a small register footprint.
 // ---------------------------------------------------------------------------
-// One launch per pass: the C-PML slab boxes come first in logical block order,
-// then the interior box; each block runs the body specialised for its box.
-// ---------------------------------------------------------------------------
-template <typename T, int V>
+template <typename T, int MODE, int V>
 __global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
 {
     extern __shared__ float tabs[];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const Box B = pick_box(A.box, A.nbox, lb);
+    const Box B = MODE == kInterior ? A.box[0] : pick_box(A.box, A.nbox, lb);
     Scales<T> S;
     load_scales(A, S, true);
     const TabP<T>*tx, *ty;
@@ -761,23 +762,20 @@ __global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
         tx = A.tab[0];
         ty = A.tab[1];
     } else {
-        stage_tables(A, tabs, B.mode == kSlab);
+        stage_tables(A, tabs, MODE == kSlab);
         tx = tabs;
         ty = tabs + 6 * A.nx;
     }
-    if (B.mode == kSlab)
-        stress_body<T, kSlab, V>(A, B, lb - B.blk0, tx, ty, S);
-    else
-        stress_body<T, kInterior, V>(A, B, lb - B.blk0, tx, ty, S);
+    stress_body<T, MODE, V>(A, B, lb - B.blk0, tx, ty, S);
     flush_maxima(A, S, 3, 18);
 }
 
-template <typename T, bool ENERGY, int V>
+template <typename T, int MODE, bool ENERGY, int V>
 __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
 {
     extern __shared__ float tabs[];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const Box B = pick_box(A.box, A.nbox, lb);
+    const Box B = MODE == kInterior ? A.box[0] : pick_box(A.box, A.nbox, lb);
     Scales<T> S;
     load_scales(A, S, false);
     const TabP<T>*tx, *ty;
@@ -785,15 +783,12 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
         tx = A.tab[0];
         ty = A.tab[1];
     } else {
-        stage_tables(A, tabs, B.mode == kSlab);
+        stage_tables(A, tabs, MODE == kSlab);
         tx = tabs;
         ty = tabs + 6 * A.nx;
     }
     double ek = 0.0, ep = 0.0;
-    if (B.mode == kSlab)
-        velocity_body<T, kSlab, ENERGY, V>(A, B, lb - B.blk0, tx, ty, ek, ep, S);
-    else
-        velocity_body<T, kInterior, ENERGY, V>(A, B, lb - B.blk0, tx, ty, ek, ep, S);
+    velocity_body<T, MODE, ENERGY, V>(A, B, lb - B.blk0, tx, ty, ek, ep, S);
     flush_maxima(A, S, 0, 3);
     flush_maxima(A, S, 18, 27);
     if (ENERGY) {  // per-block FP64 partials, summed in block order by the tail kernel
@@ -820,54 +815,68 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
 }
 
 template <typename T>
-static size_t table_lds(int nx, int ny)
+static size_t table_lds(const StepArgs<T>& a)
 {
-    return std::is_same<T, double>::value ? 0 : size_t(6) * (nx + ny) * sizeof(float);
+    if (std::is_same<T, double>::value) return 0;
+    return size_t(6) * (a.nx + (a.mode == kSlab ? a.ny : 0)) * sizeof(float);
 }
 
-template <typename T, int V>
+template <typename T, int M, int V>
 static void launch_stress_v(const StepArgs<T>& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_stress<T, V>), dim3(a.nblocks), dim3(a.block), table_lds<T>(a.nx, a.ny), s, a);
+    hipLaunchKernelGGL((k_stress<T, M, V>), dim3(a.nblocks), dim3(a.block), table_lds<T>(a), s, a);
 }
 
-template <typename T, bool E, int V>
+template <typename T, int M, bool E, int V>
 static void launch_velocity_v(const StepArgs<T>& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_velocity<T, E, V>), dim3(a.nblocks), dim3(a.block), table_lds<T>(a.nx, a.ny), s, a);
+    hipLaunchKernelGGL((k_velocity<T, M, E, V>), dim3(a.nblocks), dim3(a.block), table_lds<T>(a), s, a);
 }
 
+// interior launches: VEC 1, 2 or 4; absorbing-layer launches: VEC 1 or 2
 template <typename T>
 void launch_stress(const StepArgs<T>& a, hipStream_t s)
 {
     if (a.nblocks <= 0) return;
-    if (a.vec == 4)
-        launch_stress_v<T, 4>(a, s);
-    else if (a.vec == 2)
-        launch_stress_v<T, 2>(a, s);
-    else
-        launch_stress_v<T, 1>(a, s);
+    if (a.mode == kSlab) {
+        if (a.vec == 2)
+            launch_stress_v<T, kSlab, 2>(a, s);
+        else
+            launch_stress_v<T, kSlab, 1>(a, s);
+    } else if (a.vec == 4) {
+        launch_stress_v<T, kInterior, 4>(a, s);
+    } else if (a.vec == 2) {
+        launch_stress_v<T, kInterior, 2>(a, s);
+    } else {
+        launch_stress_v<T, kInterior, 1>(a, s);
+    }
+}
+
+template <typename T, bool E>
+static void launch_velocity_e(const StepArgs<T>& a, hipStream_t s)
+{
+    if (a.mode == kSlab) {
+        if (a.vec == 2)
+            launch_velocity_v<T, kSlab, E, 2>(a, s);
+        else
+            launch_velocity_v<T, kSlab, E, 1>(a, s);
+    } else if (a.vec == 4) {
+        launch_velocity_v<T, kInterior, E, 4>(a, s);
+    } else if (a.vec == 2) {
+        launch_velocity_v<T, kInterior, E, 2>(a, s);
+    } else {
+        launch_velocity_v<T, kInterior, E, 1>(a, s);
+    }
 }
 
 template <typename T>
 void launch_velocity(const StepArgs<T>& a, bool energy, hipStream_t s)
 {
     if (a.nblocks <= 0) return;
-    if (energy) {
-        if (a.vec == 4)
-            launch_velocity_v<T, true, 4>(a, s);
-        else if (a.vec == 2)
-            launch_velocity_v<T, true, 2>(a, s);
-        else
-            launch_velocity_v<T, true, 1>(a, s);
-    } else {
-        if (a.vec == 4)
-            launch_velocity_v<T, false, 4>(a, s);
-        else if (a.vec == 2)
-            launch_velocity_v<T, false, 2>(a, s);
-        else
-            launch_velocity_v<T, false, 1>(a, s);
-    }
+    if (energy)
+        launch_velocity_e<T, true>(a, s);
+    else
+        launch_velocity_e<T, false>(a, s);
 }
 
 template void launch_stress<float>(const StepArgs<float>&, hipStream_t);

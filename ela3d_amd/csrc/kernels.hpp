@@ -55,6 +55,7 @@ struct StepArgs {
     Box box[kMaxBox];
     int nbox, nblocks, block;
     int vec;                 // consecutive xy positions per lane: 1, 2 or 4
+    int mode;                // kInterior or kSlab: every box of this launch has this mode
 };
 
 template <typename T>

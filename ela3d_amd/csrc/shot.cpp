@@ -29,7 +29,9 @@ Tuning choose_tuning(const ShotPlan& p)
     const ela3d_config& c = p.cfg;
     if (c.block == 0 && c.zseg == 0 && cached_tuning(c, t)) return t;
     t.block = c.block > 0 ? c.block : 256;
-    t.vec = 2;
+    t.block_slab = t.block;
+    t.vec = 1;
+    t.vec_slab = 1;
     if (c.zseg > 0) {
         t.zseg_interior = c.zseg;
         t.zseg_slab = c.zseg;
@@ -49,40 +51,39 @@ Tuning choose_tuning(const ShotPlan& p)
 int64_t plane_stride(int nx, int ny) { return (int64_t(nx) * ny + 63) / 64 * 64; }
 int64_t guard_elems(int nx) { return (int64_t(nx) + 64 + 63) / 64 * 64; }
 
-static void add_box(LaunchGeom& g, int p0, int p1, int k0, int k1, int zseg, int mode)
+static void add_box(LaunchGeom& g, const BoxRange& r, int zseg)
 {
-    if (p1 <= p0 || k1 <= k0) return;
     Box& b = g.box[g.nbox++];
     const int per = g.block * g.vec;
-    b.p0 = p0; b.p1 = p1; b.k0 = k0; b.k1 = k1;
-    b.nchunk = (p1 - p0 + per - 1) / per;
+    b.p0 = r.p0; b.p1 = r.p1; b.k0 = r.k0; b.k1 = r.k1;
+    b.nchunk = (r.p1 - r.p0 + per - 1) / per;
     b.zseg = std::max(1, zseg);
     b.blk0 = g.nblocks;
-    b.mode = mode;
-    g.nblocks += b.nchunk * ((k1 - k0 + b.zseg - 1) / b.zseg);
+    b.mode = r.mode == 0 ? kInterior : kSlab;
+    g.nblocks += b.nchunk * ((r.k1 - r.k0 + b.zseg - 1) / b.zseg);
 }
 
-// One launch per pass.  The four absorbing-layer boxes come first in logical
-// block order so their slower blocks start early; the interior box follows.
-// Box starts are multiples of 64 elements, so with the padded plane stride
-// every workgroup tile begins on a 256-byte line boundary and VEC groups are
-// aligned; the rows between the y layers and those boundaries go to the slabs.
-void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& g)
+// Two launches per pass.  The absorbing-layer launch covers the z-min, z-max,
+// y-min and y-max boxes; the interior launch the interior box.  Box starts are
+// multiples of 64 elements, so with the padded plane stride every workgroup
+// tile begins on a 256-byte line boundary and VEC groups are aligned.
+void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& slab, LaunchGeom& interior)
 {
-    const ela3d_config& c = p.cfg;
-    const int nx = c.nx, nxy = c.nx * c.ny;
-    const int ylo = p.ax[1].lo, yhi = p.ax[1].hi, zlo = p.ax[2].lo, zhi = p.ax[2].hi;
-    int pa = (ylo * nx + 63) / 64 * 64, pb = (yhi * nx) / 64 * 64;
-    if (pb <= pa) pa = pb = nxy;  // too small for an interior box: all slab
-    g = LaunchGeom();
-    g.block = t.block;
-    g.vec = t.vec;
-    const int zs = t.zseg_slab, zi = t.zseg_interior;
-    add_box(g, 0, nxy, 0, zlo, zs, kSlab);                 // z-min absorbing layer
-    add_box(g, 0, nxy, zhi, c.nz, zs, kSlab);              // z-max absorbing layer
-    add_box(g, 0, pa, zlo, zhi, zi, kSlab);                // y-min absorbing layer
-    add_box(g, pb, nxy, zlo, zhi, zi, kSlab);              // y-max absorbing layer
-    add_box(g, pa, pb, zlo, zhi, zi, kInterior);
+    slab = LaunchGeom();
+    interior = LaunchGeom();
+    slab.block = t.block_slab;
+    slab.vec = t.vec_slab;
+    interior.block = t.block;
+    interior.vec = t.vec;
+    BoxRange r[5];
+    const int n = split_boxes(p, r);
+    for (int q = 0; q < n; ++q) {
+        const bool zlayer = r[q].k1 <= p.ax[2].lo || r[q].k0 >= p.ax[2].hi;
+        if (r[q].mode == 0)
+            add_box(interior, r[q], t.zseg_interior);
+        else  // z layers march in short segments; the y layers as the interior
+            add_box(slab, r[q], zlayer ? t.zseg_slab : t.zseg_interior);
+    }
 }
 
 static size_t mem_elems(const ShotPlan& p, int axis)
@@ -146,11 +147,11 @@ public:
             own_stream_ = true;
         }
         tune_ = choose_tuning(plan_);
-        geometry(plan_, tune_, geo_);
+        geometry(plan_, tune_, geo_[1], geo_[0]);
         allocate();
         upload(cp, cs, rho);
         initial_scales(cp, rho);
-        args_ = step_args(geo_, d_epart_);
+        set_args();
     }
 
     ~ShotImpl() override
@@ -300,16 +301,19 @@ public:
     void profile(int n, std::vector<ela3d_kernel_stats>& st, double& wall_ms) override
     {
         HIP_CHECK(hipSetDevice(device_));
-        static const char* names[kLaunches] = {"stress", "velocity", "tail"};
+        static const char* names[kLaunches] = {"stress_cpml", "stress_interior", "velocity_cpml",
+                                               "velocity_interior", "tail"};
         st.assign(kLaunches, ela3d_kernel_stats());
         for (int q = 0; q < kLaunches; ++q) {
             std::snprintf(st[q].name, sizeof(st[q].name), "%s", names[q]);
             st[q].launches = 0;
             st[q].total_ms = 0.0;
         }
-        st[0].alg_bytes_per_launch = plan_.alg_stress_interior + plan_.alg_stress_slab;
-        st[1].alg_bytes_per_launch = plan_.alg_velocity_interior + plan_.alg_velocity_slab;
-        st[2].alg_bytes_per_launch = 0;
+        st[0].alg_bytes_per_launch = plan_.alg_stress_slab;
+        st[1].alg_bytes_per_launch = plan_.alg_stress_interior;
+        st[2].alg_bytes_per_launch = plan_.alg_velocity_slab;
+        st[3].alg_bytes_per_launch = plan_.alg_velocity_interior;
+        st[4].alg_bytes_per_launch = 0;
         const int chunk = 64;
         const int E = kLaunches + 1;
         std::vector<hipEvent_t> ev(size_t(chunk) * E);
@@ -352,13 +356,15 @@ public:
     {
         HIP_CHECK(hipSetDevice(device_));
         HIP_CHECK(hipStreamSynchronize(stream_));
-        LaunchGeom g;
-        geometry(plan_, t, g);
-        if (g.nblocks > epart_cap_) throw Error{ELA3D_E_INVALID, "launch shape needs too many blocks"};
+        LaunchGeom g[2];
+        geometry(plan_, t, g[1], g[0]);
+        if (g[0].nblocks + g[1].nblocks > epart_cap_)
+            throw Error{ELA3D_E_INVALID, "launch shape needs too many blocks"};
         tune_ = t;
-        geo_ = g;
-        nepart_ = g.nblocks;
-        args_ = step_args(geo_, d_epart_);
+        geo_[0] = g[0];
+        geo_[1] = g[1];
+        nepart_ = g[0].nblocks + g[1].nblocks;
+        set_args();
         for (auto& x : graphs_)
             if (x) {
                 hipGraphExecDestroy(x);
@@ -382,6 +388,8 @@ public:
         o.zseg = tune_.zseg_interior;
         o.zseg_slab = tune_.zseg_slab;
         o.vec = tune_.vec;
+        o.block_slab = tune_.block_slab;
+        o.vec_slab = tune_.vec_slab;
         for (int a = 0; a < 3; ++a) {
             o.region_lo[a] = plan_.ax[a].lo;
             o.region_hi[a] = plan_.ax[a].hi;
@@ -436,7 +444,7 @@ private:
         const ela3d_config& c = plan_.cfg;
         nxyp_ = plane_stride(c.nx, c.ny);
         guard_ = guard_elems(c.nx);
-        nepart_ = geo_.nblocks;
+        nepart_ = geo_[0].nblocks + geo_[1].nblocks;
         const ArenaLayout L = arena_layout(plan_, int(sizeof(T)), nepart_);
         epart_cap_ = L.epart_cap;
         const size_t off = L.total;
@@ -577,7 +585,15 @@ private:
         a.nblocks = g.nblocks;
         a.vec = g.vec;
         a.block = g.block;
+        a.mode = g.nbox > 0 ? g.box[0].mode : kInterior;
         return a;
+    }
+
+    // absorbing-layer launch first: its energy partials come first in block order
+    void set_args()
+    {
+        args_[1] = step_args(geo_[1], d_epart_);
+        args_[0] = step_args(geo_[0], d_epart_ + 2 * size_t(geo_[1].nblocks));
     }
 
     TailArgs<T> tail_args() const
@@ -622,12 +638,16 @@ private:
     void launch_step(bool energy, hipEvent_t* ev)
     {
         if (ev) HIP_CHECK(hipEventRecord(ev[0], stream_));
-        launch_stress(args_, stream_);
+        launch_stress(args_[1], stream_);
         if (ev) HIP_CHECK(hipEventRecord(ev[1], stream_));
-        launch_velocity(args_, energy, stream_);
+        launch_stress(args_[0], stream_);
         if (ev) HIP_CHECK(hipEventRecord(ev[2], stream_));
-        launch_tail(tail_args(), stream_);
+        launch_velocity(args_[1], energy, stream_);
         if (ev) HIP_CHECK(hipEventRecord(ev[3], stream_));
+        launch_velocity(args_[0], energy, stream_);
+        if (ev) HIP_CHECK(hipEventRecord(ev[4], stream_));
+        launch_tail(tail_args(), stream_);
+        if (ev) HIP_CHECK(hipEventRecord(ev[5], stream_));
     }
 
     hipGraphExec_t capture(bool energy)
@@ -647,9 +667,10 @@ private:
     hipStream_t stream_ = nullptr;
     bool own_stream_ = true;
     Tuning tune_;
-    static constexpr int kLaunches = 3;  // stress, velocity, tail
-    LaunchGeom geo_;
-    StepArgs<T> args_;
+    // stress (absorbing layers, interior), velocity (same), tail
+    static constexpr int kLaunches = 5;
+    LaunchGeom geo_[2];   // [0] interior launch, [1] absorbing-layer launch
+    StepArgs<T> args_[2];
     void* arena_ = nullptr;
     int64_t arena_bytes_ = 0;
     T* d_f_[9] = {};

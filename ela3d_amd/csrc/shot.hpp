@@ -21,18 +21,21 @@ namespace ela3d {
                                  std::string(#expr) + ": " + hipGetErrorString(e_)};      \
     } while (0)
 
-// Launch shape shared by the stress and the velocity launch.
+// Shape of one launch (the stress and velocity passes share it).
 struct LaunchGeom {
     Box box[kMaxBox];
     int nbox = 0, nblocks = 0, block = 256, vec = 1;
 };
 
-// Launch tuning chosen for a grid (see tune.cpp).
+// Launch tuning chosen for a grid (see tune.cpp): the interior launch and the
+// absorbing-layer launch of each pass are shaped independently.
 struct Tuning {
-    int block = 256;
+    int block = 256;        // interior launch
     int zseg_interior = 16;
+    int vec = 2;            // consecutive xy positions per lane (1, 2, 4)
+    int block_slab = 256;   // absorbing-layer launch
     int zseg_slab = 8;
-    int vec = 2;  // consecutive xy positions per lane
+    int vec_slab = 1;       // 1 or 2
 };
 
 // Device layout of the 9 fields and the 3 model arrays: planes of plane_stride
@@ -81,6 +84,7 @@ std::unique_ptr<Shot> make_shot(ShotPlan plan, const float* cp, const float* cs,
 Tuning choose_tuning(const ShotPlan& p);
 bool cached_tuning(const ela3d_config& c, Tuning& out);
 void store_tuning(const ela3d_config& c, const Tuning& t, double ms);
-void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& g);
+// launch shapes of the absorbing-layer boxes (slab) and the interior box
+void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& slab, LaunchGeom& interior);
 
 }  // namespace ela3d

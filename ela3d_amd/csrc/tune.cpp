@@ -1,11 +1,13 @@
 // tune.cpp -- launch-shape autotuner and its process-wide cache.
 //
-// A launch shape is (threads per workgroup, z planes per workgroup in the
-// interior box, z planes per workgroup in the absorbing-layer boxes).  The
-// tuner times every candidate on a scratch shot of the same grid (source
-// amplitude 0, so the traffic is identical and nothing is perturbed), keeps
-// the fastest, and caches it keyed by (nx, ny, nz, storage, npoints_pml).
-// Shots created with block = 0 and zseg = 0 take the cached shape.
+// Each pass is two launches (interior box, absorbing-layer boxes) whose shapes
+// are independent: the tuner first times the interior candidates (threads per
+// workgroup x z planes per workgroup x xy positions per lane) on the interior
+// launches, then the absorbing-layer candidates on the absorbing-layer launches,
+// on a scratch shot of the same grid (source amplitude 0, so the traffic is
+// identical and nothing is perturbed).  The winner is cached keyed by
+// (nx, ny, nz, storage, npoints_pml); shots created with block = 0 and
+// zseg = 0 take the cached shape.
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -78,39 +80,68 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
         static const int zsi[] = {2, 4, 8, 16, 32};
         static const int zss[] = {2, 4, 8};
         static const int vecs[] = {1, 2, 4};
-        Tuning best;
-        double best_ms = 1e300;
+        static const int vecs_slab[] = {1, 2};
+        const bool verbose = std::getenv("ELA3D_TUNE_VERBOSE") != nullptr;
+        Tuning best = ela3d::choose_tuning(plan);
         int tried = 0;
         std::vector<ela3d_kernel_stats> st;
+        // launch times of the stress and velocity passes: [0] interior, [1] absorbing layers
+        auto measure = [&](const Tuning& t, double ms[2]) {
+            shot->retune(t);
+            double wall = 0.0;
+            shot->profile(1, st, wall);  // warm
+            shot->profile(steps, st, wall);
+            ms[0] = (st[1].total_ms + st[3].total_ms) / steps;
+            ms[1] = (st[0].total_ms + st[2].total_ms) / steps;
+            ++tried;
+        };
+        double ms[2], best_int = 1e300, best_slab = 1e300;
         for (int vv : vecs)
-        for (int b : blocks)
-            for (int zi : zsi)
-                for (int zs : zss) {
-                    Tuning t;
+            for (int b : blocks)
+                for (int zi : zsi) {
+                    Tuning t = best;
                     t.block = b;
                     t.zseg_interior = zi;
-                    t.zseg_slab = zs;
                     t.vec = vv;
-                    shot->retune(t);
-                    double wall = 0.0;
-                    shot->profile(1, st, wall);  // warm
-                    shot->profile(steps, st, wall);
-                    const double ms = (st[0].total_ms + st[1].total_ms) / steps;
-                    ++tried;
-                    if (std::getenv("ELA3D_TUNE_VERBOSE"))
-                        std::fprintf(stderr, "[tune] vec %d block %3d zseg %3d/%d: stress %.4f velocity %.4f ms\n",
-                                     vv, b, zi, zs, st[0].total_ms / steps, st[1].total_ms / steps);
-                    if (ms < best_ms) {
-                        best_ms = ms;
-                        best = t;
+                    measure(t, ms);
+                    if (verbose)
+                        std::fprintf(stderr, "[tune] interior vec %d block %3d zseg %3d: %.4f ms\n", vv, b, zi,
+                                     ms[0]);
+                    if (ms[0] < best_int) {
+                        best_int = ms[0];
+                        best.block = b;
+                        best.zseg_interior = zi;
+                        best.vec = vv;
                     }
                 }
+        for (int vv : vecs_slab)
+            for (int b : blocks)
+                for (int zs : zss) {
+                    Tuning t = best;
+                    t.block_slab = b;
+                    t.zseg_slab = zs;
+                    t.vec_slab = vv;
+                    measure(t, ms);
+                    if (verbose)
+                        std::fprintf(stderr, "[tune] layers   vec %d block %3d zseg %3d: %.4f ms\n", vv, b, zs,
+                                     ms[1]);
+                    if (ms[1] < best_slab) {
+                        best_slab = ms[1];
+                        best.block_slab = b;
+                        best.zseg_slab = zs;
+                        best.vec_slab = vv;
+                    }
+                }
+        measure(best, ms);
+        const double best_ms = ms[0] + ms[1];
         ela3d::store_tuning(*cfg, best, best_ms);
         if (out) {
             out->block = best.block;
             out->zseg_interior = best.zseg_interior;
             out->zseg_slab = best.zseg_slab;
             out->vec = best.vec;
+            out->block_slab = best.block_slab;
+            out->vec_slab = best.vec_slab;
             out->ms_per_step = best_ms;
             out->candidates = tried;
         }
@@ -127,13 +158,19 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
 int ela3d_tuning_set(const ela3d_config* cfg, const ela3d_tuning* t)
 {
     if (!cfg || !t || t->block <= 0 || t->zseg_interior <= 0 || t->zseg_slab <= 0) return ELA3D_E_INVALID;
-    if (t->block != 64 && t->block != 128 && t->block != 256) return ELA3D_E_INVALID;
+    auto ok_block = [](int b) { return b == 64 || b == 128 || b == 256; };
+    const int bs = t->block_slab ? t->block_slab : t->block;
+    const int vs = t->vec_slab ? t->vec_slab : 1;
+    if (!ok_block(t->block) || !ok_block(bs)) return ELA3D_E_INVALID;
+    if (t->vec != 1 && t->vec != 2 && t->vec != 4) return ELA3D_E_INVALID;
+    if (vs != 1 && vs != 2) return ELA3D_E_INVALID;
     Tuning u;
     u.block = t->block;
     u.zseg_interior = t->zseg_interior;
-    u.zseg_slab = t->zseg_slab;
-    if (t->vec != 1 && t->vec != 2 && t->vec != 4) return ELA3D_E_INVALID;
     u.vec = t->vec;
+    u.block_slab = bs;
+    u.zseg_slab = t->zseg_slab;
+    u.vec_slab = vs;
     ela3d::store_tuning(*cfg, u, t->ms_per_step);
     return ELA3D_OK;
 }
@@ -147,6 +184,8 @@ int ela3d_tuning_get(const ela3d_config* cfg, ela3d_tuning* t)
     t->zseg_interior = u.zseg_interior;
     t->zseg_slab = u.zseg_slab;
     t->vec = u.vec;
+    t->block_slab = u.block_slab;
+    t->vec_slab = u.vec_slab;
     t->ms_per_step = 0.0;
     t->candidates = 0;
     return ELA3D_OK;

@@ -127,15 +127,16 @@ typedef struct ela3d_info {
     int64_t cells;                 /* nx*ny*nz */
     int64_t device_bytes;          /* device memory held by the shot */
     double alg_bytes_per_step;     /* algorithmic HBM bytes of one step (two-pass schedule) */
-    double alg_bytes_stress;       /* per stress launch (interior + C-PML slabs) */
-    double alg_bytes_velocity;     /* per velocity launch (interior + C-PML slabs) */
+    double alg_bytes_stress;       /* stress pass (interior + absorbing-layer launches) */
+    double alg_bytes_velocity;     /* velocity pass (interior + absorbing-layer launches) */
     double pml_bytes_per_step;     /* the C-PML memory-variable share of alg_bytes_per_step */
     int32_t nsamples;              /* seismogram samples over nstep */
     int32_t nenergy;               /* energy samples over nstep */
     int32_t steps_done;
-    int32_t block, zseg;           /* launch shape in use: threads, interior z planes per workgroup */
-    int32_t zseg_slab, vec;        /* absorbing-layer z planes per workgroup, xy positions per lane */
+    int32_t block, zseg;           /* interior launch: threads, z planes per workgroup */
+    int32_t zseg_slab, vec;        /* absorbing-layer z planes per workgroup; interior xy positions per lane */
     int32_t region_lo[3], region_hi[3]; /* C-PML compact regions: [0,lo) u [hi,n) per axis */
+    int32_t block_slab, vec_slab;  /* absorbing-layer launch: threads, xy positions per lane */
 } ela3d_info;
 
 #define ELA3D_MAX_KERNELS 8
@@ -226,15 +227,18 @@ int ela3d_server_stats(ela3d_server* s, int64_t* out);
 void ela3d_server_destroy(ela3d_server* s);
 
 /* ---- launch-shape autotuner (no reference counterpart) ----
- * Times every candidate (threads per workgroup, z planes per workgroup in the
- * interior and in the absorbing layers) on a scratch shot of this grid and
- * caches the fastest for (nx, ny, nz, storage, npoints_pml); shots created with
- * block = 0 and zseg = 0 use the cached shape. */
+ * Each pass is two launches: the interior box and the absorbing-layer boxes.
+ * The tuner times the interior candidates (threads per workgroup, z planes per
+ * workgroup, xy positions per lane), then the absorbing-layer candidates, on a
+ * scratch shot of this grid and caches the fastest for (nx, ny, nz, storage,
+ * npoints_pml); shots created with block = 0 and zseg = 0 use the cached shape. */
 typedef struct ela3d_tuning {
     int32_t block, zseg_interior, zseg_slab;
-    int32_t vec;          /* consecutive xy positions per lane: 1, 2 or 4 */
+    int32_t vec;          /* interior: consecutive xy positions per lane, 1, 2 or 4 */
     int32_t candidates;
     double ms_per_step;   /* stress + velocity launches */
+    int32_t block_slab;   /* absorbing-layer launch: threads per workgroup (0 = block) */
+    int32_t vec_slab;     /* absorbing-layer launch: xy positions per lane, 1 or 2 (0 = 1) */
 } ela3d_tuning;
 int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, const float* rho,
                    int32_t steps, ela3d_tuning* out);

@@ -25,7 +25,12 @@ def guard(nx):
     return (nx + 64 + 63) // 64 * 64
 
 
-def violations(cfg, block, vec, zi, zs, base_at_guard=False):
+def violations(cfg, block, vec, zi, zs, base_at_guard=False, block_slab=None, vec_slab=None):
+    """Out-of-bounds accesses of the interior launch (block, vec, zseg zi) and the
+    absorbing-layer launch (block_slab, vec_slab; zseg zs on the z layers, zi on
+    the y layers) over one step."""
+    block_slab = block if block_slab is None else block_slab
+    vec_slab = min(vec, 2) if vec_slab is None else vec_slab
     p = cfg.plan()
     nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
     nxy, nxyp, G = nx * ny, plane_stride(nx, ny), guard(nx)
@@ -42,24 +47,26 @@ def violations(cfg, block, vec, zi, zs, base_at_guard=False):
                                       (pa, pb, lo[2], hi[2], zi, 0)):
         if p1 > p0 and k1 > k0:
             boxes.append((p0, p1, k0, k1, z, mode))
+    vec_i = vec
     bad = []
     # base_at_guard models pointers placed at plane 0 (the addressing that wrapped)
     cell = (lambda k, q: (k * nxyp + q) % M32) if base_at_guard else (lambda k, q: (G + k * nxyp + q) % M32)
     shift = G if base_at_guard else 0
 
-    def fld(o, tag):
+    def fld(o, tag, vec):
         for e in range(vec):
             if not (shift + (o + e) % M32 < alloc):
                 bad.append(tag)
 
-    per = block * vec
     for (p0b, p1b, k0, k1, z, mode) in boxes:
+        blk, vec = (block, vec_i) if mode == 0 else (block_slab, vec_slab)
+        per = blk * vec
         for chunk in range((p1b - p0b + per - 1) // per):
             for zsg in range((k1 - k0 + z - 1) // z):
                 kb = k0 + zsg * z
                 ke = min(kb + z, k1)
-                for t in range(block):
-                    p0 = p0b + (chunk * block + t) * vec
+                for t in range(blk):
+                    p0 = p0b + (chunk * blk + t) * vec
                     if p0 >= p1b:
                         continue
                     assert p0 % vec == 0
@@ -72,14 +79,14 @@ def violations(cfg, block, vec, zi, zs, base_at_guard=False):
                         if mode == 1 and act:
                             cy = j if j < lo[1] else (lo[1] + j - hi[1] if j >= hi[1] else -1)
                         els.append((i, j, act, cx, cy))
-                    fld(cell(kb, p0), "queue")
+                    fld(cell(kb, p0), "queue", vec)
                     if kb >= 1:
-                        fld((cell(kb, p0) - nxyp) % M32, "queue-1")
+                        fld((cell(kb, p0) - nxyp) % M32, "queue-1", vec)
                     for k in range(kb, ke):
                         o = cell(k, p0)
                         dU = 0 if k == nz - 1 else nxyp
                         for d in (0, dU, 1, -1, nx, -nx):
-                            fld((o + d) % M32, ("neighbour", k, p0, d))
+                            fld((o + d) % M32, ("neighbour", k, p0, d), vec)
                         cz = -1
                         if mode == 1:
                             cz = k if k < lo[2] else (lo[2] + k - hi[2] if k >= hi[2] else -1)
@@ -103,7 +110,8 @@ def test_every_access_in_bounds(shape, vec):
     for compat in (0, 7):
         cfg = small_cfg(nx=nx, ny=ny, nz=nz, npml=npml, compat=compat)
         for block, zi, zs in ((64, 2, 2), (256, 4, 4), (128, 32, 8)):
-            assert violations(cfg, block, vec, zi, zs) == []
+            for vs, bs in ((1, 64), (2, 256)):
+                assert violations(cfg, block, vec, zi, zs, block_slab=bs, vec_slab=vs) == []
 
 
 def test_cube_shape_in_bounds():
