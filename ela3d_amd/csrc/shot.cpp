@@ -146,6 +146,11 @@ public:
             HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
             own_stream_ = true;
         }
+        // second stream for the absorbing-layer launches, forked from and joined
+        // back into the main stream every pass (default priority, like the main one)
+        HIP_CHECK(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
+        for (auto& e : fork_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        for (auto& e : join_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         tune_ = choose_tuning(plan_);
         geometry(plan_, tune_, geo_[1], geo_[0]);
         allocate();
@@ -162,10 +167,25 @@ public:
             if (g) hipGraphExecDestroy(g);
         if (arena_) hipFree(arena_);
         if (h_flag_) hipHostFree(h_flag_);
+        if (aux_) {
+            hipStreamSynchronize(aux_);
+            hipStreamDestroy(aux_);
+        }
+        for (auto& e : fork_)
+            if (e) hipEventDestroy(e);
+        for (auto& e : join_)
+            if (e) hipEventDestroy(e);
         if (own_stream_ && stream_) hipStreamDestroy(stream_);
     }
 
     hipStream_t stream() const override { return stream_; }
+    void set_overlap(bool on) override
+    {
+        if (on == overlap_) return;
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        overlap_ = on;
+        drop_graphs();
+    }
     int steps_done() const override { return host_step_; }
     int64_t device_bytes() const override { return arena_bytes_; }
 
@@ -315,7 +335,7 @@ public:
         st[3].alg_bytes_per_launch = plan_.alg_velocity_interior;
         st[4].alg_bytes_per_launch = 0;
         const int chunk = 64;
-        const int E = kLaunches + 1;
+        const int E = 2 * kLaunches;  // start and end event of every launch
         std::vector<hipEvent_t> ev(size_t(chunk) * E);
         for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
         hipEvent_t t0, t1;
@@ -332,11 +352,11 @@ public:
                 host_step_ = it;
             }
             HIP_CHECK(hipGetLastError());
-            HIP_CHECK(hipEventSynchronize(ev[size_t(m - 1) * E + kLaunches]));
+            HIP_CHECK(hipEventSynchronize(ev[size_t(m - 1) * E + E - 1]));
             for (int s = 0; s < m; ++s)
                 for (int q = 0; q < kLaunches; ++q) {
                     float ms = 0.f;
-                    HIP_CHECK(hipEventElapsedTime(&ms, ev[size_t(s) * E + q], ev[size_t(s) * E + q + 1]));
+                    HIP_CHECK(hipEventElapsedTime(&ms, ev[size_t(s) * E + 2 * q], ev[size_t(s) * E + 2 * q + 1]));
                     st[q].total_ms += ms;
                     st[q].launches += 1;
                 }
@@ -365,6 +385,11 @@ public:
         geo_[1] = g[1];
         nepart_ = g[0].nblocks + g[1].nblocks;
         set_args();
+        drop_graphs();
+    }
+
+    void drop_graphs()
+    {
         for (auto& x : graphs_)
             if (x) {
                 hipGraphExecDestroy(x);
@@ -634,20 +659,44 @@ private:
     }
 
     // One time step: stress pass -> velocity pass -> tail (receivers, energy, source).
-    // With ev != nullptr, an event is recorded before the first and after every launch.
+    // Within a pass the absorbing-layer launch runs on the auxiliary stream, forked
+    // from the main stream, concurrently with the interior launch (they write
+    // disjoint cells and read only what the previous pass wrote); the main stream
+    // joins it before the next pass.  With overlap off both run on the main
+    // stream in sequence.  With ev != nullptr, events ev[2q], ev[2q+1] bracket
+    // launch q (stress layers, stress interior, velocity layers, velocity
+    // interior, tail) on the stream it runs on.
     void launch_step(bool energy, hipEvent_t* ev)
     {
-        if (ev) HIP_CHECK(hipEventRecord(ev[0], stream_));
-        launch_stress(args_[1], stream_);
-        if (ev) HIP_CHECK(hipEventRecord(ev[1], stream_));
-        launch_stress(args_[0], stream_);
-        if (ev) HIP_CHECK(hipEventRecord(ev[2], stream_));
-        launch_velocity(args_[1], energy, stream_);
-        if (ev) HIP_CHECK(hipEventRecord(ev[3], stream_));
-        launch_velocity(args_[0], energy, stream_);
-        if (ev) HIP_CHECK(hipEventRecord(ev[4], stream_));
+        auto mark = [&](int q, hipStream_t s) {
+            if (ev) HIP_CHECK(hipEventRecord(ev[q], s));
+        };
+        hipStream_t side = overlap_ ? aux_ : stream_;
+        for (int pass = 0; pass < 2; ++pass) {
+            if (overlap_) {
+                HIP_CHECK(hipEventRecord(fork_[pass], stream_));
+                HIP_CHECK(hipStreamWaitEvent(aux_, fork_[pass], 0));
+            }
+            mark(4 * pass + 0, side);
+            if (pass == 0)
+                launch_stress(args_[1], side);
+            else
+                launch_velocity(args_[1], energy, side);
+            mark(4 * pass + 1, side);
+            mark(4 * pass + 2, stream_);
+            if (pass == 0)
+                launch_stress(args_[0], stream_);
+            else
+                launch_velocity(args_[0], energy, stream_);
+            mark(4 * pass + 3, stream_);
+            if (overlap_) {
+                HIP_CHECK(hipEventRecord(join_[pass], aux_));
+                HIP_CHECK(hipStreamWaitEvent(stream_, join_[pass], 0));
+            }
+        }
+        mark(8, stream_);
         launch_tail(tail_args(), stream_);
-        if (ev) HIP_CHECK(hipEventRecord(ev[5], stream_));
+        mark(9, stream_);
     }
 
     hipGraphExec_t capture(bool energy)
@@ -666,6 +715,9 @@ private:
     int device_ = 0;
     hipStream_t stream_ = nullptr;
     bool own_stream_ = true;
+    hipStream_t aux_ = nullptr;       // absorbing-layer launches
+    hipEvent_t fork_[2] = {nullptr, nullptr}, join_[2] = {nullptr, nullptr};
+    bool overlap_ = true;
     Tuning tune_;
     // stress (absorbing layers, interior), velocity (same), tail
     static constexpr int kLaunches = 5;

@@ -71,6 +71,9 @@ public:
     virtual void info(ela3d_info& out) = 0;
     virtual void retune(const Tuning& t) = 0;       // new launch shape (drops captured graphs)
     virtual hipStream_t stream() const = 0;
+    // run the absorbing-layer launches concurrently with the interior ones (default)
+    // or in sequence on the main stream (drops captured graphs)
+    virtual void set_overlap(bool on) = 0;
     virtual int steps_done() const = 0;
     virtual int64_t device_bytes() const = 0;
     const ShotPlan& plan() const { return plan_; }

@@ -86,6 +86,7 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
         int tried = 0;
         std::vector<ela3d_kernel_stats> st;
         // launch times of the stress and velocity passes: [0] interior, [1] absorbing layers
+        shot->set_overlap(false);  // per-launch times of launches run one after another
         auto measure = [&](const Tuning& t, double ms[2]) {
             shot->retune(t);
             double wall = 0.0;
