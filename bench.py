@@ -297,7 +297,7 @@ def main():
         tuned = ela3d_amd.autotune(cfg, model, steps=6)
         log(f"[bench] autotune: interior block={tuned['block']} zseg={tuned['zseg_interior']} "
             f"vec={tuned['vec']}; layers block={tuned['block_slab']} zseg={tuned['zseg_slab']} "
-            f"vec={tuned['vec_slab']} ({tuned['ms_per_step']:.3f} ms/step over "
+            f"vec={tuned['vec_slab']} overlap={tuned['overlap']} ({tuned['ms_per_step']:.3f} ms/step over "
             f"{tuned['candidates']} shapes, {time.perf_counter() - t:.1f} s)")
     with Shot(cfg, *model) as shot:
         shot.step(W)
@@ -357,7 +357,7 @@ def main():
             "graph_ms_per_step": graph_ms,
             "launch_shape": {"block": info["block"], "zseg": info["zseg"], "vec": info["vec"],
                              "block_slab": info["block_slab"], "zseg_slab": info["zseg_slab"],
-                             "vec_slab": info["vec_slab"]},
+                             "vec_slab": info["vec_slab"], "overlap": info["overlap"]},
         },
         "roofline": {
             "bound": "hbm", "kernel": dom["name"], "achieved": round(achieved / 1e9, 2),

@@ -152,6 +152,7 @@ public:
         for (auto& e : fork_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         for (auto& e : join_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         tune_ = choose_tuning(plan_);
+        overlap_ = tune_.overlap != 0;
         geometry(plan_, tune_, geo_[1], geo_[0]);
         allocate();
         upload(cp, cs, rho);
@@ -184,6 +185,7 @@ public:
         if (on == overlap_) return;
         HIP_CHECK(hipStreamSynchronize(stream_));
         overlap_ = on;
+        tune_.overlap = on ? 1 : 0;
         drop_graphs();
     }
     int steps_done() const override { return host_step_; }
@@ -381,6 +383,7 @@ public:
         if (g[0].nblocks + g[1].nblocks > epart_cap_)
             throw Error{ELA3D_E_INVALID, "launch shape needs too many blocks"};
         tune_ = t;
+        overlap_ = t.overlap != 0;
         geo_[0] = g[0];
         geo_[1] = g[1];
         nepart_ = g[0].nblocks + g[1].nblocks;
@@ -415,6 +418,7 @@ public:
         o.vec = tune_.vec;
         o.block_slab = tune_.block_slab;
         o.vec_slab = tune_.vec_slab;
+        o.overlap = overlap_ ? 1 : 0;
         for (int a = 0; a < 3; ++a) {
             o.region_lo[a] = plan_.ax[a].lo;
             o.region_hi[a] = plan_.ax[a].hi;
@@ -717,7 +721,7 @@ private:
     bool own_stream_ = true;
     hipStream_t aux_ = nullptr;       // absorbing-layer launches
     hipEvent_t fork_[2] = {nullptr, nullptr}, join_[2] = {nullptr, nullptr};
-    bool overlap_ = true;
+    bool overlap_ = false;
     Tuning tune_;
     // stress (absorbing layers, interior), velocity (same), tail
     static constexpr int kLaunches = 5;

@@ -36,6 +36,7 @@ struct Tuning {
     int block_slab = 256;   // absorbing-layer launch
     int zseg_slab = 8;
     int vec_slab = 1;       // 1 or 2
+    int overlap = 0;        // 1: absorbing-layer launches concurrent with the interior ones
 };
 
 // Device layout of the 9 fields and the 3 model arrays: planes of plane_stride

@@ -133,6 +133,23 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
                         best.vec_slab = vv;
                     }
                 }
+        // concurrent absorbing-layer launches pay off when neither launch fills the GPU
+        double wall_seq = 0.0, wall_ovl = 0.0;
+        for (int ov = 0; ov < 2; ++ov) {
+            Tuning t = best;
+            t.overlap = ov;
+            shot->retune(t);
+            double wall = 0.0;
+            shot->profile(1, st, wall);
+            shot->profile(steps, st, wall);
+            (ov ? wall_ovl : wall_seq) = wall / steps;
+            ++tried;
+        }
+        best.overlap = wall_ovl < 0.97 * wall_seq ? 1 : 0;
+        if (verbose)
+            std::fprintf(stderr, "[tune] step wall: in sequence %.4f ms, overlapped %.4f ms\n", wall_seq, wall_ovl);
+        shot->retune(best);
+        shot->set_overlap(false);
         measure(best, ms);
         const double best_ms = ms[0] + ms[1];
         ela3d::store_tuning(*cfg, best, best_ms);
@@ -143,6 +160,7 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
             out->vec = best.vec;
             out->block_slab = best.block_slab;
             out->vec_slab = best.vec_slab;
+            out->overlap = best.overlap;
             out->ms_per_step = best_ms;
             out->candidates = tried;
         }
@@ -172,6 +190,7 @@ int ela3d_tuning_set(const ela3d_config* cfg, const ela3d_tuning* t)
     u.block_slab = bs;
     u.zseg_slab = t->zseg_slab;
     u.vec_slab = vs;
+    u.overlap = t->overlap ? 1 : 0;
     ela3d::store_tuning(*cfg, u, t->ms_per_step);
     return ELA3D_OK;
 }
@@ -187,6 +206,7 @@ int ela3d_tuning_get(const ela3d_config* cfg, ela3d_tuning* t)
     t->vec = u.vec;
     t->block_slab = u.block_slab;
     t->vec_slab = u.vec_slab;
+    t->overlap = u.overlap;
     t->ms_per_step = 0.0;
     t->candidates = 0;
     return ELA3D_OK;

@@ -98,6 +98,36 @@ def test_graph_and_eager_identical(gpu):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("storage", [STORAGE_FP32, STORAGE_FP64])
+def test_launch_shapes_agree(gpu, storage):
+    """Every launch shape computes the same cells: seismograms and fields are
+    bit-identical with the absorbing-layer launches overlapped or in sequence,
+    and agree to rounding across thread counts, z-segments and VEC widths (the
+    energy partial sums are grouped by workgroup, so energy agrees to 1e-12)."""
+    from ela3d_amd import set_tuning
+    cfg = small_cfg(nx=45, ny=38, nz=33, nstep=60, storage=storage)
+    model = hetero_model(cfg, seed=5)
+    shapes = [(256, 4, 1, 256, 4, 1, False), (256, 4, 1, 256, 4, 1, True),
+              (64, 2, 2, 128, 2, 2, True), (128, 8, 4, 64, 8, 1, False), (64, 32, 4, 256, 2, 2, True)]
+    out = []
+    for (b, zi, v, bs, zs, vs, ov) in shapes:
+        set_tuning(cfg, block=b, zseg_interior=zi, vec=v, block_slab=bs, zseg_slab=zs, vec_slab=vs,
+                   overlap=ov)
+        with Shot(cfg, *model) as shot:
+            inf = shot.info()
+            assert (inf["block"], inf["vec"], inf["vec_slab"], inf["overlap"]) == (b, v, vs, int(ov))
+            shot.run()
+            out.append((shot.seismograms(), shot.energy(), shot.snapshot(5), shot.snapshot(20)))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2]) and np.array_equal(out[0][3], out[1][3])
+    tol = 1e-6 if storage == STORAGE_FP32 else 1e-13
+    for o in out[2:]:
+        assert rel_l2(o[0], out[0][0]) <= tol
+        assert rel_l2(o[2], out[0][2]) <= tol and rel_l2(o[3], out[0][3]) <= tol
+        assert np.allclose(o[1], out[0][1], rtol=1e-12 if storage == STORAGE_FP64 else 1e-6)
+    set_tuning(cfg, block=256, zseg_interior=4, vec=1, block_slab=256, zseg_slab=4, vec_slab=1)
+
+
 def test_zero_source_stays_zero(gpu):
     cfg = small_cfg(factor=0.0, nstep=20)
     with Shot(cfg, *homo(cfg)) as shot:
