@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 from dataclasses import dataclass, field, replace
 from typing import Optional
 
@@ -24,7 +25,8 @@ __all__ = [
     "copy_bench", "SRC_REFERENCE_SIGMA", "SRC_FORCE", "SRC_EXPLOSIVE", "SRC_FORCE3",
     "WAVELET_GAUSSIAN", "WAVELET_DGAUSSIAN", "WAVELET_RICKER", "STORAGE_FP32", "STORAGE_FP16",
     "STORAGE_FP64", "COMPAT_XHALF_A", "COMPAT_ZHALF_MIN", "COMPAT_GHOST", "COMPAT_REFERENCE",
-    "FIELD_NAMES", "Server", "SHOT_QUEUED", "SHOT_RUNNING", "SHOT_DONE",
+    "FIELD_NAMES", "Server", "SHOT_QUEUED", "SHOT_RUNNING", "SHOT_DONE", "anticline_model",
+    "read_raw", "write_raw", "write_traces", "write_rechor",
 ]
 
 SRC_REFERENCE_SIGMA, SRC_FORCE, SRC_EXPLOSIVE, SRC_FORCE3 = 0, 1, 2, 3
@@ -235,6 +237,20 @@ class Shot:
         a = np.ascontiguousarray(np.asarray(values, dtype=np.float64))
         check(lib().ela3d_set_field(self._h, int(fid), _dptr(a), a.size))
 
+    def source_trace(self) -> np.ndarray:
+        """[nstep+1][3]: sigma_xx, sigma_yy, sigma_zz at the source after each injection."""
+        out = np.zeros((self.cfg.nstep + 1, 3), dtype=np.float64)
+        check(lib().ela3d_source_trace(self._h, _dptr(out), out.size))
+        return out
+
+    def checkpoint(self, path: str):
+        """Write the whole device state and step count to `path`."""
+        check(lib().ela3d_checkpoint_save(self._h, os.fsencode(path)))
+
+    def restore(self, path: str):
+        """Load a checkpoint written by a shot of the same config; continues bit-identically."""
+        check(lib().ela3d_checkpoint_load(self._h, os.fsencode(path)))
+
     def profile(self, nsteps: int):
         st = (KernelStats * 8)()
         nk = C.c_int32()
@@ -410,3 +426,41 @@ def layered_model(nx=401, ny=401, nz=301, h=10.0, seed=None, **kw):
 def courant_dt(h: float, vp_max: float, courant: float = 0.5) -> float:
     """dt = courant * h / (vp_max * sqrt(3)) for the 3-D staggered scheme."""
     return courant * h / (vp_max * math.sqrt(3.0))
+
+
+def anticline_model(n: int = 201):
+    """The MATLAB anticline model (generate_3d_model_anticline.m) on an n^3 grid."""
+    shape = (n, n, n)
+    cp = np.empty(shape, np.float32)
+    cs = np.empty(shape, np.float32)
+    rh = np.empty(shape, np.float32)
+    check(lib().ela3d_model_anticline(int(n), _fptr(cp), _fptr(cs), _fptr(rh)))
+    return cp, cs, rh
+
+
+def read_raw(path: str, shape, bytes_per_value: int = 8) -> np.ndarray:
+    """Raw x-fastest model file (float64 or float32) as a float32 array of `shape` (nz, ny, nx)."""
+    out = np.empty(tuple(shape), np.float32)
+    check(lib().ela3d_read_raw(os.fsencode(path), out.size, int(bytes_per_value), _fptr(out)))
+    return out
+
+
+def write_raw(path: str, values, bytes_per_value: int = 4):
+    a = np.ascontiguousarray(np.asarray(values, dtype=np.float32))
+    check(lib().ela3d_write_raw(os.fsencode(path), a.size, int(bytes_per_value), _fptr(a)))
+
+
+def write_traces(path: str, values, stride: int = 1, offset: int = 0, append: bool = False,
+                 final_newline: bool = True):
+    """Rows of values in the reference's text-trace layout (see ela3d_write_traces)."""
+    a = np.ascontiguousarray(np.asarray(values, dtype=np.float64))
+    rows = a.shape[0] if a.ndim > 0 else 1
+    cols = (a.size // max(rows, 1)) // stride
+    check(lib().ela3d_write_traces(os.fsencode(path), _dptr(a), rows, cols, int(stride), int(offset),
+                                   1 if append else 0, 1 if final_newline else 0))
+
+
+def write_rechor(directory: str, seismograms, append: bool = True):
+    """rechorvx.txt, rechorvy.txt, rechorvz.txt from seismograms [nsamples][nrec][3]."""
+    for comp, name in enumerate(("rechorvx.txt", "rechorvy.txt", "rechorvz.txt")):
+        write_traces(os.path.join(directory, name), seismograms, stride=3, offset=comp, append=append)

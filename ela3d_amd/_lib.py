@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libela3d.so")
 ABI_VERSION = 1
 E_NAMES = {
     0: "OK", -1: "E_INVALID", -2: "E_OOM", -3: "E_UNSTABLE", -4: "E_HIP", -5: "E_NODEVICE",
-    -6: "E_RANGE", -7: "E_STATE", -8: "E_BUDGET", -9: "E_NOTFOUND", -10: "E_TIMEOUT",
+    -6: "E_RANGE", -7: "E_STATE", -8: "E_BUDGET", -9: "E_NOTFOUND", -10: "E_TIMEOUT", -11: "E_IO",
 }
 
 
@@ -132,6 +132,14 @@ def lib() -> C.CDLL:
     L.ela3d_layered_default.argtypes = [P(LayeredSpec)]
     L.ela3d_layered_default.restype = None
     L.ela3d_model_layered.argtypes = [P(LayeredSpec), P(f), P(f), P(f), P(C.c_double)]
+    L.ela3d_source_trace.argtypes = [C.c_void_p, P(C.c_double), C.c_size_t]
+    L.ela3d_checkpoint_save.argtypes = [C.c_void_p, C.c_char_p]
+    L.ela3d_checkpoint_load.argtypes = [C.c_void_p, C.c_char_p]
+    L.ela3d_model_anticline.argtypes = [C.c_int32, P(f), P(f), P(f)]
+    L.ela3d_read_raw.argtypes = [C.c_char_p, C.c_int64, C.c_int32, P(f)]
+    L.ela3d_write_raw.argtypes = [C.c_char_p, C.c_int64, C.c_int32, P(f)]
+    L.ela3d_write_traces.argtypes = [C.c_char_p, P(C.c_double), C.c_int64, C.c_int64, C.c_int32,
+                                     C.c_int32, C.c_int32, C.c_int32]
     if hasattr(L, "ela3d_server_create"):
         L.ela3d_server_opts_default.argtypes = [P(ServerOpts)]
         L.ela3d_server_opts_default.restype = None

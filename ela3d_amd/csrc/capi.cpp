@@ -1,9 +1,11 @@
 // capi.cpp -- extern "C" boundary: status codes, no exceptions across the ABI.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "shot.hpp"
 
@@ -206,6 +208,124 @@ int ela3d_set_field(ela3d_shot* s, int32_t field, const double* in, size_t n)
     });
 }
 
+int ela3d_source_trace(ela3d_shot* s, double* out, size_t cap)
+{
+    return guarded([&] {
+        if (!s || !out) throw Error{ELA3D_E_INVALID, "null argument"};
+        if (cap < (size_t(s->impl->plan().cfg.nstep) + 1) * 3) throw Error{ELA3D_E_RANGE, "buffer too small"};
+        s->impl->source_trace(out);
+        return ELA3D_OK;
+    });
+}
+
+namespace {
+
+// Checkpoint file: this header, then the shot's device allocation byte for byte.
+struct CheckpointHeader {
+    char magic[8];
+    uint32_t version;
+    uint32_t storage;
+    int32_t nx, ny, nz, nstep, nrec, steps_done;
+    uint64_t state_bytes;
+    uint64_t fingerprint;
+};
+const char kMagic[8] = {'E', 'L', 'A', '3', 'D', 'C', 'K', '1'};
+
+// FNV-1a over every config value that shapes the device state
+uint64_t fingerprint(const ela3d::ShotPlan& p)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void* d, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(d);
+        for (size_t q = 0; q < n; ++q) {
+            h ^= b[q];
+            h *= 1099511628211ull;
+        }
+    };
+    const ela3d_config& c = p.cfg;
+    const int32_t ints[] = {c.nx, c.ny, c.nz, c.nstep, c.npoints_pml, c.src_kind, c.wavelet, c.isrc, c.jsrc,
+                            c.ksrc, c.nrec, c.sample_every, c.energy_every, c.storage, int32_t(c.compat_flags),
+                            c.dirichlet, c.fp16_rescale_every};
+    const double dbl[] = {c.dx, c.dy, c.dz, c.dt, c.npower, c.k_max_pml, c.alpha_max_pml, c.rcoef, c.vpml,
+                          c.f0, c.t0, c.factor, c.angle_force};
+    mix(ints, sizeof(ints));
+    mix(dbl, sizeof(dbl));
+    if (!p.rec.empty()) mix(p.rec.data(), p.rec.size() * sizeof(int32_t));
+    return h;
+}
+
+struct File {
+    std::FILE* f = nullptr;
+    ~File()
+    {
+        if (f) std::fclose(f);
+    }
+};
+
+const size_t kChunk = size_t(64) << 20;
+
+}  // namespace
+
+int ela3d_checkpoint_save(ela3d_shot* s, const char* path)
+{
+    return guarded([&] {
+        if (!s || !path) throw Error{ELA3D_E_INVALID, "null argument"};
+        const auto& p = s->impl->plan();
+        CheckpointHeader h;
+        std::memset(&h, 0, sizeof(h));
+        std::memcpy(h.magic, kMagic, 8);
+        h.version = 1;
+        h.storage = uint32_t(p.cfg.storage);
+        h.nx = p.cfg.nx; h.ny = p.cfg.ny; h.nz = p.cfg.nz;
+        h.nstep = p.cfg.nstep;
+        h.nrec = p.cfg.nrec;
+        h.steps_done = s->impl->steps_done();
+        h.state_bytes = uint64_t(s->impl->device_bytes());
+        h.fingerprint = fingerprint(p);
+        File f;
+        f.f = std::fopen(path, "wb");
+        if (!f.f) throw Error{ELA3D_E_IO, std::string("cannot open ") + path + " for writing"};
+        if (std::fwrite(&h, sizeof(h), 1, f.f) != 1) throw Error{ELA3D_E_IO, "checkpoint header write failed"};
+        std::vector<char> buf(std::min<size_t>(kChunk, h.state_bytes));
+        for (size_t off = 0; off < h.state_bytes; off += buf.size()) {
+            const size_t n = std::min<size_t>(buf.size(), h.state_bytes - off);
+            s->impl->read_state(off, n, buf.data());
+            if (std::fwrite(buf.data(), 1, n, f.f) != n) throw Error{ELA3D_E_IO, "checkpoint write failed"};
+        }
+        if (std::fclose(f.f) != 0) {
+            f.f = nullptr;
+            throw Error{ELA3D_E_IO, "checkpoint close failed"};
+        }
+        f.f = nullptr;
+        return ELA3D_OK;
+    });
+}
+
+int ela3d_checkpoint_load(ela3d_shot* s, const char* path)
+{
+    return guarded([&] {
+        if (!s || !path) throw Error{ELA3D_E_INVALID, "null argument"};
+        const auto& p = s->impl->plan();
+        File f;
+        f.f = std::fopen(path, "rb");
+        if (!f.f) throw Error{ELA3D_E_IO, std::string("cannot open ") + path};
+        CheckpointHeader h;
+        if (std::fread(&h, sizeof(h), 1, f.f) != 1 || std::memcmp(h.magic, kMagic, 8) != 0 || h.version != 1)
+            throw Error{ELA3D_E_IO, "not an ela3d checkpoint"};
+        if (h.fingerprint != fingerprint(p) || h.state_bytes != uint64_t(s->impl->device_bytes()) ||
+            h.steps_done < 0 || h.steps_done > p.cfg.nstep)
+            throw Error{ELA3D_E_INVALID, "checkpoint belongs to a different configuration"};
+        std::vector<char> buf(std::min<size_t>(kChunk, h.state_bytes));
+        for (size_t off = 0; off < h.state_bytes; off += buf.size()) {
+            const size_t n = std::min<size_t>(buf.size(), h.state_bytes - off);
+            if (std::fread(buf.data(), 1, n, f.f) != n) throw Error{ELA3D_E_IO, "checkpoint truncated"};
+            s->impl->write_state(off, n, buf.data());
+        }
+        s->impl->set_steps_done(h.steps_done);
+        return ELA3D_OK;
+    });
+}
+
 int ela3d_info_get(ela3d_shot* s, ela3d_info* out)
 {
     return guarded([&] {
@@ -291,6 +411,7 @@ const char* ela3d_strerror(int code)
     case ELA3D_E_BUDGET: return "exceeds the device-memory budget";
     case ELA3D_E_NOTFOUND: return "unknown ticket";
     case ELA3D_E_TIMEOUT: return "timed out";
+    case ELA3D_E_IO: return "file error";
     default: return "unknown status";
     }
 }

@@ -81,6 +81,8 @@ struct TailArgs {
     float* scl;              // FP16 storage scale table (see kScaleTable)
     unsigned* maxslot;       // [kMaxSlots][27] maxima folded by this step's stencil kernels
     int rescale_every;       // FP16 storage: re-choose every scale each N steps
+    double* srctrace;        // [nstep+1][3] sigma_xx, sigma_yy, sigma_zz at the source after injection
+    int nsrctrace;           // rows of srctrace
 };
 
 template <typename T>

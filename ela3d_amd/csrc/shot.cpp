@@ -121,6 +121,7 @@ ArenaLayout arena_layout(const ShotPlan& p, int elem, int nepart)
     L.ctl = off; off += al(2 * sizeof(int));
     L.scl = off; off += al(kScaleTable * sizeof(float));
     L.slot = off; off += al(size_t(kMaxSlots) * 27 * sizeof(unsigned));
+    L.srctrace = off; off += al((size_t(c.nstep) + 1) * 3 * sizeof(double));
     L.total = off;
     return L;
 }
@@ -267,6 +268,34 @@ public:
         const size_t n = size_t(plan_.nenergy) * 2;
         if (n) HIP_CHECK(hipMemcpy(out, d_energy_, n * sizeof(double), hipMemcpyDeviceToHost));
     }
+
+    void source_trace(double* out) override
+    {
+        HIP_CHECK(hipSetDevice(device_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        HIP_CHECK(hipMemcpy(out, d_srctrace_, (size_t(plan_.cfg.nstep) + 1) * 3 * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    }
+
+    void read_state(size_t offset, size_t bytes, void* host) override
+    {
+        if (offset > size_t(arena_bytes_) || bytes > size_t(arena_bytes_) - offset)
+            throw Error{ELA3D_E_RANGE, "state range outside the allocation"};
+        HIP_CHECK(hipSetDevice(device_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        HIP_CHECK(hipMemcpy(host, static_cast<char*>(arena_) + offset, bytes, hipMemcpyDeviceToHost));
+    }
+
+    void write_state(size_t offset, size_t bytes, const void* host) override
+    {
+        if (offset > size_t(arena_bytes_) || bytes > size_t(arena_bytes_) - offset)
+            throw Error{ELA3D_E_RANGE, "state range outside the allocation"};
+        HIP_CHECK(hipSetDevice(device_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        HIP_CHECK(hipMemcpy(static_cast<char*>(arena_) + offset, host, bytes, hipMemcpyHostToDevice));
+    }
+
+    void set_steps_done(int n) override { host_step_ = n; }
 
     void snapshot(int field, double* out) override
     {
@@ -482,7 +511,7 @@ private:
         const size_t* o_model = L.model;
         const size_t* o_tab = L.tab;
         const size_t o_wav = L.wav, o_rec = L.rec, o_seis = L.seis, o_en = L.energy, o_ep = L.epart;
-        const size_t o_ctl = L.ctl, o_scl = L.scl, o_slot = L.slot;
+        const size_t o_ctl = L.ctl, o_scl = L.scl, o_slot = L.slot, o_src = L.srctrace;
         arena_bytes_ = int64_t(off);
         HIP_CHECK(hipMalloc(&arena_, off));
         char* b = static_cast<char*>(arena_);
@@ -499,6 +528,7 @@ private:
         d_flag_ = d_step_ + 1;
         d_scl_ = reinterpret_cast<float*>(b + o_scl);
         d_maxslot_ = reinterpret_cast<unsigned*>(b + o_slot);
+        d_srctrace_ = reinterpret_cast<double*>(b + o_src);
         // zero everything: wavefields and memory variables (CPU:107-140), the guards
         // and plane padding (read as neighbours, never written), outputs
         HIP_CHECK(hipMemsetAsync(arena_, 0, off, stream_));
@@ -659,6 +689,8 @@ private:
         a.scl = d_scl_;
         a.maxslot = d_maxslot_;
         a.rescale_every = std::max(1, c.fp16_rescale_every);
+        a.srctrace = d_srctrace_;
+        a.nsrctrace = c.nstep + 1;
         return a;
     }
 
@@ -739,6 +771,7 @@ private:
     int64_t nxyp_ = 0, guard_ = 0;
     int host_step_ = 0;
     int* h_flag_ = nullptr;  // pinned mirror of d_flag_ (serving)
+    double* d_srctrace_ = nullptr;   // [nstep+1][3]
     float* d_scl_ = nullptr;         // FP16 scale table (kScaleTable floats)
     unsigned* d_maxslot_ = nullptr;  // [kMaxSlots][27]
 

@@ -46,7 +46,7 @@ int64_t guard_elems(int nx);
 
 struct ArenaLayout {
     size_t f[9], m[18], model[3], tab[3];
-    size_t wav, rec, seis, energy, epart, ctl, scl, slot, total;
+    size_t wav, rec, seis, energy, epart, ctl, scl, slot, srctrace, total;
     int epart_cap;
 };
 ArenaLayout arena_layout(const ShotPlan& p, int elem_bytes, int nepart);
@@ -66,6 +66,12 @@ public:
     virtual bool unstable_seen() const = 0;
     virtual void seismograms(double* out) = 0;      // nsamples*nrec*3
     virtual void energy(double* out) = 0;           // nenergy*2
+    virtual void source_trace(double* out) = 0;     // (nstep+1)*3
+    // raw access to the whole device state (every buffer of the single allocation,
+    // device_bytes() long) for checkpoints; set_steps_done after a full restore
+    virtual void read_state(size_t offset, size_t bytes, void* host) = 0;
+    virtual void write_state(size_t offset, size_t bytes, const void* host) = 0;
+    virtual void set_steps_done(int n) = 0;
     virtual void snapshot(int field, double* out) = 0;
     virtual void set_field(int field, const double* in) = 0;
     virtual void profile(int n, std::vector<ela3d_kernel_stats>& st, double& wall_ms) = 0;

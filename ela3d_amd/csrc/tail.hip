@@ -125,6 +125,12 @@ __global__ void __launch_bounds__(256) k_tail(const TailArgs<T> A)
             }
             }
         }
+        if (it < A.nsrctrace) {  // stresses at the source cell after injection (UNI:463-465)
+            const int o = A.src;
+            A.srctrace[3 * it] = (double)ldf(A.f[3], o, nisc<T>(A, 3));
+            A.srctrace[3 * it + 1] = (double)ldf(A.f[4], o, nisc<T>(A, 4));
+            A.srctrace[3 * it + 2] = (double)ldf(A.f[5], o, nisc<T>(A, 5));
+        }
         if (bad) *A.flag = 1;
         *A.step = it;
     }

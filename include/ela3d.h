@@ -44,6 +44,7 @@ extern "C" {
 #define ELA3D_E_BUDGET    -8  /* shot can never fit the server's device-memory budget    */
 #define ELA3D_E_NOTFOUND  -9  /* unknown ticket                                           */
 #define ELA3D_E_TIMEOUT  -10  /* wait timed out                                           */
+#define ELA3D_E_IO       -11  /* file could not be opened, read or written                */
 
 /* ---- source kinds (CPU:611-622, ZC:446-459) ---- */
 #define ELA3D_SRC_REFERENCE_SIGMA 0 /* CPU:620-622: sxx,syy += fx*dt*(l+2m); szz += fy*dt*(l+2m) */
@@ -169,6 +170,15 @@ int ela3d_energy(ela3d_shot* shot, double* out, size_t cap);
 int ela3d_snapshot(ela3d_shot* shot, int32_t field, double* out, size_t cap);
 /* Overwrite one field from FP64 values (rounded to the storage type). */
 int ela3d_set_field(ela3d_shot* shot, int32_t field, const double* in, size_t n);
+/* sigma_xx, sigma_yy, sigma_zz at the source cell right after each step's injection,
+ * [nstep+1][3] FP64 with row 0 zero (the lesrcx/y/z arrays, UNI:463-465, 1202-1223). */
+int ela3d_source_trace(ela3d_shot* shot, double* out, size_t cap);
+/* Checkpoint / resume (no reference counterpart; the reference snapshots cannot
+ * resume).  Writes the shot's whole device state and step count to a file; a shot
+ * created from the same config and model can load it and continue bit-identically.
+ * ELA3D_E_INVALID if the file belongs to another config, ELA3D_E_IO on file errors. */
+int ela3d_checkpoint_save(ela3d_shot* shot, const char* path);
+int ela3d_checkpoint_load(ela3d_shot* shot, const char* path);
 int ela3d_info_get(ela3d_shot* shot, ela3d_info* out);
 /* Run n steps eagerly with hipEvents around every launch; per-kernel totals. */
 int ela3d_profile(ela3d_shot* shot, int32_t nsteps, ela3d_kernel_stats* out, int32_t cap,
@@ -273,6 +283,27 @@ typedef struct ela3d_layered_spec {
 void ela3d_layered_default(ela3d_layered_spec* s);
 int ela3d_model_layered(const ela3d_layered_spec* s, float* cp, float* cs, float* rh,
                         double* vp_max);
+/* The MATLAB anticline (generate_3d_model_anticline.m): an n^3 grid (n = 201 there)
+ * with interfaces h - d*Z(x,y), Z = max(0, sin x + sin y - 1) on x, y in [0, pi],
+ * h = 121, 161, 201 (scaled by n/201), d = 100*n/201, five media (MAT:72-76)
+ * including the oil-bearing sandstone above k = 80*n/201.  cp, cs, rho: n^3 each. */
+int ela3d_model_anticline(int32_t n, float* cp, float* cs, float* rh);
+
+/* ---- the reference's file formats (host only) ---- */
+/* Raw x-fastest model files (modelcp.bin / modelcs.bin / modelrh.bin): float64 as
+ * read at UNI:595-644 (bytes_per_value 8) or float32 as at ZC:544-595 (4); read
+ * into FP32.  Writing uses the same layouts (snapshots v?%05i.bin are float32,
+ * CPU:685-722). */
+int ela3d_read_raw(const char* path, int64_t n, int32_t bytes_per_value, float* out);
+int ela3d_write_raw(const char* path, int64_t n, int32_t bytes_per_value, const float* in);
+/* Text traces in the layout of rechorvx/y/z.txt (CPU:636-670): one line per row,
+ * each of ncols values as the reference's float stream prints it (6 significant
+ * digits) followed by a space.  Value (r, c) = v[(r*ncols + c)*stride + offset]:
+ * seismograms [nsamples][nrec][3] with stride 3, offset = component.  append = 1
+ * appends as the reference does; final_newline = 0 omits the last newline
+ * (recsrc*.txt, UNI:1202-1223, is one line without one). */
+int ela3d_write_traces(const char* path, const double* v, int64_t nrows, int64_t ncols,
+                       int32_t stride, int32_t offset, int32_t append, int32_t final_newline);
 
 #ifdef __cplusplus
 }
