@@ -46,6 +46,7 @@ class InfoStruct(C.Structure):
         ("block", C.c_int32), ("zseg", C.c_int32), ("zseg_slab", C.c_int32), ("vec", C.c_int32),
         ("region_lo", C.c_int32 * 3), ("region_hi", C.c_int32 * 3),
         ("block_slab", C.c_int32), ("vec_slab", C.c_int32), ("overlap", C.c_int32),
+        ("merged", C.c_int32),
     ]
 
 
@@ -57,7 +58,8 @@ class KernelStats(C.Structure):
 class TuningStruct(C.Structure):
     _fields_ = [("block", C.c_int32), ("zseg_interior", C.c_int32), ("zseg_slab", C.c_int32),
                 ("vec", C.c_int32), ("candidates", C.c_int32), ("ms_per_step", C.c_double),
-                ("block_slab", C.c_int32), ("vec_slab", C.c_int32), ("overlap", C.c_int32)]
+                ("block_slab", C.c_int32), ("vec_slab", C.c_int32), ("overlap", C.c_int32),
+                ("merged", C.c_int32)]
 
 
 class LayeredSpec(C.Structure):

@@ -745,9 +745,10 @@ __device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B
     }
 }
 
-// Two launches per pass: one over the absorbing-layer boxes, one over the
-// interior box.  Each is compiled for its own mode, so the interior kernel
-// carries none of the y/z C-PML state and keeps a small register footprint.
+// Two launches per pass (one over the absorbing-layer boxes, one over the
+// interior box, each compiled for its own mode so the interior kernel carries
+// none of the y/z C-PML state), or one mixed launch over all boxes in which
+// each block branches to its box's body (one kernel boundary less per pass).
 // ---------------------------------------------------------------------------
 template <typename T, int MODE, int V>
 __global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
@@ -762,11 +763,14 @@ __global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
         tx = A.tab[0];
         ty = A.tab[1];
     } else {
-        stage_tables(A, tabs, MODE == kSlab);
+        stage_tables(A, tabs, MODE == kSlab || (MODE == kMixed && B.mode == kSlab));
         tx = tabs;
         ty = tabs + 6 * A.nx;
     }
-    stress_body<T, MODE, V>(A, B, lb - B.blk0, tx, ty, S);
+    if (MODE == kSlab || (MODE == kMixed && B.mode == kSlab))
+        stress_body<T, kSlab, V>(A, B, lb - B.blk0, tx, ty, S);
+    else
+        stress_body<T, kInterior, V>(A, B, lb - B.blk0, tx, ty, S);
     flush_maxima(A, S, 3, 18);
 }
 
@@ -783,12 +787,15 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
         tx = A.tab[0];
         ty = A.tab[1];
     } else {
-        stage_tables(A, tabs, MODE == kSlab);
+        stage_tables(A, tabs, MODE == kSlab || (MODE == kMixed && B.mode == kSlab));
         tx = tabs;
         ty = tabs + 6 * A.nx;
     }
     double ek = 0.0, ep = 0.0;
-    velocity_body<T, MODE, ENERGY, V>(A, B, lb - B.blk0, tx, ty, ek, ep, S);
+    if (MODE == kSlab || (MODE == kMixed && B.mode == kSlab))
+        velocity_body<T, kSlab, ENERGY, V>(A, B, lb - B.blk0, tx, ty, ek, ep, S);
+    else
+        velocity_body<T, kInterior, ENERGY, V>(A, B, lb - B.blk0, tx, ty, ek, ep, S);
     flush_maxima(A, S, 0, 3);
     flush_maxima(A, S, 18, 27);
     if (ENERGY) {  // per-block FP64 partials, summed in block order by the tail kernel
@@ -818,7 +825,7 @@ template <typename T>
 static size_t table_lds(const StepArgs<T>& a)
 {
     if (std::is_same<T, double>::value) return 0;
-    return size_t(6) * (a.nx + (a.mode == kSlab ? a.ny : 0)) * sizeof(float);
+    return size_t(6) * (a.nx + (a.mode != kInterior ? a.ny : 0)) * sizeof(float);
 }
 
 template <typename T, int M, int V>
@@ -833,12 +840,19 @@ static void launch_velocity_v(const StepArgs<T>& a, hipStream_t s)
     hipLaunchKernelGGL((k_velocity<T, M, E, V>), dim3(a.nblocks), dim3(a.block), table_lds<T>(a), s, a);
 }
 
-// interior launches: VEC 1, 2 or 4; absorbing-layer launches: VEC 1 or 2
+// interior and mixed launches: VEC 1, 2 or 4; absorbing-layer launches: VEC 1 or 2
 template <typename T>
 void launch_stress(const StepArgs<T>& a, hipStream_t s)
 {
     if (a.nblocks <= 0) return;
-    if (a.mode == kSlab) {
+    if (a.mode == kMixed) {
+        if (a.vec == 4)
+            launch_stress_v<T, kMixed, 4>(a, s);
+        else if (a.vec == 2)
+            launch_stress_v<T, kMixed, 2>(a, s);
+        else
+            launch_stress_v<T, kMixed, 1>(a, s);
+    } else if (a.mode == kSlab) {
         if (a.vec == 2)
             launch_stress_v<T, kSlab, 2>(a, s);
         else
@@ -855,7 +869,14 @@ void launch_stress(const StepArgs<T>& a, hipStream_t s)
 template <typename T, bool E>
 static void launch_velocity_e(const StepArgs<T>& a, hipStream_t s)
 {
-    if (a.mode == kSlab) {
+    if (a.mode == kMixed) {
+        if (a.vec == 4)
+            launch_velocity_v<T, kMixed, E, 4>(a, s);
+        else if (a.vec == 2)
+            launch_velocity_v<T, kMixed, E, 2>(a, s);
+        else
+            launch_velocity_v<T, kMixed, E, 1>(a, s);
+    } else if (a.mode == kSlab) {
         if (a.vec == 2)
             launch_velocity_v<T, kSlab, E, 2>(a, s);
         else

@@ -11,6 +11,7 @@ typedef _Float16 half_t;
 // box modes
 constexpr int kInterior = 0;  // y and z outside the C-PML: only x is checked per lane
 constexpr int kSlab = 1;      // y or z inside the C-PML: every axis checked
+constexpr int kMixed = 2;     // launch mode only: boxes of both kinds, each block runs its box's body
 
 // FP16 storage keeps one power-of-two scale per field (stored = value * scale)
 // in a device table of kScaleTable floats, so replayed graphs see live values:
@@ -55,7 +56,7 @@ struct StepArgs {
     Box box[kMaxBox];
     int nbox, nblocks, block;
     int vec;                 // consecutive xy positions per lane: 1, 2 or 4
-    int mode;                // kInterior or kSlab: every box of this launch has this mode
+    int mode;                // kInterior or kSlab (every box of the launch), or kMixed
 };
 
 template <typename T>

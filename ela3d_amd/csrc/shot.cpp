@@ -63,8 +63,10 @@ static void add_box(LaunchGeom& g, const BoxRange& r, int zseg)
     g.nblocks += b.nchunk * ((r.k1 - r.k0 + b.zseg - 1) / b.zseg);
 }
 
-// Two launches per pass.  The absorbing-layer launch covers the z-min, z-max,
-// y-min and y-max boxes; the interior launch the interior box.  Box starts are
+// Two launches per pass: the absorbing-layer launch covers the z-min, z-max,
+// y-min and y-max boxes, the interior launch the interior box.  Merged: one
+// mixed launch with the absorbing-layer boxes first in block order (their
+// slower blocks start early), then the interior box.  Box starts are
 // multiples of 64 elements, so with the padded plane stride every workgroup
 // tile begins on a 256-byte line boundary and VEC groups are aligned.
 void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& slab, LaunchGeom& interior)
@@ -73,16 +75,18 @@ void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& slab, LaunchGeom& 
     interior = LaunchGeom();
     slab.block = t.block_slab;
     slab.vec = t.vec_slab;
+    slab.mode = kSlab;
     interior.block = t.block;
     interior.vec = t.vec;
+    interior.mode = t.merged ? kMixed : kInterior;
     BoxRange r[5];
     const int n = split_boxes(p, r);
-    for (int q = 0; q < n; ++q) {
+    for (int q = 0; q < n; ++q) {  // absorbing-layer boxes come first in split_boxes order
         const bool zlayer = r[q].k1 <= p.ax[2].lo || r[q].k0 >= p.ax[2].hi;
         if (r[q].mode == 0)
             add_box(interior, r[q], t.zseg_interior);
         else  // z layers march in short segments; the y layers as the interior
-            add_box(slab, r[q], zlayer ? t.zseg_slab : t.zseg_interior);
+            add_box(t.merged ? interior : slab, r[q], zlayer ? t.zseg_slab : t.zseg_interior);
     }
 }
 
@@ -352,18 +356,21 @@ public:
     void profile(int n, std::vector<ela3d_kernel_stats>& st, double& wall_ms) override
     {
         HIP_CHECK(hipSetDevice(device_));
-        static const char* names[kLaunches] = {"stress_cpml", "stress_interior", "velocity_cpml",
+        static const char* split[kLaunches] = {"stress_cpml", "stress_interior", "velocity_cpml",
                                                "velocity_interior", "tail"};
+        static const char* merged[kLaunches] = {"stress_cpml", "stress", "velocity_cpml", "velocity", "tail"};
+        const char* const* names = tune_.merged ? merged : split;
         st.assign(kLaunches, ela3d_kernel_stats());
         for (int q = 0; q < kLaunches; ++q) {
             std::snprintf(st[q].name, sizeof(st[q].name), "%s", names[q]);
             st[q].launches = 0;
             st[q].total_ms = 0.0;
         }
-        st[0].alg_bytes_per_launch = plan_.alg_stress_slab;
-        st[1].alg_bytes_per_launch = plan_.alg_stress_interior;
-        st[2].alg_bytes_per_launch = plan_.alg_velocity_slab;
-        st[3].alg_bytes_per_launch = plan_.alg_velocity_interior;
+        const bool m = tune_.merged != 0;  // the absorbing-layer launches are then empty
+        st[0].alg_bytes_per_launch = m ? 0.0 : plan_.alg_stress_slab;
+        st[1].alg_bytes_per_launch = plan_.alg_stress_interior + (m ? plan_.alg_stress_slab : 0.0);
+        st[2].alg_bytes_per_launch = m ? 0.0 : plan_.alg_velocity_slab;
+        st[3].alg_bytes_per_launch = plan_.alg_velocity_interior + (m ? plan_.alg_velocity_slab : 0.0);
         st[4].alg_bytes_per_launch = 0;
         const int chunk = 64;
         const int E = 2 * kLaunches;  // start and end event of every launch
@@ -448,6 +455,7 @@ public:
         o.block_slab = tune_.block_slab;
         o.vec_slab = tune_.vec_slab;
         o.overlap = overlap_ ? 1 : 0;
+        o.merged = tune_.merged;
         for (int a = 0; a < 3; ++a) {
             o.region_lo[a] = plan_.ax[a].lo;
             o.region_hi[a] = plan_.ax[a].hi;
@@ -644,7 +652,7 @@ private:
         a.nblocks = g.nblocks;
         a.vec = g.vec;
         a.block = g.block;
-        a.mode = g.nbox > 0 ? g.box[0].mode : kInterior;
+        a.mode = g.mode;
         return a;
     }
 

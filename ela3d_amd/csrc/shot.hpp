@@ -25,6 +25,7 @@ namespace ela3d {
 struct LaunchGeom {
     Box box[kMaxBox];
     int nbox = 0, nblocks = 0, block = 256, vec = 1;
+    int mode = kInterior;  // kInterior, kSlab or kMixed
 };
 
 // Launch tuning chosen for a grid (see tune.cpp): the interior launch and the
@@ -37,6 +38,8 @@ struct Tuning {
     int zseg_slab = 8;
     int vec_slab = 1;       // 1 or 2
     int overlap = 0;        // 1: absorbing-layer launches concurrent with the interior ones
+    int merged = 0;         // 1: one mixed launch per pass over all boxes (interior shape;
+                            //    z layers with zseg_slab)
 };
 
 // Device layout of the 9 fields and the 3 model arrays: planes of plane_stride

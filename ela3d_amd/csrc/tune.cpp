@@ -133,21 +133,28 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
                         best.vec_slab = vv;
                     }
                 }
-        // concurrent absorbing-layer launches pay off when neither launch fills the GPU
-        double wall_seq = 0.0, wall_ovl = 0.0;
-        for (int ov = 0; ov < 2; ++ov) {
+        // schedule: the two launches in sequence, overlapped (pays off when neither
+        // fills the GPU), or one mixed launch per pass (one kernel boundary less)
+        double wall3[3] = {0.0, 0.0, 0.0};
+        for (int sc = 0; sc < 3; ++sc) {
             Tuning t = best;
-            t.overlap = ov;
+            t.overlap = sc == 1;
+            t.merged = sc == 2;
             shot->retune(t);
             double wall = 0.0;
             shot->profile(1, st, wall);
             shot->profile(steps, st, wall);
-            (ov ? wall_ovl : wall_seq) = wall / steps;
+            wall3[sc] = wall / steps;
             ++tried;
         }
-        best.overlap = wall_ovl < 0.97 * wall_seq ? 1 : 0;
+        int pick = 0;
+        if (wall3[2] < 0.99 * wall3[pick]) pick = 2;
+        if (wall3[1] < 0.97 * wall3[pick]) pick = 1;
+        best.overlap = pick == 1;
+        best.merged = pick == 2;
         if (verbose)
-            std::fprintf(stderr, "[tune] step wall: in sequence %.4f ms, overlapped %.4f ms\n", wall_seq, wall_ovl);
+            std::fprintf(stderr, "[tune] step wall: split %.4f ms, overlapped %.4f ms, merged %.4f ms\n",
+                         wall3[0], wall3[1], wall3[2]);
         shot->retune(best);
         shot->set_overlap(false);
         measure(best, ms);
@@ -161,6 +168,7 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
             out->block_slab = best.block_slab;
             out->vec_slab = best.vec_slab;
             out->overlap = best.overlap;
+            out->merged = best.merged;
             out->ms_per_step = best_ms;
             out->candidates = tried;
         }
@@ -191,6 +199,7 @@ int ela3d_tuning_set(const ela3d_config* cfg, const ela3d_tuning* t)
     u.zseg_slab = t->zseg_slab;
     u.vec_slab = vs;
     u.overlap = t->overlap ? 1 : 0;
+    u.merged = t->merged ? 1 : 0;
     ela3d::store_tuning(*cfg, u, t->ms_per_step);
     return ELA3D_OK;
 }
@@ -207,6 +216,7 @@ int ela3d_tuning_get(const ela3d_config* cfg, ela3d_tuning* t)
     t->block_slab = u.block_slab;
     t->vec_slab = u.vec_slab;
     t->overlap = u.overlap;
+    t->merged = u.merged;
     t->ms_per_step = 0.0;
     t->candidates = 0;
     return ELA3D_OK;

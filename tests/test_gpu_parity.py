@@ -107,15 +107,18 @@ def test_launch_shapes_agree(gpu, storage):
     from ela3d_amd import set_tuning
     cfg = small_cfg(nx=45, ny=38, nz=33, nstep=60, storage=storage)
     model = hetero_model(cfg, seed=5)
-    shapes = [(256, 4, 1, 256, 4, 1, False), (256, 4, 1, 256, 4, 1, True),
-              (64, 2, 2, 128, 2, 2, True), (128, 8, 4, 64, 8, 1, False), (64, 32, 4, 256, 2, 2, True)]
+    shapes = [(256, 4, 1, 256, 4, 1, False, False), (256, 4, 1, 256, 4, 1, True, False),
+              (64, 2, 2, 128, 2, 2, True, False), (128, 8, 4, 64, 8, 1, False, False),
+              (64, 32, 4, 256, 2, 2, True, False), (256, 4, 4, 256, 2, 1, False, True),
+              (128, 2, 2, 128, 8, 1, False, True)]
     out = []
-    for (b, zi, v, bs, zs, vs, ov) in shapes:
+    for (b, zi, v, bs, zs, vs, ov, mg) in shapes:
         set_tuning(cfg, block=b, zseg_interior=zi, vec=v, block_slab=bs, zseg_slab=zs, vec_slab=vs,
-                   overlap=ov)
+                   overlap=ov, merged=mg)
         with Shot(cfg, *model) as shot:
             inf = shot.info()
-            assert (inf["block"], inf["vec"], inf["vec_slab"], inf["overlap"]) == (b, v, vs, int(ov))
+            assert (inf["block"], inf["vec"], inf["vec_slab"], inf["overlap"], inf["merged"]) == \
+                (b, v, vs, int(ov), int(mg))
             shot.run()
             out.append((shot.seismograms(), shot.energy(), shot.snapshot(5), shot.snapshot(20)))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])

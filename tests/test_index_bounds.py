@@ -110,7 +110,8 @@ def test_every_access_in_bounds(shape, vec):
     for compat in (0, 7):
         cfg = small_cfg(nx=nx, ny=ny, nz=nz, npml=npml, compat=compat)
         for block, zi, zs in ((64, 2, 2), (256, 4, 4), (128, 32, 8)):
-            for vs, bs in ((1, 64), (2, 256), (2, 128)):
+            # split launches, and the merged launch (absorbing layers with the interior shape)
+            for vs, bs in ((1, 64), (2, 256), (2, 128), (vec, block)):
                 assert violations(cfg, block, vec, zi, zs, block_slab=bs, vec_slab=vs) == []
 
 
