@@ -17,8 +17,9 @@
 //  * C-PML memory variables live in compact per-axis slabs.  Interior boxes test
 //    x per element only (y and z lie outside the C-PML by construction); the
 //    absorbing-layer boxes test all three axes.  Each pass is two launches, one
-//    per box mode, each compiled for its mode.  The FP64 a/b/K tables of x (and
-//    of y in absorbing-layer launches) are staged per workgroup into LDS as FP32.
+//    per box mode, each compiled for its mode (or one mixed launch).  The a/b/K
+//    tables of x and y are read as FP32 copies, through the caches or staged per
+//    workgroup in LDS (a launch-shape choice).
 //  * Arithmetic is FP32 (FP64 for the FP64 storage instantiation); energy
 //    accumulates in FP64.
 //
@@ -200,10 +201,29 @@ using TabP = typename std::conditional<std::is_same<T, double>::value, double, f
 template <typename T>
 __device__ __forceinline__ void stage_tables(const StepArgs<T>& A, float* s, bool with_y)
 {
-    const int nx6 = 6 * A.nx, n = nx6 + (with_y ? 6 * A.ny : 0);
-    for (int q = threadIdx.x; q < n; q += blockDim.x)
-        s[q] = q < nx6 ? (float)A.tab[0][q] : (float)A.tab[1][q - nx6];
+    const int n = 6 * A.nx + (with_y ? 6 * A.ny : 0);
+    for (int q = threadIdx.x; q < n; q += blockDim.x) s[q] = A.tabf[q];
     __syncthreads();
+}
+
+// x and y coefficient tables of a pass: the FP64 tables for FP64 arithmetic,
+// otherwise the FP32 copies, staged in LDS or read through L1/L2.
+template <typename T>
+__device__ __forceinline__ void pick_tables(const StepArgs<T>& A, float* lds, bool with_y, const TabP<T>*& tx,
+                                            const TabP<T>*& ty)
+{
+    if constexpr (std::is_same<T, double>::value) {
+        tx = A.tab[0];
+        ty = A.tab[1];
+    } else {
+        const float* t = A.tabf;
+        if (A.tab_lds) {
+            stage_tables(A, lds, with_y);
+            t = lds;
+        }
+        tx = t;
+        ty = t + 6 * A.nx;
+    }
 }
 
 // Per-lane geometry of the VEC elements p0..p0+VEC-1.
@@ -759,14 +779,7 @@ __global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
     Scales<T> S;
     load_scales(A, S, true);
     const TabP<T>*tx, *ty;
-    if constexpr (std::is_same<T, double>::value) {
-        tx = A.tab[0];
-        ty = A.tab[1];
-    } else {
-        stage_tables(A, tabs, MODE == kSlab || (MODE == kMixed && B.mode == kSlab));
-        tx = tabs;
-        ty = tabs + 6 * A.nx;
-    }
+    pick_tables(A, tabs, MODE == kSlab || (MODE == kMixed && B.mode == kSlab), tx, ty);
     if (MODE == kSlab || (MODE == kMixed && B.mode == kSlab))
         stress_body<T, kSlab, V>(A, B, lb - B.blk0, tx, ty, S);
     else
@@ -783,14 +796,7 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
     Scales<T> S;
     load_scales(A, S, false);
     const TabP<T>*tx, *ty;
-    if constexpr (std::is_same<T, double>::value) {
-        tx = A.tab[0];
-        ty = A.tab[1];
-    } else {
-        stage_tables(A, tabs, MODE == kSlab || (MODE == kMixed && B.mode == kSlab));
-        tx = tabs;
-        ty = tabs + 6 * A.nx;
-    }
+    pick_tables(A, tabs, MODE == kSlab || (MODE == kMixed && B.mode == kSlab), tx, ty);
     double ek = 0.0, ep = 0.0;
     if (MODE == kSlab || (MODE == kMixed && B.mode == kSlab))
         velocity_body<T, kSlab, ENERGY, V>(A, B, lb - B.blk0, tx, ty, ek, ep, S);
@@ -824,7 +830,7 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
 template <typename T>
 static size_t table_lds(const StepArgs<T>& a)
 {
-    if (std::is_same<T, double>::value) return 0;
+    if (std::is_same<T, double>::value || !a.tab_lds) return 0;
     return size_t(6) * (a.nx + (a.mode != kInterior ? a.ny : 0)) * sizeof(float);
 }
 

@@ -126,6 +126,7 @@ ArenaLayout arena_layout(const ShotPlan& p, int elem, int nepart)
     L.scl = off; off += al(kScaleTable * sizeof(float));
     L.slot = off; off += al(size_t(kMaxSlots) * 27 * sizeof(unsigned));
     L.srctrace = off; off += al((size_t(c.nstep) + 1) * 3 * sizeof(double));
+    L.tabf = off; off += al(6 * size_t(c.nx + c.ny) * sizeof(float));
     L.total = off;
     return L;
 }
@@ -456,6 +457,7 @@ public:
         o.vec_slab = tune_.vec_slab;
         o.overlap = overlap_ ? 1 : 0;
         o.merged = tune_.merged;
+        o.tab_lds = tune_.tab_lds;
         for (int a = 0; a < 3; ++a) {
             o.region_lo[a] = plan_.ax[a].lo;
             o.region_hi[a] = plan_.ax[a].hi;
@@ -537,6 +539,7 @@ private:
         d_scl_ = reinterpret_cast<float*>(b + o_scl);
         d_maxslot_ = reinterpret_cast<unsigned*>(b + o_slot);
         d_srctrace_ = reinterpret_cast<double*>(b + o_src);
+        d_tabf_ = reinterpret_cast<float*>(b + L.tabf);
         // zero everything: wavefields and memory variables (CPU:107-140), the guards
         // and plane padding (read as neighbours, never written), outputs
         HIP_CHECK(hipMemsetAsync(arena_, 0, off, stream_));
@@ -605,6 +608,11 @@ private:
             }
             HIP_CHECK(hipMemcpyAsync(d_tab_[a], t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
             HIP_CHECK(hipStreamSynchronize(stream_));
+            if (a < 2) {  // FP32 copies of the x and y tables, back to back
+                const std::vector<float> f(t.begin(), t.end());
+                HIP_CHECK(hipMemcpy(d_tabf_ + (a == 0 ? 0 : 6 * size_t(c.nx)), f.data(), f.size() * sizeof(float),
+                                    hipMemcpyHostToDevice));
+            }
         }
         if (!plan_.wavelet.empty())
             HIP_CHECK(hipMemcpyAsync(d_wav_, plan_.wavelet.data(), plan_.wavelet.size() * sizeof(double),
@@ -629,6 +637,8 @@ private:
         a.cs = d_model_[1] - guard_;
         a.rho = d_model_[2] - guard_;
         for (int q = 0; q < 3; ++q) a.tab[q] = d_tab_[q];
+        a.tabf = d_tabf_;
+        a.tab_lds = tune_.tab_lds;
         a.nx = c.nx; a.ny = c.ny; a.nz = c.nz; a.nxy = c.nx * c.ny;
         a.nxyp = int(nxyp_);
         a.g0 = int(guard_);
@@ -780,6 +790,7 @@ private:
     int host_step_ = 0;
     int* h_flag_ = nullptr;  // pinned mirror of d_flag_ (serving)
     double* d_srctrace_ = nullptr;   // [nstep+1][3]
+    float* d_tabf_ = nullptr;        // FP32 x and y tables [6][nx], [6][ny]
     float* d_scl_ = nullptr;         // FP16 scale table (kScaleTable floats)
     unsigned* d_maxslot_ = nullptr;  // [kMaxSlots][27]
 

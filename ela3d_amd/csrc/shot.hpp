@@ -40,6 +40,7 @@ struct Tuning {
     int overlap = 0;        // 1: absorbing-layer launches concurrent with the interior ones
     int merged = 0;         // 1: one mixed launch per pass over all boxes (interior shape;
                             //    z layers with zseg_slab)
+    int tab_lds = 0;        // 1: stage the x/y coefficient tables in LDS per workgroup
 };
 
 // Device layout of the 9 fields and the 3 model arrays: planes of plane_stride
@@ -49,7 +50,7 @@ int64_t guard_elems(int nx);
 
 struct ArenaLayout {
     size_t f[9], m[18], model[3], tab[3];
-    size_t wav, rec, seis, energy, epart, ctl, scl, slot, srctrace, total;
+    size_t wav, rec, seis, energy, epart, ctl, scl, slot, srctrace, tabf, total;
     int epart_cap;
 };
 ArenaLayout arena_layout(const ShotPlan& p, int elem_bytes, int nepart);

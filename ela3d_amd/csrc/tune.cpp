@@ -97,6 +97,18 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
             ++tried;
         };
         double ms[2], best_int = 1e300, best_slab = 1e300;
+        {  // coefficient tables through the caches or staged in LDS
+            double t2[2];
+            for (int l = 0; l < 2; ++l) {
+                Tuning t = best;
+                t.tab_lds = l;
+                measure(t, ms);
+                t2[l] = ms[0] + ms[1];
+            }
+            best.tab_lds = t2[1] < t2[0] ? 1 : 0;
+            if (verbose)
+                std::fprintf(stderr, "[tune] tables via caches %.4f ms, via LDS %.4f ms\n", t2[0], t2[1]);
+        }
         for (int vv : vecs)
             for (int b : blocks)
                 for (int zi : zsi) {
@@ -135,23 +147,46 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
                 }
         // schedule: the two launches in sequence, overlapped (pays off when neither
         // fills the GPU), or one mixed launch per pass (one kernel boundary less)
-        double wall3[3] = {0.0, 0.0, 0.0};
-        for (int sc = 0; sc < 3; ++sc) {
-            Tuning t = best;
-            t.overlap = sc == 1;
-            t.merged = sc == 2;
+        auto wall_of = [&](const Tuning& t) {
             shot->retune(t);
             double wall = 0.0;
             shot->profile(1, st, wall);
             shot->profile(steps, st, wall);
-            wall3[sc] = wall / steps;
             ++tried;
-        }
+            return wall / steps;
+        };
+        double wall3[3] = {0.0, 0.0, 1e300};
+        Tuning cand[3] = {best, best, best};
+        cand[1].overlap = 1;
+        wall3[0] = wall_of(cand[0]);
+        wall3[1] = wall_of(cand[1]);
+        static const int mblocks[] = {128, 256};
+        static const int mzi[] = {4, 8, 16};
+        static const int mzs[] = {2, 4};
+        static const int mvec[] = {1, 2};
+        for (int vv : mvec)
+            for (int b : mblocks)
+                for (int zi : mzi)
+                    for (int zs : mzs) {
+                        Tuning t = best;
+                        t.merged = 1;
+                        t.block = b;
+                        t.zseg_interior = zi;
+                        t.zseg_slab = zs;
+                        t.vec = vv;
+                        const double w = wall_of(t);
+                        if (verbose)
+                            std::fprintf(stderr, "[tune] merged vec %d block %3d zseg %2d/%d: %.4f ms\n", vv, b, zi,
+                                         zs, w);
+                        if (w < wall3[2]) {
+                            wall3[2] = w;
+                            cand[2] = t;
+                        }
+                    }
         int pick = 0;
         if (wall3[2] < 0.99 * wall3[pick]) pick = 2;
         if (wall3[1] < 0.97 * wall3[pick]) pick = 1;
-        best.overlap = pick == 1;
-        best.merged = pick == 2;
+        best = cand[pick];
         if (verbose)
             std::fprintf(stderr, "[tune] step wall: split %.4f ms, overlapped %.4f ms, merged %.4f ms\n",
                          wall3[0], wall3[1], wall3[2]);
@@ -169,6 +204,7 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
             out->vec_slab = best.vec_slab;
             out->overlap = best.overlap;
             out->merged = best.merged;
+            out->tab_lds = best.tab_lds;
             out->ms_per_step = best_ms;
             out->candidates = tried;
         }
@@ -200,6 +236,7 @@ int ela3d_tuning_set(const ela3d_config* cfg, const ela3d_tuning* t)
     u.vec_slab = vs;
     u.overlap = t->overlap ? 1 : 0;
     u.merged = t->merged ? 1 : 0;
+    u.tab_lds = t->tab_lds ? 1 : 0;
     ela3d::store_tuning(*cfg, u, t->ms_per_step);
     return ELA3D_OK;
 }
@@ -217,6 +254,7 @@ int ela3d_tuning_get(const ela3d_config* cfg, ela3d_tuning* t)
     t->vec_slab = u.vec_slab;
     t->overlap = u.overlap;
     t->merged = u.merged;
+    t->tab_lds = u.tab_lds;
     t->ms_per_step = 0.0;
     t->candidates = 0;
     return ELA3D_OK;
