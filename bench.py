@@ -248,7 +248,7 @@ def main():
                     help="copy kernel for --calibrate (0-3); default: all four")
     ap.add_argument("--shape", default="",
                     help="fixed launch shape block,zseg,vec[,block_slab,zseg_slab,vec_slab"
-                         "[,overlap,merged,tab_lds]] (interior, then absorbing layers; skips the tuner)")
+                         "[,overlap,merged]] (interior, then absorbing layers; skips the tuner)")
     ap.add_argument("--shots", type=int, default=256, help="serving: shots in the stream")
     ap.add_argument("--seed", type=int, default=7, help="serving: stream seed")
     ap.add_argument("--max-active", type=int, default=8, help="serving: concurrent shots")
@@ -286,12 +286,12 @@ def main():
         if len(v) == 3:
             v += [0, 4, 1]
         if len(v) == 6:
-            v += [0, 0, 0]
-        if len(v) != 9:
-            raise SystemExit("--shape takes 3, 6 or 9 integers")
-        b, zi, pp, bs, zs, ps, ov, mg, tl = v
+            v += [0, 0]
+        if len(v) != 8:
+            raise SystemExit("--shape takes 3, 6 or 8 integers")
+        b, zi, pp, bs, zs, ps, ov, mg = v
         ela3d_amd.set_tuning(cfg, block=b, zseg_interior=zi, vec=pp, block_slab=bs, zseg_slab=zs,
-                             vec_slab=ps, overlap=bool(ov), merged=bool(mg), tab_lds=bool(tl))
+                             vec_slab=ps, overlap=bool(ov), merged=bool(mg))
         log(f"[bench] fixed launch shape interior block={b} zseg={zi} vec={pp}; "
             f"layers block={bs or b} zseg={zs} vec={ps}")
     elif not args.no_tune:
@@ -300,7 +300,6 @@ def main():
         log(f"[bench] autotune: interior block={tuned['block']} zseg={tuned['zseg_interior']} "
             f"vec={tuned['vec']}; layers block={tuned['block_slab']} zseg={tuned['zseg_slab']} "
             f"vec={tuned['vec_slab']} overlap={tuned['overlap']} merged={tuned['merged']} "
-            f"tab_lds={tuned['tab_lds']} "
             f"({tuned['ms_per_step']:.3f} ms/step over "
             f"{tuned['candidates']} shapes, {time.perf_counter() - t:.1f} s)")
     with Shot(cfg, *model) as shot:
@@ -362,7 +361,7 @@ def main():
             "launch_shape": {"block": info["block"], "zseg": info["zseg"], "vec": info["vec"],
                              "block_slab": info["block_slab"], "zseg_slab": info["zseg_slab"],
                              "vec_slab": info["vec_slab"], "overlap": info["overlap"],
-                             "merged": info["merged"], "tab_lds": info["tab_lds"]},
+                             "merged": info["merged"]},
         },
         "roofline": {
             "bound": "hbm", "kernel": dom["name"], "achieved": round(achieved / 1e9, 2),

@@ -17,9 +17,9 @@
 //  * C-PML memory variables live in compact per-axis slabs.  Interior boxes test
 //    x per element only (y and z lie outside the C-PML by construction); the
 //    absorbing-layer boxes test all three axes.  Each pass is two launches, one
-//    per box mode, each compiled for its mode (or one mixed launch).  The a/b/K
-//    tables of x and y are read as FP32 copies, through the caches or staged per
-//    workgroup in LDS (a launch-shape choice).
+//    per box mode, each compiled for its mode (or one mixed launch).  FP32
+//    copies of the a/b/K tables of x (and y in absorbing-layer boxes) are staged
+//    per workgroup in LDS.
 //  * Arithmetic is FP32 (FP64 for the FP64 storage instantiation); energy
 //    accumulates in FP64.
 //
@@ -207,7 +207,8 @@ __device__ __forceinline__ void stage_tables(const StepArgs<T>& A, float* s, boo
 }
 
 // x and y coefficient tables of a pass: the FP64 tables for FP64 arithmetic,
-// otherwise the FP32 copies, staged in LDS or read through L1/L2.
+// otherwise the FP32 copies staged in LDS (32-bit LDS addressing keeps the
+// register footprint below that of reading them through the caches).
 template <typename T>
 __device__ __forceinline__ void pick_tables(const StepArgs<T>& A, float* lds, bool with_y, const TabP<T>*& tx,
                                             const TabP<T>*& ty)
@@ -216,13 +217,9 @@ __device__ __forceinline__ void pick_tables(const StepArgs<T>& A, float* lds, bo
         tx = A.tab[0];
         ty = A.tab[1];
     } else {
-        const float* t = A.tabf;
-        if (A.tab_lds) {
-            stage_tables(A, lds, with_y);
-            t = lds;
-        }
-        tx = t;
-        ty = t + 6 * A.nx;
+        stage_tables(A, lds, with_y);
+        tx = lds;
+        ty = lds + 6 * A.nx;
     }
 }
 
@@ -830,7 +827,7 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
 template <typename T>
 static size_t table_lds(const StepArgs<T>& a)
 {
-    if (std::is_same<T, double>::value || !a.tab_lds) return 0;
+    if (std::is_same<T, double>::value) return 0;
     return size_t(6) * (a.nx + (a.mode != kInterior ? a.ny : 0)) * sizeof(float);
 }
 

@@ -43,7 +43,6 @@ struct StepArgs {
     const float* rho;
     const double* tab[3];  // per axis [6][n]: a, b, K (full grid), a, b, K (half grid)
     const float* tabf;     // FP32 copies of the x then y tables: [6][nx] then [6][ny]
-    int tab_lds;           // 1: each workgroup stages tabf into LDS; 0: read it through the caches
     int nx, ny, nz, nxy;
     int nxyp;                // padded plane stride of the 9 fields and the model (elements)
     int g0;                  // guard elements before plane 0; f[] and cp/cs/rho point at the guard

@@ -457,7 +457,6 @@ public:
         o.vec_slab = tune_.vec_slab;
         o.overlap = overlap_ ? 1 : 0;
         o.merged = tune_.merged;
-        o.tab_lds = tune_.tab_lds;
         for (int a = 0; a < 3; ++a) {
             o.region_lo[a] = plan_.ax[a].lo;
             o.region_hi[a] = plan_.ax[a].hi;
@@ -638,7 +637,6 @@ private:
         a.rho = d_model_[2] - guard_;
         for (int q = 0; q < 3; ++q) a.tab[q] = d_tab_[q];
         a.tabf = d_tabf_;
-        a.tab_lds = tune_.tab_lds;
         a.nx = c.nx; a.ny = c.ny; a.nz = c.nz; a.nxy = c.nx * c.ny;
         a.nxyp = int(nxyp_);
         a.g0 = int(guard_);

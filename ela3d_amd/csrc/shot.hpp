@@ -40,7 +40,6 @@ struct Tuning {
     int overlap = 0;        // 1: absorbing-layer launches concurrent with the interior ones
     int merged = 0;         // 1: one mixed launch per pass over all boxes (interior shape;
                             //    z layers with zseg_slab)
-    int tab_lds = 0;        // 1: stage the x/y coefficient tables in LDS per workgroup
 };
 
 // Device layout of the 9 fields and the 3 model arrays: planes of plane_stride

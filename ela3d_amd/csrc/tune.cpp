@@ -97,18 +97,6 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
             ++tried;
         };
         double ms[2], best_int = 1e300, best_slab = 1e300;
-        {  // coefficient tables through the caches or staged in LDS
-            double t2[2];
-            for (int l = 0; l < 2; ++l) {
-                Tuning t = best;
-                t.tab_lds = l;
-                measure(t, ms);
-                t2[l] = ms[0] + ms[1];
-            }
-            best.tab_lds = t2[1] < t2[0] ? 1 : 0;
-            if (verbose)
-                std::fprintf(stderr, "[tune] tables via caches %.4f ms, via LDS %.4f ms\n", t2[0], t2[1]);
-        }
         for (int vv : vecs)
             for (int b : blocks)
                 for (int zi : zsi) {
@@ -204,7 +192,6 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
             out->vec_slab = best.vec_slab;
             out->overlap = best.overlap;
             out->merged = best.merged;
-            out->tab_lds = best.tab_lds;
             out->ms_per_step = best_ms;
             out->candidates = tried;
         }
@@ -236,7 +223,6 @@ int ela3d_tuning_set(const ela3d_config* cfg, const ela3d_tuning* t)
     u.vec_slab = vs;
     u.overlap = t->overlap ? 1 : 0;
     u.merged = t->merged ? 1 : 0;
-    u.tab_lds = t->tab_lds ? 1 : 0;
     ela3d::store_tuning(*cfg, u, t->ms_per_step);
     return ELA3D_OK;
 }
@@ -254,7 +240,6 @@ int ela3d_tuning_get(const ela3d_config* cfg, ela3d_tuning* t)
     t->vec_slab = u.vec_slab;
     t->overlap = u.overlap;
     t->merged = u.merged;
-    t->tab_lds = u.tab_lds;
     t->ms_per_step = 0.0;
     t->candidates = 0;
     return ELA3D_OK;

@@ -114,7 +114,7 @@ def test_launch_shapes_agree(gpu, storage):
     out = []
     for (b, zi, v, bs, zs, vs, ov, mg) in shapes:
         set_tuning(cfg, block=b, zseg_interior=zi, vec=v, block_slab=bs, zseg_slab=zs, vec_slab=vs,
-                   overlap=ov, merged=mg, tab_lds=(b == 128))  # tables via LDS or the caches
+                   overlap=ov, merged=mg)
         with Shot(cfg, *model) as shot:
             inf = shot.info()
             assert (inf["block"], inf["vec"], inf["vec_slab"], inf["overlap"], inf["merged"]) == \
