@@ -68,7 +68,7 @@ def cpu_baseline(cfg, model, budget_s=15.0):
     t = time.perf_counter()
     oracle.run(sub, model, 1, st=st, dtype=np.float32, threads=threads)
     one = time.perf_counter() - t
-    nsteps = int(max(2, min(400, budget_s / max(one, 1e-6))))
+    nsteps = int(max(2, min(4000, budget_s / max(one, 1e-6))))
     t = time.perf_counter()
     oracle.run(sub, model, nsteps, st=st, it0=1, dtype=np.float32, threads=threads)
     wall = time.perf_counter() - t
@@ -259,12 +259,15 @@ def main():
     if args.calibrate > 0:
         import ela3d_amd
         nbytes = int(args.calibrate * (1 << 30))
-        variants = [args.copy_variant] if args.copy_variant >= 0 else [0, 1, 2, 3]
+        variants = [args.copy_variant] if args.copy_variant >= 0 else [0, 1, 2, 3, 4, 5, 6, 7]
+        arrays = {4: (12, 6), 5: (10, 3), 6: (12, 9), 7: (12, 6)}
         for v in variants:
-            ms = ela3d_amd.copy_bench(nbytes, reps=20, variant=v)
-            print(json.dumps({"copy_variant": v, "copy_bytes_read": nbytes,
-                              "copy_bytes_written": nbytes, "ms_per_copy": ms,
-                              "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+            nr, nw = arrays.get(v, (1, 1))
+            size = nbytes if v < 4 else nbytes // 8  # multi-stream: every array this long
+            ms = ela3d_amd.copy_bench(size, reps=20, variant=v)
+            print(json.dumps({"copy_variant": v, "arrays_read": nr, "arrays_written": nw,
+                              "bytes_per_array": size, "ms_per_copy": ms,
+                              "GBps": (nr + nw) * size / (ms * 1e-3) / 1e9}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the time loop "

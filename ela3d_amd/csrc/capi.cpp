@@ -431,10 +431,36 @@ int ela3d_copy_bench(int32_t device, int64_t bytes, int32_t reps, int32_t varian
     return guarded([&] {
         int rc = need_device(device);
         if (rc) return rc;
-        if (bytes < 16 || reps < 1 || !ms_per_copy || variant < 0 || variant > 3 ||
+        if (bytes < 16 || reps < 1 || !ms_per_copy || variant < 0 || variant > 7 ||
             bytes > (int64_t(1) << 36))
             throw Error{ELA3D_E_INVALID, "bad argument"};
         HIP_CHECK(hipSetDevice(device));
+        if (variant >= 4) {  // multi-stream: every array holds `bytes`
+            int nr = 0, nw = 0;
+            ela3d::stream_counts(variant, nr, nw);
+            const long n = long(bytes / 16) * 4;  // floats per array, a multiple of 4
+            void* base = nullptr;
+            HIP_CHECK(hipMalloc(&base, size_t(n) * 4 * size_t(nr + nw)));
+            hipStream_t st;
+            hipEvent_t t0, t1;
+            HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            HIP_CHECK(hipEventCreate(&t0));
+            HIP_CHECK(hipEventCreate(&t1));
+            HIP_CHECK(hipMemsetAsync(base, 0, size_t(n) * 4 * size_t(nr + nw), st));
+            ela3d::launch_streams(static_cast<float*>(base), n, variant, st);  // warm-up
+            HIP_CHECK(hipEventRecord(t0, st));
+            for (int r = 0; r < reps; ++r) ela3d::launch_streams(static_cast<float*>(base), n, variant, st);
+            HIP_CHECK(hipEventRecord(t1, st));
+            HIP_CHECK(hipEventSynchronize(t1));
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+            *ms_per_copy = ms / reps;
+            hipEventDestroy(t0);
+            hipEventDestroy(t1);
+            hipStreamDestroy(st);
+            hipFree(base);
+            return ELA3D_OK;
+        }
         const long n4 = long(bytes / 16);
         void *a = nullptr, *b = nullptr;
         HIP_CHECK(hipMalloc(&a, size_t(n4) * 16));

@@ -94,5 +94,8 @@ void launch_velocity(const StepArgs<T>& a, bool energy, hipStream_t s);
 template <typename T>
 void launch_tail(const TailArgs<T>& a, hipStream_t s);
 void launch_copy(const float4* src, float4* dst, long n4, int variant, hipStream_t s);
+// multi-stream calibration (variants 4-7): arrays of n floats packed back to back at base
+int stream_counts(int variant, int& nreads, int& nwrites);  // returns floats per lane
+void launch_streams(float* base, long n, int variant, hipStream_t s);
 
 }  // namespace ela3d

@@ -205,6 +205,64 @@ __global__ void __launch_bounds__(256) k_copyf(const float* __restrict__ s, floa
     if (i < n) d[i] = s[i];
 }
 
+// Multi-stream calibration: NR arrays read and NW arrays written element-wise in
+// one pass, W floats per lane -- the access pattern of a stencil pass without
+// its neighbour loads (stress pass: 12 read, 6 written; velocity: 10 and 3;
+// one fused step: 12 and 9).
+struct StreamSet {
+    const float* in[12];
+    float* out[9];
+    long n;  // floats per array
+};
+
+template <int NR, int NW, int W>
+__global__ void __launch_bounds__(256) k_streams(const StreamSet S)
+{
+    typedef float vt __attribute__((ext_vector_type(W)));
+    const long i = (blockIdx.x * (long)blockDim.x + threadIdx.x) * W;
+    if (i + W > S.n) return;
+    vt acc = 0.f;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        vt v;
+        __builtin_memcpy(&v, S.in[r] + i, sizeof(vt));
+        acc += v;
+    }
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        const vt o = acc + (float)w;
+        __builtin_memcpy(S.out[w] + i, &o, sizeof(vt));
+    }
+}
+
+int stream_counts(int variant, int& nr, int& nw)
+{
+    switch (variant) {
+    case 4: nr = 12; nw = 6; return 1;
+    case 5: nr = 10; nw = 3; return 1;
+    case 6: nr = 12; nw = 9; return 1;
+    case 7: nr = 12; nw = 6; return 4;
+    default: nr = 1; nw = 1; return 4;
+    }
+}
+
+void launch_streams(float* base, long n, int variant, hipStream_t s)
+{
+    int nr = 0, nw = 0;
+    const int w = stream_counts(variant, nr, nw);
+    StreamSet S;
+    for (int r = 0; r < 12; ++r) S.in[r] = base + long(r < nr ? r : 0) * n;
+    for (int q = 0; q < 9; ++q) S.out[q] = base + long(nr + (q < nw ? q : 0)) * n;
+    S.n = n;
+    const unsigned blocks = (unsigned)((n / w + 255) / 256);
+    switch (variant) {
+    case 4: hipLaunchKernelGGL((k_streams<12, 6, 1>), dim3(blocks), dim3(256), 0, s, S); break;
+    case 5: hipLaunchKernelGGL((k_streams<10, 3, 1>), dim3(blocks), dim3(256), 0, s, S); break;
+    case 6: hipLaunchKernelGGL((k_streams<12, 9, 1>), dim3(blocks), dim3(256), 0, s, S); break;
+    default: hipLaunchKernelGGL((k_streams<12, 6, 4>), dim3(blocks), dim3(256), 0, s, S); break;
+    }
+}
+
 // ---------------------------------------------------------------------------
 template <typename T>
 void launch_tail(const TailArgs<T>& a, hipStream_t s)

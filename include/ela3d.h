@@ -264,7 +264,10 @@ const char* ela3d_last_error(void);
 int ela3d_device_count(int32_t* out);
 /* Stream-copy kernel of n bytes, timed with hipEvents (HBM ceiling calibration).
  * variant 0: grid-stride float4; 1: one float4 per thread; 2: four float4 per
- * thread, nontemporal; 3: one float per thread. */
+ * thread, nontemporal; 3: one float per thread.  Multi-stream variants, every
+ * array `bytes` long, element-wise in one pass: 4: 12 arrays read + 6 written,
+ * one float per lane (a stress pass); 5: 10 + 3 (a velocity pass); 6: 12 + 9
+ * (one fused step); 7: 12 + 6 with float4 per lane. */
 int ela3d_copy_bench(int32_t device, int64_t bytes, int32_t reps, int32_t variant, double* ms_per_copy);
 
 /* ---- seeded synthetic models (CPU:41-59 homogeneous; layered per the workload spec) ---- */
