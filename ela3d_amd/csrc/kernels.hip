@@ -12,8 +12,9 @@
 //    that hit L1/L2.
 //  * Each z step is load-all -> compute -> predicated store: one memory round
 //    trip per plane.
-//  * Workgroups are renumbered so each of the 8 XCDs walks a contiguous run of
-//    (chunk, z-segment) tiles, keeping in-plane halos inside one XCD's L2.
+//  * Workgroups are renumbered so each of the 8 XCDs takes a contiguous run of
+//    blocks, and a box's blocks are enumerated strip-major, so each XCD marches
+//    its own xy strip through z and in-plane and z-segment halos stay in its L2.
 //  * C-PML memory variables live in compact per-axis slabs.  Interior boxes test
 //    x per element only (y and z lie outside the C-PML by construction); the
 //    absorbing-layer boxes test all three axes.  Each pass is two launches, one
@@ -38,6 +39,27 @@ __device__ __forceinline__ int xcd_block(int b, int n)
     const int q = n / X, r = n % X;
     const int x = b % X, l = b / X;
     return x * q + min(x, r) + l;
+}
+
+// Tile (xy chunk, z segment) of a box's local block index, strip-major: the
+// chunks are cut into up to 8 strips, and each strip's chunks are enumerated
+// for every z segment before the next strip starts.  With the contiguous
+// per-XCD block ranges of xcd_block, each XCD then marches one xy strip through
+// z, so the plane above a z segment and the rows beside a chunk are re-read
+// from that XCD's L2 rather than from HBM.  A bijection onto
+// [0, nchunk) x [0, nzs) for local in [0, nchunk*nzs).
+__device__ __forceinline__ void tile_of(const Box& B, int local, int& chunk, int& zs)
+{
+    const int nzs = (B.k1 - B.k0 + B.zseg - 1) / B.zseg;
+    const int ns = min(8, B.nchunk);
+    const int w = B.nchunk / ns, r = B.nchunk % ns;  // strips 0..r-1 hold w+1 chunks
+    const int cp = local / nzs;
+    const int s = cp < r * (w + 1) ? cp / (w + 1) : r + (cp - r * (w + 1)) / w;
+    const int ws = w + (s < r ? 1 : 0);
+    const int c0 = s * w + min(s, r);
+    const int off = local - c0 * nzs;
+    chunk = c0 + off % ws;
+    zs = off / ws;
 }
 
 __device__ __forceinline__ Box pick_box(const Box* bx, int nbox, int lb)
@@ -479,10 +501,11 @@ __device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, 
                                             const TabP<T>* ty, Scales<T>& S)
 {
     typedef typename Arith<T>::R R;
-    const int chunk = local % B.nchunk, zs = local / B.nchunk;
+    int chunk, zs;
+    tile_of(B, local, chunk, zs);
     const int p0 = B.p0 + (chunk * A.block + (int)threadIdx.x) * V;
-    if (p0 >= B.p1) return;
     const int kb = B.k0 + zs * B.zseg;
+    if (p0 >= B.p1 || kb >= B.k1) return;
     const int ke = min(kb + B.zseg, B.k1);
     Lane<V> L;
     make_lane<T, MODE, V>(A, p0, B.p1, L);
@@ -726,10 +749,11 @@ __device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B
                                               const TabP<T>* ty, double& ek, double& ep, Scales<T>& S)
 {
     typedef typename Arith<T>::R R;
-    const int chunk = local % B.nchunk, zs = local / B.nchunk;
+    int chunk, zs;
+    tile_of(B, local, chunk, zs);
     const int p0 = B.p0 + (chunk * A.block + (int)threadIdx.x) * V;
-    if (p0 >= B.p1) return;
     const int kb = B.k0 + zs * B.zseg;
+    if (p0 >= B.p1 || kb >= B.k1) return;
     const int ke = min(kb + B.zseg, B.k1);
     Lane<V> L;
     make_lane<T, MODE, V>(A, p0, B.p1, L);
