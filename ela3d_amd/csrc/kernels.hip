@@ -62,6 +62,18 @@ __device__ __forceinline__ void tile_of(const Box& B, int local, int& chunk, int
     zs = off / ws;
 }
 
+// This workgroup's tile and the rows [jlo, jhi] its xy range spans.
+template <int V, typename T>
+__device__ __forceinline__ void block_tile(const StepArgs<T>& A, const Box& B, int local, int& chunk, int& zs,
+                                           int& jlo, int& jhi)
+{
+    tile_of(B, local, chunk, zs);
+    const int ps = B.p0 + chunk * A.block * V;
+    const int pe = min(ps + A.block * V, B.p1) - 1;
+    jlo = ps / A.nx;
+    jhi = min(pe / A.nx, A.ny - 1);
+}
+
 __device__ __forceinline__ Box pick_box(const Box* bx, int nbox, int lb)
 {
     Box B = bx[0];
@@ -220,11 +232,20 @@ using TabP = typename std::conditional<std::is_same<T, double>::value, double, f
 // (a, b, K on the full grid, a, b, K on the half grid).
 // Interior workgroups read only the x coefficients, so only absorbing-layer
 // workgroups stage the y tables.
+// The x tables are staged whole; of the y tables only rows [jlo, jhi] (the rows
+// this workgroup's xy range touches), at their places in the [6][ny] layout.
 template <typename T>
-__device__ __forceinline__ void stage_tables(const StepArgs<T>& A, float* s, bool with_y)
+__device__ __forceinline__ void stage_tables(const StepArgs<T>& A, float* s, bool with_y, int jlo, int jhi)
 {
-    const int n = 6 * A.nx + (with_y ? 6 * A.ny : 0);
-    for (int q = threadIdx.x; q < n; q += blockDim.x) s[q] = A.tabf[q];
+    const int nx6 = 6 * A.nx;
+    for (int q = threadIdx.x; q < nx6; q += blockDim.x) s[q] = A.tabf[q];
+    if (with_y) {
+        const int w = jhi - jlo + 1;
+        for (int q = threadIdx.x; q < 6 * w; q += blockDim.x) {
+            const int t = q / w, j = jlo + (q - t * w);
+            s[nx6 + t * A.ny + j] = A.tabf[nx6 + t * A.ny + j];
+        }
+    }
     __syncthreads();
 }
 
@@ -232,14 +253,14 @@ __device__ __forceinline__ void stage_tables(const StepArgs<T>& A, float* s, boo
 // otherwise the FP32 copies staged in LDS (32-bit LDS addressing keeps the
 // register footprint below that of reading them through the caches).
 template <typename T>
-__device__ __forceinline__ void pick_tables(const StepArgs<T>& A, float* lds, bool with_y, const TabP<T>*& tx,
-                                            const TabP<T>*& ty)
+__device__ __forceinline__ void pick_tables(const StepArgs<T>& A, float* lds, bool with_y, int jlo, int jhi,
+                                            const TabP<T>*& tx, const TabP<T>*& ty)
 {
     if constexpr (std::is_same<T, double>::value) {
         tx = A.tab[0];
         ty = A.tab[1];
     } else {
-        stage_tables(A, lds, with_y);
+        stage_tables(A, lds, with_y, jlo, jhi);
         tx = lds;
         ty = lds + 6 * A.nx;
     }
@@ -497,12 +518,10 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
 }
 
 template <typename T, int MODE, int V>
-__device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, int local, const TabP<T>* tx,
+__device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, int chunk, int zs, const TabP<T>* tx,
                                             const TabP<T>* ty, Scales<T>& S)
 {
     typedef typename Arith<T>::R R;
-    int chunk, zs;
-    tile_of(B, local, chunk, zs);
     const int p0 = B.p0 + (chunk * A.block + (int)threadIdx.x) * V;
     const int kb = B.k0 + zs * B.zseg;
     if (p0 >= B.p1 || kb >= B.k1) return;
@@ -745,12 +764,11 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
 }
 
 template <typename T, int MODE, bool ENERGY, int V>
-__device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B, int local, const TabP<T>* tx,
-                                              const TabP<T>* ty, double& ek, double& ep, Scales<T>& S)
+__device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B, int chunk, int zs,
+                                              const TabP<T>* tx, const TabP<T>* ty, double& ek, double& ep,
+                                              Scales<T>& S)
 {
     typedef typename Arith<T>::R R;
-    int chunk, zs;
-    tile_of(B, local, chunk, zs);
     const int p0 = B.p0 + (chunk * A.block + (int)threadIdx.x) * V;
     const int kb = B.k0 + zs * B.zseg;
     if (p0 >= B.p1 || kb >= B.k1) return;
@@ -797,14 +815,17 @@ __global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
     extern __shared__ float tabs[];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const Box B = MODE == kInterior ? A.box[0] : pick_box(A.box, A.nbox, lb);
+    int chunk, zs, jlo, jhi;
+    block_tile<V>(A, B, lb - B.blk0, chunk, zs, jlo, jhi);
     Scales<T> S;
     load_scales(A, S, true);
+    const bool slab = MODE == kSlab || (MODE == kMixed && B.mode == kSlab);
     const TabP<T>*tx, *ty;
-    pick_tables(A, tabs, MODE == kSlab || (MODE == kMixed && B.mode == kSlab), tx, ty);
-    if (MODE == kSlab || (MODE == kMixed && B.mode == kSlab))
-        stress_body<T, kSlab, V>(A, B, lb - B.blk0, tx, ty, S);
+    pick_tables(A, tabs, slab, jlo, jhi, tx, ty);
+    if (slab)
+        stress_body<T, kSlab, V>(A, B, chunk, zs, tx, ty, S);
     else
-        stress_body<T, kInterior, V>(A, B, lb - B.blk0, tx, ty, S);
+        stress_body<T, kInterior, V>(A, B, chunk, zs, tx, ty, S);
     flush_maxima(A, S, 3, 18);
 }
 
@@ -814,15 +835,18 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
     extern __shared__ float tabs[];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const Box B = MODE == kInterior ? A.box[0] : pick_box(A.box, A.nbox, lb);
+    int chunk, zs, jlo, jhi;
+    block_tile<V>(A, B, lb - B.blk0, chunk, zs, jlo, jhi);
     Scales<T> S;
     load_scales(A, S, false);
+    const bool slab = MODE == kSlab || (MODE == kMixed && B.mode == kSlab);
     const TabP<T>*tx, *ty;
-    pick_tables(A, tabs, MODE == kSlab || (MODE == kMixed && B.mode == kSlab), tx, ty);
+    pick_tables(A, tabs, slab, jlo, jhi, tx, ty);
     double ek = 0.0, ep = 0.0;
-    if (MODE == kSlab || (MODE == kMixed && B.mode == kSlab))
-        velocity_body<T, kSlab, ENERGY, V>(A, B, lb - B.blk0, tx, ty, ek, ep, S);
+    if (slab)
+        velocity_body<T, kSlab, ENERGY, V>(A, B, chunk, zs, tx, ty, ek, ep, S);
     else
-        velocity_body<T, kInterior, ENERGY, V>(A, B, lb - B.blk0, tx, ty, ek, ep, S);
+        velocity_body<T, kInterior, ENERGY, V>(A, B, chunk, zs, tx, ty, ek, ep, S);
     flush_maxima(A, S, 0, 3);
     flush_maxima(A, S, 18, 27);
     if (ENERGY) {  // per-block FP64 partials, summed in block order by the tail kernel

@@ -30,11 +30,13 @@ __device__ __forceinline__ void inject(const TailArgs<T>& A, int q, int o, typen
 // Per-step tail (one workgroup): receivers (before the source, as CPU:593-602
 // precedes CPU:604-622), energy reduction, source injection, step counter.
 // ---------------------------------------------------------------------------
+constexpr int kTailThreads = 1024;  // wide enough to sum ~10^5 energy partials quickly
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_tail(const TailArgs<T> A)
+__global__ void __launch_bounds__(kTailThreads) k_tail(const TailArgs<T> A)
 {
     typedef typename Arith<T>::R R;
-    __shared__ double red[2][4];
+    __shared__ double red[2][kTailThreads / 64];
     __shared__ int bad;
     __shared__ float inj[9];  // |value| written by the source injection, per field
     const int tid = threadIdx.x;
@@ -267,7 +269,7 @@ void launch_streams(float* base, long n, int variant, hipStream_t s)
 template <typename T>
 void launch_tail(const TailArgs<T>& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_tail<T>), dim3(1), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_tail<T>), dim3(1), dim3(kTailThreads), 0, s, a);
 }
 
 void launch_copy(const float4* src, float4* dst, long n4, int variant, hipStream_t s)
