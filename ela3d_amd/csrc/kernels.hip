@@ -347,7 +347,7 @@ __device__ __forceinline__ void stress_load(const StepArgs<T>& A, const Lane<V>&
     const unsigned o = cell(A, k, L.p0);
     const bool top = (k == A.nz - 1);
     const unsigned dU = top ? 0u : nxyp;
-    const T *vx = A.f[0], *vy = A.f[1], *vz = A.f[2];
+    const T *vx = A.fi[0], *vy = A.fi[1], *vz = A.fi[2];
     ldv<T, V>(vz, o, S.r[2], P.vz_c);
     ldm<V>(A.cp, o, P.cp_c);
     ldv<T, V>(vx, o + dU, S.r[0], P.vx_u);
@@ -364,33 +364,33 @@ __device__ __forceinline__ void stress_load(const StepArgs<T>& A, const Lane<V>&
     ldm<V>(A.cs, o + nx, P.csN);
     ldm<V>(A.rho, o - 1, P.rhoW);
     ldm<V>(A.rho, o + nx, P.rhoN);
-    ldv<T, V>(A.f[3], o, S.r[3], P.sxx);
-    ldv<T, V>(A.f[4], o, S.r[4], P.syy);
-    ldv<T, V>(A.f[5], o, S.r[5], P.szz);
-    ldv<T, V>(A.f[6], o, S.r[6], P.sxy);
-    ldv<T, V>(A.f[7], o, S.r[7], P.sxz);
-    ldv<T, V>(A.f[8], o, S.r[8], P.syz);
+    ldv<T, V>(A.fi[3], o, S.r[3], P.sxx);
+    ldv<T, V>(A.fi[4], o, S.r[4], P.syy);
+    ldv<T, V>(A.fi[5], o, S.r[5], P.szz);
+    ldv<T, V>(A.fi[6], o, S.r[6], P.sxy);
+    ldv<T, V>(A.fi[7], o, S.r[7], P.sxz);
+    ldv<T, V>(A.fi[8], o, S.r[8], P.syz);
     const int cz = MODE == kSlab ? (k < A.lo[2] ? k : (k >= A.hi[2] ? A.lo[2] + (k - A.hi[2]) : -1)) : -1;
 #pragma unroll
     for (int e = 0; e < V; ++e) {
         if (L.cx[e] >= 0) {
             const unsigned om = unsigned((k * A.ny + L.j[e]) * A.w[0] + L.cx[e]);
-            P.mx[0][e] = ldf(A.m[0], om, S.r[9]);
-            P.mx[1][e] = ldf(A.m[3], om, S.r[12]);
-            P.mx[2][e] = ldf(A.m[5], om, S.r[14]);
+            P.mx[0][e] = ldf(A.mi[0], om, S.r[9]);
+            P.mx[1][e] = ldf(A.mi[3], om, S.r[12]);
+            P.mx[2][e] = ldf(A.mi[5], om, S.r[14]);
         }
         if (MODE == kSlab) {
             if (L.cy[e] >= 0) {
                 const unsigned om = unsigned((k * A.w[1] + L.cy[e]) * A.nx + L.i[e]);
-                P.my[0][e] = ldf(A.m[1], om, S.r[10]);
-                P.my[1][e] = ldf(A.m[4], om, S.r[13]);
-                P.my[2][e] = ldf(A.m[7], om, S.r[16]);
+                P.my[0][e] = ldf(A.mi[1], om, S.r[10]);
+                P.my[1][e] = ldf(A.mi[4], om, S.r[13]);
+                P.my[2][e] = ldf(A.mi[7], om, S.r[16]);
             }
             if (cz >= 0 && L.act[e]) {  // elements past the box end may sit past the last row
                 const unsigned om = unsigned((cz * A.ny + L.j[e]) * A.nx + L.i[e]);
-                P.mz[0][e] = ldf(A.m[2], om, S.r[11]);
-                P.mz[1][e] = ldf(A.m[6], om, S.r[15]);
-                P.mz[2][e] = ldf(A.m[8], om, S.r[17]);
+                P.mz[0][e] = ldf(A.mi[2], om, S.r[11]);
+                P.mz[1][e] = ldf(A.mi[6], om, S.r[15]);
+                P.mz[2][e] = ldf(A.mi[8], om, S.r[17]);
             }
         }
     }
@@ -436,16 +436,16 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvz_dz = (P.vz_c[e] - Q.vz_m[e]) * idz;
             if (cx >= 0) {
                 const R m = cpml(P.mx[0][e], (R)tx[3 * nx + i], (R)tx[4 * nx + i], (R)tx[5 * nx + i], dvx_dx);
-                if (onN[e]) stm(A.m[0], omx, m, S, 9);
+                if (onN[e]) stm(A.mo[0], omx, m, S, 9);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[0][e], (R)ty[j], (R)ty[ny + j], (R)ty[2 * ny + j], dvy_dy);
-                    if (onN[e]) stm(A.m[1], omy, m, S, 10);
+                    if (onN[e]) stm(A.mo[1], omy, m, S, 10);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[0][e], Z.az, Z.bz, Z.Kz, dvz_dz);
-                    if (onN[e]) stm(A.m[2], omz, m, S, 11);
+                    if (onN[e]) stm(A.mo[2], omz, m, S, 11);
                 }
             }
             nsxx[e] = DT_l2m * dvx_dx + DT_l * (dvy_dy + dvz_dz) + P.sxx[e];
@@ -462,11 +462,11 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvx_dy = (P.vxN[e] - vx_c) * idy;
             if (cx >= 0) {
                 const R m = cpml(P.mx[1][e], (R)tx[i], (R)tx[nx + i], (R)tx[2 * nx + i], dvy_dx);
-                if (onXY[e]) stm(A.m[3], omx, m, S, 12);
+                if (onXY[e]) stm(A.mo[3], omx, m, S, 12);
             }
             if (MODE == kSlab && cy >= 0) {
                 const R m = cpml(P.my[1][e], (R)ty[3 * ny + j], (R)ty[4 * ny + j], (R)ty[5 * ny + j], dvx_dy);
-                if (onXY[e]) stm(A.m[4], omy, m, S, 13);
+                if (onXY[e]) stm(A.mo[4], omy, m, S, 13);
             }
             nsxy[e] = DT_mu * (dvy_dx + dvx_dy) + P.sxy[e];
         }
@@ -480,11 +480,11 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvx_dz = (P.vx_u[e] - vx_c) * idz;
             if (cx >= 0) {
                 const R m = cpml(P.mx[2][e], (R)tx[i], (R)tx[nx + i], (R)tx[2 * nx + i], dvz_dx);
-                if (onXZ[e]) stm(A.m[5], omx, m, S, 14);
+                if (onXZ[e]) stm(A.mo[5], omx, m, S, 14);
             }
             if (MODE == kSlab && Z.cz >= 0) {
                 const R m = cpml(P.mz[1][e], Z.azh, Z.bzh, Z.Kzh, dvx_dz);
-                if (onXZ[e]) stm(A.m[6], omz, m, S, 15);
+                if (onXZ[e]) stm(A.mo[6], omz, m, S, 15);
             }
             nsxz[e] = DT_mu * (dvz_dx + dvx_dz) + P.sxz[e];
         }
@@ -499,22 +499,22 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[2][e], (R)ty[3 * ny + j], (R)ty[4 * ny + j], (R)ty[5 * ny + j], dvz_dy);
-                    if (onYZ[e]) stm(A.m[7], omy, m, S, 16);
+                    if (onYZ[e]) stm(A.mo[7], omy, m, S, 16);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[2][e], Z.azh, Z.bzh, Z.Kzh, dvy_dz);
-                    if (onYZ[e]) stm(A.m[8], omz, m, S, 17);
+                    if (onYZ[e]) stm(A.mo[8], omz, m, S, 17);
                 }
             }
             nsyz[e] = DT_mu * (dvz_dy + dvy_dz) + P.syz[e];
         }
     }
-    stv<T, V>(A.f[3], o, nsxx, onN, S, 3);
-    stv<T, V>(A.f[4], o, nsyy, onN, S, 4);
-    stv<T, V>(A.f[5], o, nszz, onN, S, 5);
-    stv<T, V>(A.f[6], o, nsxy, onXY, S, 6);
-    stv<T, V>(A.f[7], o, nsxz, onXZ, S, 7);
-    stv<T, V>(A.f[8], o, nsyz, onYZ, S, 8);
+    stv<T, V>(A.fo[3], o, nsxx, onN, S, 3);
+    stv<T, V>(A.fo[4], o, nsyy, onN, S, 4);
+    stv<T, V>(A.fo[5], o, nszz, onN, S, 5);
+    stv<T, V>(A.fo[6], o, nsxy, onXY, S, 6);
+    stv<T, V>(A.fo[7], o, nsxz, onXZ, S, 7);
+    stv<T, V>(A.fo[8], o, nsyz, onYZ, S, 8);
 }
 
 template <typename T, int MODE, int V>
@@ -531,12 +531,12 @@ __device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, 
     SQueue<R, V> Q;
     {
         const unsigned o = cell(A, kb, p0);
-        ldv<T, V>(A.f[0], o, S.r[0], Q.vx_c);
-        ldv<T, V>(A.f[1], o, S.r[1], Q.vy_c);
+        ldv<T, V>(A.fi[0], o, S.r[0], Q.vx_c);
+        ldv<T, V>(A.fi[1], o, S.r[1], Q.vy_c);
         ldm<V>(A.cs, o, Q.cs_c);
         ldm<V>(A.rho, o, Q.rho_c);
         if (kb >= 1)
-            ldv<T, V>(A.f[2], o - unsigned(A.nxyp), S.r[2], Q.vz_m);
+            ldv<T, V>(A.fi[2], o - unsigned(A.nxyp), S.r[2], Q.vz_m);
         else
             for (int e = 0; e < V; ++e) Q.vz_m[e] = 0;
     }
@@ -581,7 +581,7 @@ __device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V
     const unsigned o = cell(A, k, L.p0);
     const bool top = (k == A.nz - 1);
     const unsigned dU = top ? 0u : nxyp;
-    const T *sxx = A.f[3], *syy = A.f[4], *szz = A.f[5], *sxy = A.f[6], *sxz = A.f[7], *syz = A.f[8];
+    const T *sxx = A.fi[3], *syy = A.fi[4], *szz = A.fi[5], *sxy = A.fi[6], *sxz = A.fi[7], *syz = A.fi[8];
     ldv<T, V>(sxx, o, S.r[3], P.sxx);
     ldv<T, V>(syy, o, S.r[4], P.syy);
     ldv<T, V>(sxy, o, S.r[6], P.sxy);
@@ -597,9 +597,9 @@ __device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V
     ldm<V>(A.rho, o + nx, P.rhoN);
     ldv<T, V>(sxz, o + 1, S.r[7], P.sxzE);
     ldv<T, V>(syz, o - nx, S.r[8], P.syzS);
-    ldv<T, V>(A.f[0], o, S.r[0], P.vx);
-    ldv<T, V>(A.f[1], o, S.r[1], P.vy);
-    ldv<T, V>(A.f[2], o, S.r[2], P.vz);
+    ldv<T, V>(A.fi[0], o, S.r[0], P.vx);
+    ldv<T, V>(A.fi[1], o, S.r[1], P.vy);
+    ldv<T, V>(A.fi[2], o, S.r[2], P.vz);
     if (ENERGY) {
         ldm<V>(A.cp, o, P.cp);
         ldm<V>(A.cs, o, P.cs);
@@ -609,22 +609,22 @@ __device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V
     for (int e = 0; e < V; ++e) {
         if (L.cx[e] >= 0) {
             const unsigned om = unsigned((k * A.ny + L.j[e]) * A.w[0] + L.cx[e]);
-            P.mx[0][e] = ldf(A.m[9], om, S.r[18]);
-            P.mx[1][e] = ldf(A.m[12], om, S.r[21]);
-            P.mx[2][e] = ldf(A.m[15], om, S.r[24]);
+            P.mx[0][e] = ldf(A.mi[9], om, S.r[18]);
+            P.mx[1][e] = ldf(A.mi[12], om, S.r[21]);
+            P.mx[2][e] = ldf(A.mi[15], om, S.r[24]);
         }
         if (MODE == kSlab) {
             if (L.cy[e] >= 0) {
                 const unsigned om = unsigned((k * A.w[1] + L.cy[e]) * A.nx + L.i[e]);
-                P.my[0][e] = ldf(A.m[10], om, S.r[19]);
-                P.my[1][e] = ldf(A.m[13], om, S.r[22]);
-                P.my[2][e] = ldf(A.m[16], om, S.r[25]);
+                P.my[0][e] = ldf(A.mi[10], om, S.r[19]);
+                P.my[1][e] = ldf(A.mi[13], om, S.r[22]);
+                P.my[2][e] = ldf(A.mi[16], om, S.r[25]);
             }
             if (cz >= 0 && L.act[e]) {  // elements past the box end may sit past the last row
                 const unsigned om = unsigned((cz * A.ny + L.j[e]) * A.nx + L.i[e]);
-                P.mz[0][e] = ldf(A.m[11], om, S.r[20]);
-                P.mz[1][e] = ldf(A.m[14], om, S.r[23]);
-                P.mz[2][e] = ldf(A.m[17], om, S.r[26]);
+                P.mz[0][e] = ldf(A.mi[11], om, S.r[20]);
+                P.mz[1][e] = ldf(A.mi[14], om, S.r[23]);
+                P.mz[2][e] = ldf(A.mi[17], om, S.r[26]);
             }
         }
     }
@@ -667,16 +667,16 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d3 = (P.sxz[e] - Q.sxz_m[e]) * idz;
             if (cx >= 0) {
                 const R m = cpml(P.mx[0][e], (R)tx[i], (R)tx[nx + i], (R)tx[2 * nx + i], d1);
-                if (onx) stm(A.m[9], omx, m, S, 18);
+                if (onx) stm(A.mo[9], omx, m, S, 18);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[0][e], (R)ty[j], (R)ty[ny + j], (R)ty[2 * ny + j], d2);
-                    if (onx) stm(A.m[10], omy, m, S, 19);
+                    if (onx) stm(A.mo[10], omy, m, S, 19);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[0][e], Z.az, Z.bz, Z.Kz, d3);
-                    if (onx) stm(A.m[11], omz, m, S, 20);
+                    if (onx) stm(A.mo[11], omz, m, S, 20);
                 }
             }
             if (onx) nvx[e] = DT_over_rho * (d1 + d2 + d3) + P.vx[e];
@@ -692,16 +692,16 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d3 = (P.syz[e] - Q.syz_m[e]) * idz;
             if (cx >= 0) {
                 const R m = cpml(P.mx[1][e], (R)tx[3 * nx + i], (R)tx[4 * nx + i], (R)tx[5 * nx + i], d1);
-                if (ony) stm(A.m[12], omx, m, S, 21);
+                if (ony) stm(A.mo[12], omx, m, S, 21);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[1][e], (R)ty[3 * ny + j], (R)ty[4 * ny + j], (R)ty[5 * ny + j], d2);
-                    if (ony) stm(A.m[13], omy, m, S, 22);
+                    if (ony) stm(A.mo[13], omy, m, S, 22);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[1][e], Z.az, Z.bz, Z.Kz, d3);
-                    if (ony) stm(A.m[14], omz, m, S, 23);
+                    if (ony) stm(A.mo[14], omz, m, S, 23);
                 }
             }
             if (ony) nvy[e] = DT_over_rho * (d1 + d2 + d3) + P.vy[e];
@@ -717,16 +717,16 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d3 = (P.szz_u[e] - Q.szz_c[e]) * idz;
             if (cx >= 0) {
                 const R m = cpml(P.mx[2][e], (R)tx[3 * nx + i], (R)tx[4 * nx + i], (R)tx[5 * nx + i], d1);
-                if (onz) stm(A.m[15], omx, m, S, 24);
+                if (onz) stm(A.mo[15], omx, m, S, 24);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[2][e], (R)ty[j], (R)ty[ny + j], (R)ty[2 * ny + j], d2);
-                    if (onz) stm(A.m[16], omy, m, S, 25);
+                    if (onz) stm(A.mo[16], omy, m, S, 25);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[2][e], Z.azh, Z.bzh, Z.Kzh, d3);
-                    if (onz) stm(A.m[17], omz, m, S, 26);
+                    if (onz) stm(A.mo[17], omz, m, S, 26);
                 }
             }
             if (onz) nvz[e] = DT_over_rho * (d1 + d2 + d3) + P.vz[e];
@@ -758,9 +758,9 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             }
         }
     }
-    stv<T, V>(A.f[0], o, nvx, wx, S, 0);
-    stv<T, V>(A.f[1], o, nvy, wy, S, 1);
-    stv<T, V>(A.f[2], o, nvz, wz, S, 2);
+    stv<T, V>(A.fo[0], o, nvx, wx, S, 0);
+    stv<T, V>(A.fo[1], o, nvy, wy, S, 1);
+    stv<T, V>(A.fo[2], o, nvz, wz, S, 2);
 }
 
 template <typename T, int MODE, bool ENERGY, int V>
@@ -779,15 +779,15 @@ __device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B
     {
         const unsigned o = cell(A, kb, p0);
         if (kb >= 1) {
-            ldv<T, V>(A.f[7], o - unsigned(A.nxyp), S.r[7], Q.sxz_m);
-            ldv<T, V>(A.f[8], o - unsigned(A.nxyp), S.r[8], Q.syz_m);
+            ldv<T, V>(A.fi[7], o - unsigned(A.nxyp), S.r[7], Q.sxz_m);
+            ldv<T, V>(A.fi[8], o - unsigned(A.nxyp), S.r[8], Q.syz_m);
         } else {
             for (int e = 0; e < V; ++e) {
                 Q.sxz_m[e] = 0;
                 Q.syz_m[e] = 0;
             }
         }
-        ldv<T, V>(A.f[5], o, S.r[5], Q.szz_c);
+        ldv<T, V>(A.fi[5], o, S.r[5], Q.szz_c);
         ldm<V>(A.rho, o, Q.rho_c);
     }
     for (int k = kb; k < ke; ++k) {

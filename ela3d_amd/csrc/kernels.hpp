@@ -34,10 +34,17 @@ struct Box {
     int mode;    // kInterior or kSlab
 };
 
+// Fields are read through fi and written through fo, memory variables read
+// through mi and written through mo.  In place, fi == fo and mi == mo.  Double
+// buffered, the stress pass reads v and sigma of the current set and writes
+// sigma to the next set, and the velocity pass reads the new sigma and the
+// current v and writes v to the next set.
 template <typename T>
 struct StepArgs {
-    T* f[9];           // vx vy vz sxx syy szz sxy sxz syz, x-fastest nx*ny*nz
-    T* m[18];          // memory variables, compact per axis family (see shot.cpp)
+    const T* fi[9];    // vx vy vz sxx syy szz sxy sxz syz, padded planes (see shot.cpp)
+    T* fo[9];
+    const T* mi[18];   // memory variables, compact per axis family (see shot.cpp)
+    T* mo[18];
     const float* cp;
     const float* cs;
     const float* rho;

@@ -102,7 +102,7 @@ static size_t mem_elems(const ShotPlan& p, int axis)
 // Byte offsets of every buffer in a shot's single device allocation.  Each
 // buffer starts on a 256-byte boundary; the energy-partial buffer has room for
 // any launch shape the tuner may pick (block >= 64, vec >= 1, zseg >= 1).
-ArenaLayout arena_layout(const ShotPlan& p, int elem, int nepart)
+ArenaLayout arena_layout(const ShotPlan& p, int elem, int nepart, bool dbuf)
 {
     const ela3d_config& c = p.cfg;
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
@@ -112,6 +112,14 @@ ArenaLayout arena_layout(const ShotPlan& p, int elem, int nepart)
     size_t off = 0;
     for (int q = 0; q < 9; ++q) { L.f[q] = off; off += al(padded * elem); }
     for (int q = 0; q < 18; ++q) { L.m[q] = off; off += al(mem_elems(p, kMemAxis[q]) * elem); }
+    for (int q = 0; q < 9; ++q) {  // second set of a double-buffered shot
+        L.f2[q] = L.f[q];
+        if (dbuf) { L.f2[q] = off; off += al(padded * elem); }
+    }
+    for (int q = 0; q < 18; ++q) {
+        L.m2[q] = L.m[q];
+        if (dbuf) { L.m2[q] = off; off += al(mem_elems(p, kMemAxis[q]) * elem); }
+    }
     for (int q = 0; q < 3; ++q) { L.model[q] = off; off += al(padded * sizeof(float)); }
     const int ns[3] = {c.nx, c.ny, c.nz};
     for (int q = 0; q < 3; ++q) { L.tab[q] = off; off += al(6 * size_t(ns[q]) * sizeof(double)); }
@@ -133,13 +141,13 @@ ArenaLayout arena_layout(const ShotPlan& p, int elem, int nepart)
 
 int64_t shot_device_bytes(const ShotPlan& p)
 {
-    return int64_t(arena_layout(p, storage_bytes(p.cfg.storage), 0).total);
+    return int64_t(arena_layout(p, storage_bytes(p.cfg.storage), 0, choose_tuning(p).dbuf != 0).total);
 }
 
 template <typename T>
 class ShotImpl final : public Shot {
 public:
-    ShotImpl(ShotPlan plan, const float* cp, const float* cs, const float* rho, hipStream_t ext)
+    ShotImpl(ShotPlan plan, const float* cp, const float* cs, const float* rho, hipStream_t ext, bool dbuf)
     {
         plan_ = std::move(plan);
         const ela3d_config& c = plan_.cfg;
@@ -158,6 +166,8 @@ public:
         for (auto& e : fork_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         for (auto& e : join_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         tune_ = choose_tuning(plan_);
+        dbuf_ = dbuf || tune_.dbuf != 0;
+        tune_.dbuf = dbuf_ ? 1 : 0;
         overlap_ = tune_.overlap != 0;
         geometry(plan_, tune_, geo_[1], geo_[0]);
         allocate();
@@ -203,12 +213,13 @@ public:
         for (int s = 0; s < n; ++s) {
             const int it = host_step_ + 1;
             const bool e = energy_step(it);
+            const int par = cur();
             if (plan_.cfg.use_graph) {
-                hipGraphExec_t& g = graphs_[e ? 1 : 0];
-                if (!g) g = capture(e);
+                hipGraphExec_t& g = graphs_[2 * par + (e ? 1 : 0)];
+                if (!g) g = capture(e, par);
                 HIP_CHECK(hipGraphLaunch(g, stream_));
             } else {
-                launch_step(e, nullptr);
+                launch_step(e, nullptr, par);
             }
             host_step_ = it;
         }
@@ -225,7 +236,7 @@ public:
             const int it = done + 1;
             HIP_CHECK(hipMemcpyAsync(d_step_, &done, sizeof(int), hipMemcpyHostToDevice, stream_));
             HIP_CHECK(hipStreamSynchronize(stream_));
-            launch_step(energy_step(it), nullptr);
+            launch_step(energy_step(it), nullptr, cur());
             HIP_CHECK(hipGetLastError());
             if (c.nrec > 0 && it % c.sample_every == 0 && it / c.sample_every - 1 < plan_.nsamples) {
                 const size_t r = size_t(it / c.sample_every - 1) * c.nrec * 3;
@@ -310,7 +321,7 @@ public:
         const size_t cells = size_t(plan_.cells);
         if (field < 9) {
             std::vector<T> h(cells);
-            copy_planes(h.data(), d_f_[field], sizeof(T), hipMemcpyDeviceToHost);
+            copy_planes(h.data(), d_f_[cur()][field], sizeof(T), hipMemcpyDeviceToHost);
             const double inv = 1.0 / current_scale(field);
             for (size_t q = 0; q < cells; ++q) out[q] = double(h[q]) * inv;
             return;
@@ -318,7 +329,7 @@ public:
         const int m = field - 9, axis = kMemAxis[m];
         const size_t nm = mem_count(axis);
         std::vector<T> h(nm);
-        if (nm) HIP_CHECK(hipMemcpy(h.data(), d_m_[m], nm * sizeof(T), hipMemcpyDeviceToHost));
+        if (nm) HIP_CHECK(hipMemcpy(h.data(), d_m_[cur()][m], nm * sizeof(T), hipMemcpyDeviceToHost));
         std::fill(out, out + cells, 0.0);
         const double inv = 1.0 / current_scale(field);
         for_each_compact(axis, [&](size_t full, size_t comp) { out[full] = double(h[comp]) * inv; });
@@ -334,7 +345,8 @@ public:
             std::vector<T> h(cells);
             const double sc = current_scale(field);
             for (size_t q = 0; q < cells; ++q) h[q] = T(in[q] * sc);
-            copy_planes(d_f_[field], h.data(), sizeof(T), hipMemcpyHostToDevice);
+            for (int b = 0; b < (dbuf_ ? 2 : 1); ++b)  // both sets: cells a step never writes keep it
+                copy_planes(d_f_[b][field], h.data(), sizeof(T), hipMemcpyHostToDevice);
             return;
         }
         const int m = field - 9, axis = kMemAxis[m];
@@ -351,7 +363,8 @@ public:
                 throw Error{ELA3D_E_INVALID,
                             "memory variables are stored only inside the C-PML layers; "
                             "non-zero value outside them"};
-        if (nm) HIP_CHECK(hipMemcpy(d_m_[m], h.data(), nm * sizeof(T), hipMemcpyHostToDevice));
+        for (int b = 0; b < (dbuf_ ? 2 : 1); ++b)
+            if (nm) HIP_CHECK(hipMemcpy(d_m_[b][m], h.data(), nm * sizeof(T), hipMemcpyHostToDevice));
     }
 
     void profile(int n, std::vector<ela3d_kernel_stats>& st, double& wall_ms) override
@@ -387,7 +400,7 @@ public:
             const int m = std::min(chunk, n - done);
             for (int s = 0; s < m; ++s) {
                 const int it = host_step_ + 1;
-                launch_step(energy_step(it), &ev[size_t(s) * E]);
+                launch_step(energy_step(it), &ev[size_t(s) * E], cur());
                 host_step_ = it;
             }
             HIP_CHECK(hipGetLastError());
@@ -415,11 +428,14 @@ public:
     {
         HIP_CHECK(hipSetDevice(device_));
         HIP_CHECK(hipStreamSynchronize(stream_));
+        if ((t.dbuf != 0) && !dbuf_)
+            throw Error{ELA3D_E_INVALID, "double buffering is fixed when the shot is created"};
         LaunchGeom g[2];
         geometry(plan_, t, g[1], g[0]);
         if (g[0].nblocks + g[1].nblocks > epart_cap_)
             throw Error{ELA3D_E_INVALID, "launch shape needs too many blocks"};
         tune_ = t;
+        tune_.dbuf = dbuf_ ? 1 : 0;
         overlap_ = t.overlap != 0;
         geo_[0] = g[0];
         geo_[1] = g[1];
@@ -457,6 +473,7 @@ public:
         o.vec_slab = tune_.vec_slab;
         o.overlap = overlap_ ? 1 : 0;
         o.merged = tune_.merged;
+        o.dbuf = dbuf_ ? 1 : 0;
         for (int a = 0; a < 3; ++a) {
             o.region_lo[a] = plan_.ax[a].lo;
             o.region_hi[a] = plan_.ax[a].hi;
@@ -512,7 +529,7 @@ private:
         nxyp_ = plane_stride(c.nx, c.ny);
         guard_ = guard_elems(c.nx);
         nepart_ = geo_[0].nblocks + geo_[1].nblocks;
-        const ArenaLayout L = arena_layout(plan_, int(sizeof(T)), nepart_);
+        const ArenaLayout L = arena_layout(plan_, int(sizeof(T)), nepart_, dbuf_);
         epart_cap_ = L.epart_cap;
         const size_t off = L.total;
         const size_t* o_f = L.f;
@@ -524,8 +541,14 @@ private:
         arena_bytes_ = int64_t(off);
         HIP_CHECK(hipMalloc(&arena_, off));
         char* b = static_cast<char*>(arena_);
-        for (int q = 0; q < 9; ++q) d_f_[q] = reinterpret_cast<T*>(b + o_f[q]) + guard_;
-        for (int q = 0; q < 18; ++q) d_m_[q] = reinterpret_cast<T*>(b + o_m[q]);
+        for (int q = 0; q < 9; ++q) {
+            d_f_[0][q] = reinterpret_cast<T*>(b + o_f[q]) + guard_;
+            d_f_[1][q] = reinterpret_cast<T*>(b + L.f2[q]) + guard_;
+        }
+        for (int q = 0; q < 18; ++q) {
+            d_m_[0][q] = reinterpret_cast<T*>(b + o_m[q]);
+            d_m_[1][q] = reinterpret_cast<T*>(b + L.m2[q]);
+        }
         for (int q = 0; q < 3; ++q) d_model_[q] = reinterpret_cast<float*>(b + o_model[q]) + guard_;
         for (int q = 0; q < 3; ++q) d_tab_[q] = reinterpret_cast<double*>(b + o_tab[q]);
         d_wav_ = reinterpret_cast<double*>(b + o_wav);
@@ -623,15 +646,26 @@ private:
         (void)c;
     }
 
-    StepArgs<T> step_args(const LaunchGeom& g, double* epart) const
+    // Arguments of one launch for one pass (0 stress, 1 velocity) of a step that
+    // starts from field set `par`: reads the current set (sigma of the velocity
+    // pass from the set just written), writes the other set (the same set in place).
+    StepArgs<T> step_args(const LaunchGeom& g, double* epart, int pass, int par) const
     {
         const ela3d_config& c = plan_.cfg;
         StepArgs<T> a;
         std::memset(&a, 0, sizeof(a));
         // stencil kernels address fields and model from the allocation start
         // (guard_ elements before plane 0) so neighbour offsets never go negative
-        for (int q = 0; q < 9; ++q) a.f[q] = d_f_[q] - guard_;
-        for (int q = 0; q < 18; ++q) a.m[q] = d_m_[q];
+        const int in = par, out = dbuf_ ? 1 - par : par;
+        for (int q = 0; q < 9; ++q) {
+            const int src = (pass == 1 && q >= 3) ? out : in;
+            a.fi[q] = d_f_[src][q] - guard_;
+            a.fo[q] = d_f_[out][q] - guard_;
+        }
+        for (int q = 0; q < 18; ++q) {
+            a.mi[q] = d_m_[in][q];
+            a.mo[q] = d_m_[out][q];
+        }
         a.cp = d_model_[0] - guard_;
         a.cs = d_model_[1] - guard_;
         a.rho = d_model_[2] - guard_;
@@ -667,16 +701,20 @@ private:
     // absorbing-layer launch first: its energy partials come first in block order
     void set_args()
     {
-        args_[1] = step_args(geo_[1], d_epart_);
-        args_[0] = step_args(geo_[0], d_epart_ + 2 * size_t(geo_[1].nblocks));
+        for (int par = 0; par < 2; ++par)
+            for (int pass = 0; pass < 2; ++pass) {
+                args_[par][1][pass] = step_args(geo_[1], d_epart_, pass, par);
+                args_[par][0][pass] = step_args(geo_[0], d_epart_ + 2 * size_t(geo_[1].nblocks), pass, par);
+            }
     }
 
-    TailArgs<T> tail_args() const
+    // tail of a step that starts from set `par` (works on the set that step writes)
+    TailArgs<T> tail_args(int par) const
     {
         const ela3d_config& c = plan_.cfg;
         TailArgs<T> a;
         std::memset(&a, 0, sizeof(a));
-        for (int q = 0; q < 9; ++q) a.f[q] = d_f_[q];
+        for (int q = 0; q < 9; ++q) a.f[q] = d_f_[dbuf_ ? 1 - par : par][q];
         a.cp = d_model_[0];
         a.rho = d_model_[2];
         a.rec = d_rec_;
@@ -718,7 +756,7 @@ private:
     // stream in sequence.  With ev != nullptr, events ev[2q], ev[2q+1] bracket
     // launch q (stress layers, stress interior, velocity layers, velocity
     // interior, tail) on the stream it runs on.
-    void launch_step(bool energy, hipEvent_t* ev)
+    void launch_step(bool energy, hipEvent_t* ev, int par)
     {
         auto mark = [&](int q, hipStream_t s) {
             if (ev) HIP_CHECK(hipEventRecord(ev[q], s));
@@ -731,15 +769,15 @@ private:
             }
             mark(4 * pass + 0, side);
             if (pass == 0)
-                launch_stress(args_[1], side);
+                launch_stress(args_[par][1][0], side);
             else
-                launch_velocity(args_[1], energy, side);
+                launch_velocity(args_[par][1][1], energy, side);
             mark(4 * pass + 1, side);
             mark(4 * pass + 2, stream_);
             if (pass == 0)
-                launch_stress(args_[0], stream_);
+                launch_stress(args_[par][0][0], stream_);
             else
-                launch_velocity(args_[0], energy, stream_);
+                launch_velocity(args_[par][0][1], energy, stream_);
             mark(4 * pass + 3, stream_);
             if (overlap_) {
                 HIP_CHECK(hipEventRecord(join_[pass], aux_));
@@ -747,16 +785,16 @@ private:
             }
         }
         mark(8, stream_);
-        launch_tail(tail_args(), stream_);
+        launch_tail(tail_args(par), stream_);
         mark(9, stream_);
     }
 
-    hipGraphExec_t capture(bool energy)
+    hipGraphExec_t capture(bool energy, int par)
     {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-        launch_step(energy, nullptr);
+        launch_step(energy, nullptr, par);
         HIP_CHECK(hipStreamEndCapture(stream_, &graph));
         hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         hipGraphDestroy(graph);
@@ -774,11 +812,14 @@ private:
     // stress (absorbing layers, interior), velocity (same), tail
     static constexpr int kLaunches = 5;
     LaunchGeom geo_[2];   // [0] interior launch, [1] absorbing-layer launch
-    StepArgs<T> args_[2];
+    StepArgs<T> args_[2][2][2];  // [field set of the step's start][launch][pass]
     void* arena_ = nullptr;
     int64_t arena_bytes_ = 0;
-    T* d_f_[9] = {};
-    T* d_m_[18] = {};
+    T* d_f_[2][9] = {};   // two sets when double buffered; set 1 aliases set 0 in place
+    T* d_m_[2][18] = {};
+    bool dbuf_ = false;
+    // set holding the current state: steps alternate sets when double buffered
+    int cur() const { return dbuf_ ? (host_step_ & 1) : 0; }
     float* d_model_[3] = {};
     double* d_tab_[3] = {};
     double *d_wav_ = nullptr, *d_seis_ = nullptr, *d_energy_ = nullptr, *d_epart_ = nullptr;
@@ -800,19 +841,19 @@ private:
         HIP_CHECK(hipMemcpy(&s, d_scl_ + q, sizeof(float), hipMemcpyDeviceToHost));
         return s;
     }
-    hipGraphExec_t graphs_[2] = {nullptr, nullptr};
+    hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};  // [set][energy]
 };
 
 std::unique_ptr<Shot> make_shot(ShotPlan plan, const float* cp, const float* cs, const float* rho,
-                                hipStream_t ext)
+                                hipStream_t ext, bool dbuf)
 {
     switch (plan.cfg.storage) {
     case ELA3D_STORAGE_FP64:
-        return std::unique_ptr<Shot>(new ShotImpl<double>(std::move(plan), cp, cs, rho, ext));
+        return std::unique_ptr<Shot>(new ShotImpl<double>(std::move(plan), cp, cs, rho, ext, dbuf));
     case ELA3D_STORAGE_FP16:
-        return std::unique_ptr<Shot>(new ShotImpl<half_t>(std::move(plan), cp, cs, rho, ext));
+        return std::unique_ptr<Shot>(new ShotImpl<half_t>(std::move(plan), cp, cs, rho, ext, dbuf));
     default:
-        return std::unique_ptr<Shot>(new ShotImpl<float>(std::move(plan), cp, cs, rho, ext));
+        return std::unique_ptr<Shot>(new ShotImpl<float>(std::move(plan), cp, cs, rho, ext, dbuf));
     }
 }
 

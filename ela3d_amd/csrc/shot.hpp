@@ -40,6 +40,7 @@ struct Tuning {
     int overlap = 0;        // 1: absorbing-layer launches concurrent with the interior ones
     int merged = 0;         // 1: one mixed launch per pass over all boxes (interior shape;
                             //    z layers with zseg_slab)
+    int dbuf = 0;           // 1: two field sets, each step reads one and writes the other
 };
 
 // Device layout of the 9 fields and the 3 model arrays: planes of plane_stride
@@ -48,11 +49,11 @@ int64_t plane_stride(int nx, int ny);
 int64_t guard_elems(int nx);
 
 struct ArenaLayout {
-    size_t f[9], m[18], model[3], tab[3];
+    size_t f[9], m[18], f2[9], m2[18], model[3], tab[3];
     size_t wav, rec, seis, energy, epart, ctl, scl, slot, srctrace, tabf, total;
     int epart_cap;
 };
-ArenaLayout arena_layout(const ShotPlan& p, int elem_bytes, int nepart);
+ArenaLayout arena_layout(const ShotPlan& p, int elem_bytes, int nepart, bool dbuf);
 
 class Shot {
 public:
@@ -92,8 +93,10 @@ protected:
     ShotPlan plan_;
 };
 
+// double_buffer: allocate a second field set even if the chosen tuning does not
+// need one (the tuner's scratch shot, which tries every schedule)
 std::unique_ptr<Shot> make_shot(ShotPlan plan, const float* cp, const float* cs, const float* rho,
-                                hipStream_t external_stream = nullptr);
+                                hipStream_t external_stream = nullptr, bool double_buffer = false);
 Tuning choose_tuning(const ShotPlan& p);
 bool cached_tuning(const ela3d_config& c, Tuning& out);
 void store_tuning(const ela3d_config& c, const Tuning& t, double ms);

@@ -192,6 +192,7 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
             out->vec_slab = best.vec_slab;
             out->overlap = best.overlap;
             out->merged = best.merged;
+            out->dbuf = best.dbuf;
             out->ms_per_step = best_ms;
             out->candidates = tried;
         }
@@ -223,6 +224,7 @@ int ela3d_tuning_set(const ela3d_config* cfg, const ela3d_tuning* t)
     u.vec_slab = vs;
     u.overlap = t->overlap ? 1 : 0;
     u.merged = t->merged ? 1 : 0;
+    u.dbuf = t->dbuf ? 1 : 0;
     ela3d::store_tuning(*cfg, u, t->ms_per_step);
     return ELA3D_OK;
 }
@@ -240,6 +242,7 @@ int ela3d_tuning_get(const ela3d_config* cfg, ela3d_tuning* t)
     t->vec_slab = u.vec_slab;
     t->overlap = u.overlap;
     t->merged = u.merged;
+    t->dbuf = u.dbuf;
     t->ms_per_step = 0.0;
     t->candidates = 0;
     return ELA3D_OK;

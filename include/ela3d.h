@@ -140,6 +140,7 @@ typedef struct ela3d_info {
     int32_t block_slab, vec_slab;  /* absorbing-layer launch: threads, xy positions per lane */
     int32_t overlap;               /* absorbing-layer launches concurrent with the interior ones */
     int32_t merged;                /* one mixed launch per pass */
+    int32_t dbuf;                  /* two field sets (double buffered) */
 } ela3d_info;
 
 #define ELA3D_MAX_KERNELS 8
@@ -253,6 +254,7 @@ typedef struct ela3d_tuning {
     int32_t vec_slab;     /* absorbing-layer launch: xy positions per lane, 1 or 2 (0 = 1) */
     int32_t overlap;      /* 1: absorbing-layer launches run concurrently with the interior ones */
     int32_t merged;       /* 1: one mixed launch per pass over every box (interior shape) */
+    int32_t dbuf;         /* 1: two field sets; each step reads one and writes the other */
 } ela3d_tuning;
 int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, const float* rho,
                    int32_t steps, ela3d_tuning* out);

@@ -20,9 +20,12 @@ pytestmark = pytest.mark.gpu
 CLI = os.path.join(os.path.dirname(ela3d_amd.__file__), "bin", "ela3d_run")
 
 
-@pytest.mark.parametrize("storage", [0, STORAGE_FP16])
-def test_checkpoint_resume_bit_identical(gpu, tmp_path, storage):
-    cfg = small_cfg(nstep=90, storage=storage, src_kind=SRC_EXPLOSIVE)
+@pytest.mark.parametrize("storage,dbuf", [(0, False), (STORAGE_FP16, False), (0, True)])
+def test_checkpoint_resume_bit_identical(gpu, tmp_path, storage, dbuf):
+    """Also with two field sets: the set holding the state follows the step count."""
+    cfg = small_cfg(nx=31 if dbuf else 29, nstep=90, storage=storage, src_kind=SRC_EXPLOSIVE)
+    if dbuf:
+        ela3d_amd.set_tuning(cfg, block=256, zseg_interior=4, zseg_slab=4, dbuf=True)
     model = hetero_model(cfg, seed=2)
     with Shot(cfg, *model) as a:
         a.run()
