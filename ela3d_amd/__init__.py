@@ -371,7 +371,7 @@ def autotune(cfg: Config, model, steps: int = 8) -> dict:
 
 def set_tuning(cfg: Config, block: int, zseg_interior: int, zseg_slab: int, vec: int = 1,
                block_slab: int = 0, vec_slab: int = 1, overlap: bool = False, merged: bool = False,
-               dbuf: bool = False):
+               dbuf: bool = False, fused: bool = False, tile: int = 1024):
     """Pin the launch shape later shots of this grid use (as if the tuner had chosen it).
 
     ``block``/``zseg_interior``/``vec`` shape the interior launches,
@@ -382,7 +382,7 @@ def set_tuning(cfg: Config, block: int, zseg_interior: int, zseg_slab: int, vec:
     t = TuningStruct(block=block, zseg_interior=zseg_interior, zseg_slab=zseg_slab, vec=vec,
                      candidates=0, ms_per_step=0.0, block_slab=block_slab, vec_slab=vec_slab,
                      overlap=1 if overlap else 0, merged=1 if merged else 0,
-                     dbuf=1 if dbuf else 0)
+                     dbuf=1 if (dbuf or fused) else 0, fused=1 if fused else 0, tile=int(tile))
     check(lib().ela3d_tuning_set(C.byref(s), C.byref(t)))
 
 

@@ -237,12 +237,16 @@ ShotPlan make_plan(const ela3d_config& in)
         if (boxes[q].mode == 0) {
             P.alg_stress_interior += bs;
             P.alg_velocity_interior += bv;
+            P.interior_cells += int64_t(boxes[q].p1 - boxes[q].p0) * (boxes[q].k1 - boxes[q].k0);
         } else {
             P.alg_stress_slab += bs;
             P.alg_velocity_slab += bv;
         }
     }
     P.alg_pml = P.alg_step() - (stress_cell_bytes(c) + velocity_cell_bytes(c)) * double(cells);
+    // two passes re-read v, sigma and rho: 27 fields + 4 model values per cell against
+    // 18 fields + 3 model values in one fused pass
+    P.fused_saving = stress_cell_bytes(c) + velocity_cell_bytes(c) - (18.0 * storage_bytes(c.storage) + 12.0);
     return P;
 }
 

@@ -44,9 +44,16 @@ struct ShotPlan {
     double alg_stress_interior = 0, alg_stress_slab = 0;
     double alg_velocity_interior = 0, alg_velocity_slab = 0;
     double alg_pml = 0;
+    int64_t interior_cells = 0;     // cells of the interior box
+    double fused_saving = 0;        // bytes per interior cell the fused schedule does not move
     double alg_step() const {
         return alg_stress_interior + alg_stress_slab + alg_velocity_interior + alg_velocity_slab;
     }
+    // fused interior step: every field read and written once, the model read once
+    double alg_fused_interior() const {
+        return alg_stress_interior + alg_velocity_interior - fused_saving * double(interior_cells);
+    }
+    double alg_step_fused() const { return alg_fused_interior() + alg_stress_slab + alg_velocity_slab; }
 };
 
 // An xy range [p0,p1) of the flattened plane (p = i + j*nx) times planes

@@ -405,10 +405,20 @@ __device__ __forceinline__ void stress_load(const StepArgs<T>& A, const Lane<V>&
     }
 }
 
+// Stresses after one stress update of a cell: the new value where the cell is
+// in the update range, the old one elsewhere (what memory holds after the pass).
+template <typename R, int V>
+struct SOut {
+    R sxx[V], syy[V], szz[V], sxy[V], sxz[V], syz[V];
+};
+
+// `store` = false computes without writing anything (a halo cell recomputed by
+// a neighbouring fused workgroup); `out`, if given, receives the stresses.
 template <typename T, int MODE, int V, typename R>
 __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V>& L, const TabP<T>* tx,
                                               const TabP<T>* ty, int k, const SPlane<R, V>& P,
-                                              const SQueue<R, V>& Q, Scales<T>& S)
+                                              const SQueue<R, V>& Q, Scales<T>& S, bool store = true,
+                                              SOut<R, V>* out = nullptr)
 {
     const unsigned o = cell(A, k, L.p0);
     const R dt = (R)A.dt, idx = (R)A.idx, idy = (R)A.idy, idz = (R)A.idz;
@@ -436,16 +446,16 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvz_dz = (P.vz_c[e] - Q.vz_m[e]) * idz;
             if (cx >= 0) {
                 const R m = cpml(P.mx[0][e], (R)tx[3 * nx + i], (R)tx[4 * nx + i], (R)tx[5 * nx + i], dvx_dx);
-                if (onN[e]) stm(A.mo[0], omx, m, S, 9);
+                if (store && onN[e]) stm(A.mo[0], omx, m, S, 9);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[0][e], (R)ty[j], (R)ty[ny + j], (R)ty[2 * ny + j], dvy_dy);
-                    if (onN[e]) stm(A.mo[1], omy, m, S, 10);
+                    if (store && onN[e]) stm(A.mo[1], omy, m, S, 10);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[0][e], Z.az, Z.bz, Z.Kz, dvz_dz);
-                    if (onN[e]) stm(A.mo[2], omz, m, S, 11);
+                    if (store && onN[e]) stm(A.mo[2], omz, m, S, 11);
                 }
             }
             nsxx[e] = DT_l2m * dvx_dx + DT_l * (dvy_dy + dvz_dz) + P.sxx[e];
@@ -462,11 +472,11 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvx_dy = (P.vxN[e] - vx_c) * idy;
             if (cx >= 0) {
                 const R m = cpml(P.mx[1][e], (R)tx[i], (R)tx[nx + i], (R)tx[2 * nx + i], dvy_dx);
-                if (onXY[e]) stm(A.mo[3], omx, m, S, 12);
+                if (store && onXY[e]) stm(A.mo[3], omx, m, S, 12);
             }
             if (MODE == kSlab && cy >= 0) {
                 const R m = cpml(P.my[1][e], (R)ty[3 * ny + j], (R)ty[4 * ny + j], (R)ty[5 * ny + j], dvx_dy);
-                if (onXY[e]) stm(A.mo[4], omy, m, S, 13);
+                if (store && onXY[e]) stm(A.mo[4], omy, m, S, 13);
             }
             nsxy[e] = DT_mu * (dvy_dx + dvx_dy) + P.sxy[e];
         }
@@ -480,11 +490,11 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvx_dz = (P.vx_u[e] - vx_c) * idz;
             if (cx >= 0) {
                 const R m = cpml(P.mx[2][e], (R)tx[i], (R)tx[nx + i], (R)tx[2 * nx + i], dvz_dx);
-                if (onXZ[e]) stm(A.mo[5], omx, m, S, 14);
+                if (store && onXZ[e]) stm(A.mo[5], omx, m, S, 14);
             }
             if (MODE == kSlab && Z.cz >= 0) {
                 const R m = cpml(P.mz[1][e], Z.azh, Z.bzh, Z.Kzh, dvx_dz);
-                if (onXZ[e]) stm(A.mo[6], omz, m, S, 15);
+                if (store && onXZ[e]) stm(A.mo[6], omz, m, S, 15);
             }
             nsxz[e] = DT_mu * (dvz_dx + dvx_dz) + P.sxz[e];
         }
@@ -499,16 +509,28 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[2][e], (R)ty[3 * ny + j], (R)ty[4 * ny + j], (R)ty[5 * ny + j], dvz_dy);
-                    if (onYZ[e]) stm(A.mo[7], omy, m, S, 16);
+                    if (store && onYZ[e]) stm(A.mo[7], omy, m, S, 16);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[2][e], Z.azh, Z.bzh, Z.Kzh, dvy_dz);
-                    if (onYZ[e]) stm(A.mo[8], omz, m, S, 17);
+                    if (store && onYZ[e]) stm(A.mo[8], omz, m, S, 17);
                 }
             }
             nsyz[e] = DT_mu * (dvz_dy + dvy_dz) + P.syz[e];
         }
     }
+    if (out) {
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            out->sxx[e] = onN[e] ? nsxx[e] : P.sxx[e];
+            out->syy[e] = onN[e] ? nsyy[e] : P.syy[e];
+            out->szz[e] = onN[e] ? nszz[e] : P.szz[e];
+            out->sxy[e] = onXY[e] ? nsxy[e] : P.sxy[e];
+            out->sxz[e] = onXZ[e] ? nsxz[e] : P.sxz[e];
+            out->syz[e] = onYZ[e] ? nsyz[e] : P.syz[e];
+        }
+    }
+    if (!store) return;
     stv<T, V>(A.fo[3], o, nsxx, onN, S, 3);
     stv<T, V>(A.fo[4], o, nsyy, onN, S, 4);
     stv<T, V>(A.fo[5], o, nszz, onN, S, 5);
@@ -950,6 +972,242 @@ void launch_velocity(const StepArgs<T>& a, bool energy, hipStream_t s)
     else
         launch_velocity_e<T, false>(a, s);
 }
+
+// ===========================================================================
+// Fused step over the interior box (double-buffered fields, FP32 or FP64).
+//
+// A workgroup owns an xy tile [q0, q1) of the flattened interior box and marches
+// its z-segment [kb, ke).  Per plane it computes the new stresses of the tile
+// and of one row of halo on each side (p in [q0-nx, q1+nx)) into LDS, and the
+// new velocities of the tile one plane behind, reading the in-plane stress
+// neighbours from LDS and its own column's z neighbours from registers.  Halo
+// cells inside the interior box are recomputed from the previous step's set
+// (bit-identical to what their owner computes; never written); halo cells in
+// the absorbing layers, and the planes just below and above the interior,
+// take the new stresses the absorbing-layer stress launch wrote.  The update
+// arithmetic is the two-pass kernels' own (stress_update, velocity_update).
+// ===========================================================================
+constexpr int kFusedThreads = 256;
+
+template <typename T>
+size_t fused_lds_bytes(int nx, int tile)
+{
+    typedef typename Arith<T>::R R;
+    const size_t tab = std::is_same<T, double>::value ? 0 : (size_t(6) * nx * sizeof(float) + 15) / 16 * 16;
+    const size_t bytes = tab + size_t(2) * 5 * size_t(tile + 2 * nx) * sizeof(R);
+    return bytes <= 65536 ? bytes : 0;
+}
+
+// new stresses of cell p at plane k: recomputed inside the interior box, else
+// read from the set the absorbing-layer stress launch wrote
+template <typename T, typename R>
+__device__ __forceinline__ void fused_sigma(const StepArgs<T>& A, const Box& B, const TabP<T>* tx, int p, int k,
+                                            bool store, Scales<T>& S, Scales<T>& Sv, SOut<R, 1>& out)
+{
+    if (p >= B.p0 && p < B.p1 && k >= B.k0 && k < B.k1) {
+        Lane<1> L;
+        make_lane<T, kInterior, 1>(A, p, B.p1, L);
+        SQueue<R, 1> Q;
+        const unsigned o = cell(A, k, p);
+        Q.vx_c[0] = ldf(A.fi[0], o, S.r[0]);
+        Q.vy_c[0] = ldf(A.fi[1], o, S.r[1]);
+        Q.cs_c[0] = (R)A.cs[o];
+        Q.rho_c[0] = (R)A.rho[o];
+        Q.vz_m[0] = k >= 1 ? (R)ldf(A.fi[2], o - unsigned(A.nxyp), S.r[2]) : (R)0;
+        SPlane<R, 1> P;
+        stress_load<T, kInterior, 1, R>(A, L, k, P, S);
+        stress_update<T, kInterior, 1, R>(A, L, tx, tx, k, P, Q, S, store, &out);
+    } else {
+        const unsigned o = cell(A, k, p);
+        out.sxx[0] = ldf((const T*)A.fo[3], o, Sv.r[3]);
+        out.syy[0] = ldf((const T*)A.fo[4], o, Sv.r[4]);
+        out.szz[0] = ldf((const T*)A.fo[5], o, Sv.r[5]);
+        out.sxy[0] = ldf((const T*)A.fo[6], o, Sv.r[6]);
+        out.sxz[0] = ldf((const T*)A.fo[7], o, Sv.r[7]);
+        out.syz[0] = ldf((const T*)A.fo[8], o, Sv.r[8]);
+    }
+}
+
+template <typename T, bool ENERGY, int NU>
+__global__ void __launch_bounds__(kFusedThreads) k_fused(const StepArgs<T> A)
+{
+    typedef typename Arith<T>::R R;
+    extern __shared__ float smem[];
+    const int tid = threadIdx.x;
+    const int nx = A.nx;
+    const Box B = A.box[0];
+    int chunk, zs;
+    tile_of(B, xcd_block(blockIdx.x, gridDim.x), chunk, zs);
+    const int q0 = B.p0 + chunk * A.tile, q1 = min(q0 + A.tile, B.p1);
+    const int kb = B.k0 + zs * B.zseg, ke = min(kb + B.zseg, B.k1);
+    const int rlo = -nx, rhi = (q1 - q0) + nx;  // region, relative to q0: [rlo, rhi)
+    const int LR = A.ldsreg;                     // stride of one component plane
+    Scales<T> S, Sv;
+    load_scales(A, S, true);
+    load_scales(A, Sv, false);
+    const TabP<T>* tx;
+    R* lds;
+    if constexpr (std::is_same<T, double>::value) {
+        tx = A.tab[0];
+        lds = reinterpret_cast<R*>(smem);
+    } else {
+        for (int q = tid; q < 6 * nx; q += kFusedThreads) smem[q] = A.tabf[q];
+        tx = smem;
+        lds = reinterpret_cast<R*>(smem + (6 * nx + 3) / 4 * 4);
+    }
+    // lds[(b*5 + comp)*LR + (c - rlo)]: comp 0 sxx, 1 sxy, 2 syy, 3 sxz, 4 syz of plane b = k & 1
+    auto at = [&](int b, int comp, int c) -> R& { return lds[(b * 5 + comp) * LR + (c - rlo)]; };
+    // own cells q0 + tid + 256*u: column queues across planes
+    R sxz_m[NU], syz_m[NU], szz_c[NU], szz_u[NU];
+    // the plane below the segment: own sxz, syz
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        const int c = tid + kFusedThreads * u;
+        sxz_m[u] = 0;
+        syz_m[u] = 0;
+        szz_c[u] = 0;
+        szz_u[u] = 0;
+        if (c < q1 - q0) {
+            SOut<R, 1> o;
+            fused_sigma<T, R>(A, B, tx, q0 + c, kb - 1, false, S, Sv, o);
+            sxz_m[u] = o.sxz[0];
+            syz_m[u] = o.syz[0];
+        }
+    }
+    // plane kb: the whole region into LDS, own cells stored
+    {
+        const int b = kb & 1;
+        for (int c = rlo + ((tid - rlo) % kFusedThreads + kFusedThreads) % kFusedThreads; c < rhi;
+             c += kFusedThreads) {
+            const bool own = c >= 0 && c < q1 - q0;
+            SOut<R, 1> o;
+            fused_sigma<T, R>(A, B, tx, q0 + c, kb, own, S, Sv, o);
+            at(b, 0, c) = o.sxx[0];
+            at(b, 1, c) = o.sxy[0];
+            at(b, 2, c) = o.syy[0];
+            at(b, 3, c) = o.sxz[0];
+            at(b, 4, c) = o.syz[0];
+            if (own) szz_c[(c - tid) / kFusedThreads] = o.szz[0];
+        }
+    }
+    __syncthreads();
+    double ek = 0.0, ep = 0.0;
+    for (int k = kb; k < ke; ++k) {
+        const int b = k & 1, nb = b ^ 1;
+        const bool last = (k + 1 == ke);
+        // stresses of plane k+1: the region (own cells stored) or, past the
+        // segment, the own cells' sigma_zz only
+        for (int c = (last ? tid : rlo + ((tid - rlo) % kFusedThreads + kFusedThreads) % kFusedThreads);
+             c < (last ? q1 - q0 : rhi); c += kFusedThreads) {
+            const bool own = c >= 0 && c < q1 - q0;
+            SOut<R, 1> o;
+            fused_sigma<T, R>(A, B, tx, q0 + c, k + 1, own && !last, S, Sv, o);
+            if (!last) {
+                at(nb, 0, c) = o.sxx[0];
+                at(nb, 1, c) = o.sxy[0];
+                at(nb, 2, c) = o.syy[0];
+                at(nb, 3, c) = o.sxz[0];
+                at(nb, 4, c) = o.syz[0];
+            }
+            if (own) szz_u[(c - tid) / kFusedThreads] = o.szz[0];
+        }
+        // velocities of plane k
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            const int c = tid + kFusedThreads * u;
+            if (c >= q1 - q0) continue;
+            const int p = q0 + c;
+            Lane<1> L;
+            make_lane<T, kInterior, 1>(A, p, B.p1, L);
+            const unsigned o = cell(A, k, p);
+            VPlane<R, 1> P;
+            VQueue<R, 1> Q;
+            P.sxx[0] = at(b, 0, c);
+            P.sxxW[0] = at(b, 0, c - 1);
+            P.sxy[0] = at(b, 1, c);
+            P.sxyS[0] = at(b, 1, c - nx);
+            P.sxyE[0] = at(b, 1, c + 1);
+            P.syy[0] = at(b, 2, c);
+            P.syyN[0] = at(b, 2, c + nx);
+            P.sxz[0] = at(b, 3, c);
+            P.sxzE[0] = at(b, 3, c + 1);
+            P.syz[0] = at(b, 4, c);
+            P.syzS[0] = at(b, 4, c - nx);
+            P.szz_u[0] = szz_u[u];
+            P.rho_u[0] = (R)A.rho[o + unsigned(A.nxyp)];
+            P.rhoW[0] = (R)A.rho[o - 1];
+            P.rhoN[0] = (R)A.rho[o + unsigned(nx)];
+            P.vx[0] = ldf(A.fi[0], o, Sv.r[0]);
+            P.vy[0] = ldf(A.fi[1], o, Sv.r[1]);
+            P.vz[0] = ldf(A.fi[2], o, Sv.r[2]);
+            if (ENERGY) {
+                P.cp[0] = (R)A.cp[o];
+                P.cs[0] = (R)A.cs[o];
+            }
+            if (L.cx[0] >= 0) {
+                const unsigned om = unsigned((k * A.ny + L.j[0]) * A.w[0] + L.cx[0]);
+                P.mx[0][0] = ldf(A.mi[9], om, Sv.r[18]);
+                P.mx[1][0] = ldf(A.mi[12], om, Sv.r[21]);
+                P.mx[2][0] = ldf(A.mi[15], om, Sv.r[24]);
+            }
+            Q.sxz_m[0] = sxz_m[u];
+            Q.syz_m[0] = syz_m[u];
+            Q.szz_c[0] = szz_c[u];
+            Q.rho_c[0] = (R)A.rho[o];
+            velocity_update<T, kInterior, ENERGY, 1, R>(A, L, tx, tx, k, P, Q, ek, ep, Sv);
+            sxz_m[u] = P.sxz[0];
+            syz_m[u] = P.syz[0];
+            szz_c[u] = szz_u[u];
+        }
+        __syncthreads();
+    }
+    if (ENERGY) {  // per-block FP64 partials, summed in block order by the tail kernel
+        __shared__ double red[2][kFusedThreads / 64];
+        ek = wave_sum(ek);
+        ep = wave_sum(ep);
+        const int lane = tid & 63, wid = tid >> 6;
+        if (lane == 0) {
+            red[0][wid] = ek;
+            red[1][wid] = ep;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double a = 0.0, bsum = 0.0;
+            for (int w = 0; w < kFusedThreads / 64; ++w) {
+                a += red[0][w];
+                bsum += red[1][w];
+            }
+            A.epart[2 * blockIdx.x] = a;
+            A.epart[2 * blockIdx.x + 1] = bsum;
+        }
+    }
+}
+
+template <typename T, bool E>
+static void launch_fused_e(const StepArgs<T>& a, hipStream_t s)
+{
+    const size_t lds = fused_lds_bytes<T>(a.nx, a.tile);
+    if (a.tile <= 2 * kFusedThreads)
+        hipLaunchKernelGGL((k_fused<T, E, 2>), dim3(a.nblocks), dim3(kFusedThreads), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_fused<T, E, 4>), dim3(a.nblocks), dim3(kFusedThreads), lds, s, a);
+}
+
+template <typename T>
+void launch_fused(const StepArgs<T>& a, bool energy, hipStream_t s)
+{
+    if (a.nblocks <= 0) return;
+    if (energy)
+        launch_fused_e<T, true>(a, s);
+    else
+        launch_fused_e<T, false>(a, s);
+}
+
+template void launch_fused<float>(const StepArgs<float>&, bool, hipStream_t);
+template void launch_fused<double>(const StepArgs<double>&, bool, hipStream_t);
+template size_t fused_lds_bytes<float>(int, int);
+template size_t fused_lds_bytes<double>(int, int);
+template size_t fused_lds_bytes<half_t>(int, int);
 
 template void launch_stress<float>(const StepArgs<float>&, hipStream_t);
 template void launch_stress<double>(const StepArgs<double>&, hipStream_t);

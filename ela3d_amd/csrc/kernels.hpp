@@ -12,6 +12,7 @@ typedef _Float16 half_t;
 constexpr int kInterior = 0;  // y and z outside the C-PML: only x is checked per lane
 constexpr int kSlab = 1;      // y or z inside the C-PML: every axis checked
 constexpr int kMixed = 2;     // launch mode only: boxes of both kinds, each block runs its box's body
+constexpr int kFused = 3;     // launch mode only: one fused stress+velocity step over the interior box
 
 // FP16 storage keeps one power-of-two scale per field (stored = value * scale)
 // in a device table of kScaleTable floats, so replayed graphs see live values:
@@ -64,7 +65,9 @@ struct StepArgs {
     Box box[kMaxBox];
     int nbox, nblocks, block;
     int vec;                 // consecutive xy positions per lane: 1, 2 or 4
-    int mode;                // kInterior or kSlab (every box of the launch), or kMixed
+    int mode;                // kInterior or kSlab (every box of the launch), kMixed or kFused
+    int tile;                // kFused: cells of the flattened interior box per workgroup
+    int ldsreg;              // kFused: cells of one LDS stress plane (tile + 2*nx)
 };
 
 template <typename T>
@@ -98,6 +101,13 @@ template <typename T>
 void launch_stress(const StepArgs<T>& a, hipStream_t s);
 template <typename T>
 void launch_velocity(const StepArgs<T>& a, bool energy, hipStream_t s);
+// fused step over the interior box; needs double-buffered fields and the
+// absorbing-layer stress launch of the same step to have run
+template <typename T>
+void launch_fused(const StepArgs<T>& a, bool energy, hipStream_t s);
+// LDS bytes of a fused launch (0: not launchable with these sizes)
+template <typename T>
+size_t fused_lds_bytes(int nx, int tile);
 template <typename T>
 void launch_tail(const TailArgs<T>& a, hipStream_t s);
 void launch_copy(const float4* src, float4* dst, long n4, int variant, hipStream_t s);

@@ -78,16 +78,49 @@ void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& slab, LaunchGeom& 
     slab.mode = kSlab;
     interior.block = t.block;
     interior.vec = t.vec;
-    interior.mode = t.merged ? kMixed : kInterior;
+    interior.mode = t.fused ? kFused : t.merged ? kMixed : kInterior;
+    if (t.fused) {  // one workgroup per (tile, z segment): 256 threads over `tile` cells
+        interior.block = 256;
+        interior.vec = 1;
+        interior.tile = t.tile;
+    }
     BoxRange r[5];
     const int n = split_boxes(p, r);
     for (int q = 0; q < n; ++q) {  // absorbing-layer boxes come first in split_boxes order
         const bool zlayer = r[q].k1 <= p.ax[2].lo || r[q].k0 >= p.ax[2].hi;
-        if (r[q].mode == 0)
-            add_box(interior, r[q], t.zseg_interior);
-        else  // z layers march in short segments; the y layers as the interior
+        if (r[q].mode == 0) {
+            if (t.fused) {
+                Box& b = interior.box[interior.nbox++];
+                b.p0 = r[q].p0; b.p1 = r[q].p1; b.k0 = r[q].k0; b.k1 = r[q].k1;
+                b.nchunk = (r[q].p1 - r[q].p0 + t.tile - 1) / t.tile;
+                b.zseg = std::max(1, t.zseg_interior);
+                b.blk0 = 0;
+                b.mode = kInterior;
+                interior.nblocks = b.nchunk * ((b.k1 - b.k0 + b.zseg - 1) / b.zseg);
+            } else {
+                add_box(interior, r[q], t.zseg_interior);
+            }
+        } else {  // z layers march in short segments; the y layers as the interior
             add_box(t.merged ? interior : slab, r[q], zlayer ? t.zseg_slab : t.zseg_interior);
+        }
     }
+}
+
+std::string fused_unsupported(const ShotPlan& p, int tile)
+{
+    const ela3d_config& c = p.cfg;
+    if (c.storage == ELA3D_STORAGE_FP16) return "the fused schedule runs FP32 and FP64 storage only";
+    if (p.ax[1].lo < 1 || p.ax[2].lo < 1 || p.ax[1].hi > c.ny - 1 || p.ax[2].hi > c.nz - 1)
+        return "the fused schedule needs absorbing layers on every y and z face";
+    if (tile < 64 || tile > 1024 || tile % 64) return "fused tile must be a multiple of 64 in [64, 1024]";
+    const size_t lds = c.storage == ELA3D_STORAGE_FP64 ? fused_lds_bytes<double>(c.nx, tile)
+                                                       : fused_lds_bytes<float>(c.nx, tile);
+    if (lds == 0) return "fused tile and nx need more LDS than one workgroup has";
+    BoxRange r[5];
+    const int n = split_boxes(p, r);
+    for (int q = 0; q < n; ++q)
+        if (r[q].mode == 0) return "";
+    return "the grid has no interior box";
 }
 
 static size_t mem_elems(const ShotPlan& p, int axis)
@@ -166,7 +199,8 @@ public:
         for (auto& e : fork_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         for (auto& e : join_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         tune_ = choose_tuning(plan_);
-        dbuf_ = dbuf || tune_.dbuf != 0;
+        if (tune_.fused && !fused_unsupported(plan_, tune_.tile).empty()) tune_.fused = 0;
+        dbuf_ = dbuf || tune_.dbuf != 0 || tune_.fused != 0;
         tune_.dbuf = dbuf_ ? 1 : 0;
         overlap_ = tune_.overlap != 0;
         geometry(plan_, tune_, geo_[1], geo_[0]);
@@ -373,7 +407,8 @@ public:
         static const char* split[kLaunches] = {"stress_cpml", "stress_interior", "velocity_cpml",
                                                "velocity_interior", "tail"};
         static const char* merged[kLaunches] = {"stress_cpml", "stress", "velocity_cpml", "velocity", "tail"};
-        const char* const* names = tune_.merged ? merged : split;
+        static const char* fused[kLaunches] = {"stress_cpml", "fused_interior", "velocity_cpml", "-", "tail"};
+        const char* const* names = tune_.fused ? fused : tune_.merged ? merged : split;
         st.assign(kLaunches, ela3d_kernel_stats());
         for (int q = 0; q < kLaunches; ++q) {
             std::snprintf(st[q].name, sizeof(st[q].name), "%s", names[q]);
@@ -386,6 +421,10 @@ public:
         st[2].alg_bytes_per_launch = m ? 0.0 : plan_.alg_velocity_slab;
         st[3].alg_bytes_per_launch = plan_.alg_velocity_interior + (m ? plan_.alg_velocity_slab : 0.0);
         st[4].alg_bytes_per_launch = 0;
+        if (tune_.fused) {
+            st[1].alg_bytes_per_launch = plan_.alg_fused_interior();
+            st[3].alg_bytes_per_launch = 0;
+        }
         const int chunk = 64;
         const int E = 2 * kLaunches;  // start and end event of every launch
         std::vector<hipEvent_t> ev(size_t(chunk) * E);
@@ -428,8 +467,12 @@ public:
     {
         HIP_CHECK(hipSetDevice(device_));
         HIP_CHECK(hipStreamSynchronize(stream_));
-        if ((t.dbuf != 0) && !dbuf_)
+        if ((t.dbuf != 0 || t.fused != 0) && !dbuf_)
             throw Error{ELA3D_E_INVALID, "double buffering is fixed when the shot is created"};
+        if (t.fused) {
+            const std::string why = fused_unsupported(plan_, t.tile);
+            if (!why.empty()) throw Error{ELA3D_E_INVALID, why};
+        }
         LaunchGeom g[2];
         geometry(plan_, t, g[1], g[0]);
         if (g[0].nblocks + g[1].nblocks > epart_cap_)
@@ -458,7 +501,7 @@ public:
         std::memset(&o, 0, sizeof(o));
         o.cells = plan_.cells;
         o.device_bytes = arena_bytes_;
-        o.alg_bytes_per_step = plan_.alg_step();
+        o.alg_bytes_per_step = tune_.fused ? plan_.alg_step_fused() : plan_.alg_step();
         o.alg_bytes_stress = plan_.alg_stress_interior + plan_.alg_stress_slab;
         o.alg_bytes_velocity = plan_.alg_velocity_interior + plan_.alg_velocity_slab;
         o.pml_bytes_per_step = plan_.alg_pml;
@@ -474,6 +517,8 @@ public:
         o.overlap = overlap_ ? 1 : 0;
         o.merged = tune_.merged;
         o.dbuf = dbuf_ ? 1 : 0;
+        o.fused = tune_.fused;
+        o.tile = tune_.fused ? tune_.tile : 0;
         for (int a = 0; a < 3; ++a) {
             o.region_lo[a] = plan_.ax[a].lo;
             o.region_hi[a] = plan_.ax[a].hi;
@@ -695,6 +740,8 @@ private:
         a.vec = g.vec;
         a.block = g.block;
         a.mode = g.mode;
+        a.tile = g.tile;
+        a.ldsreg = g.tile + 2 * c.nx;
         return a;
     }
 
@@ -761,6 +808,26 @@ private:
         auto mark = [&](int q, hipStream_t s) {
             if (ev) HIP_CHECK(hipEventRecord(ev[q], s));
         };
+        if (tune_.fused) {  // layers' stresses, the fused interior step, layers' velocities
+            mark(0, stream_);
+            launch_stress(args_[par][1][0], stream_);
+            mark(1, stream_);
+            mark(2, stream_);
+            if constexpr (std::is_same<T, half_t>::value)
+                throw Error{ELA3D_E_INVALID, "the fused schedule does not run FP16 storage"};
+            else
+                launch_fused(args_[par][0][0], energy, stream_);
+            mark(3, stream_);
+            mark(4, stream_);
+            launch_velocity(args_[par][1][1], energy, stream_);
+            mark(5, stream_);
+            mark(6, stream_);
+            mark(7, stream_);
+            mark(8, stream_);
+            launch_tail(tail_args(par), stream_);
+            mark(9, stream_);
+            return;
+        }
         hipStream_t side = overlap_ ? aux_ : stream_;
         for (int pass = 0; pass < 2; ++pass) {
             if (overlap_) {

@@ -25,7 +25,8 @@ namespace ela3d {
 struct LaunchGeom {
     Box box[kMaxBox];
     int nbox = 0, nblocks = 0, block = 256, vec = 1;
-    int mode = kInterior;  // kInterior, kSlab or kMixed
+    int mode = kInterior;  // kInterior, kSlab, kMixed or kFused
+    int tile = 0;          // kFused: cells per workgroup
 };
 
 // Launch tuning chosen for a grid (see tune.cpp): the interior launch and the
@@ -41,6 +42,9 @@ struct Tuning {
     int merged = 0;         // 1: one mixed launch per pass over all boxes (interior shape;
                             //    z layers with zseg_slab)
     int dbuf = 0;           // 1: two field sets, each step reads one and writes the other
+    int fused = 0;          // 1: one fused stress+velocity launch over the interior box
+                            //    (needs dbuf; zseg_interior planes per workgroup)
+    int tile = 1024;        // fused: interior cells of the flattened plane per workgroup
 };
 
 // Device layout of the 9 fields and the 3 model arrays: planes of plane_stride
@@ -102,5 +106,7 @@ bool cached_tuning(const ela3d_config& c, Tuning& out);
 void store_tuning(const ela3d_config& c, const Tuning& t, double ms);
 // launch shapes of the absorbing-layer boxes (slab) and the interior box
 void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& slab, LaunchGeom& interior);
+// whether the fused schedule can run this plan with this tile (empty: yes)
+std::string fused_unsupported(const ShotPlan& p, int tile);
 
 }  // namespace ela3d

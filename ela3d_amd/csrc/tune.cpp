@@ -75,7 +75,9 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
         ela3d::ShotPlan plan = ela3d::make_plan(c);
         int n = 0;
         if (hipGetDeviceCount(&n) != hipSuccess || n <= c.device) throw Error{ELA3D_E_NODEVICE, "no HIP device"};
-        std::unique_ptr<ela3d::Shot> shot = ela3d::make_shot(plan, cp, cs, rho);
+        // a second field set lets the scratch shot also try the fused schedule
+        const bool fusable = ela3d::fused_unsupported(plan, 1024).empty() || ela3d::fused_unsupported(plan, 512).empty();
+        std::unique_ptr<ela3d::Shot> shot = ela3d::make_shot(plan, cp, cs, rho, nullptr, fusable);
         static const int blocks[] = {64, 128, 256};
         static const int zsi[] = {2, 4, 8, 16, 32};
         static const int zss[] = {2, 4, 8};
@@ -175,13 +177,34 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
         if (wall3[2] < 0.99 * wall3[pick]) pick = 2;
         if (wall3[1] < 0.97 * wall3[pick]) pick = 1;
         best = cand[pick];
+        double best_wall = wall3[pick];
+        if (fusable) {  // fused interior step: tile x z segment
+            static const int tiles[] = {512, 1024};
+            static const int fzs[] = {4, 8, 16, 32};
+            for (int tl : tiles) {
+                if (!ela3d::fused_unsupported(plan, tl).empty()) continue;
+                for (int z : fzs) {
+                    Tuning t = best;
+                    t.overlap = 0;
+                    t.merged = 0;
+                    t.fused = 1;
+                    t.dbuf = 1;
+                    t.tile = tl;
+                    t.zseg_interior = z;
+                    const double w = wall_of(t);
+                    if (verbose) std::fprintf(stderr, "[tune] fused tile %4d zseg %2d: %.4f ms\n", tl, z, w);
+                    if (w < 0.99 * best_wall) {
+                        best_wall = w;
+                        best = t;
+                    }
+                }
+            }
+        }
         if (verbose)
             std::fprintf(stderr, "[tune] step wall: split %.4f ms, overlapped %.4f ms, merged %.4f ms\n",
                          wall3[0], wall3[1], wall3[2]);
         shot->retune(best);
-        shot->set_overlap(false);
-        measure(best, ms);
-        const double best_ms = ms[0] + ms[1];
+        const double best_ms = wall_of(best);
         ela3d::store_tuning(*cfg, best, best_ms);
         if (out) {
             out->block = best.block;
@@ -193,6 +216,8 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
             out->overlap = best.overlap;
             out->merged = best.merged;
             out->dbuf = best.dbuf;
+            out->fused = best.fused;
+            out->tile = best.tile;
             out->ms_per_step = best_ms;
             out->candidates = tried;
         }
@@ -225,6 +250,9 @@ int ela3d_tuning_set(const ela3d_config* cfg, const ela3d_tuning* t)
     u.overlap = t->overlap ? 1 : 0;
     u.merged = t->merged ? 1 : 0;
     u.dbuf = t->dbuf ? 1 : 0;
+    u.fused = t->fused ? 1 : 0;
+    u.tile = t->tile > 0 ? t->tile : 1024;
+    if (u.fused) u.dbuf = 1;
     ela3d::store_tuning(*cfg, u, t->ms_per_step);
     return ELA3D_OK;
 }
@@ -243,6 +271,8 @@ int ela3d_tuning_get(const ela3d_config* cfg, ela3d_tuning* t)
     t->overlap = u.overlap;
     t->merged = u.merged;
     t->dbuf = u.dbuf;
+    t->fused = u.fused;
+    t->tile = u.tile;
     t->ms_per_step = 0.0;
     t->candidates = 0;
     return ELA3D_OK;

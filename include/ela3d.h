@@ -141,6 +141,7 @@ typedef struct ela3d_info {
     int32_t overlap;               /* absorbing-layer launches concurrent with the interior ones */
     int32_t merged;                /* one mixed launch per pass */
     int32_t dbuf;                  /* two field sets (double buffered) */
+    int32_t fused, tile;           /* fused interior schedule and its tile */
 } ela3d_info;
 
 #define ELA3D_MAX_KERNELS 8
@@ -255,6 +256,8 @@ typedef struct ela3d_tuning {
     int32_t overlap;      /* 1: absorbing-layer launches run concurrently with the interior ones */
     int32_t merged;       /* 1: one mixed launch per pass over every box (interior shape) */
     int32_t dbuf;         /* 1: two field sets; each step reads one and writes the other */
+    int32_t fused;        /* 1: one fused stress+velocity launch over the interior box (needs dbuf) */
+    int32_t tile;         /* fused: interior cells per workgroup (multiple of 64, <= 1024) */
 } ela3d_tuning;
 int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, const float* rho,
                    int32_t steps, ela3d_tuning* out);

@@ -125,3 +125,102 @@ def test_model_catches_wrapped_offsets():
     """Indexing from plane 0 with unsigned offsets wraps at the first row (o - 1, o - nx)."""
     bad = violations(small_cfg(), 256, 2, 4, 4, base_at_guard=True)
     assert ("neighbour", 0, 0, -1) in bad and ("neighbour", 0, 0, -29) in bad
+
+
+def fused_violations(cfg, tile, zseg, lds_bytes_limit=65536, elem=4):
+    """Every global and LDS index the fused interior kernel forms (k_fused and
+    fused_sigma in kernels.hip), for all tiles and z segments, checked against
+    the allocation, memory-variable and LDS sizes."""
+    p = cfg.plan()
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    nxy, nxyp, G = nx * ny, plane_stride(nx, ny), guard(nx)
+    alloc = nz * nxyp + 2 * G
+    lo, hi = p["region_lo"], p["region_hi"]
+    w0 = lo[0] + nx - hi[0]
+    memsize0 = nz * ny * w0
+    pa, pb = (lo[1] * nx + 63) // 64 * 64, (hi[1] * nx) // 64 * 64
+    assert pb > pa
+    zlo, zhi = lo[2], hi[2]
+    LR = tile + 2 * nx
+    tab = 0 if elem == 8 else (6 * nx * 4 + 15) // 16 * 16
+    if tab + 2 * 5 * LR * elem > lds_bytes_limit:
+        return None  # the library refuses this tile (fused_unsupported)
+    bad = []
+    nchunk = (pb - pa + tile - 1) // tile
+    nzs = (zhi - zlo + zseg - 1) // zseg
+
+    def cell(k, q):
+        return G + k * nxyp + q
+
+    def fld(o, tag):
+        o = np.asarray(o)
+        if np.any((o < 0) | (o >= alloc)):
+            bad.append(tag)
+
+    def mem(k, q):
+        j, i = np.divmod(q, nx)
+        cx = np.where(i < lo[0], i, np.where(i >= hi[0], lo[0] + i - hi[0], -1))
+        om = (k * ny + j) * w0 + cx
+        if np.any((cx >= 0) & ((om < 0) | (om >= memsize0))):
+            bad.append(("mem", k))
+
+    def sigma(q, k):  # fused_sigma at cells q (array) of plane k
+        if not (0 <= k < nz):
+            bad.append(("plane", k))
+            return
+        if np.any((q < 0) | (q >= nxy)):
+            bad.append(("region", k))
+        inside = (q >= pa) & (q < pb) & (zlo <= k < zhi)
+        qi = q[inside]
+        if qi.size:
+            o = cell(k, qi)
+            dU = 0 if k == nz - 1 else nxyp
+            for d in (0, dU, 1, -1, nx, -nx):
+                fld(o + d, ("stress", k, d))
+            if k >= 1:
+                fld(o - nxyp, ("queue", k))
+            mem(k, qi)
+        qo = q[~inside]
+        if qo.size:
+            fld(cell(k, qo), ("fresh", k))
+
+    for chunk in range(nchunk):
+        q0 = pa + chunk * tile
+        q1 = min(q0 + tile, pb)
+        region = np.arange(q0 - nx, q1 + nx)
+        own = np.arange(q0, q1)
+        c_own = own - q0
+        for zs in range(nzs):
+            kb = zlo + zs * zseg
+            ke = min(kb + zseg, zhi)
+            sigma(own, kb - 1)
+            sigma(region, kb)
+            for k in range(kb, ke):
+                sigma(own if k + 1 == ke else region, k + 1)
+                o = cell(k, own)
+                for d in (0, nxyp, -1, nx):
+                    fld(o + d, ("velocity", k, d))
+                mem(k, own)
+                for d in (0, -1, 1, -nx, nx):  # LDS neighbours, relative to the region start
+                    idx = c_own + d + nx
+                    if np.any((idx < 0) | (idx >= LR)):
+                        bad.append(("lds", k, d))
+    return bad
+
+
+@pytest.mark.parametrize("shape", [(29, 23, 26, 4), (40, 13, 15, 4), (17, 19, 18, 3), (64, 64, 9, 3)])
+@pytest.mark.parametrize("tile,zseg", [(64, 2), (512, 4), (1024, 8), (256, 32)])
+def test_fused_accesses_in_bounds(shape, tile, zseg):
+    nx, ny, nz, npml = shape
+    for compat in (0, 7):
+        cfg = small_cfg(nx=nx, ny=ny, nz=nz, npml=npml, compat=compat)
+        assert fused_violations(cfg, tile, zseg) == []
+        assert fused_violations(cfg, tile, zseg, elem=8) in ([], None)
+
+
+def test_fused_cube_and_classic_in_bounds():
+    cfg, _ = workloads.homogeneous_cube(n=64, nstep=1)
+    for tile, zseg in ((512, 8), (1024, 16)):
+        assert fused_violations(cfg, tile, zseg) == []
+    cfg, _ = workloads.classic(nstep=1)
+    assert fused_violations(cfg, 1024, 16) == []
