@@ -995,7 +995,7 @@ size_t fused_lds_bytes(int nx, int tile)
     typedef typename Arith<T>::R R;
     const size_t tab = std::is_same<T, double>::value ? 0 : (size_t(6) * nx * sizeof(float) + 15) / 16 * 16;
     const size_t bytes = tab + size_t(2) * 5 * size_t(tile + 2 * nx) * sizeof(R);
-    return bytes <= 65536 ? bytes : 0;
+    return bytes <= 65536 - 1024 ? bytes : 0;  // room for the kernel's static LDS
 }
 
 // new stresses of cell p at plane k: recomputed inside the interior box, else
@@ -1055,6 +1055,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_fused(const StepArgs<T> A)
         tx = smem;
         lds = reinterpret_cast<R*>(smem + (6 * nx + 3) / 4 * 4);
     }
+    __syncthreads();  // tables staged before the first stress update reads them
     // lds[(b*5 + comp)*LR + (c - rlo)]: comp 0 sxx, 1 sxy, 2 syy, 3 sxz, 4 syz of plane b = k & 1
     auto at = [&](int b, int comp, int c) -> R& { return lds[(b * 5 + comp) * LR + (c - rlo)]; };
     // own cells q0 + tid + 256*u: column queues across planes
@@ -1187,6 +1188,7 @@ template <typename T, bool E>
 static void launch_fused_e(const StepArgs<T>& a, hipStream_t s)
 {
     const size_t lds = fused_lds_bytes<T>(a.nx, a.tile);
+    if (lds == 0 || a.tile < 1 || a.tile > 4 * kFusedThreads) return;  // refused on the host first
     if (a.tile <= 2 * kFusedThreads)
         hipLaunchKernelGGL((k_fused<T, E, 2>), dim3(a.nblocks), dim3(kFusedThreads), lds, s, a);
     else

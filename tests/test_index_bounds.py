@@ -127,7 +127,7 @@ def test_model_catches_wrapped_offsets():
     assert ("neighbour", 0, 0, -1) in bad and ("neighbour", 0, 0, -29) in bad
 
 
-def fused_violations(cfg, tile, zseg, lds_bytes_limit=65536, elem=4):
+def fused_violations(cfg, tile, zseg, lds_bytes_limit=65536 - 1024, elem=4):
     """Every global and LDS index the fused interior kernel forms (k_fused and
     fused_sigma in kernels.hip), for all tiles and z segments, checked against
     the allocation, memory-variable and LDS sizes."""
