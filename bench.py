@@ -39,20 +39,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_workload(name, nstep):
+def make_workload(name, nstep=None):
+    """(Config, model, description) of a named workload; nstep None keeps its own NSTEP."""
     from ela3d_amd import workloads
+    kw = {} if nstep is None else {"nstep": nstep}
     if name == "classic":
-        cfg, model = workloads.classic(nstep=nstep)
+        cfg, model = workloads.classic(**kw)
         desc = ("classic SEISMIC_CPML 101x641x640 homogeneous, cp=3300 m/s, dx=10 m, dt=1.6e-3 s, "
                 "10-point C-PML, Ricker force f0=7 Hz, 65 receivers, energy every 10 steps")
     elif name == "layered":
-        cfg, model = workloads.layered(nstep=nstep)
+        cfg, model = workloads.layered(**kw)
         desc = "seeded 6-layer anticline 401x401x301, explosive source f0=10 Hz, FP32"
     elif name == "layered_fp16":
-        cfg, model = workloads.layered(nstep=nstep, fp16=True)
+        cfg, model = workloads.layered(fp16=True, **kw)
         desc = "seeded 6-layer anticline 401x401x301, FP16 storage with power-of-two scales"
     elif name == "homogeneous_cube":
-        cfg, model = workloads.homogeneous_cube(nstep=nstep)
+        cfg, model = workloads.homogeneous_cube(**kw)
         desc = "homogeneous cube 101^3, Ricker force f0=7 Hz, 42 receivers, energy every step"
     else:
         raise SystemExit(f"unknown workload {name}")
@@ -101,6 +103,29 @@ def load_traffic(workload, kernel):
         d = json.load(f)
     k = d.get("kernels", {}).get(kernel)
     return (None if k is None else k.get("hbm_bytes_per_dispatch")), d.get("launch_shape")
+
+
+def accuracy_run(workload):
+    """The workload's full NSTEP in its own storage and in FP64 storage (the GPU FP64
+    path, itself pinned to the CPU FP64 oracle by the parity tests): seismogram
+    relative L2 and the largest energy deviation relative to the FP64 energy peak."""
+    from ela3d_amd import STORAGE_FP64, Shot
+    cfg, model, _ = make_workload(workload)
+    out = {}
+    for name, c in (("run", cfg), ("fp64", cfg.replace(storage=STORAGE_FP64))):
+        t = time.perf_counter()
+        with Shot(c, *model) as s:
+            s.run()
+            out[name] = (s.seismograms(), s.energy())
+        log(f"[bench] accuracy leg {name}: {c.nstep} steps in {time.perf_counter() - t:.1f} s")
+    s32, e32 = out["run"]
+    s64, e64 = out["fp64"]
+    rel = float(np.linalg.norm((s32 - s64).ravel()) / max(np.linalg.norm(s64.ravel()), 1e-300))
+    tot32, tot64 = e32.sum(1), e64.sum(1)
+    edev = float(np.max(np.abs(tot32 - tot64)) / max(np.max(tot64), 1e-300)) if tot64.size else None
+    log(f"[bench] accuracy: seismogram rel L2 vs FP64 {rel:.3e}, max energy deviation {edev}")
+    return {"nstep": cfg.nstep, "seismogram_rel_l2_vs_fp64": rel, "max_energy_dev_vs_fp64": edev,
+            "reference": "same workload with FP64 storage on the GPU"}
 
 
 def percentile(a, q):
@@ -255,6 +280,9 @@ def main():
     ap.add_argument("--round", type=int, default=50, help="serving: steps per round")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serving: HBM budget (GB)")
     ap.add_argument("--no-plain", action="store_true", help="serving: skip the plain-loop leg")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="also run the whole NSTEP of the workload in its storage and in FP64 storage "
+                         "and report the seismogram relative L2 and the largest energy deviation")
     args = ap.parse_args()
     if args.calibrate > 0:
         import ela3d_amd
@@ -326,6 +354,9 @@ def main():
                 sg.sync()
                 graph_ms = (time.perf_counter() - t) * 1e3 / g_steps
             log(f"[bench] graph replay: {graph_ms:.4f} ms/step")
+    accuracy = None
+    if args.accuracy:
+        accuracy = accuracy_run(args.workload)
     ms_per_step = host_s * 1e3 / K
     mcells = cfg.cells * K / host_s / 1e6
     for s in stats:
@@ -361,6 +392,7 @@ def main():
             "alg_bytes_per_step": info["alg_bytes_per_step"],
             "step_hbm_frac": round(step_bw / PEAK_HBM, 4),
             "graph_ms_per_step": graph_ms,
+            "accuracy": accuracy,
             "launch_shape": {"block": info["block"], "zseg": info["zseg"], "vec": info["vec"],
                              "block_slab": info["block_slab"], "zseg_slab": info["zseg_slab"],
                              "vec_slab": info["vec_slab"], "overlap": info["overlap"],
