@@ -61,7 +61,7 @@ def make_workload(name, nstep=None):
     return cfg, model, desc
 
 
-def cpu_baseline(cfg, model, budget_s=15.0):
+def cpu_baseline(cfg, model, budget_s=20.0):
     """Time the oracle's FP32 restatement of the reference loop on this host's cores."""
     import oracle
     threads = min(os.cpu_count() or 1, 16)
