@@ -8,11 +8,13 @@
 // identical and nothing is perturbed).  The winner is cached keyed by
 // (nx, ny, nz, storage, npoints_pml); shots created with block = 0 and
 // zseg = 0 take the cached shape.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <vector>
 
 #include "shot.hpp"
 
@@ -98,7 +100,33 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
             ms[1] = (st[0].total_ms + st[2].total_ms) / steps;
             ++tried;
         };
-        double ms[2], best_int = 1e300, best_slab = 1e300;
+        double ms[2];
+        // each stage times every candidate once over `steps` steps, then re-times
+        // its best few over four times as many and keeps the fastest of those
+        auto refine = [&](std::vector<std::pair<double, Tuning>>& c, int which) {
+            std::sort(c.begin(), c.end(),
+                      [](const std::pair<double, Tuning>& a, const std::pair<double, Tuning>& b) {
+                          return a.first < b.first;
+                      });
+            const int keep = std::min<int>(5, int(c.size()));
+            Tuning win = c[0].second;
+            double win_ms = 1e300;
+            for (int q = 0; q < keep; ++q) {
+                shot->retune(c[q].second);
+                double wall = 0.0;
+                shot->profile(1, st, wall);
+                shot->profile(4 * steps, st, wall);
+                ++tried;
+                const double t = which == 0 ? (st[1].total_ms + st[3].total_ms) / (4 * steps)
+                                            : (st[0].total_ms + st[2].total_ms) / (4 * steps);
+                if (t < win_ms) {
+                    win_ms = t;
+                    win = c[q].second;
+                }
+            }
+            return win;
+        };
+        std::vector<std::pair<double, Tuning>> cands;
         for (int vv : vecs)
             for (int b : blocks)
                 for (int zi : zsi) {
@@ -110,13 +138,10 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
                     if (verbose)
                         std::fprintf(stderr, "[tune] interior vec %d block %3d zseg %3d: %.4f ms\n", vv, b, zi,
                                      ms[0]);
-                    if (ms[0] < best_int) {
-                        best_int = ms[0];
-                        best.block = b;
-                        best.zseg_interior = zi;
-                        best.vec = vv;
-                    }
+                    cands.emplace_back(ms[0], t);
                 }
+        best = refine(cands, 0);
+        cands.clear();
         for (int vv : vecs_slab)
             for (int b : blocks)
                 for (int zs : zss) {
@@ -128,13 +153,9 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
                     if (verbose)
                         std::fprintf(stderr, "[tune] layers   vec %d block %3d zseg %3d: %.4f ms\n", vv, b, zs,
                                      ms[1]);
-                    if (ms[1] < best_slab) {
-                        best_slab = ms[1];
-                        best.block_slab = b;
-                        best.zseg_slab = zs;
-                        best.vec_slab = vv;
-                    }
+                    cands.emplace_back(ms[1], t);
                 }
+        best = refine(cands, 1);
         // schedule: the two launches in sequence, overlapped (pays off when neither
         // fills the GPU), or one mixed launch per pass (one kernel boundary less)
         auto wall_of = [&](const Tuning& t) {
