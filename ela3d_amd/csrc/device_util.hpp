@@ -36,32 +36,6 @@ __device__ __forceinline__ void stf(T* p, unsigned o, typename Arith<T>::R v, fl
         p[o] = v;
 }
 
-// Streaming (nontemporal) variants for data a pass touches once -- a cell's own
-// stresses in the stress pass, its own velocities in the velocity pass, memory
-// variables -- so that L2 keeps the arrays that are re-read as neighbours.
-constexpr bool kStreamHints = true;
-
-template <typename T>
-__device__ __forceinline__ typename Arith<T>::R ldf_s(const T* p, unsigned o, float isc)
-{
-    if constexpr (!kStreamHints) return ldf(p, o, isc);
-    const T v = __builtin_nontemporal_load(p + o);
-    if constexpr (std::is_same<T, half_t>::value)
-        return (float)v * isc;
-    else
-        return v;
-}
-
-template <typename T>
-__device__ __forceinline__ void stf_s(T* p, unsigned o, typename Arith<T>::R v, float sc)
-{
-    if constexpr (!kStreamHints) return stf(p, o, v, sc);
-    if constexpr (std::is_same<T, half_t>::value)
-        __builtin_nontemporal_store((half_t)(v * sc), p + o);
-    else
-        __builtin_nontemporal_store((T)v, p + o);
-}
-
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll

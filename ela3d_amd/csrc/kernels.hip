@@ -94,20 +94,16 @@ __device__ __forceinline__ unsigned cell(const StepArgs<T>& A, int k, int p)
     return unsigned(A.g0) + unsigned(k) * unsigned(A.nxyp) + unsigned(p);
 }
 
-// VEC-wide load/store of storage T at element offset o (element-aligned only;
-// NT: a streaming access at a VEC-aligned own-cell offset)
-template <typename T, int V, bool NT = false>
+// VEC-wide load/store of storage T at element offset o (element-aligned only)
+template <typename T, int V>
 __device__ __forceinline__ void ldv(const T* p, unsigned o, float isc, typename Arith<T>::R (&out)[V])
 {
     if constexpr (V == 1) {
-        out[0] = NT ? ldf_s(p, o, isc) : ldf(p, o, isc);
+        out[0] = ldf(p, o, isc);
     } else {
         typedef T vt __attribute__((ext_vector_type(V)));
         vt v;
-        if constexpr (NT && kStreamHints)
-            v = __builtin_nontemporal_load(reinterpret_cast<const vt*>(p + o));
-        else
-            __builtin_memcpy(&v, p + o, sizeof(vt));
+        __builtin_memcpy(&v, p + o, sizeof(vt));
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             if constexpr (std::is_same<T, half_t>::value)
@@ -182,12 +178,12 @@ __device__ __forceinline__ void flush_maxima(const StepArgs<T>& A, Scales<T>& S,
 template <typename T>
 __device__ __forceinline__ void stm(T* p, unsigned o, typename Arith<T>::R v, Scales<T>& S, int q)
 {
-    stf_s(p, o, v, S.w[q]);
+    stf(p, o, v, S.w[q]);
     S.track(q, (float)v);
 }
 
 // store the enabled elements; one vector store when all are enabled
-template <typename T, int V, bool NT = false>
+template <typename T, int V>
 __device__ __forceinline__ void stv(T* p, unsigned o, const typename Arith<T>::R (&val)[V], const bool (&en)[V],
                                     Scales<T>& S, int q)
 {
@@ -209,21 +205,13 @@ __device__ __forceinline__ void stv(T* p, unsigned o, const typename Arith<T>::R
                 else
                     v[e] = val[e];
             }
-            if constexpr (NT && kStreamHints)
-                __builtin_nontemporal_store(v, reinterpret_cast<vt*>(p + o));
-            else
-                __builtin_memcpy(p + o, &v, sizeof(vt));
+            __builtin_memcpy(p + o, &v, sizeof(vt));
             return;
         }
     }
 #pragma unroll
     for (int e = 0; e < V; ++e)
-        if (en[e]) {
-            if constexpr (NT)
-                stf_s(p, o + e, val[e], sc);
-            else
-                stf(p, o + e, val[e], sc);
-        }
+        if (en[e]) stf(p, o + e, val[e], sc);
 }
 
 // memory-variable recurrence psi <- b psi + a D;  D <- D/K + psi  (CPU:395-405)
@@ -376,33 +364,33 @@ __device__ __forceinline__ void stress_load(const StepArgs<T>& A, const Lane<V>&
     ldm<V>(A.cs, o + nx, P.csN);
     ldm<V>(A.rho, o - 1, P.rhoW);
     ldm<V>(A.rho, o + nx, P.rhoN);
-    ldv<T, V, true>(A.fi[3], o, S.r[3], P.sxx);
-    ldv<T, V, true>(A.fi[4], o, S.r[4], P.syy);
-    ldv<T, V, true>(A.fi[5], o, S.r[5], P.szz);
-    ldv<T, V, true>(A.fi[6], o, S.r[6], P.sxy);
-    ldv<T, V, true>(A.fi[7], o, S.r[7], P.sxz);
-    ldv<T, V, true>(A.fi[8], o, S.r[8], P.syz);
+    ldv<T, V>(A.fi[3], o, S.r[3], P.sxx);
+    ldv<T, V>(A.fi[4], o, S.r[4], P.syy);
+    ldv<T, V>(A.fi[5], o, S.r[5], P.szz);
+    ldv<T, V>(A.fi[6], o, S.r[6], P.sxy);
+    ldv<T, V>(A.fi[7], o, S.r[7], P.sxz);
+    ldv<T, V>(A.fi[8], o, S.r[8], P.syz);
     const int cz = MODE == kSlab ? (k < A.lo[2] ? k : (k >= A.hi[2] ? A.lo[2] + (k - A.hi[2]) : -1)) : -1;
 #pragma unroll
     for (int e = 0; e < V; ++e) {
         if (L.cx[e] >= 0) {
             const unsigned om = unsigned((k * A.ny + L.j[e]) * A.w[0] + L.cx[e]);
-            P.mx[0][e] = ldf_s(A.mi[0], om, S.r[9]);
-            P.mx[1][e] = ldf_s(A.mi[3], om, S.r[12]);
-            P.mx[2][e] = ldf_s(A.mi[5], om, S.r[14]);
+            P.mx[0][e] = ldf(A.mi[0], om, S.r[9]);
+            P.mx[1][e] = ldf(A.mi[3], om, S.r[12]);
+            P.mx[2][e] = ldf(A.mi[5], om, S.r[14]);
         }
         if (MODE == kSlab) {
             if (L.cy[e] >= 0) {
                 const unsigned om = unsigned((k * A.w[1] + L.cy[e]) * A.nx + L.i[e]);
-                P.my[0][e] = ldf_s(A.mi[1], om, S.r[10]);
-                P.my[1][e] = ldf_s(A.mi[4], om, S.r[13]);
-                P.my[2][e] = ldf_s(A.mi[7], om, S.r[16]);
+                P.my[0][e] = ldf(A.mi[1], om, S.r[10]);
+                P.my[1][e] = ldf(A.mi[4], om, S.r[13]);
+                P.my[2][e] = ldf(A.mi[7], om, S.r[16]);
             }
             if (cz >= 0 && L.act[e]) {  // elements past the box end may sit past the last row
                 const unsigned om = unsigned((cz * A.ny + L.j[e]) * A.nx + L.i[e]);
-                P.mz[0][e] = ldf_s(A.mi[2], om, S.r[11]);
-                P.mz[1][e] = ldf_s(A.mi[6], om, S.r[15]);
-                P.mz[2][e] = ldf_s(A.mi[8], om, S.r[17]);
+                P.mz[0][e] = ldf(A.mi[2], om, S.r[11]);
+                P.mz[1][e] = ldf(A.mi[6], om, S.r[15]);
+                P.mz[2][e] = ldf(A.mi[8], om, S.r[17]);
             }
         }
     }
@@ -543,12 +531,12 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
         }
     }
     if (!store) return;
-    stv<T, V, true>(A.fo[3], o, nsxx, onN, S, 3);
-    stv<T, V, true>(A.fo[4], o, nsyy, onN, S, 4);
-    stv<T, V, true>(A.fo[5], o, nszz, onN, S, 5);
-    stv<T, V, true>(A.fo[6], o, nsxy, onXY, S, 6);
-    stv<T, V, true>(A.fo[7], o, nsxz, onXZ, S, 7);
-    stv<T, V, true>(A.fo[8], o, nsyz, onYZ, S, 8);
+    stv<T, V>(A.fo[3], o, nsxx, onN, S, 3);
+    stv<T, V>(A.fo[4], o, nsyy, onN, S, 4);
+    stv<T, V>(A.fo[5], o, nszz, onN, S, 5);
+    stv<T, V>(A.fo[6], o, nsxy, onXY, S, 6);
+    stv<T, V>(A.fo[7], o, nsxz, onXZ, S, 7);
+    stv<T, V>(A.fo[8], o, nsyz, onYZ, S, 8);
 }
 
 template <typename T, int MODE, int V>
@@ -631,9 +619,9 @@ __device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V
     ldm<V>(A.rho, o + nx, P.rhoN);
     ldv<T, V>(sxz, o + 1, S.r[7], P.sxzE);
     ldv<T, V>(syz, o - nx, S.r[8], P.syzS);
-    ldv<T, V, true>(A.fi[0], o, S.r[0], P.vx);
-    ldv<T, V, true>(A.fi[1], o, S.r[1], P.vy);
-    ldv<T, V, true>(A.fi[2], o, S.r[2], P.vz);
+    ldv<T, V>(A.fi[0], o, S.r[0], P.vx);
+    ldv<T, V>(A.fi[1], o, S.r[1], P.vy);
+    ldv<T, V>(A.fi[2], o, S.r[2], P.vz);
     if (ENERGY) {
         ldm<V>(A.cp, o, P.cp);
         ldm<V>(A.cs, o, P.cs);
@@ -643,22 +631,22 @@ __device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V
     for (int e = 0; e < V; ++e) {
         if (L.cx[e] >= 0) {
             const unsigned om = unsigned((k * A.ny + L.j[e]) * A.w[0] + L.cx[e]);
-            P.mx[0][e] = ldf_s(A.mi[9], om, S.r[18]);
-            P.mx[1][e] = ldf_s(A.mi[12], om, S.r[21]);
-            P.mx[2][e] = ldf_s(A.mi[15], om, S.r[24]);
+            P.mx[0][e] = ldf(A.mi[9], om, S.r[18]);
+            P.mx[1][e] = ldf(A.mi[12], om, S.r[21]);
+            P.mx[2][e] = ldf(A.mi[15], om, S.r[24]);
         }
         if (MODE == kSlab) {
             if (L.cy[e] >= 0) {
                 const unsigned om = unsigned((k * A.w[1] + L.cy[e]) * A.nx + L.i[e]);
-                P.my[0][e] = ldf_s(A.mi[10], om, S.r[19]);
-                P.my[1][e] = ldf_s(A.mi[13], om, S.r[22]);
-                P.my[2][e] = ldf_s(A.mi[16], om, S.r[25]);
+                P.my[0][e] = ldf(A.mi[10], om, S.r[19]);
+                P.my[1][e] = ldf(A.mi[13], om, S.r[22]);
+                P.my[2][e] = ldf(A.mi[16], om, S.r[25]);
             }
             if (cz >= 0 && L.act[e]) {  // elements past the box end may sit past the last row
                 const unsigned om = unsigned((cz * A.ny + L.j[e]) * A.nx + L.i[e]);
-                P.mz[0][e] = ldf_s(A.mi[11], om, S.r[20]);
-                P.mz[1][e] = ldf_s(A.mi[14], om, S.r[23]);
-                P.mz[2][e] = ldf_s(A.mi[17], om, S.r[26]);
+                P.mz[0][e] = ldf(A.mi[11], om, S.r[20]);
+                P.mz[1][e] = ldf(A.mi[14], om, S.r[23]);
+                P.mz[2][e] = ldf(A.mi[17], om, S.r[26]);
             }
         }
     }
@@ -792,9 +780,9 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             }
         }
     }
-    stv<T, V, true>(A.fo[0], o, nvx, wx, S, 0);
-    stv<T, V, true>(A.fo[1], o, nvy, wy, S, 1);
-    stv<T, V, true>(A.fo[2], o, nvz, wz, S, 2);
+    stv<T, V>(A.fo[0], o, nvx, wx, S, 0);
+    stv<T, V>(A.fo[1], o, nvy, wy, S, 1);
+    stv<T, V>(A.fo[2], o, nvz, wz, S, 2);
 }
 
 template <typename T, int MODE, bool ENERGY, int V>
@@ -1159,9 +1147,9 @@ __global__ void __launch_bounds__(kFusedThreads) k_fused(const StepArgs<T> A)
             }
             if (L.cx[0] >= 0) {
                 const unsigned om = unsigned((k * A.ny + L.j[0]) * A.w[0] + L.cx[0]);
-                P.mx[0][0] = ldf_s(A.mi[9], om, Sv.r[18]);
-                P.mx[1][0] = ldf_s(A.mi[12], om, Sv.r[21]);
-                P.mx[2][0] = ldf_s(A.mi[15], om, Sv.r[24]);
+                P.mx[0][0] = ldf(A.mi[9], om, Sv.r[18]);
+                P.mx[1][0] = ldf(A.mi[12], om, Sv.r[21]);
+                P.mx[2][0] = ldf(A.mi[15], om, Sv.r[24]);
             }
             Q.sxz_m[0] = sxz_m[u];
             Q.syz_m[0] = syz_m[u];
