@@ -843,7 +843,14 @@ __global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
     load_scales(A, S, true);
     const bool slab = MODE == kSlab || (MODE == kMixed && B.mode == kSlab);
     const TabP<T>*tx, *ty;
-    pick_tables(A, tabs, slab, jlo, jhi, tx, ty);
+    if (MODE == kInterior && !std::is_same<T, double>::value) {
+        // interior launches: x coefficients straight from the FP32 copies (only
+        // x C-PML lanes read them), no staging and no barrier
+        tx = reinterpret_cast<const TabP<T>*>(A.tabf);
+        ty = tx;
+    } else {
+        pick_tables(A, tabs, slab, jlo, jhi, tx, ty);
+    }
     if (slab)
         stress_body<T, kSlab, V>(A, B, chunk, zs, tx, ty, S);
     else
@@ -863,7 +870,14 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
     load_scales(A, S, false);
     const bool slab = MODE == kSlab || (MODE == kMixed && B.mode == kSlab);
     const TabP<T>*tx, *ty;
-    pick_tables(A, tabs, slab, jlo, jhi, tx, ty);
+    if (MODE == kInterior && !std::is_same<T, double>::value) {
+        // interior launches: x coefficients straight from the FP32 copies (only
+        // x C-PML lanes read them), no staging and no barrier
+        tx = reinterpret_cast<const TabP<T>*>(A.tabf);
+        ty = tx;
+    } else {
+        pick_tables(A, tabs, slab, jlo, jhi, tx, ty);
+    }
     double ek = 0.0, ep = 0.0;
     if (slab)
         velocity_body<T, kSlab, ENERGY, V>(A, B, chunk, zs, tx, ty, ek, ep, S);
@@ -897,8 +911,8 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
 template <typename T>
 static size_t table_lds(const StepArgs<T>& a)
 {
-    if (std::is_same<T, double>::value) return 0;
-    return size_t(6) * (a.nx + (a.mode != kInterior ? a.ny : 0)) * sizeof(float);
+    if (std::is_same<T, double>::value || a.mode == kInterior) return 0;
+    return size_t(6) * (a.nx + a.ny) * sizeof(float);
 }
 
 template <typename T, int M, int V>
