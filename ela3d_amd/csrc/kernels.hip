@@ -18,9 +18,9 @@
 //  * C-PML memory variables live in compact per-axis slabs.  Interior boxes test
 //    x per element only (y and z lie outside the C-PML by construction); the
 //    absorbing-layer boxes test all three axes.  Each pass is two launches, one
-//    per box mode, each compiled for its mode (or one mixed launch).  FP32
-//    copies of the a/b/K tables of x (and y in absorbing-layer boxes) are staged
-//    per workgroup in LDS.
+//    per box mode, each compiled for its mode (or one mixed launch).  A lane
+//    keeps its xy positions through its z-march, so its C-PML a/b/K coefficients
+//    (FP32 copies, or the FP64 tables) are loaded once into registers.
 //  * Arithmetic is FP32 (FP64 for the FP64 storage instantiation); energy
 //    accumulates in FP64.
 //
@@ -60,18 +60,6 @@ __device__ __forceinline__ void tile_of(const Box& B, int local, int& chunk, int
     const int off = local - c0 * nzs;
     chunk = c0 + off % ws;
     zs = off / ws;
-}
-
-// This workgroup's tile and the rows [jlo, jhi] its xy range spans.
-template <int V, typename T>
-__device__ __forceinline__ void block_tile(const StepArgs<T>& A, const Box& B, int local, int& chunk, int& zs,
-                                           int& jlo, int& jhi)
-{
-    tile_of(B, local, chunk, zs);
-    const int ps = B.p0 + chunk * A.block * V;
-    const int pe = min(ps + A.block * V, B.p1) - 1;
-    jlo = ps / A.nx;
-    jhi = min(pe / A.nx, A.ny - 1);
 }
 
 __device__ __forceinline__ Box pick_box(const Box* bx, int nbox, int lb)
@@ -232,40 +220,6 @@ using TabP = typename std::conditional<std::is_same<T, double>::value, double, f
 // (a, b, K on the full grid, a, b, K on the half grid).
 // Interior workgroups read only the x coefficients, so only absorbing-layer
 // workgroups stage the y tables.
-// The x tables are staged whole; of the y tables only rows [jlo, jhi] (the rows
-// this workgroup's xy range touches), at their places in the [6][ny] layout.
-template <typename T>
-__device__ __forceinline__ void stage_tables(const StepArgs<T>& A, float* s, bool with_y, int jlo, int jhi)
-{
-    const int nx6 = 6 * A.nx;
-    for (int q = threadIdx.x; q < nx6; q += blockDim.x) s[q] = A.tabf[q];
-    if (with_y) {
-        const int w = jhi - jlo + 1;
-        for (int q = threadIdx.x; q < 6 * w; q += blockDim.x) {
-            const int t = q / w, j = jlo + (q - t * w);
-            s[nx6 + t * A.ny + j] = A.tabf[nx6 + t * A.ny + j];
-        }
-    }
-    __syncthreads();
-}
-
-// x and y coefficient tables of a pass: the FP64 tables for FP64 arithmetic,
-// otherwise the FP32 copies staged in LDS (32-bit LDS addressing keeps the
-// register footprint below that of reading them through the caches).
-template <typename T>
-__device__ __forceinline__ void pick_tables(const StepArgs<T>& A, float* lds, bool with_y, int jlo, int jhi,
-                                            const TabP<T>*& tx, const TabP<T>*& ty)
-{
-    if constexpr (std::is_same<T, double>::value) {
-        tx = A.tab[0];
-        ty = A.tab[1];
-    } else {
-        stage_tables(A, lds, with_y, jlo, jhi);
-        tx = lds;
-        ty = lds + 6 * A.nx;
-    }
-}
-
 // Per-lane geometry of the VEC elements p0..p0+VEC-1.
 template <int V>
 struct Lane {
@@ -294,6 +248,47 @@ __device__ __forceinline__ void make_lane(const StepArgs<T>& A, int p0, int p1, 
         if (++i == nx) {
             i = 0;
             ++j;
+        }
+    }
+}
+
+// C-PML coefficients of a lane's elements: a, b, K on the full grid then on the
+// half grid, for x (x C-PML lanes) and y (absorbing-layer boxes, y C-PML lanes).
+// A lane keeps its xy positions through its whole z-march, so these are loaded
+// once per workgroup into registers.
+template <typename C, int V>
+struct LaneTabs {
+    C x[6][V];
+    C y[6][V];
+};
+
+template <typename T, int MODE, int V>
+__device__ __forceinline__ void lane_tabs(const StepArgs<T>& A, const Lane<V>& L, LaneTabs<TabP<T>, V>& LT)
+{
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            LT.x[q][e] = 0;
+            LT.y[q][e] = 0;
+        }
+        if (L.cx[e] >= 0) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                if constexpr (std::is_same<T, double>::value)
+                    LT.x[q][e] = A.tab[0][q * A.nx + L.i[e]];
+                else
+                    LT.x[q][e] = A.tabf[q * A.nx + L.i[e]];
+            }
+        }
+        if (MODE == kSlab && L.cy[e] >= 0) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                if constexpr (std::is_same<T, double>::value)
+                    LT.y[q][e] = A.tab[1][q * A.ny + L.j[e]];
+                else
+                    LT.y[q][e] = A.tabf[6 * A.nx + q * A.ny + L.j[e]];
+            }
         }
     }
 }
@@ -415,8 +410,7 @@ struct SOut {
 // `store` = false computes without writing anything (a halo cell recomputed by
 // a neighbouring fused workgroup); `out`, if given, receives the stresses.
 template <typename T, int MODE, int V, typename R>
-__device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V>& L, const TabP<T>* tx,
-                                              const TabP<T>* ty, int k, const SPlane<R, V>& P,
+__device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V>& L, const LaneTabs<TabP<T>, V>& LT, int k, const SPlane<R, V>& P,
                                               const SQueue<R, V>& Q, Scales<T>& S, bool store = true,
                                               SOut<R, V>* out = nullptr)
 {
@@ -445,12 +439,12 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvy_dy = (vy_c - P.vyS[e]) * idy;
             R dvz_dz = (P.vz_c[e] - Q.vz_m[e]) * idz;
             if (cx >= 0) {
-                const R m = cpml(P.mx[0][e], (R)tx[3 * nx + i], (R)tx[4 * nx + i], (R)tx[5 * nx + i], dvx_dx);
+                const R m = cpml(P.mx[0][e], (R)LT.x[3][e], (R)LT.x[4][e], (R)LT.x[5][e], dvx_dx);
                 if (store && onN[e]) stm(A.mo[0], omx, m, S, 9);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
-                    const R m = cpml(P.my[0][e], (R)ty[j], (R)ty[ny + j], (R)ty[2 * ny + j], dvy_dy);
+                    const R m = cpml(P.my[0][e], (R)LT.y[0][e], (R)LT.y[1][e], (R)LT.y[2][e], dvy_dy);
                     if (store && onN[e]) stm(A.mo[1], omy, m, S, 10);
                 }
                 if (Z.cz >= 0) {
@@ -471,11 +465,11 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvy_dx = (vy_c - P.vyW[e]) * idx;
             R dvx_dy = (P.vxN[e] - vx_c) * idy;
             if (cx >= 0) {
-                const R m = cpml(P.mx[1][e], (R)tx[i], (R)tx[nx + i], (R)tx[2 * nx + i], dvy_dx);
+                const R m = cpml(P.mx[1][e], (R)LT.x[0][e], (R)LT.x[1][e], (R)LT.x[2][e], dvy_dx);
                 if (store && onXY[e]) stm(A.mo[3], omx, m, S, 12);
             }
             if (MODE == kSlab && cy >= 0) {
-                const R m = cpml(P.my[1][e], (R)ty[3 * ny + j], (R)ty[4 * ny + j], (R)ty[5 * ny + j], dvx_dy);
+                const R m = cpml(P.my[1][e], (R)LT.y[3][e], (R)LT.y[4][e], (R)LT.y[5][e], dvx_dy);
                 if (store && onXY[e]) stm(A.mo[4], omy, m, S, 13);
             }
             nsxy[e] = DT_mu * (dvy_dx + dvx_dy) + P.sxy[e];
@@ -489,7 +483,7 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvz_dx = (P.vz_c[e] - P.vzW[e]) * idx;
             R dvx_dz = (P.vx_u[e] - vx_c) * idz;
             if (cx >= 0) {
-                const R m = cpml(P.mx[2][e], (R)tx[i], (R)tx[nx + i], (R)tx[2 * nx + i], dvz_dx);
+                const R m = cpml(P.mx[2][e], (R)LT.x[0][e], (R)LT.x[1][e], (R)LT.x[2][e], dvz_dx);
                 if (store && onXZ[e]) stm(A.mo[5], omx, m, S, 14);
             }
             if (MODE == kSlab && Z.cz >= 0) {
@@ -508,7 +502,7 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvy_dz = (P.vy_u[e] - vy_c) * idz;
             if (MODE == kSlab) {
                 if (cy >= 0) {
-                    const R m = cpml(P.my[2][e], (R)ty[3 * ny + j], (R)ty[4 * ny + j], (R)ty[5 * ny + j], dvz_dy);
+                    const R m = cpml(P.my[2][e], (R)LT.y[3][e], (R)LT.y[4][e], (R)LT.y[5][e], dvz_dy);
                     if (store && onYZ[e]) stm(A.mo[7], omy, m, S, 16);
                 }
                 if (Z.cz >= 0) {
@@ -540,8 +534,7 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
 }
 
 template <typename T, int MODE, int V>
-__device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, int chunk, int zs, const TabP<T>* tx,
-                                            const TabP<T>* ty, Scales<T>& S)
+__device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, int chunk, int zs, Scales<T>& S)
 {
     typedef typename Arith<T>::R R;
     const int p0 = B.p0 + (chunk * A.block + (int)threadIdx.x) * V;
@@ -550,6 +543,8 @@ __device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, 
     const int ke = min(kb + B.zseg, B.k1);
     Lane<V> L;
     make_lane<T, MODE, V>(A, p0, B.p1, L);
+    LaneTabs<TabP<T>, V> LT;
+    lane_tabs<T, MODE, V>(A, L, LT);
     SQueue<R, V> Q;
     {
         const unsigned o = cell(A, kb, p0);
@@ -565,7 +560,7 @@ __device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, 
     for (int k = kb; k < ke; ++k) {
         SPlane<R, V> P;
         stress_load<T, MODE, V, R>(A, L, k, P, S);
-        stress_update<T, MODE, V, R>(A, L, tx, ty, k, P, Q, S);
+        stress_update<T, MODE, V, R>(A, L, LT, k, P, Q, S);
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             Q.vx_c[e] = P.vx_u[e];
@@ -660,8 +655,7 @@ __device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V
 }
 
 template <typename T, int MODE, bool ENERGY, int V, typename R>
-__device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane<V>& L, const TabP<T>* tx,
-                                                const TabP<T>* ty, int k, const VPlane<R, V>& P,
+__device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane<V>& L, const LaneTabs<TabP<T>, V>& LT, int k, const VPlane<R, V>& P,
                                                 const VQueue<R, V>& Q, double& ek, double& ep,
                                                 Scales<T>& S)
 {
@@ -688,12 +682,12 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d2 = (P.sxy[e] - P.sxyS[e]) * idy;
             R d3 = (P.sxz[e] - Q.sxz_m[e]) * idz;
             if (cx >= 0) {
-                const R m = cpml(P.mx[0][e], (R)tx[i], (R)tx[nx + i], (R)tx[2 * nx + i], d1);
+                const R m = cpml(P.mx[0][e], (R)LT.x[0][e], (R)LT.x[1][e], (R)LT.x[2][e], d1);
                 if (onx) stm(A.mo[9], omx, m, S, 18);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
-                    const R m = cpml(P.my[0][e], (R)ty[j], (R)ty[ny + j], (R)ty[2 * ny + j], d2);
+                    const R m = cpml(P.my[0][e], (R)LT.y[0][e], (R)LT.y[1][e], (R)LT.y[2][e], d2);
                     if (onx) stm(A.mo[10], omy, m, S, 19);
                 }
                 if (Z.cz >= 0) {
@@ -713,12 +707,12 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d2 = (P.syyN[e] - P.syy[e]) * idy;
             R d3 = (P.syz[e] - Q.syz_m[e]) * idz;
             if (cx >= 0) {
-                const R m = cpml(P.mx[1][e], (R)tx[3 * nx + i], (R)tx[4 * nx + i], (R)tx[5 * nx + i], d1);
+                const R m = cpml(P.mx[1][e], (R)LT.x[3][e], (R)LT.x[4][e], (R)LT.x[5][e], d1);
                 if (ony) stm(A.mo[12], omx, m, S, 21);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
-                    const R m = cpml(P.my[1][e], (R)ty[3 * ny + j], (R)ty[4 * ny + j], (R)ty[5 * ny + j], d2);
+                    const R m = cpml(P.my[1][e], (R)LT.y[3][e], (R)LT.y[4][e], (R)LT.y[5][e], d2);
                     if (ony) stm(A.mo[13], omy, m, S, 22);
                 }
                 if (Z.cz >= 0) {
@@ -738,12 +732,12 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d2 = (P.syz[e] - P.syzS[e]) * idy;
             R d3 = (P.szz_u[e] - Q.szz_c[e]) * idz;
             if (cx >= 0) {
-                const R m = cpml(P.mx[2][e], (R)tx[3 * nx + i], (R)tx[4 * nx + i], (R)tx[5 * nx + i], d1);
+                const R m = cpml(P.mx[2][e], (R)LT.x[3][e], (R)LT.x[4][e], (R)LT.x[5][e], d1);
                 if (onz) stm(A.mo[15], omx, m, S, 24);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
-                    const R m = cpml(P.my[2][e], (R)ty[j], (R)ty[ny + j], (R)ty[2 * ny + j], d2);
+                    const R m = cpml(P.my[2][e], (R)LT.y[0][e], (R)LT.y[1][e], (R)LT.y[2][e], d2);
                     if (onz) stm(A.mo[16], omy, m, S, 25);
                 }
                 if (Z.cz >= 0) {
@@ -786,9 +780,8 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
 }
 
 template <typename T, int MODE, bool ENERGY, int V>
-__device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B, int chunk, int zs,
-                                              const TabP<T>* tx, const TabP<T>* ty, double& ek, double& ep,
-                                              Scales<T>& S)
+__device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B, int chunk, int zs, double& ek,
+                                              double& ep, Scales<T>& S)
 {
     typedef typename Arith<T>::R R;
     const int p0 = B.p0 + (chunk * A.block + (int)threadIdx.x) * V;
@@ -797,6 +790,8 @@ __device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B
     const int ke = min(kb + B.zseg, B.k1);
     Lane<V> L;
     make_lane<T, MODE, V>(A, p0, B.p1, L);
+    LaneTabs<TabP<T>, V> LT;
+    lane_tabs<T, MODE, V>(A, L, LT);
     VQueue<R, V> Q;
     {
         const unsigned o = cell(A, kb, p0);
@@ -815,7 +810,7 @@ __device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B
     for (int k = kb; k < ke; ++k) {
         VPlane<R, V> P;
         velocity_load<T, MODE, ENERGY, V, R>(A, L, k, P, S);
-        velocity_update<T, MODE, ENERGY, V, R>(A, L, tx, ty, k, P, Q, ek, ep, S);
+        velocity_update<T, MODE, ENERGY, V, R>(A, L, LT, k, P, Q, ek, ep, S);
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             Q.sxz_m[e] = P.sxz[e];
@@ -834,55 +829,35 @@ __device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B
 template <typename T, int MODE, int V>
 __global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
 {
-    extern __shared__ float tabs[];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const Box B = MODE == kInterior ? A.box[0] : pick_box(A.box, A.nbox, lb);
-    int chunk, zs, jlo, jhi;
-    block_tile<V>(A, B, lb - B.blk0, chunk, zs, jlo, jhi);
+    int chunk, zs;
+    tile_of(B, lb - B.blk0, chunk, zs);
     Scales<T> S;
     load_scales(A, S, true);
     const bool slab = MODE == kSlab || (MODE == kMixed && B.mode == kSlab);
-    const TabP<T>*tx, *ty;
-    if (MODE == kInterior && !std::is_same<T, double>::value) {
-        // interior launches: x coefficients straight from the FP32 copies (only
-        // x C-PML lanes read them), no staging and no barrier
-        tx = reinterpret_cast<const TabP<T>*>(A.tabf);
-        ty = tx;
-    } else {
-        pick_tables(A, tabs, slab, jlo, jhi, tx, ty);
-    }
     if (slab)
-        stress_body<T, kSlab, V>(A, B, chunk, zs, tx, ty, S);
+        stress_body<T, kSlab, V>(A, B, chunk, zs, S);
     else
-        stress_body<T, kInterior, V>(A, B, chunk, zs, tx, ty, S);
+        stress_body<T, kInterior, V>(A, B, chunk, zs, S);
     flush_maxima(A, S, 3, 18);
 }
 
 template <typename T, int MODE, bool ENERGY, int V>
 __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
 {
-    extern __shared__ float tabs[];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const Box B = MODE == kInterior ? A.box[0] : pick_box(A.box, A.nbox, lb);
-    int chunk, zs, jlo, jhi;
-    block_tile<V>(A, B, lb - B.blk0, chunk, zs, jlo, jhi);
+    int chunk, zs;
+    tile_of(B, lb - B.blk0, chunk, zs);
     Scales<T> S;
     load_scales(A, S, false);
     const bool slab = MODE == kSlab || (MODE == kMixed && B.mode == kSlab);
-    const TabP<T>*tx, *ty;
-    if (MODE == kInterior && !std::is_same<T, double>::value) {
-        // interior launches: x coefficients straight from the FP32 copies (only
-        // x C-PML lanes read them), no staging and no barrier
-        tx = reinterpret_cast<const TabP<T>*>(A.tabf);
-        ty = tx;
-    } else {
-        pick_tables(A, tabs, slab, jlo, jhi, tx, ty);
-    }
     double ek = 0.0, ep = 0.0;
     if (slab)
-        velocity_body<T, kSlab, ENERGY, V>(A, B, chunk, zs, tx, ty, ek, ep, S);
+        velocity_body<T, kSlab, ENERGY, V>(A, B, chunk, zs, ek, ep, S);
     else
-        velocity_body<T, kInterior, ENERGY, V>(A, B, chunk, zs, tx, ty, ek, ep, S);
+        velocity_body<T, kInterior, ENERGY, V>(A, B, chunk, zs, ek, ep, S);
     flush_maxima(A, S, 0, 3);
     flush_maxima(A, S, 18, 27);
     if (ENERGY) {  // per-block FP64 partials, summed in block order by the tail kernel
@@ -908,23 +883,16 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
     }
 }
 
-template <typename T>
-static size_t table_lds(const StepArgs<T>& a)
-{
-    if (std::is_same<T, double>::value || a.mode == kInterior) return 0;
-    return size_t(6) * (a.nx + a.ny) * sizeof(float);
-}
-
 template <typename T, int M, int V>
 static void launch_stress_v(const StepArgs<T>& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_stress<T, M, V>), dim3(a.nblocks), dim3(a.block), table_lds<T>(a), s, a);
+    hipLaunchKernelGGL((k_stress<T, M, V>), dim3(a.nblocks), dim3(a.block), 0, s, a);
 }
 
 template <typename T, int M, bool E, int V>
 static void launch_velocity_v(const StepArgs<T>& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_velocity<T, M, E, V>), dim3(a.nblocks), dim3(a.block), table_lds<T>(a), s, a);
+    hipLaunchKernelGGL((k_velocity<T, M, E, V>), dim3(a.nblocks), dim3(a.block), 0, s, a);
 }
 
 // interior and mixed launches: VEC 1, 2 or 4; absorbing-layer launches: VEC 1 or 2
@@ -1007,20 +975,21 @@ template <typename T>
 size_t fused_lds_bytes(int nx, int tile)
 {
     typedef typename Arith<T>::R R;
-    const size_t tab = std::is_same<T, double>::value ? 0 : (size_t(6) * nx * sizeof(float) + 15) / 16 * 16;
-    const size_t bytes = tab + size_t(2) * 5 * size_t(tile + 2 * nx) * sizeof(R);
+    const size_t bytes = size_t(2) * 5 * size_t(tile + 2 * nx) * sizeof(R);
     return bytes <= 65536 - 1024 ? bytes : 0;  // room for the kernel's static LDS
 }
 
 // new stresses of cell p at plane k: recomputed inside the interior box, else
 // read from the set the absorbing-layer stress launch wrote
 template <typename T, typename R>
-__device__ __forceinline__ void fused_sigma(const StepArgs<T>& A, const Box& B, const TabP<T>* tx, int p, int k,
+__device__ __forceinline__ void fused_sigma(const StepArgs<T>& A, const Box& B, int p, int k,
                                             bool store, Scales<T>& S, Scales<T>& Sv, SOut<R, 1>& out)
 {
     if (p >= B.p0 && p < B.p1 && k >= B.k0 && k < B.k1) {
         Lane<1> L;
         make_lane<T, kInterior, 1>(A, p, B.p1, L);
+        LaneTabs<TabP<T>, 1> LT;
+        lane_tabs<T, kInterior, 1>(A, L, LT);
         SQueue<R, 1> Q;
         const unsigned o = cell(A, k, p);
         Q.vx_c[0] = ldf(A.fi[0], o, S.r[0]);
@@ -1030,7 +999,7 @@ __device__ __forceinline__ void fused_sigma(const StepArgs<T>& A, const Box& B, 
         Q.vz_m[0] = k >= 1 ? (R)ldf(A.fi[2], o - unsigned(A.nxyp), S.r[2]) : (R)0;
         SPlane<R, 1> P;
         stress_load<T, kInterior, 1, R>(A, L, k, P, S);
-        stress_update<T, kInterior, 1, R>(A, L, tx, tx, k, P, Q, S, store, &out);
+        stress_update<T, kInterior, 1, R>(A, L, LT, k, P, Q, S, store, &out);
     } else {
         const unsigned o = cell(A, k, p);
         out.sxx[0] = ldf((const T*)A.fo[3], o, Sv.r[3]);
@@ -1059,17 +1028,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_fused(const StepArgs<T> A)
     Scales<T> S, Sv;
     load_scales(A, S, true);
     load_scales(A, Sv, false);
-    const TabP<T>* tx;
-    R* lds;
-    if constexpr (std::is_same<T, double>::value) {
-        tx = A.tab[0];
-        lds = reinterpret_cast<R*>(smem);
-    } else {
-        for (int q = tid; q < 6 * nx; q += kFusedThreads) smem[q] = A.tabf[q];
-        tx = smem;
-        lds = reinterpret_cast<R*>(smem + (6 * nx + 3) / 4 * 4);
-    }
-    __syncthreads();  // tables staged before the first stress update reads them
+    R* lds = reinterpret_cast<R*>(smem);
     // lds[(b*5 + comp)*LR + (c - rlo)]: comp 0 sxx, 1 sxy, 2 syy, 3 sxz, 4 syz of plane b = k & 1
     auto at = [&](int b, int comp, int c) -> R& { return lds[(b * 5 + comp) * LR + (c - rlo)]; };
     // own cells q0 + tid + 256*u: column queues across planes
@@ -1084,7 +1043,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_fused(const StepArgs<T> A)
         szz_u[u] = 0;
         if (c < q1 - q0) {
             SOut<R, 1> o;
-            fused_sigma<T, R>(A, B, tx, q0 + c, kb - 1, false, S, Sv, o);
+            fused_sigma<T, R>(A, B, q0 + c, kb - 1, false, S, Sv, o);
             sxz_m[u] = o.sxz[0];
             syz_m[u] = o.syz[0];
         }
@@ -1096,7 +1055,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_fused(const StepArgs<T> A)
              c += kFusedThreads) {
             const bool own = c >= 0 && c < q1 - q0;
             SOut<R, 1> o;
-            fused_sigma<T, R>(A, B, tx, q0 + c, kb, own, S, Sv, o);
+            fused_sigma<T, R>(A, B, q0 + c, kb, own, S, Sv, o);
             at(b, 0, c) = o.sxx[0];
             at(b, 1, c) = o.sxy[0];
             at(b, 2, c) = o.syy[0];
@@ -1116,7 +1075,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_fused(const StepArgs<T> A)
              c < (last ? q1 - q0 : rhi); c += kFusedThreads) {
             const bool own = c >= 0 && c < q1 - q0;
             SOut<R, 1> o;
-            fused_sigma<T, R>(A, B, tx, q0 + c, k + 1, own && !last, S, Sv, o);
+            fused_sigma<T, R>(A, B, q0 + c, k + 1, own && !last, S, Sv, o);
             if (!last) {
                 at(nb, 0, c) = o.sxx[0];
                 at(nb, 1, c) = o.sxy[0];
@@ -1134,6 +1093,8 @@ __global__ void __launch_bounds__(kFusedThreads) k_fused(const StepArgs<T> A)
             const int p = q0 + c;
             Lane<1> L;
             make_lane<T, kInterior, 1>(A, p, B.p1, L);
+            LaneTabs<TabP<T>, 1> LT;
+            lane_tabs<T, kInterior, 1>(A, L, LT);
             const unsigned o = cell(A, k, p);
             VPlane<R, 1> P;
             VQueue<R, 1> Q;
@@ -1169,7 +1130,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_fused(const StepArgs<T> A)
             Q.syz_m[0] = syz_m[u];
             Q.szz_c[0] = szz_c[u];
             Q.rho_c[0] = (R)A.rho[o];
-            velocity_update<T, kInterior, ENERGY, 1, R>(A, L, tx, tx, k, P, Q, ek, ep, Sv);
+            velocity_update<T, kInterior, ENERGY, 1, R>(A, L, LT, k, P, Q, ek, ep, Sv);
             sxz_m[u] = P.sxz[0];
             syz_m[u] = P.syz[0];
             szz_c[u] = szz_u[u];

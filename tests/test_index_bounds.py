@@ -142,8 +142,7 @@ def fused_violations(cfg, tile, zseg, lds_bytes_limit=65536 - 1024, elem=4):
     assert pb > pa
     zlo, zhi = lo[2], hi[2]
     LR = tile + 2 * nx
-    tab = 0 if elem == 8 else (6 * nx * 4 + 15) // 16 * 16
-    if tab + 2 * 5 * LR * elem > lds_bytes_limit:
+    if 2 * 5 * LR * elem > lds_bytes_limit:
         return None  # the library refuses this tile (fused_unsupported)
     bad = []
     nchunk = (pb - pa + tile - 1) // tile
