@@ -186,6 +186,15 @@ ShotPlan make_plan(const ela3d_config& in)
         fail("grid too small: each of nx, ny, nz must be >= max(3, 2*npoints_pml+3)");
     const int64_t cells = int64_t(c.nx) * c.ny * c.nz;
     if (cells > (int64_t(1) << 30)) fail("grid has more than 2^30 cells (32-bit offsets)");
+    {
+        // kernels address each field and model array by a 32-bit byte offset from
+        // its start: a padded array (guard | nz planes of stride roundup(nx*ny, 64) | guard)
+        // must stay below 4 GiB at the widest element (FP64 storage 8 B, else the FP32 model's 4 B)
+        const int64_t padded = (int64_t(c.nx) * c.ny + 63) / 64 * 64 * c.nz + 2 * ((int64_t(c.nx) + 127) / 64 * 64);
+        const int64_t elem = c.storage == ELA3D_STORAGE_FP64 ? 8 : 4;
+        if (padded * elem >= (int64_t(1) << 32))
+            fail("grid too large: a padded field or model array would exceed the 4 GiB its 32-bit byte offsets address");
+    }
     if (c.nx + c.ny > 2700) fail("nx + ny must be <= 2700 (x and y C-PML tables are staged in LDS)");
     if (!(c.dx > 0 && c.dy > 0 && c.dz > 0)) fail("dx, dy, dz must be > 0");
     if (!(c.dt > 0)) fail("dt must be > 0");

@@ -17,23 +17,37 @@ struct Arith<double> {
     typedef double R;
 };
 
-// Offsets are unsigned 32-bit so loads use the SGPR-base + 32-bit VGPR-offset form.
+// Element o of p, addressed as the (uniform) base plus a 32-bit byte offset, so
+// a load or store takes the SGPR-base + VGPR-offset form with no 64-bit address
+// arithmetic per access.  Every allocation addressed this way is below 4 GiB
+// (checked on the host when a shot is created).
+template <typename T>
+__device__ __forceinline__ T& el(T* p, unsigned o)
+{
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + o * unsigned(sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ const T& el(const T* p, unsigned o)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + o * unsigned(sizeof(T)));
+}
+
 template <typename T>
 __device__ __forceinline__ typename Arith<T>::R ldf(const T* p, unsigned o, float isc)
 {
     if constexpr (std::is_same<T, half_t>::value)
-        return (float)p[o] * isc;
+        return (float)el(p, o) * isc;
     else
-        return p[o];
+        return el(p, o);
 }
 
 template <typename T>
 __device__ __forceinline__ void stf(T* p, unsigned o, typename Arith<T>::R v, float sc)
 {
     if constexpr (std::is_same<T, half_t>::value)
-        p[o] = (half_t)(v * sc);
+        el(p, o) = (half_t)(v * sc);
     else
-        p[o] = v;
+        el(p, o) = v;
 }
 
 __device__ __forceinline__ double wave_sum(double v)

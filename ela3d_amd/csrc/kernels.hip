@@ -91,7 +91,7 @@ __device__ __forceinline__ void ldv(const T* p, unsigned o, float isc, typename 
     } else {
         typedef T vt __attribute__((ext_vector_type(V)));
         vt v;
-        __builtin_memcpy(&v, p + o, sizeof(vt));
+        __builtin_memcpy(&v, &el(p, o), sizeof(vt));
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             if constexpr (std::is_same<T, half_t>::value)
@@ -107,11 +107,11 @@ template <int V, typename R>
 __device__ __forceinline__ void ldm(const float* p, unsigned o, R (&out)[V])
 {
     if constexpr (V == 1) {
-        out[0] = (R)p[o];
+        out[0] = (R)el(p, o);
     } else {
         typedef float vt __attribute__((ext_vector_type(V)));
         vt v;
-        __builtin_memcpy(&v, p + o, sizeof(vt));
+        __builtin_memcpy(&v, &el(p, o), sizeof(vt));
 #pragma unroll
         for (int e = 0; e < V; ++e) out[e] = (R)v[e];
     }
@@ -193,7 +193,7 @@ __device__ __forceinline__ void stv(T* p, unsigned o, const typename Arith<T>::R
                 else
                     v[e] = val[e];
             }
-            __builtin_memcpy(p + o, &v, sizeof(vt));
+            __builtin_memcpy(&el(p, o), &v, sizeof(vt));
             return;
         }
     }
@@ -409,7 +409,12 @@ struct SOut {
 
 // `store` = false computes without writing anything (a halo cell recomputed by
 // a neighbouring fused workgroup); `out`, if given, receives the stresses.
-template <typename T, int MODE, int V, typename R>
+// BFX (branch-free x C-PML, fused kernel): every lane runs the x recurrences,
+// lanes outside the x C-PML with a = b = 0, K = 1 (which leaves the derivative
+// unchanged), and only the memory-variable stores are conditional.  No loaded
+// value is then used under a branch only, so the compiler cannot sink a load
+// into one (where its wait would drain every access in flight).
+template <typename T, int MODE, int V, typename R, bool BFX = false>
 __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V>& L, const LaneTabs<TabP<T>, V>& LT, int k, const SPlane<R, V>& P,
                                               const SQueue<R, V>& Q, Scales<T>& S, bool store = true,
                                               SOut<R, V>* out = nullptr)
@@ -438,9 +443,9 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvx_dx = (P.vxE[e] - vx_c) * idx;
             R dvy_dy = (vy_c - P.vyS[e]) * idy;
             R dvz_dz = (P.vz_c[e] - Q.vz_m[e]) * idz;
-            if (cx >= 0) {
+            if (BFX || cx >= 0) {
                 const R m = cpml(P.mx[0][e], (R)LT.x[3][e], (R)LT.x[4][e], (R)LT.x[5][e], dvx_dx);
-                if (store && onN[e]) stm(A.mo[0], omx, m, S, 9);
+                if (store && onN[e] && (!BFX || cx >= 0)) stm(A.mo[0], omx, m, S, 9);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
@@ -464,9 +469,9 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             const R DT_mu = dt * (rh * vs * vs);
             R dvy_dx = (vy_c - P.vyW[e]) * idx;
             R dvx_dy = (P.vxN[e] - vx_c) * idy;
-            if (cx >= 0) {
+            if (BFX || cx >= 0) {
                 const R m = cpml(P.mx[1][e], (R)LT.x[0][e], (R)LT.x[1][e], (R)LT.x[2][e], dvy_dx);
-                if (store && onXY[e]) stm(A.mo[3], omx, m, S, 12);
+                if (store && onXY[e] && (!BFX || cx >= 0)) stm(A.mo[3], omx, m, S, 12);
             }
             if (MODE == kSlab && cy >= 0) {
                 const R m = cpml(P.my[1][e], (R)LT.y[3][e], (R)LT.y[4][e], (R)LT.y[5][e], dvx_dy);
@@ -482,9 +487,9 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             const R DT_mu = dt * (rh * vs * vs);
             R dvz_dx = (P.vz_c[e] - P.vzW[e]) * idx;
             R dvx_dz = (P.vx_u[e] - vx_c) * idz;
-            if (cx >= 0) {
+            if (BFX || cx >= 0) {
                 const R m = cpml(P.mx[2][e], (R)LT.x[0][e], (R)LT.x[1][e], (R)LT.x[2][e], dvz_dx);
-                if (store && onXZ[e]) stm(A.mo[5], omx, m, S, 14);
+                if (store && onXZ[e] && (!BFX || cx >= 0)) stm(A.mo[5], omx, m, S, 14);
             }
             if (MODE == kSlab && Z.cz >= 0) {
                 const R m = cpml(P.mz[1][e], Z.azh, Z.bzh, Z.Kzh, dvx_dz);
@@ -654,7 +659,8 @@ __device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V
     }
 }
 
-template <typename T, int MODE, bool ENERGY, int V, typename R>
+// BFX: branch-free x C-PML and value selects, as for stress_update.
+template <typename T, int MODE, bool ENERGY, int V, typename R, bool BFX = false>
 __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane<V>& L, const LaneTabs<TabP<T>, V>& LT, int k, const VPlane<R, V>& P,
                                                 const VQueue<R, V>& Q, double& ek, double& ep,
                                                 Scales<T>& S)
@@ -681,9 +687,9 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d1 = (P.sxx[e] - P.sxxW[e]) * idx;
             R d2 = (P.sxy[e] - P.sxyS[e]) * idy;
             R d3 = (P.sxz[e] - Q.sxz_m[e]) * idz;
-            if (cx >= 0) {
+            if (BFX || cx >= 0) {
                 const R m = cpml(P.mx[0][e], (R)LT.x[0][e], (R)LT.x[1][e], (R)LT.x[2][e], d1);
-                if (onx) stm(A.mo[9], omx, m, S, 18);
+                if (onx && (!BFX || cx >= 0)) stm(A.mo[9], omx, m, S, 18);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
@@ -695,7 +701,10 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
                     if (onx) stm(A.mo[11], omz, m, S, 20);
                 }
             }
-            if (onx) nvx[e] = DT_over_rho * (d1 + d2 + d3) + P.vx[e];
+            if (BFX)
+                nvx[e] = onx ? DT_over_rho * (d1 + d2 + d3) + P.vx[e] : P.vx[e];
+            else if (onx)
+                nvx[e] = DT_over_rho * (d1 + d2 + d3) + P.vx[e];
         }
         // vy: i in [0,nx-2], j in [0,ny-2], k in [1,nz-1] (CPU:531-558)
         const bool ony = L.act[e] && (i <= nx - 2) && (j <= ny - 2) && (k >= 1);
@@ -706,9 +715,9 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d1 = (P.sxyE[e] - P.sxy[e]) * idx;
             R d2 = (P.syyN[e] - P.syy[e]) * idy;
             R d3 = (P.syz[e] - Q.syz_m[e]) * idz;
-            if (cx >= 0) {
+            if (BFX || cx >= 0) {
                 const R m = cpml(P.mx[1][e], (R)LT.x[3][e], (R)LT.x[4][e], (R)LT.x[5][e], d1);
-                if (ony) stm(A.mo[12], omx, m, S, 21);
+                if (ony && (!BFX || cx >= 0)) stm(A.mo[12], omx, m, S, 21);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
@@ -720,7 +729,10 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
                     if (ony) stm(A.mo[14], omz, m, S, 23);
                 }
             }
-            if (ony) nvy[e] = DT_over_rho * (d1 + d2 + d3) + P.vy[e];
+            if (BFX)
+                nvy[e] = ony ? DT_over_rho * (d1 + d2 + d3) + P.vy[e] : P.vy[e];
+            else if (ony)
+                nvy[e] = DT_over_rho * (d1 + d2 + d3) + P.vy[e];
         }
         // vz: i in [0,nx-2], j in [1,ny-1], k in [0,nz-2] (CPU:562-590)
         const bool onz = L.act[e] && (i <= nx - 2) && (j >= 1) && (k <= nz - 2);
@@ -731,9 +743,9 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d1 = (P.sxzE[e] - P.sxz[e]) * idx;
             R d2 = (P.syz[e] - P.syzS[e]) * idy;
             R d3 = (P.szz_u[e] - Q.szz_c[e]) * idz;
-            if (cx >= 0) {
+            if (BFX || cx >= 0) {
                 const R m = cpml(P.mx[2][e], (R)LT.x[3][e], (R)LT.x[4][e], (R)LT.x[5][e], d1);
-                if (onz) stm(A.mo[15], omx, m, S, 24);
+                if (onz && (!BFX || cx >= 0)) stm(A.mo[15], omx, m, S, 24);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
@@ -745,7 +757,10 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
                     if (onz) stm(A.mo[17], omz, m, S, 26);
                 }
             }
-            if (onz) nvz[e] = DT_over_rho * (d1 + d2 + d3) + P.vz[e];
+            if (BFX)
+                nvz[e] = onz ? DT_over_rho * (d1 + d2 + d3) + P.vz[e] : P.vz[e];
+            else if (onz)
+                nvz[e] = DT_over_rho * (d1 + d2 + d3) + P.vz[e];
         }
         // rigid faces: the memory variables above are still advanced, then v = 0
         const bool f = face && L.act[e];
@@ -759,8 +774,9 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
         wz[e] = onz || f;
         if (ENERGY) {
             const int em = A.emargin;
-            if (L.act[e] && i >= em && i < nx - 1 - em && j >= em && j < ny - 1 - em && k >= em &&
-                k < nz - 1 - em) {
+            const bool in_e = L.act[e] && i >= em && i < nx - 1 - em && j >= em && j < ny - 1 - em && k >= em &&
+                              k < nz - 1 - em;
+            if (BFX || in_e) {
                 // state after this velocity update: cell-local values, FP64 terms
                 const double r = Q.rho_c[e], vp = P.cp[e], vs = P.cs[e];
                 const double mu = r * vs * vs, lam = r * (vp * vp - 2.0 * vs * vs);
@@ -769,8 +785,10 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
                 const double tr = sxx + syy + szz;
                 const double ss = sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + sxz * sxz + syz * syz);
                 const double dvx = nvx[e], dvy = nvy[e], dvz = nvz[e];
-                ek += 0.5 * r * (dvx * dvx + dvy * dvy + dvz * dvz);
-                ep += 0.5 * (ss / (2.0 * mu) - lam * tr * tr / (2.0 * mu * (3.0 * lam + 2.0 * mu)));
+                const double tk = 0.5 * r * (dvx * dvx + dvy * dvy + dvz * dvz);
+                const double tp = 0.5 * (ss / (2.0 * mu) - lam * tr * tr / (2.0 * mu * (3.0 * lam + 2.0 * mu)));
+                ek += (!BFX || in_e) ? tk : 0.0;
+                ep += (!BFX || in_e) ? tp : 0.0;
             }
         }
     }
@@ -958,15 +976,24 @@ void launch_velocity(const StepArgs<T>& a, bool energy, hipStream_t s)
 // ===========================================================================
 // Fused step over the interior box (double-buffered fields, FP32 or FP64).
 //
-// A workgroup owns an xy tile [q0, q1) of the flattened interior box and marches
-// its z-segment [kb, ke).  Per plane it computes the new stresses of the tile
-// and of one row of halo on each side (p in [q0-nx, q1+nx)) into LDS, and the
-// new velocities of the tile one plane behind, reading the in-plane stress
-// neighbours from LDS and its own column's z neighbours from registers.  Halo
-// cells inside the interior box are recomputed from the previous step's set
-// (bit-identical to what their owner computes; never written); halo cells in
-// the absorbing layers, and the planes just below and above the interior,
-// take the new stresses the absorbing-layer stress launch wrote.  The update
+// A workgroup of 256 threads owns a tile [q0, q0+n) of the flattened interior
+// box (n <= tile) and marches its z-segment [kb, ke).  The region of a plane is
+// the tile plus one row on each side: c in [-nx, n+nx), relative to q0.  Thread
+// t holds the region cells c = t + 256*(s - NH), s in [0, NH + NU + NH) (its
+// slots); slots NH .. NH+NU-1 hold its own cells.  Each slot keeps its z queue
+// in registers, so one plane costs a slot one batch of loads.  Per plane k:
+//  * the new stresses of every region cell at plane k+1.  Own cells are stored;
+//    halo cells inside the interior box are recomputed from the previous set
+//    (bit-identical to what their owner computes, never written); halo cells
+//    outside it, and the plane above the box, are read from the set the
+//    absorbing-layer stress launch wrote, which runs first.  sxx, sxy, syy, sxz
+//    and syz go to the LDS plane buffer (k+1) & 1; the own cells' szz stays in
+//    registers.
+//  * the new velocities of the own cells at plane k: in-plane stress
+//    neighbours from the LDS buffer k & 1, the own column's z neighbours from
+//    registers, old velocities and density from the previous set.
+//  * one barrier.
+// Every load of a plane is issued before any of its arithmetic.  The update
 // arithmetic is the two-pass kernels' own (stress_update, velocity_update).
 // ===========================================================================
 constexpr int kFusedThreads = 256;
@@ -975,168 +1002,264 @@ template <typename T>
 size_t fused_lds_bytes(int nx, int tile)
 {
     typedef typename Arith<T>::R R;
+    if (nx < 1 || nx > 2 * kFusedThreads || tile < 1 || tile > 4 * kFusedThreads) return 0;
     const size_t bytes = size_t(2) * 5 * size_t(tile + 2 * nx) * sizeof(R);
     return bytes <= 65536 - 1024 ? bytes : 0;  // room for the kernel's static LDS
 }
 
-// new stresses of cell p at plane k: recomputed inside the interior box, else
-// read from the set the absorbing-layer stress launch wrote
+// Loads of one interior cell's stress update at plane k: stress_load of an
+// interior lane, except that the x memory variables are read unconditionally
+// (at offset 0 of their slab for cells outside the x C-PML; the field sxx when
+// the grid has no x C-PML) so that a plane's loads form one branch-free batch.
 template <typename T, typename R>
-__device__ __forceinline__ void fused_sigma(const StepArgs<T>& A, const Box& B, int p, int k,
-                                            bool store, Scales<T>& S, Scales<T>& Sv, SOut<R, 1>& out)
+__device__ __forceinline__ void fused_sload(const StepArgs<T>& A, const Lane<1>& L, int k, SPlane<R, 1>& P)
 {
-    if (p >= B.p0 && p < B.p1 && k >= B.k0 && k < B.k1) {
-        Lane<1> L;
-        make_lane<T, kInterior, 1>(A, p, B.p1, L);
-        LaneTabs<TabP<T>, 1> LT;
-        lane_tabs<T, kInterior, 1>(A, L, LT);
-        SQueue<R, 1> Q;
-        const unsigned o = cell(A, k, p);
-        Q.vx_c[0] = ldf(A.fi[0], o, S.r[0]);
-        Q.vy_c[0] = ldf(A.fi[1], o, S.r[1]);
-        Q.cs_c[0] = (R)A.cs[o];
-        Q.rho_c[0] = (R)A.rho[o];
-        Q.vz_m[0] = k >= 1 ? (R)ldf(A.fi[2], o - unsigned(A.nxyp), S.r[2]) : (R)0;
-        SPlane<R, 1> P;
-        stress_load<T, kInterior, 1, R>(A, L, k, P, S);
-        stress_update<T, kInterior, 1, R>(A, L, LT, k, P, Q, S, store, &out);
-    } else {
-        const unsigned o = cell(A, k, p);
-        out.sxx[0] = ldf((const T*)A.fo[3], o, Sv.r[3]);
-        out.syy[0] = ldf((const T*)A.fo[4], o, Sv.r[4]);
-        out.szz[0] = ldf((const T*)A.fo[5], o, Sv.r[5]);
-        out.sxy[0] = ldf((const T*)A.fo[6], o, Sv.r[6]);
-        out.sxz[0] = ldf((const T*)A.fo[7], o, Sv.r[7]);
-        out.syz[0] = ldf((const T*)A.fo[8], o, Sv.r[8]);
+    const unsigned nxyp = A.nxyp, nx = A.nx;
+    const unsigned o = cell(A, k, L.p0);
+    const T *vx = A.fi[0], *vy = A.fi[1], *vz = A.fi[2];
+    P.vz_c[0] = el(vz, o);
+    P.cp_c[0] = (R)el(A.cp, o);
+    P.vx_u[0] = el(vx, o + nxyp);
+    P.vy_u[0] = el(vy, o + nxyp);
+    P.cs_u[0] = (R)el(A.cs, o + nxyp);
+    P.rho_u[0] = (R)el(A.rho, o + nxyp);
+    P.vxE[0] = el(vx, o + 1);
+    P.vyS[0] = el(vy, o - nx);
+    P.vyW[0] = el(vy, o - 1);
+    P.vxN[0] = el(vx, o + nx);
+    P.vzW[0] = el(vz, o - 1);
+    P.vzN[0] = el(vz, o + nx);
+    P.csW[0] = (R)el(A.cs, o - 1);
+    P.csN[0] = (R)el(A.cs, o + nx);
+    P.rhoW[0] = (R)el(A.rho, o - 1);
+    P.rhoN[0] = (R)el(A.rho, o + nx);
+    P.sxx[0] = el(A.fi[3], o);
+    P.syy[0] = el(A.fi[4], o);
+    P.szz[0] = el(A.fi[5], o);
+    P.sxy[0] = el(A.fi[6], o);
+    P.sxz[0] = el(A.fi[7], o);
+    P.syz[0] = el(A.fi[8], o);
+    const bool mx = A.w[0] > 0;
+    const unsigned om = mx ? unsigned((k * A.ny + L.j[0]) * A.w[0] + max(L.cx[0], 0)) : 0u;
+    P.mx[0][0] = el((mx ? A.mi[0] : A.fi[3]), om);
+    P.mx[1][0] = el((mx ? A.mi[3] : A.fi[3]), om);
+    P.mx[2][0] = el((mx ? A.mi[5] : A.fi[3]), om);
+}
+
+// Per-thread slot state of the fused kernel.
+template <typename T, int NS>
+struct FSlots {
+    typedef typename Arith<T>::R R;
+    int c[NS];                      // region offset of the slot's cell (relative to q0)
+    bool act[NS];                   // the slot holds a region cell
+    bool own[NS];                   // ... one of the tile's own cells
+    bool inb[NS];                   // ... inside the interior box (recomputed, not read)
+    Lane<1> L[NS];                  // geometry (inactive slots alias cell q0)
+    LaneTabs<TabP<T>, 1> LT[NS];    // x C-PML coefficients
+    SQueue<R, 1> Q[NS];             // z queue at the next plane the slot computes
+};
+
+// z queues of slots [S0, S1) at plane m (velocity and density of plane m, vz of m-1)
+template <typename T, int NS, int S0, int S1>
+__device__ __forceinline__ void fused_qinit(const StepArgs<T>& A, FSlots<T, NS>& F, int m)
+{
+#pragma unroll
+    for (int s = S0; s < S1; ++s) {
+        const unsigned o = cell(A, m, F.L[s].p0);
+        F.Q[s].vx_c[0] = el(A.fi[0], o);
+        F.Q[s].vy_c[0] = el(A.fi[1], o);
+        F.Q[s].cs_c[0] = el(A.cs, o);
+        F.Q[s].rho_c[0] = el(A.rho, o);
+        F.Q[s].vz_m[0] = el(A.fi[2], o - unsigned(A.nxyp));
     }
 }
 
-template <typename T, bool ENERGY, int NU>
+// new stresses of slots [S0, S1) at plane m into O; own cells stored when
+// `store`; with LDSW, the region cells' sxx, sxy, syy, sxz, syz into the LDS
+// plane buffer `lb` (component stride LR, cell c at c + nx).  Advances the queues.
+template <typename T, int NS, int S0, int S1, bool EDGE, bool LDSW>
+__device__ __forceinline__ void fused_plane(const StepArgs<T>& A, FSlots<T, NS>& F, int m, bool store,
+                                            typename Arith<T>::R* lb, int LR,
+                                            SOut<typename Arith<T>::R, 1> (&O)[NS], Scales<T>& S)
+{
+    typedef typename Arith<T>::R R;
+    SPlane<R, 1> P[NS];
+#pragma unroll
+    for (int s = S0; s < S1; ++s) fused_sload<T, R>(A, F.L[s], m, P[s]);
+#pragma unroll
+    for (int s = S0; s < S1; ++s) {
+        if (!EDGE || F.inb[s]) {
+            stress_update<T, kInterior, 1, R, true>(A, F.L[s], F.LT[s], m, P[s], F.Q[s], S, store && F.own[s], &O[s]);
+        } else {  // outside the interior box: this step's values, from the absorbing-layer launch
+            const unsigned o = cell(A, m, F.L[s].p0);
+            O[s].sxx[0] = el(A.fo[3], o);
+            O[s].syy[0] = el(A.fo[4], o);
+            O[s].szz[0] = el(A.fo[5], o);
+            O[s].sxy[0] = el(A.fo[6], o);
+            O[s].sxz[0] = el(A.fo[7], o);
+            O[s].syz[0] = el(A.fo[8], o);
+        }
+        F.Q[s].vx_c[0] = P[s].vx_u[0];
+        F.Q[s].vy_c[0] = P[s].vy_u[0];
+        F.Q[s].cs_c[0] = P[s].cs_u[0];
+        F.Q[s].rho_c[0] = P[s].rho_u[0];
+        F.Q[s].vz_m[0] = P[s].vz_c[0];
+        if (LDSW && F.act[s]) {
+            R* d = lb + (F.c[s] + A.nx);
+            d[0] = O[s].sxx[0];
+            d[LR] = O[s].sxy[0];
+            d[2 * LR] = O[s].syy[0];
+            d[3 * LR] = O[s].sxz[0];
+            d[4 * LR] = O[s].syz[0];
+        }
+    }
+}
+
+// this step's stresses of slots [S0, S1) at plane m, as written by the
+// absorbing-layer stress launch (planes outside the interior box)
+template <typename T, int NS, int S0, int S1>
+__device__ __forceinline__ void fused_fresh(const StepArgs<T>& A, const FSlots<T, NS>& F, int m,
+                                            SOut<typename Arith<T>::R, 1> (&O)[NS])
+{
+#pragma unroll
+    for (int s = S0; s < S1; ++s) {
+        const unsigned o = cell(A, m, F.L[s].p0);
+        O[s].sxx[0] = el(A.fo[3], o);
+        O[s].syy[0] = el(A.fo[4], o);
+        O[s].szz[0] = el(A.fo[5], o);
+        O[s].sxy[0] = el(A.fo[6], o);
+        O[s].sxz[0] = el(A.fo[7], o);
+        O[s].syz[0] = el(A.fo[8], o);
+    }
+}
+
+template <typename T, bool ENERGY, int NU, int NH, bool EDGE>
+__device__ __forceinline__ void fused_body(const StepArgs<T>& A, const Box& B, int q0, int n, int kb, int ke,
+                                           typename Arith<T>::R* lds, double& ek, double& ep, Scales<T>& S)
+{
+    typedef typename Arith<T>::R R;
+    constexpr int NS = 2 * NH + NU;
+    const int tid = threadIdx.x, nx = A.nx, LR = A.ldsreg;
+    FSlots<T, NS> F;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int c = tid + kFusedThreads * (s - NH);
+        F.c[s] = c;
+        F.act[s] = c >= -nx && c < n + nx;
+        F.own[s] = c >= 0 && c < n;
+        const int p = q0 + (F.act[s] ? c : 0);
+        F.inb[s] = p >= B.p0 && p < B.p1;
+        make_lane<T, kInterior, 1>(A, p, B.p1, F.L[s]);
+        lane_tabs<T, kInterior, 1>(A, F.L[s], F.LT[s]);
+        if (F.L[s].cx[0] < 0) {  // neutral x coefficients for the branch-free recurrences
+            F.LT[s].x[2][0] = 1;
+            F.LT[s].x[5][0] = 1;
+        }
+    }
+    SOut<R, 1> O[NS];
+    VQueue<R, 1> VQ[NU];
+    // own sxz, syz of the plane below the segment
+    if (kb - 1 >= B.k0) {
+        fused_qinit<T, NS, NH, NH + NU>(A, F, kb - 1);
+        fused_plane<T, NS, NH, NH + NU, EDGE, false>(A, F, kb - 1, false, nullptr, LR, O, S);
+        fused_qinit<T, NS, 0, NH>(A, F, kb);
+        fused_qinit<T, NS, NH + NU, NS>(A, F, kb);
+    } else {
+        fused_fresh<T, NS, NH, NH + NU>(A, F, kb - 1, O);
+        fused_qinit<T, NS, 0, NS>(A, F, kb);
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        VQ[u].sxz_m[0] = O[NH + u].sxz[0];
+        VQ[u].syz_m[0] = O[NH + u].syz[0];
+        VQ[u].rho_c[0] = F.Q[NH + u].rho_c[0];
+    }
+    // plane kb: the region into LDS, own cells stored
+    fused_plane<T, NS, 0, NS, EDGE, true>(A, F, kb, true, lds + (kb & 1) * 5 * LR, LR, O, S);
+#pragma unroll
+    for (int u = 0; u < NU; ++u) VQ[u].szz_c[0] = O[NH + u].szz[0];
+    __syncthreads();
+    for (int k = kb; k < ke; ++k) {
+        const int m = k + 1;
+        // own cells' velocity-pass loads of plane k (old v, density, x memory variables)
+        VPlane<R, 1> V[NU];
+        const bool mx = A.w[0] > 0;
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            const Lane<1>& L = F.L[NH + u];
+            const unsigned o = cell(A, k, L.p0);
+            V[u].vx[0] = el(A.fi[0], o);
+            V[u].vy[0] = el(A.fi[1], o);
+            V[u].vz[0] = F.Q[NH + u].vz_m[0];
+            V[u].rhoW[0] = el(A.rho, o - 1);
+            V[u].rhoN[0] = el(A.rho, o + unsigned(nx));
+            V[u].rho_u[0] = F.Q[NH + u].rho_c[0];
+            if (ENERGY) {
+                V[u].cp[0] = el(A.cp, o);
+                V[u].cs[0] = el(A.cs, o);
+            }
+            const unsigned om = mx ? unsigned((k * A.ny + L.j[0]) * A.w[0] + max(L.cx[0], 0)) : 0u;
+            V[u].mx[0][0] = el((mx ? A.mi[9] : A.fi[0]), om);
+            V[u].mx[1][0] = el((mx ? A.mi[12] : A.fi[0]), om);
+            V[u].mx[2][0] = el((mx ? A.mi[15] : A.fi[0]), om);
+        }
+        // stresses of plane k+1
+        if (m < ke)  // the region, own cells stored
+            fused_plane<T, NS, 0, NS, EDGE, true>(A, F, m, true, lds + (m & 1) * 5 * LR, LR, O, S);
+        else if (m < B.k1)  // above the segment: the own cells' szz only, not stored
+            fused_plane<T, NS, NH, NH + NU, EDGE, false>(A, F, m, false, nullptr, LR, O, S);
+        else  // above the interior box
+            fused_fresh<T, NS, NH, NH + NU>(A, F, m, O);
+        // velocities of plane k
+        const R* lb = lds + (k & 1) * 5 * LR + nx;
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            const int s = NH + u;
+            if (!F.own[s]) continue;
+            const int c = F.c[s];
+            VPlane<R, 1>& P = V[u];
+            P.sxx[0] = lb[c];
+            P.sxxW[0] = lb[c - 1];
+            P.sxy[0] = lb[LR + c];
+            P.sxyS[0] = lb[LR + c - nx];
+            P.sxyE[0] = lb[LR + c + 1];
+            P.syy[0] = lb[2 * LR + c];
+            P.syyN[0] = lb[2 * LR + c + nx];
+            P.sxz[0] = lb[3 * LR + c];
+            P.sxzE[0] = lb[3 * LR + c + 1];
+            P.syz[0] = lb[4 * LR + c];
+            P.syzS[0] = lb[4 * LR + c - nx];
+            P.szz_u[0] = O[s].szz[0];
+            velocity_update<T, kInterior, ENERGY, 1, R, true>(A, F.L[s], F.LT[s], k, P, VQ[u], ek, ep, S);
+            VQ[u].sxz_m[0] = P.sxz[0];
+            VQ[u].syz_m[0] = P.syz[0];
+            VQ[u].szz_c[0] = O[s].szz[0];
+            VQ[u].rho_c[0] = P.rho_u[0];
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, bool ENERGY, int NU, int NH>
 __global__ void __launch_bounds__(kFusedThreads) k_fused(const StepArgs<T> A)
 {
     typedef typename Arith<T>::R R;
     extern __shared__ float smem[];
-    const int tid = threadIdx.x;
-    const int nx = A.nx;
+    const int tid = threadIdx.x, nx = A.nx;
     const Box B = A.box[0];
     int chunk, zs;
     tile_of(B, xcd_block(blockIdx.x, gridDim.x), chunk, zs);
-    const int q0 = B.p0 + chunk * A.tile, q1 = min(q0 + A.tile, B.p1);
+    const int q0 = B.p0 + chunk * A.tile, n = min(A.tile, B.p1 - q0);
     const int kb = B.k0 + zs * B.zseg, ke = min(kb + B.zseg, B.k1);
-    const int rlo = -nx, rhi = (q1 - q0) + nx;  // region, relative to q0: [rlo, rhi)
-    const int LR = A.ldsreg;                     // stride of one component plane
-    Scales<T> S, Sv;
+    Scales<T> S;
     load_scales(A, S, true);
-    load_scales(A, Sv, false);
     R* lds = reinterpret_cast<R*>(smem);
-    // lds[(b*5 + comp)*LR + (c - rlo)]: comp 0 sxx, 1 sxy, 2 syy, 3 sxz, 4 syz of plane b = k & 1
-    auto at = [&](int b, int comp, int c) -> R& { return lds[(b * 5 + comp) * LR + (c - rlo)]; };
-    // own cells q0 + tid + 256*u: column queues across planes
-    R sxz_m[NU], syz_m[NU], szz_c[NU], szz_u[NU];
-    // the plane below the segment: own sxz, syz
-#pragma unroll
-    for (int u = 0; u < NU; ++u) {
-        const int c = tid + kFusedThreads * u;
-        sxz_m[u] = 0;
-        syz_m[u] = 0;
-        szz_c[u] = 0;
-        szz_u[u] = 0;
-        if (c < q1 - q0) {
-            SOut<R, 1> o;
-            fused_sigma<T, R>(A, B, q0 + c, kb - 1, false, S, Sv, o);
-            sxz_m[u] = o.sxz[0];
-            syz_m[u] = o.syz[0];
-        }
-    }
-    // plane kb: the whole region into LDS, own cells stored
-    {
-        const int b = kb & 1;
-        for (int c = rlo + ((tid - rlo) % kFusedThreads + kFusedThreads) % kFusedThreads; c < rhi;
-             c += kFusedThreads) {
-            const bool own = c >= 0 && c < q1 - q0;
-            SOut<R, 1> o;
-            fused_sigma<T, R>(A, B, q0 + c, kb, own, S, Sv, o);
-            at(b, 0, c) = o.sxx[0];
-            at(b, 1, c) = o.sxy[0];
-            at(b, 2, c) = o.syy[0];
-            at(b, 3, c) = o.sxz[0];
-            at(b, 4, c) = o.syz[0];
-            if (own) szz_c[(c - tid) / kFusedThreads] = o.szz[0];
-        }
-    }
-    __syncthreads();
     double ek = 0.0, ep = 0.0;
-    for (int k = kb; k < ke; ++k) {
-        const int b = k & 1, nb = b ^ 1;
-        const bool last = (k + 1 == ke);
-        // stresses of plane k+1: the region (own cells stored) or, past the
-        // segment, the own cells' sigma_zz only
-        for (int c = (last ? tid : rlo + ((tid - rlo) % kFusedThreads + kFusedThreads) % kFusedThreads);
-             c < (last ? q1 - q0 : rhi); c += kFusedThreads) {
-            const bool own = c >= 0 && c < q1 - q0;
-            SOut<R, 1> o;
-            fused_sigma<T, R>(A, B, q0 + c, k + 1, own && !last, S, Sv, o);
-            if (!last) {
-                at(nb, 0, c) = o.sxx[0];
-                at(nb, 1, c) = o.sxy[0];
-                at(nb, 2, c) = o.syy[0];
-                at(nb, 3, c) = o.sxz[0];
-                at(nb, 4, c) = o.syz[0];
-            }
-            if (own) szz_u[(c - tid) / kFusedThreads] = o.szz[0];
-        }
-        // velocities of plane k
-#pragma unroll
-        for (int u = 0; u < NU; ++u) {
-            const int c = tid + kFusedThreads * u;
-            if (c >= q1 - q0) continue;
-            const int p = q0 + c;
-            Lane<1> L;
-            make_lane<T, kInterior, 1>(A, p, B.p1, L);
-            LaneTabs<TabP<T>, 1> LT;
-            lane_tabs<T, kInterior, 1>(A, L, LT);
-            const unsigned o = cell(A, k, p);
-            VPlane<R, 1> P;
-            VQueue<R, 1> Q;
-            P.sxx[0] = at(b, 0, c);
-            P.sxxW[0] = at(b, 0, c - 1);
-            P.sxy[0] = at(b, 1, c);
-            P.sxyS[0] = at(b, 1, c - nx);
-            P.sxyE[0] = at(b, 1, c + 1);
-            P.syy[0] = at(b, 2, c);
-            P.syyN[0] = at(b, 2, c + nx);
-            P.sxz[0] = at(b, 3, c);
-            P.sxzE[0] = at(b, 3, c + 1);
-            P.syz[0] = at(b, 4, c);
-            P.syzS[0] = at(b, 4, c - nx);
-            P.szz_u[0] = szz_u[u];
-            P.rho_u[0] = (R)A.rho[o + unsigned(A.nxyp)];
-            P.rhoW[0] = (R)A.rho[o - 1];
-            P.rhoN[0] = (R)A.rho[o + unsigned(nx)];
-            P.vx[0] = ldf(A.fi[0], o, Sv.r[0]);
-            P.vy[0] = ldf(A.fi[1], o, Sv.r[1]);
-            P.vz[0] = ldf(A.fi[2], o, Sv.r[2]);
-            if (ENERGY) {
-                P.cp[0] = (R)A.cp[o];
-                P.cs[0] = (R)A.cs[o];
-            }
-            if (L.cx[0] >= 0) {
-                const unsigned om = unsigned((k * A.ny + L.j[0]) * A.w[0] + L.cx[0]);
-                P.mx[0][0] = ldf(A.mi[9], om, Sv.r[18]);
-                P.mx[1][0] = ldf(A.mi[12], om, Sv.r[21]);
-                P.mx[2][0] = ldf(A.mi[15], om, Sv.r[24]);
-            }
-            Q.sxz_m[0] = sxz_m[u];
-            Q.syz_m[0] = syz_m[u];
-            Q.szz_c[0] = szz_c[u];
-            Q.rho_c[0] = (R)A.rho[o];
-            velocity_update<T, kInterior, ENERGY, 1, R>(A, L, LT, k, P, Q, ek, ep, Sv);
-            sxz_m[u] = P.sxz[0];
-            syz_m[u] = P.syz[0];
-            szz_c[u] = szz_u[u];
-        }
-        __syncthreads();
-    }
+    // tiles whose halo rows leave the interior box take the slow per-slot path
+    if (q0 - nx < B.p0 || q0 + n + nx > B.p1)
+        fused_body<T, ENERGY, NU, NH, true>(A, B, q0, n, kb, ke, lds, ek, ep, S);
+    else
+        fused_body<T, ENERGY, NU, NH, false>(A, B, q0, n, kb, ke, lds, ek, ep, S);
     if (ENERGY) {  // per-block FP64 partials, summed in block order by the tail kernel
         __shared__ double red[2][kFusedThreads / 64];
         ek = wave_sum(ek);
@@ -1159,15 +1282,26 @@ __global__ void __launch_bounds__(kFusedThreads) k_fused(const StepArgs<T> A)
     }
 }
 
+template <typename T, bool E, int NH>
+static void launch_fused_h(const StepArgs<T>& a, size_t lds, hipStream_t s)
+{
+    if (a.tile <= kFusedThreads)
+        hipLaunchKernelGGL((k_fused<T, E, 1, NH>), dim3(a.nblocks), dim3(kFusedThreads), lds, s, a);
+    else if (a.tile <= 2 * kFusedThreads)
+        hipLaunchKernelGGL((k_fused<T, E, 2, NH>), dim3(a.nblocks), dim3(kFusedThreads), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_fused<T, E, 4, NH>), dim3(a.nblocks), dim3(kFusedThreads), lds, s, a);
+}
+
 template <typename T, bool E>
 static void launch_fused_e(const StepArgs<T>& a, hipStream_t s)
 {
     const size_t lds = fused_lds_bytes<T>(a.nx, a.tile);
-    if (lds == 0 || a.tile < 1 || a.tile > 4 * kFusedThreads) return;  // refused on the host first
-    if (a.tile <= 2 * kFusedThreads)
-        hipLaunchKernelGGL((k_fused<T, E, 2>), dim3(a.nblocks), dim3(kFusedThreads), lds, s, a);
+    if (lds == 0 || a.ldsreg < a.tile + 2 * a.nx) return;  // refused on the host first
+    if (a.nx <= kFusedThreads)
+        launch_fused_h<T, E, 1>(a, lds, s);
     else
-        hipLaunchKernelGGL((k_fused<T, E, 4>), dim3(a.nblocks), dim3(kFusedThreads), lds, s, a);
+        launch_fused_h<T, E, 2>(a, lds, s);
 }
 
 template <typename T>

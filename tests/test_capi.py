@@ -46,6 +46,18 @@ def test_config_default_and_validate():
         cfg.replace(dt=-1.0).validate()
 
 
+def test_padded_arrays_fit_32bit_byte_offsets():
+    """Kernels address every field and model array by a 32-bit byte offset: a grid
+    whose padded array would reach 4 GiB is refused, per storage element size."""
+    from ela3d_amd import STORAGE_FP64
+    big = Config().replace(nx=1024, ny=1024, nz=520, isrc=512, jsrc=512, ksrc=260)
+    big.validate()  # FP32: 2.2 GB per padded array
+    with pytest.raises(Ela3dError) as e:
+        big.replace(storage=STORAGE_FP64).validate()  # FP64: 4.4 GB
+    assert "4 GiB" in str(e.value)
+    big.replace(nz=511, ksrc=255, storage=STORAGE_FP64).validate()  # 4.29e9 - 4 MB: still fits
+
+
 def test_struct_size_guard():
     s, keep = Config().to_struct()
     s.struct_size = 12

@@ -127,10 +127,12 @@ def test_model_catches_wrapped_offsets():
     assert ("neighbour", 0, 0, -1) in bad and ("neighbour", 0, 0, -29) in bad
 
 
-def fused_violations(cfg, tile, zseg, lds_bytes_limit=65536 - 1024, elem=4):
-    """Every global and LDS index the fused interior kernel forms (k_fused and
-    fused_sigma in kernels.hip), for all tiles and z segments, checked against
-    the allocation, memory-variable and LDS sizes."""
+def fused_violations(cfg, tile, zseg, lds_bytes_limit=65536 - 1024, elem=4, threads=256, ldsreg=None):
+    """Every global and LDS index the fused interior kernel forms (k_fused,
+    fused_body, fused_sload, fused_qinit, fused_plane and fused_fresh in
+    kernels.hip), for all tiles and z segments, checked against the allocation,
+    memory-variable and LDS sizes.  Also checks that the slots of a workgroup
+    cover the region it writes to LDS and every LDS cell it reads."""
     p = cfg.plan()
     nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
     nxy, nxyp, G = nx * ny, plane_stride(nx, ny), guard(nx)
@@ -141,12 +143,16 @@ def fused_violations(cfg, tile, zseg, lds_bytes_limit=65536 - 1024, elem=4):
     pa, pb = (lo[1] * nx + 63) // 64 * 64, (hi[1] * nx) // 64 * 64
     assert pb > pa
     zlo, zhi = lo[2], hi[2]
-    LR = tile + 2 * nx
-    if 2 * 5 * LR * elem > lds_bytes_limit:
-        return None  # the library refuses this tile (fused_unsupported)
+    LR = tile + 2 * nx if ldsreg is None else ldsreg
+    if nx > 2 * threads or tile > 4 * threads or 2 * 5 * LR * elem > lds_bytes_limit:
+        return None  # the library refuses this tile (fused_lds_bytes, fused_unsupported)
+    NU = 1 if tile <= threads else (2 if tile <= 2 * threads else 4)
+    NH = 1 if nx <= threads else 2
+    NS = 2 * NH + NU
     bad = []
     nchunk = (pb - pa + tile - 1) // tile
     nzs = (zhi - zlo + zseg - 1) // zseg
+    t = np.arange(threads)
 
     def cell(k, q):
         return G + k * nxyp + q
@@ -156,54 +162,86 @@ def fused_violations(cfg, tile, zseg, lds_bytes_limit=65536 - 1024, elem=4):
         if np.any((o < 0) | (o >= alloc)):
             bad.append(tag)
 
-    def mem(k, q):
+    def mem(k, q):  # x memory variables, read at offset 0 outside the x C-PML
         j, i = np.divmod(q, nx)
         cx = np.where(i < lo[0], i, np.where(i >= hi[0], lo[0] + i - hi[0], -1))
-        om = (k * ny + j) * w0 + cx
-        if np.any((cx >= 0) & ((om < 0) | (om >= memsize0))):
-            bad.append(("mem", k))
+        if np.any(j >= ny):
+            bad.append(("row", k))
+        if w0 > 0:
+            om = (k * ny + j) * w0 + np.maximum(cx, 0)
+            if np.any((om < 0) | (om >= memsize0)):
+                bad.append(("mem", k))
 
-    def sigma(q, k):  # fused_sigma at cells q (array) of plane k
+    def sload(q, k):  # fused_sload
         if not (0 <= k < nz):
             bad.append(("plane", k))
-            return
-        if np.any((q < 0) | (q >= nxy)):
-            bad.append(("region", k))
-        inside = (q >= pa) & (q < pb) & (zlo <= k < zhi)
-        qi = q[inside]
-        if qi.size:
-            o = cell(k, qi)
-            dU = 0 if k == nz - 1 else nxyp
-            for d in (0, dU, 1, -1, nx, -nx):
-                fld(o + d, ("stress", k, d))
-            if k >= 1:
-                fld(o - nxyp, ("queue", k))
-            mem(k, qi)
-        qo = q[~inside]
-        if qo.size:
-            fld(cell(k, qo), ("fresh", k))
+        o = cell(k, q)
+        for d in (0, nxyp, 1, -1, nx, -nx):
+            fld(o + d, ("stress", k, d))
+        mem(k, q)
+
+    def qinit(q, k):
+        if not (1 <= k < nz):
+            bad.append(("qplane", k))
+        fld(cell(k, q), ("queue", k))
+        fld(cell(k, q) - nxyp, ("queue-1", k))
 
     for chunk in range(nchunk):
         q0 = pa + chunk * tile
-        q1 = min(q0 + tile, pb)
-        region = np.arange(q0 - nx, q1 + nx)
-        own = np.arange(q0, q1)
-        c_own = own - q0
+        n = min(tile, pb - q0)
+        c = np.stack([t + threads * (s - NH) for s in range(NS)])  # [slot][thread]
+        act = (c >= -nx) & (c < n + nx)
+        own = (c >= 0) & (c < n)
+        pc = q0 + np.where(act, c, 0)
+        if np.any((pc < 0) | (pc >= nxy)):
+            bad.append(("slot", chunk))
+        # the active slots cover the region exactly once
+        cov = np.sort(c[act])
+        if not np.array_equal(cov, np.arange(-nx, n + nx)):
+            bad.append(("coverage", chunk))
+        if not np.array_equal(np.sort(c[own]), np.arange(n)):
+            bad.append(("own", chunk))
+        if np.any(own[:NH]) or np.any(own[NH + NU:]):
+            bad.append(("own-slot", chunk))
+        if np.any((c[act] + nx < 0) | (c[act] + nx >= LR)):
+            bad.append(("lds-write", chunk))
+        inb = (pc >= pa) & (pc < pb)
+        ownp = pc[NH:NH + NU].ravel()
         for zs in range(nzs):
             kb = zlo + zs * zseg
             ke = min(kb + zseg, zhi)
-            sigma(own, kb - 1)
-            sigma(region, kb)
+            if kb - 1 >= zlo:
+                qinit(ownp, kb - 1)
+                sload(ownp, kb - 1)
+                qinit(pc[:NH].ravel(), kb)
+                qinit(pc[NH + NU:].ravel(), kb)
+            else:
+                fld(cell(kb - 1, ownp), ("fresh", kb - 1))
+                qinit(pc.ravel(), kb)
+            sload(pc.ravel(), kb)
+            fld(cell(kb, pc[~inb]), ("fresh", kb))
             for k in range(kb, ke):
-                sigma(own if k + 1 == ke else region, k + 1)
-                o = cell(k, own)
-                for d in (0, nxyp, -1, nx):
+                o = cell(k, ownp)
+                for d in (0, -1, nx):  # old v, rhoW, rhoN, cp, cs
                     fld(o + d, ("velocity", k, d))
-                mem(k, own)
-                for d in (0, -1, 1, -nx, nx):  # LDS neighbours, relative to the region start
-                    idx = c_own + d + nx
+                mem(k, ownp)
+                m = k + 1
+                if m < ke:
+                    sload(pc.ravel(), m)
+                    fld(cell(m, pc[~inb]), ("fresh", m))
+                elif m < zhi:
+                    sload(ownp, m)
+                    fld(cell(m, pc[NH:NH + NU][~inb[NH:NH + NU]]), ("fresh", m))
+                else:
+                    fld(cell(m, ownp), ("fresh", m))
+                co = c[own]
+                for d in (0, -1, 1, -nx, nx):  # LDS neighbours of own cells
+                    idx = co + d + nx
                     if np.any((idx < 0) | (idx >= LR)):
                         bad.append(("lds", k, d))
+                    # every LDS cell read was written this plane by an active slot
+                    if not np.all(np.isin(co + d, c[act])):
+                        bad.append(("lds-unwritten", k, d))
     return bad
 
 
@@ -223,3 +261,16 @@ def test_fused_cube_and_classic_in_bounds():
         assert fused_violations(cfg, tile, zseg) == []
     cfg, _ = workloads.classic(nstep=1)
     assert fused_violations(cfg, 1024, 16) == []
+
+
+def test_fused_layered_in_bounds():
+    """nx = 401 needs two halo slots per side; 1024-cell tiles exceed the LDS."""
+    cfg, _ = workloads.layered(nstep=1)
+    assert fused_violations(cfg, 512, 16) == []
+    assert fused_violations(cfg, 1024, 16) is None
+
+
+def test_fused_model_catches_short_lds_plane():
+    cfg = small_cfg(nx=29, ny=23, nz=26, npml=4)
+    bad = fused_violations(cfg, 256, 4, ldsreg=256 + 29)
+    assert any(b[0] == "lds-write" for b in bad) and any(b[0] == "lds" for b in bad)
