@@ -111,6 +111,12 @@ size_t fused_lds_bytes(int nx, int tile);
 template <typename T>
 void launch_tail(const TailArgs<T>& a, hipStream_t s);
 void launch_copy(const float4* src, float4* dst, long n4, int variant, hipStream_t s);
+// First level of the energy reduction on energy steps: the n per-block partial
+// pairs in `in` are summed in a fixed order by `nout` workgroups into
+// out[nout][2], which the tail kernel then sums (kEpartBlocks at most).
+constexpr int kEpartBlocks = 256;
+constexpr int kEpartDirect = 2048;  // up to this many partials the tail sums them itself
+void launch_epart(const double* in, int n, double* out, int nout, hipStream_t s);
 // multi-stream calibration (variants 4-7): arrays of n floats packed back to back at base
 int stream_counts(int variant, int& nreads, int& nwrites);  // returns floats per lane
 void launch_streams(float* base, long n, int variant, hipStream_t s);

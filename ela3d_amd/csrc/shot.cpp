@@ -163,6 +163,7 @@ ArenaLayout arena_layout(const ShotPlan& p, int elem, int nepart, bool dbuf)
     L.energy = off; off += al(std::max<size_t>(1, size_t(p.nenergy) * 2) * sizeof(double));
     L.epart_cap = int((nxyp / 64 + kMaxBox + 1) * int64_t(c.nz) + 8);
     L.epart = off; off += al(size_t(std::max(L.epart_cap, nepart)) * 2 * sizeof(double));
+    L.epart2 = off; off += al(size_t(kEpartBlocks) * 2 * sizeof(double));
     L.ctl = off; off += al(2 * sizeof(int));
     L.scl = off; off += al(kScaleTable * sizeof(float));
     L.slot = off; off += al(size_t(kMaxSlots) * 27 * sizeof(unsigned));
@@ -601,6 +602,7 @@ private:
         d_seis_ = reinterpret_cast<double*>(b + o_seis);
         d_energy_ = reinterpret_cast<double*>(b + o_en);
         d_epart_ = reinterpret_cast<double*>(b + o_ep);
+        d_epart2_ = reinterpret_cast<double*>(b + L.epart2);
         d_step_ = reinterpret_cast<int*>(b + o_ctl);
         d_flag_ = d_step_ + 1;
         d_scl_ = reinterpret_cast<float*>(b + o_scl);
@@ -755,6 +757,14 @@ private:
             }
     }
 
+    // workgroups of the first energy-reduction level (0: the tail sums the
+    // per-block partials itself)
+    int epart_blocks() const
+    {
+        if (nepart_ <= kEpartDirect) return 0;
+        return std::min(kEpartBlocks, (nepart_ + 1023) / 1024);
+    }
+
     // tail of a step that starts from set `par` (works on the set that step writes)
     TailArgs<T> tail_args(int par) const
     {
@@ -769,8 +779,8 @@ private:
         a.seis = d_seis_;
         a.nsamples = plan_.nsamples;
         a.sample_every = c.sample_every;
-        a.epart = d_epart_;
-        a.nepart = nepart_;
+        a.epart = epart_blocks() > 0 ? d_epart2_ : d_epart_;
+        a.nepart = epart_blocks() > 0 ? epart_blocks() : nepart_;
         a.energy = d_energy_;
         a.nenergy = plan_.nenergy;
         a.energy_every = c.energy_every;
@@ -824,6 +834,7 @@ private:
             mark(6, stream_);
             mark(7, stream_);
             mark(8, stream_);
+            if (energy) launch_epart(d_epart_, nepart_, d_epart2_, epart_blocks(), stream_);
             launch_tail(tail_args(par), stream_);
             mark(9, stream_);
             return;
@@ -852,6 +863,7 @@ private:
             }
         }
         mark(8, stream_);
+        if (energy) launch_epart(d_epart_, nepart_, d_epart2_, epart_blocks(), stream_);
         launch_tail(tail_args(par), stream_);
         mark(9, stream_);
     }
@@ -890,6 +902,7 @@ private:
     float* d_model_[3] = {};
     double* d_tab_[3] = {};
     double *d_wav_ = nullptr, *d_seis_ = nullptr, *d_energy_ = nullptr, *d_epart_ = nullptr;
+    double* d_epart2_ = nullptr;  // first-level sums of the energy partials
     int *d_rec_ = nullptr, *d_step_ = nullptr, *d_flag_ = nullptr;
     int nepart_ = 0, epart_cap_ = 0;
     int64_t nxyp_ = 0, guard_ = 0;

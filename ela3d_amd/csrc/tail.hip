@@ -265,6 +265,47 @@ void launch_streams(float* base, long n, int variant, hipStream_t s)
     }
 }
 
+// First level of the energy reduction: workgroup b sums the partial pairs
+// [b*per, min((b+1)*per, n)) into out[b].  The assignment of partials to lanes
+// and the reduction tree are fixed, so the sum is the same on every run.
+constexpr int kEpartThreads = 256;
+
+__global__ void __launch_bounds__(kEpartThreads) k_epart(const double* __restrict__ in, int n, int per,
+                                                        double* __restrict__ out)
+{
+    __shared__ double red[2][kEpartThreads / 64];
+    const int tid = threadIdx.x;
+    const int q0 = blockIdx.x * per, q1 = min(q0 + per, n);
+    double a = 0.0, b = 0.0;
+    for (int q = q0 + tid; q < q1; q += kEpartThreads) {
+        a += in[2 * q];
+        b += in[2 * q + 1];
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if ((tid & 63) == 0) {
+        red[0][tid >> 6] = a;
+        red[1][tid >> 6] = b;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double x = 0.0, y = 0.0;
+        for (int w = 0; w < kEpartThreads / 64; ++w) {
+            x += red[0][w];
+            y += red[1][w];
+        }
+        out[2 * blockIdx.x] = x;
+        out[2 * blockIdx.x + 1] = y;
+    }
+}
+
+void launch_epart(const double* in, int n, double* out, int nout, hipStream_t s)
+{
+    if (n <= 0 || nout <= 0 || nout > kEpartBlocks) return;
+    const int per = (n + nout - 1) / nout;
+    hipLaunchKernelGGL(k_epart, dim3(nout), dim3(kEpartThreads), 0, s, in, n, per, out);
+}
+
 // ---------------------------------------------------------------------------
 template <typename T>
 void launch_tail(const TailArgs<T>& a, hipStream_t s)
