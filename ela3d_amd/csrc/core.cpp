@@ -195,7 +195,6 @@ ShotPlan make_plan(const ela3d_config& in)
         if (padded * elem >= (int64_t(1) << 32))
             fail("grid too large: a padded field or model array would exceed the 4 GiB its 32-bit byte offsets address");
     }
-    if (c.nx + c.ny > 2700) fail("nx + ny must be <= 2700 (x and y C-PML tables are staged in LDS)");
     if (!(c.dx > 0 && c.dy > 0 && c.dz > 0)) fail("dx, dy, dz must be > 0");
     if (!(c.dt > 0)) fail("dt must be > 0");
     if (c.nstep < 0) fail("nstep must be >= 0");

@@ -211,15 +211,13 @@ __device__ __forceinline__ R cpml(R psi_old, R a, R b, R K, R& d)
     return psi;
 }
 
-// Element type the x/y C-PML coefficients are read in: FP32 copies staged in
-// LDS for FP32 arithmetic, the FP64 tables themselves for the FP64 instantiation.
+// Element type the x/y C-PML coefficients are read in: the FP32 copies of the
+// x and y tables ([6][nx] then [6][ny]: a, b, K on the full grid, a, b, K on the
+// half grid) for FP32 arithmetic, the FP64 tables themselves for the FP64
+// instantiation.
 template <typename T>
 using TabP = typename std::conditional<std::is_same<T, double>::value, double, float>::type;
 
-// FP32 copies of the x and y tables, staged per workgroup: [6][nx] then [6][ny]
-// (a, b, K on the full grid, a, b, K on the half grid).
-// Interior workgroups read only the x coefficients, so only absorbing-layer
-// workgroups stage the y tables.
 // Per-lane geometry of the VEC elements p0..p0+VEC-1.
 template <int V>
 struct Lane {
