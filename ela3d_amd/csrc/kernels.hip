@@ -202,19 +202,21 @@ __device__ __forceinline__ void stv(T* p, unsigned o, const typename Arith<T>::R
         if (en[e]) stf(p, o + e, val[e], sc);
 }
 
-// memory-variable recurrence psi <- b psi + a D;  D <- D/K + psi  (CPU:395-405)
+// memory-variable recurrence psi <- b psi + a D;  D <- D/K + psi  (CPU:395-405),
+// with 1/K precomputed per table entry: identical to the division for K = 1
+// (the reference's K_MAX_PML = 1, CPU:75), within one rounding otherwise.
 template <typename R>
-__device__ __forceinline__ R cpml(R psi_old, R a, R b, R K, R& d)
+__device__ __forceinline__ R cpml(R psi_old, R a, R b, R Kinv, R& d)
 {
     const R psi = b * psi_old + a * d;
-    d = d / K + psi;
+    d = d * Kinv + psi;
     return psi;
 }
 
 // Element type the x/y C-PML coefficients are read in: the FP32 copies of the
-// x and y tables ([6][nx] then [6][ny]: a, b, K on the full grid, a, b, K on the
-// half grid) for FP32 arithmetic, the FP64 tables themselves for the FP64
-// instantiation.
+// x and y tables ([6][nx] then [6][ny]: a, b, 1/K on the full grid, a, b, 1/K on
+// the half grid) for FP32 arithmetic, the FP64 tables themselves (same layout)
+// for the FP64 instantiation.
 template <typename T>
 using TabP = typename std::conditional<std::is_same<T, double>::value, double, float>::type;
 
@@ -250,10 +252,12 @@ __device__ __forceinline__ void make_lane(const StepArgs<T>& A, int p0, int p1, 
     }
 }
 
-// C-PML coefficients of a lane's elements: a, b, K on the full grid then on the
-// half grid, for x (x C-PML lanes) and y (absorbing-layer boxes, y C-PML lanes).
-// A lane keeps its xy positions through its whole z-march, so these are loaded
-// once per workgroup into registers.
+// C-PML coefficients of a lane's elements: a, b, 1/K on the full grid then on
+// the half grid, for x (x C-PML lanes) and y (absorbing-layer boxes, y C-PML
+// lanes).  Elements outside a layer hold a = b = 0, 1/K = 1, with which the
+// recurrence leaves the derivative unchanged.  A lane keeps its xy positions
+// through its whole z-march, so these are loaded once per workgroup into
+// registers.
 template <typename C, int V>
 struct LaneTabs {
     C x[6][V];
@@ -267,8 +271,8 @@ __device__ __forceinline__ void lane_tabs(const StepArgs<T>& A, const Lane<V>& L
     for (int e = 0; e < V; ++e) {
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            LT.x[q][e] = 0;
-            LT.y[q][e] = 0;
+            LT.x[q][e] = (q == 2 || q == 5) ? 1 : 0;
+            LT.y[q][e] = (q == 2 || q == 5) ? 1 : 0;
         }
         if (L.cx[e] >= 0) {
 #pragma unroll
@@ -291,7 +295,7 @@ __device__ __forceinline__ void lane_tabs(const StepArgs<T>& A, const Lane<V>& L
     }
 }
 
-// z C-PML coefficients of plane k (uniform across the workgroup)
+// z C-PML coefficients of plane k (uniform across the workgroup); Kz, Kzh hold 1/K
 template <typename R>
 struct ZTab {
     int cz;
@@ -314,6 +318,20 @@ __device__ __forceinline__ ZTab<R> ztab(const StepArgs<T>& A, int k)
         }
     }
     return z;
+}
+
+// Offset of element e's x memory variables at plane k, as the fused kernel reads
+// them: for every element, so that a plane's loads form one batch and no load is
+// left under a branch (a load under a branch is issued only there, and waiting
+// for it then also waits for every access issued before it).  Elements outside
+// the x C-PML read offset 0 of the slab (their recurrence has a = b = 0),
+// elements past the box end offset 0, and a grid without x C-PML (w[0] = 0)
+// reads guard element 0 of a field instead.
+template <typename T, int V>
+__device__ __forceinline__ unsigned xmem_offset(const StepArgs<T>& A, const Lane<V>& L, int e, int k)
+{
+    if (A.w[0] <= 0 || !L.act[e]) return 0u;
+    return unsigned((k * A.ny + L.j[e]) * A.w[0] + max(L.cx[e], 0));
 }
 
 // ===========================================================================
@@ -1038,7 +1056,7 @@ __device__ __forceinline__ void fused_sload(const StepArgs<T>& A, const Lane<1>&
     P.sxz[0] = el(A.fi[7], o);
     P.syz[0] = el(A.fi[8], o);
     const bool mx = A.w[0] > 0;
-    const unsigned om = mx ? unsigned((k * A.ny + L.j[0]) * A.w[0] + max(L.cx[0], 0)) : 0u;
+    const unsigned om = xmem_offset(A, L, 0, k);
     P.mx[0][0] = el((mx ? A.mi[0] : A.fi[3]), om);
     P.mx[1][0] = el((mx ? A.mi[3] : A.fi[3]), om);
     P.mx[2][0] = el((mx ? A.mi[5] : A.fi[3]), om);
@@ -1149,10 +1167,6 @@ __device__ __forceinline__ void fused_body(const StepArgs<T>& A, const Box& B, i
         F.inb[s] = p >= B.p0 && p < B.p1;
         make_lane<T, kInterior, 1>(A, p, B.p1, F.L[s]);
         lane_tabs<T, kInterior, 1>(A, F.L[s], F.LT[s]);
-        if (F.L[s].cx[0] < 0) {  // neutral x coefficients for the branch-free recurrences
-            F.LT[s].x[2][0] = 1;
-            F.LT[s].x[5][0] = 1;
-        }
     }
     SOut<R, 1> O[NS];
     VQueue<R, 1> VQ[NU];
@@ -1196,7 +1210,7 @@ __device__ __forceinline__ void fused_body(const StepArgs<T>& A, const Box& B, i
                 V[u].cp[0] = el(A.cp, o);
                 V[u].cs[0] = el(A.cs, o);
             }
-            const unsigned om = mx ? unsigned((k * A.ny + L.j[0]) * A.w[0] + max(L.cx[0], 0)) : 0u;
+            const unsigned om = xmem_offset(A, L, 0, k);
             V[u].mx[0][0] = el((mx ? A.mi[9] : A.fi[0]), om);
             V[u].mx[1][0] = el((mx ? A.mi[12] : A.fi[0]), om);
             V[u].mx[2][0] = el((mx ? A.mi[15] : A.fi[0]), om);

@@ -49,8 +49,8 @@ struct StepArgs {
     const float* cp;
     const float* cs;
     const float* rho;
-    const double* tab[3];  // per axis [6][n]: a, b, K (full grid), a, b, K (half grid)
-    const float* tabf;     // FP32 copies of the x then y tables: [6][nx] then [6][ny]
+    const double* tab[3];  // per axis [6][n]: a, b, 1/K (full grid), a, b, 1/K (half grid)
+    const float* tabf;     // FP32 x then y tables [6][nx], [6][ny]: a, b, 1/K, a, b, 1/K (full, half grid)
     int nx, ny, nz, nxy;
     int nxyp;                // padded plane stride of the 9 fields and the model (elements)
     int g0;                  // guard elements before plane 0; f[] and cp/cs/rho point at the guard

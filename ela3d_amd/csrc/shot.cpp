@@ -671,9 +671,9 @@ private:
         for (int a = 0; a < 3; ++a) {
             const PmlAxis& A = plan_.ax[a];
             std::vector<double> t(6 * size_t(A.n));
-            for (int i = 0; i < A.n; ++i) {
-                t[i] = A.a[i]; t[A.n + i] = A.b[i]; t[2 * A.n + i] = A.K[i];
-                t[3 * A.n + i] = A.a_h[i]; t[4 * A.n + i] = A.b_h[i]; t[5 * A.n + i] = A.K_h[i];
+            for (int i = 0; i < A.n; ++i) {  // the kernels multiply by 1/K (exact for K = 1)
+                t[i] = A.a[i]; t[A.n + i] = A.b[i]; t[2 * A.n + i] = 1.0 / A.K[i];
+                t[3 * A.n + i] = A.a_h[i]; t[4 * A.n + i] = A.b_h[i]; t[5 * A.n + i] = 1.0 / A.K_h[i];
             }
             HIP_CHECK(hipMemcpyAsync(d_tab_[a], t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
             HIP_CHECK(hipStreamSynchronize(stream_));
