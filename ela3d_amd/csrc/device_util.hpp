@@ -57,10 +57,25 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-__device__ __forceinline__ float wave_max(float v)
+// Largest of the wave's 64 non-negative values, given and returned as float
+// bits (unsigned order = float order for non-negative floats), valid in lane 63.
+// DPP lane moves on the VALU instead of LDS permutes: row_shr 1, 2, 4, 8 leave
+// each 16-lane row's maximum in its last lane; row_bcast 15 and 31 carry those
+// into the rows above.  Lanes a move does not reach keep 0 (the neutral value).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ unsigned dpp_move(unsigned x)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWS, 0xf, false);
+}
+
+__device__ __forceinline__ unsigned wave_max_last(unsigned v)
+{
+    v = max(v, dpp_move<0x111, 0xf>(v));  // row_shr:1
+    v = max(v, dpp_move<0x112, 0xf>(v));  // row_shr:2
+    v = max(v, dpp_move<0x114, 0xf>(v));  // row_shr:4
+    v = max(v, dpp_move<0x118, 0xf>(v));  // row_shr:8
+    v = max(v, dpp_move<0x142, 0xa>(v));  // row_bcast:15 into rows 1 and 3
+    v = max(v, dpp_move<0x143, 0xc>(v));  // row_bcast:31 into rows 2 and 3
     return v;
 }
 

@@ -156,8 +156,8 @@ __device__ __forceinline__ void flush_maxima(const StepArgs<T>& A, Scales<T>& S,
     if constexpr (Scales<T>::on) {
         unsigned* slot = A.maxslot + (blockIdx.x & (kMaxSlots - 1)) * 27;
         for (int q = q0; q < q1; ++q) {
-            const float m = wave_max(S.mx[q]);
-            if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(slot + q, __float_as_uint(m));
+            const unsigned m = wave_max_last(__float_as_uint(S.mx[q]));
+            if ((threadIdx.x & 63) == 63 && m > 0u) atomicMax(slot + q, m);
         }
     }
 }
