@@ -195,3 +195,28 @@ def test_homogeneous_cube_vs_fp64_oracle(gpu, compat):
     assert np.all(tot[peak:] <= tot[peak])
     if compat == 0:
         assert tot[-1] < 0.02 * tot[peak]
+
+
+def test_empty_receivers_energy_off_and_zero_steps(gpu):
+    """Edge cases of the boundary.  An empty receiver list with the energy off runs,
+    returns empty outputs and matches the oracle's fields after its steps; a
+    config with nstep = 0 creates, runs nothing and returns empty outputs."""
+    cfg = small_cfg(receivers=np.zeros((0, 3), np.int32), energy_every=0, nstep=3)
+    model = hetero_model(cfg, seed=9)
+    st = random_state(cfg, seed=10)
+    with Shot(cfg, *model) as shot:
+        push_state(shot, st)
+        shot.run()
+        shot.sync()
+        inf = shot.info()
+        assert inf["steps_done"] == 3 and inf["nenergy"] == 0
+        assert shot.seismograms().shape == (inf["nsamples"], 0, 3)
+        assert shot.energy().shape == (0, 2)
+        _, _, st = oracle.run(cfg, model, 3, st=st)
+        compare_state(shot, st, TOL[STORAGE_FP32])
+    cfg0 = small_cfg(nstep=0)
+    with Shot(cfg0, *homo(cfg0)) as shot:
+        shot.run()
+        shot.sync()
+        assert shot.info()["steps_done"] == 0
+        assert shot.seismograms().size == 0 and shot.energy().size == 0
