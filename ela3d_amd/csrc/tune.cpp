@@ -81,8 +81,8 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
         const bool fusable = ela3d::fused_unsupported(plan, 1024).empty() || ela3d::fused_unsupported(plan, 512).empty();
         std::unique_ptr<ela3d::Shot> shot = ela3d::make_shot(plan, cp, cs, rho, nullptr, fusable);
         static const int blocks[] = {64, 128, 256};
-        static const int zsi[] = {2, 4, 8, 16, 32};
-        static const int zss[] = {2, 4, 8};
+        static const int zsi[] = {1, 2, 4, 8, 16, 32};  // 1: more workgroups for small grids
+        static const int zss[] = {1, 2, 4, 8};
         static const int vecs[] = {1, 2, 4};
         static const int vecs_slab[] = {1, 2};
         const bool verbose = std::getenv("ELA3D_TUNE_VERBOSE") != nullptr;
