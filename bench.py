@@ -265,6 +265,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="classic")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--profile-every", type=int, default=10,
+                    help="within the K timed steps, every N-th step runs eagerly with hipEvents around each "
+                         "launch (the per-kernel roofline); the others replay the step graphs (default 10; "
+                         "1 instruments every step)")
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="skip timing the hipGraph replay path (the production path, reported in config)")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
@@ -339,7 +343,7 @@ def main():
         shot.sync()
         info = shot.info()
         t0 = time.perf_counter()
-        stats, wall_ms = shot.profile(K)
+        stats, wall_ms = shot.profile(K, every=args.profile_every)
         shot.sync()
         host_s = time.perf_counter() - t0
         log(f"[bench] timed {K} steps: event wall {wall_ms:.1f} ms, host wall {host_s * 1e3:.1f} ms")
@@ -390,6 +394,8 @@ def main():
             "workload": desc, "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz,
             "cells": cfg.cells, "npoints_pml": cfg.npoints_pml, "nrec": cfg.nrec,
             "energy_every": cfg.energy_every, "schedule": "two-pass (stress, velocity)",
+            "timing": (f"{K} steps: every {args.profile_every}-th eager with hipEvents around each launch "
+                       "(roofline), the others hipGraph replays"),
             "alg_bytes_per_step": info["alg_bytes_per_step"],
             "step_hbm_frac": round(step_bw / PEAK_HBM, 4),
             "graph_ms_per_step": None if graph_ms is None else round(graph_ms, 5),

@@ -251,11 +251,15 @@ class Shot:
         """Load a checkpoint written by a shot of the same config; continues bit-identically."""
         check(lib().ela3d_checkpoint_load(self._h, os.fsencode(path)))
 
-    def profile(self, nsteps: int):
+    def profile(self, nsteps: int, every: int = 1):
+        """Run nsteps; step s eagerly with hipEvents around every launch when
+        s % every == 0, the others as graph replays.  Returns the per-launch
+        totals of the instrumented steps and the event wall time of all nsteps."""
         st = (KernelStats * 8)()
         nk = C.c_int32()
         wall = C.c_double()
-        check(lib().ela3d_profile(self._h, int(nsteps), st, 8, C.byref(nk), C.byref(wall)))
+        check(lib().ela3d_profile_sampled(self._h, int(nsteps), int(every), st, 8, C.byref(nk),
+                                          C.byref(wall)))
         ks = [dict(name=st[q].name.decode(), launches=st[q].launches, total_ms=st[q].total_ms,
                    alg_bytes_per_launch=st[q].alg_bytes_per_launch) for q in range(nk.value)]
         return ks, wall.value

@@ -117,6 +117,8 @@ def lib() -> C.CDLL:
     L.ela3d_info_get.argtypes = [C.c_void_p, P(InfoStruct)]
     L.ela3d_profile.argtypes = [C.c_void_p, C.c_int32, P(KernelStats), C.c_int32, P(C.c_int32),
                                 P(C.c_double)]
+    L.ela3d_profile_sampled.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(KernelStats), C.c_int32,
+                                        P(C.c_int32), P(C.c_double)]
     L.ela3d_destroy.argtypes = [C.c_void_p]
     L.ela3d_destroy.restype = None
     L.ela3d_plan.argtypes = [P(ConfigStruct), P(InfoStruct)]

@@ -338,11 +338,17 @@ int ela3d_info_get(ela3d_shot* s, ela3d_info* out)
 int ela3d_profile(ela3d_shot* s, int32_t nsteps, ela3d_kernel_stats* out, int32_t cap,
                   int32_t* nkernels, double* wall_ms)
 {
+    return ela3d_profile_sampled(s, nsteps, 1, out, cap, nkernels, wall_ms);
+}
+
+int ela3d_profile_sampled(ela3d_shot* s, int32_t nsteps, int32_t every, ela3d_kernel_stats* out, int32_t cap,
+                          int32_t* nkernels, double* wall_ms)
+{
     return guarded([&] {
-        if (!s || nsteps < 0) throw Error{ELA3D_E_INVALID, "bad argument"};
+        if (!s || nsteps < 0 || every < 1) throw Error{ELA3D_E_INVALID, "bad argument"};
         std::vector<ela3d_kernel_stats> st;
         double w = 0.0;
-        s->impl->profile(nsteps, st, w);
+        s->impl->profile(nsteps, st, w, every);
         if (nkernels) *nkernels = int32_t(st.size());
         if (wall_ms) *wall_ms = w;
         if (out) {

@@ -242,22 +242,27 @@ public:
     int steps_done() const override { return host_step_; }
     int64_t device_bytes() const override { return arena_bytes_; }
 
+    // one step the production way: a replay of its captured graph (captured on
+    // first use), or eager launches when graphs are off
+    void graph_step()
+    {
+        const int it = host_step_ + 1;
+        const bool e = energy_step(it);
+        const int par = cur();
+        if (plan_.cfg.use_graph) {
+            hipGraphExec_t& g = graphs_[2 * par + (e ? 1 : 0)];
+            if (!g) g = capture(e, par);
+            HIP_CHECK(hipGraphLaunch(g, stream_));
+        } else {
+            launch_step(e, nullptr, par);
+        }
+        host_step_ = it;
+    }
+
     void enqueue_steps(int n) override
     {
         HIP_CHECK(hipSetDevice(device_));
-        for (int s = 0; s < n; ++s) {
-            const int it = host_step_ + 1;
-            const bool e = energy_step(it);
-            const int par = cur();
-            if (plan_.cfg.use_graph) {
-                hipGraphExec_t& g = graphs_[2 * par + (e ? 1 : 0)];
-                if (!g) g = capture(e, par);
-                HIP_CHECK(hipGraphLaunch(g, stream_));
-            } else {
-                launch_step(e, nullptr, par);
-            }
-            host_step_ = it;
-        }
+        for (int s = 0; s < n; ++s) graph_step();
         HIP_CHECK(hipGetLastError());
     }
 
@@ -402,7 +407,7 @@ public:
             if (nm) HIP_CHECK(hipMemcpy(d_m_[b][m], h.data(), nm * sizeof(T), hipMemcpyHostToDevice));
     }
 
-    void profile(int n, std::vector<ela3d_kernel_stats>& st, double& wall_ms) override
+    void profile(int n, std::vector<ela3d_kernel_stats>& st, double& wall_ms, int every) override
     {
         HIP_CHECK(hipSetDevice(device_));
         static const char* split[kLaunches] = {"stress_cpml", "stress_interior", "velocity_cpml",
@@ -426,7 +431,10 @@ public:
             st[1].alg_bytes_per_launch = plan_.alg_fused_interior();
             st[3].alg_bytes_per_launch = 0;
         }
-        const int chunk = 64;
+        // step s of the run is instrumented (eager launches bracketed by events)
+        // when s % every == 0; the others replay the step graphs
+        every = std::max(1, every);
+        const int chunk = 64;         // instrumented steps per host wait
         const int E = 2 * kLaunches;  // start and end event of every launch
         std::vector<hipEvent_t> ev(size_t(chunk) * E);
         for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
@@ -437,13 +445,20 @@ public:
         HIP_CHECK(hipEventRecord(t0, stream_));
         int done = 0;
         while (done < n) {
-            const int m = std::min(chunk, n - done);
-            for (int s = 0; s < m; ++s) {
-                const int it = host_step_ + 1;
-                launch_step(energy_step(it), &ev[size_t(s) * E], cur());
-                host_step_ = it;
+            int m = 0;  // instrumented steps of this chunk
+            while (done < n && (m < chunk || done % every != 0)) {
+                if (done % every == 0) {
+                    const int it = host_step_ + 1;
+                    launch_step(energy_step(it), &ev[size_t(m) * E], cur());
+                    host_step_ = it;
+                    ++m;
+                } else {
+                    graph_step();
+                }
+                ++done;
             }
             HIP_CHECK(hipGetLastError());
+            if (m == 0) continue;
             HIP_CHECK(hipEventSynchronize(ev[size_t(m - 1) * E + E - 1]));
             for (int s = 0; s < m; ++s)
                 for (int q = 0; q < kLaunches; ++q) {
@@ -452,7 +467,6 @@ public:
                     st[q].total_ms += ms;
                     st[q].launches += 1;
                 }
-            done += m;
         }
         HIP_CHECK(hipEventRecord(t1, stream_));
         HIP_CHECK(hipEventSynchronize(t1));

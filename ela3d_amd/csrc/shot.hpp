@@ -82,7 +82,9 @@ public:
     virtual void set_steps_done(int n) = 0;
     virtual void snapshot(int field, double* out) = 0;
     virtual void set_field(int field, const double* in) = 0;
-    virtual void profile(int n, std::vector<ela3d_kernel_stats>& st, double& wall_ms) = 0;
+    // n steps; every `every`-th one eager with hipEvents around each launch (the
+    // per-launch totals), the others graph replays; wall_ms spans all n
+    virtual void profile(int n, std::vector<ela3d_kernel_stats>& st, double& wall_ms, int every = 1) = 0;
     virtual void info(ela3d_info& out) = 0;
     virtual void retune(const Tuning& t) = 0;       // new launch shape (drops captured graphs)
     virtual hipStream_t stream() const = 0;

@@ -186,6 +186,11 @@ int ela3d_info_get(ela3d_shot* shot, ela3d_info* out);
 /* Run n steps eagerly with hipEvents around every launch; per-kernel totals. */
 int ela3d_profile(ela3d_shot* shot, int32_t nsteps, ela3d_kernel_stats* out, int32_t cap,
                   int32_t* nkernels, double* wall_ms);
+/* Run n steps: step s (0-based) eagerly with hipEvents around every launch when
+ * s % every == 0, the others as graph replays (the production path); per-kernel
+ * totals over the instrumented steps, wall time over all n. */
+int ela3d_profile_sampled(ela3d_shot* shot, int32_t nsteps, int32_t every, ela3d_kernel_stats* out,
+                          int32_t cap, int32_t* nkernels, double* wall_ms);
 void ela3d_destroy(ela3d_shot* shot);
 /* Host-only introspection (no GPU): the plan a config expands to (cells, regions,
  * algorithmic bytes) and its FP64 C-PML tables of one axis, 10*n doubles:

@@ -220,3 +220,22 @@ def test_empty_receivers_energy_off_and_zero_steps(gpu):
         shot.sync()
         assert shot.info()["steps_done"] == 0
         assert shot.seismograms().size == 0 and shot.energy().size == 0
+
+
+def test_sampled_profile_same_results_as_plain_steps(gpu):
+    """profile(n, every) interleaves instrumented eager steps (hipEvents around each
+    launch) with graph replays: the fields, seismograms and energy equal n plain
+    steps bit for bit, and the per-launch totals cover exactly the instrumented steps."""
+    cfg = small_cfg(nstep=24)
+    model = hetero_model(cfg, seed=12)
+    with Shot(cfg, *model) as a, Shot(cfg, *model) as b:
+        a.step(23)
+        a.sync()
+        ks, wall = b.profile(23, every=5)  # steps 0, 5, 10, 15, 20 instrumented
+        b.sync()
+        assert wall > 0 and b.info()["steps_done"] == 23
+        assert [k["launches"] for k in ks] == [5] * len(ks)
+        for q in range(27):
+            np.testing.assert_array_equal(a.snapshot(q), b.snapshot(q))
+        np.testing.assert_array_equal(a.seismograms(), b.seismograms())
+        np.testing.assert_array_equal(a.energy(), b.energy())
