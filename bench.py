@@ -265,6 +265,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="classic")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--fused-tile", type=int, default=0,
+                    help="with --shape: run the fused interior step with this many cells per workgroup "
+                         "(its z-segment is the shape's interior zseg)")
     ap.add_argument("--profile-every", type=int, default=10,
                     help="within the K timed steps, every N-th step runs eagerly with hipEvents around each "
                          "launch (the per-kernel roofline); the others replay the step graphs (default 10; "
@@ -327,7 +330,8 @@ def main():
             raise SystemExit("--shape takes 3, 6 or 8 integers")
         b, zi, pp, bs, zs, ps, ov, mg = v
         ela3d_amd.set_tuning(cfg, block=b, zseg_interior=zi, vec=pp, block_slab=bs, zseg_slab=zs,
-                             vec_slab=ps, overlap=bool(ov), merged=bool(mg))
+                             vec_slab=ps, overlap=bool(ov), merged=bool(mg), fused=args.fused_tile > 0,
+                             tile=args.fused_tile if args.fused_tile > 0 else 1024)
         log(f"[bench] fixed launch shape interior block={b} zseg={zi} vec={pp}; "
             f"layers block={bs or b} zseg={zs} vec={ps}")
     elif not args.no_tune:
@@ -393,7 +397,8 @@ def main():
         "config": {
             "workload": desc, "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz,
             "cells": cfg.cells, "npoints_pml": cfg.npoints_pml, "nrec": cfg.nrec,
-            "energy_every": cfg.energy_every, "schedule": "two-pass (stress, velocity)",
+            "energy_every": cfg.energy_every,
+            "schedule": "fused interior step" if info["fused"] else "two-pass (stress, velocity)",
             "timing": (f"{K} steps: every {args.profile_every}-th eager with hipEvents around each launch "
                        "(roofline), the others hipGraph replays"),
             "alg_bytes_per_step": info["alg_bytes_per_step"],

@@ -1068,6 +1068,7 @@ struct FSlots {
     typedef typename Arith<T>::R R;
     int c[NS];                      // region offset of the slot's cell (relative to q0)
     bool act[NS];                   // the slot holds a region cell
+    bool wact[NS];                  // some lane of this wave holds one (wave-uniform)
     bool own[NS];                   // ... one of the tile's own cells
     bool inb[NS];                   // ... inside the interior box (recomputed, not read)
     Lane<1> L[NS];                  // geometry (inactive slots alias cell q0)
@@ -1101,9 +1102,11 @@ __device__ __forceinline__ void fused_plane(const StepArgs<T>& A, FSlots<T, NS>&
     typedef typename Arith<T>::R R;
     SPlane<R, 1> P[NS];
 #pragma unroll
-    for (int s = S0; s < S1; ++s) fused_sload<T, R>(A, F.L[s], m, P[s]);
+    for (int s = S0; s < S1; ++s)
+        if (F.wact[s]) fused_sload<T, R>(A, F.L[s], m, P[s]);
 #pragma unroll
     for (int s = S0; s < S1; ++s) {
+        if (!F.wact[s]) continue;  // a wave with no region cell in this slot skips it
         if (!EDGE || F.inb[s]) {
             stress_update<T, kInterior, 1, R, true>(A, F.L[s], F.LT[s], m, P[s], F.Q[s], S, store && F.own[s], &O[s]);
         } else {  // outside the interior box: this step's values, from the absorbing-layer launch
@@ -1162,6 +1165,7 @@ __device__ __forceinline__ void fused_body(const StepArgs<T>& A, const Box& B, i
         const int c = tid + kFusedThreads * (s - NH);
         F.c[s] = c;
         F.act[s] = c >= -nx && c < n + nx;
+        F.wact[s] = __ballot(F.act[s]) != 0;
         F.own[s] = c >= 0 && c < n;
         const int p = q0 + (F.act[s] ? c : 0);
         F.inb[s] = p >= B.p0 && p < B.p1;
