@@ -82,3 +82,35 @@ def test_classic_shape_fp32_vs_gpu_fp64(gpu):
     print(f"classic 101x641x640, 600 steps: FP32 vs FP64 seismogram rel L2 {err:.3e}")
     assert err <= 1e-4
     assert rel_l2(e32, e64) <= 1e-4
+
+
+@pytest.mark.parametrize("storage", [STORAGE_FP32, STORAGE_FP64])
+def test_classic_shape_exact_linearity(gpu, storage):
+    """A size-independent property at the full 101 x 641 x 640 size: the loop is linear
+    in the source, and scaling by a power of two commutes with every rounding of normal
+    numbers (FMA included), so doubling the source amplitude doubles every seismogram
+    sample and quadruples the energy.
+
+    The leading edge of the wavefront passes through the subnormal range, whose
+    rounding (to a fixed grid of multiples) does not commute with doubling.  FP64:
+    bit for bit wherever a sample is a normal number (after 700 steps one sample at the
+    front's edge is subnormal).  FP32: after 100 steps every differing cell lies at that
+    edge, below 1e-18 of its field's maximum; the differences ride on the front, and
+    after 700 steps, as it reaches the receivers, they measure 2.5e-7 of the largest
+    sample, so FP32 is held to 1e-6 (small grids, where no subnormals arise, are exact,
+    and reruns are bit-identical)."""
+    cfg, model = workloads.classic(nstep=700)  # the wave reaches the surface receivers after ~590 steps
+    cfg = cfg.replace(storage=storage)
+    s1, e1 = run(cfg, model)
+    s2, e2 = run(cfg.replace(factor=2.0 * cfg.factor), model)
+    assert np.abs(s1).max() > 0 and np.abs(e1).max() > 0 and np.all(np.isfinite(s2))
+    if storage == STORAGE_FP64:
+        normal = np.abs(2.0 * s1) >= np.finfo(np.float64).tiny
+        np.testing.assert_array_equal(s2[normal], 2.0 * s1[normal])
+        assert np.abs(s2 - 2.0 * s1).max() <= 1e-320  # a few subnormal units at the front's edge
+        np.testing.assert_array_equal(e2, 4.0 * e1)
+    else:
+        err = np.abs(s2 - 2.0 * s1).max() / np.abs(2.0 * s1).max()
+        print(f"classic FP32, 700 steps: |s(2f) - 2 s(f)| / max {err:.2e}")
+        assert err <= 1e-6
+        assert np.abs(e2 - 4.0 * e1).max() <= 1e-7 * np.abs(4.0 * e1).max()  # measured 2.1e-10
