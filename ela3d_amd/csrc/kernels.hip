@@ -42,16 +42,18 @@ __device__ __forceinline__ int xcd_block(int b, int n)
 }
 
 // Tile (xy chunk, z segment) of a box's local block index, strip-major: the
-// chunks are cut into up to 8 strips, and each strip's chunks are enumerated
-// for every z segment before the next strip starts.  With the contiguous
-// per-XCD block ranges of xcd_block, each XCD then marches one xy strip through
-// z, so the plane above a z segment and the rows beside a chunk are re-read
-// from that XCD's L2 rather than from HBM.  A bijection onto
-// [0, nchunk) x [0, nzs) for local in [0, nchunk*nzs).
+// chunks are cut into up to 8 strips of at least 8 chunks, and each strip's
+// chunks are enumerated for every z segment before the next strip starts.
+// With the contiguous per-XCD block ranges of xcd_block, each XCD then marches
+// one xy strip through z (a narrow box, such as a y absorbing layer, is one
+// strip: each XCD takes a run of its whole planes), so the plane above a z
+// segment and the rows beside a chunk are re-read from that XCD's L2 rather
+// than from HBM.  A bijection onto [0, nchunk) x [0, nzs) for local in
+// [0, nchunk*nzs).
 __device__ __forceinline__ void tile_of(const Box& B, int local, int& chunk, int& zs)
 {
     const int nzs = (B.k1 - B.k0 + B.zseg - 1) / B.zseg;
-    const int ns = min(8, B.nchunk);
+    const int ns = min(8, max(1, B.nchunk / 8));
     const int w = B.nchunk / ns, r = B.nchunk % ns;  // strips 0..r-1 hold w+1 chunks
     const int cp = local / nzs;
     const int s = cp < r * (w + 1) ? cp / (w + 1) : r + (cp - r * (w + 1)) / w;

@@ -274,3 +274,33 @@ def test_fused_model_catches_short_lds_plane():
     cfg = small_cfg(nx=29, ny=23, nz=26, npml=4)
     bad = fused_violations(cfg, 256, 4, ldsreg=256 + 29)
     assert any(b[0] == "lds-write" for b in bad) and any(b[0] == "lds" for b in bad)
+
+
+def xcd_block(b, n):
+    """The renumbering of k_stress/k_velocity/k_fused (kernels.hip): XCD x owns a
+    contiguous run of logical blocks."""
+    q, r = divmod(n, 8)
+    x, l = b % 8, b // 8
+    return x * q + min(x, r) + l
+
+
+def tile_of(nchunk, nzs, local):
+    """The strip-major tile order of kernels.hip: up to 8 strips of at least 8 chunks."""
+    ns = min(8, max(1, nchunk // 8))
+    w, r = divmod(nchunk, ns)
+    cp = local // nzs
+    s = cp // (w + 1) if cp < r * (w + 1) else r + (cp - r * (w + 1)) // w
+    ws = w + (1 if s < r else 0)
+    c0 = s * w + min(s, r)
+    off = local - c0 * nzs
+    return c0 + off % ws, off // ws
+
+
+@pytest.mark.parametrize("nzs", [1, 2, 3, 5, 310])
+def test_block_order_is_a_bijection(nzs):
+    """Every (chunk, z segment) tile of a box is run by exactly one workgroup, for
+    narrow boxes (one strip), wide ones (8 strips) and everything between."""
+    for nchunk in list(range(1, 70)) + [490, 506]:
+        n = nchunk * nzs
+        tiles = {tile_of(nchunk, nzs, xcd_block(b, n)) for b in range(n)}
+        assert tiles == {(c, z) for c in range(nchunk) for z in range(nzs)}
