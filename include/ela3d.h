@@ -153,7 +153,9 @@ typedef struct ela3d_kernel_stats {
 } ela3d_kernel_stats;
 
 /* ---- shot API (replaces main() CPU:17-726 / ZC:487-1506) ---- */
+/* The reference's literal parameter block (CPU:19-78, 146-153) as defaults. */
 void ela3d_config_default(ela3d_config* cfg);
+/* Full config check without a GPU (the reference's only check is CPU:631-634). */
 int ela3d_validate(const ela3d_config* cfg);
 /* Create a shot: validate, build FP64 C-PML tables (CPU:142-361) and the FP64
  * wavelet (CPU:604-610), copy the model (CPU:41-59), zero the 27 fields (CPU:107-140). */
@@ -167,11 +169,12 @@ int ela3d_sync(ela3d_shot* shot);
 int ela3d_run(ela3d_shot* shot);
 /* Seismograms [nsamples][nrec][3] as FP64 (replaces rechorv*.txt, CPU:624-671). */
 int ela3d_seismograms(ela3d_shot* shot, double* out, size_t cap);
-/* Energy [nenergy][2] = (kinetic, potential) per energy step, FP64. */
+/* Energy [nenergy][2] = (kinetic, potential) per energy step, FP64 (no reference counterpart). */
 int ela3d_energy(ela3d_shot* shot, double* out, size_t cap);
 /* Full-grid copy of one field (ids above) as FP64, nx*ny*nz values (CPU:685-722 wrote vx,vy,vz). */
 int ela3d_snapshot(ela3d_shot* shot, int32_t field, double* out, size_t cap);
-/* Overwrite one field from FP64 values (rounded to the storage type). */
+/* Overwrite one field from FP64 values (rounded to the storage type); the reference
+ * starts from zero fields only (CPU:107-140). */
 int ela3d_set_field(ela3d_shot* shot, int32_t field, const double* in, size_t n);
 /* sigma_xx, sigma_yy, sigma_zz at the source cell right after each step's injection,
  * [nstep+1][3] FP64 with row 0 zero (the lesrcx/y/z arrays, UNI:463-465, 1202-1223). */
@@ -182,8 +185,10 @@ int ela3d_source_trace(ela3d_shot* shot, double* out, size_t cap);
  * ELA3D_E_INVALID if the file belongs to another config, ELA3D_E_IO on file errors. */
 int ela3d_checkpoint_save(ela3d_shot* shot, const char* path);
 int ela3d_checkpoint_load(ela3d_shot* shot, const char* path);
+/* Sizes, regions, launch shape and algorithmic bytes of a shot (no reference counterpart). */
 int ela3d_info_get(ela3d_shot* shot, ela3d_info* out);
-/* Run n steps eagerly with hipEvents around every launch; per-kernel totals. */
+/* Instrumentation (no reference counterpart).
+ * Run n steps eagerly with hipEvents around every launch; per-kernel totals. */
 int ela3d_profile(ela3d_shot* shot, int32_t nsteps, ela3d_kernel_stats* out, int32_t cap,
                   int32_t* nkernels, double* wall_ms);
 /* Run n steps: step s (0-based) eagerly with hipEvents around every launch when
@@ -269,10 +274,13 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
 int ela3d_tuning_set(const ela3d_config* cfg, const ela3d_tuning* t);
 int ela3d_tuning_get(const ela3d_config* cfg, ela3d_tuning* t);
 
+/* Error text (replaces HANDLE_ERROR's printf + exit, HOM:153-162). */
 const char* ela3d_strerror(int code);
 const char* ela3d_last_error(void);
+/* Visible HIP devices (no reference counterpart). */
 int ela3d_device_count(int32_t* out);
-/* Stream-copy kernel of n bytes, timed with hipEvents (HBM ceiling calibration).
+/* Stream-copy kernel of n bytes, timed with hipEvents (HBM ceiling calibration; no
+ * reference counterpart).
  * variant 0: grid-stride float4; 1: one float4 per thread; 2: four float4 per
  * thread, nontemporal; 3: one float per thread.  Multi-stream variants, every
  * array `bytes` long, element-wise in one pass: 4: 12 arrays read + 6 written,
