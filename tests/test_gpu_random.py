@@ -1,0 +1,95 @@
+"""Seeded randomized parity: grids, C-PML widths, quirk flags, source kinds, wavelets,
+rigid faces, energy cadence, storage, receivers, launch shapes and schedules drawn at
+random (fixed seeds), each compared with the oracle after one step from a random
+state.  Every case's device address arithmetic is first checked by the CPU model of
+tests/test_index_bounds.py; the CPU test below runs that check for all cases, and the
+GPU test repeats it before launching anything."""
+import numpy as np
+import pytest
+
+import oracle
+from ela3d_amd import (COMPAT_GHOST, COMPAT_REFERENCE, COMPAT_XHALF_A, COMPAT_ZHALF_MIN, Ela3dError,
+                       SRC_EXPLOSIVE, SRC_FORCE, SRC_FORCE3, SRC_REFERENCE_SIGMA, STORAGE_FP32,
+                       STORAGE_FP64, WAVELET_DGAUSSIAN, WAVELET_GAUSSIAN, WAVELET_RICKER, Shot,
+                       set_tuning)
+from helpers import compare_state, hetero_model, max_rel, push_state, random_state, small_cfg
+from test_index_bounds import fused_violations, violations
+
+NCASES = 24
+TOL = {STORAGE_FP32: 2e-6, STORAGE_FP64: 1e-12}
+DT = {STORAGE_FP32: np.float32, STORAGE_FP64: np.float64}
+
+
+def draw(seed):
+    """One random case: (config, launch shape), regenerated until the config is valid."""
+    rng = np.random.default_rng(1000 + seed)
+    pick = lambda xs: xs[int(rng.integers(len(xs)))]
+    while True:
+        nx, ny, nz = int(rng.integers(9, 61)), int(rng.integers(9, 49)), int(rng.integers(9, 41))
+        npml = int(rng.integers(0, 6))
+        rec = np.stack([rng.integers(0, nx, 8), rng.integers(0, ny, 8), rng.integers(0, nz, 8)], 1)
+        rec = rec[: int(rng.integers(0, 9))].astype(np.int32)
+        cfg = small_cfg(nx=nx, ny=ny, nz=nz, npml=npml, nstep=4,
+                        compat=pick([0, COMPAT_REFERENCE, COMPAT_XHALF_A, COMPAT_ZHALF_MIN, COMPAT_GHOST]),
+                        src_kind=pick([SRC_REFERENCE_SIGMA, SRC_FORCE, SRC_EXPLOSIVE, SRC_FORCE3]),
+                        wavelet=pick([WAVELET_GAUSSIAN, WAVELET_DGAUSSIAN, WAVELET_RICKER]),
+                        storage=STORAGE_FP64 if rng.random() < 0.25 else STORAGE_FP32,
+                        isrc=int(rng.integers(1, nx - 1)), jsrc=int(rng.integers(1, ny - 1)),
+                        ksrc=int(rng.integers(1, nz - 1)), receivers=rec, angle_force=float(rng.uniform(0, 90)),
+                        energy_every=pick([0, 1, 3]))
+        cfg = cfg.replace(dirichlet=bool(rng.random() < 0.5))
+        try:
+            cfg.validate()
+        except Ela3dError:
+            continue
+        shape = dict(block=pick([64, 128, 256]), zseg_interior=pick([1, 2, 3, 4, 8]), vec=pick([1, 2, 4]),
+                     block_slab=pick([64, 128, 256]), zseg_slab=pick([1, 2, 4]), vec_slab=pick([1, 2]),
+                     overlap=bool(rng.random() < 0.3), merged=bool(rng.random() < 0.25),
+                     dbuf=bool(rng.random() < 0.3))
+        if rng.random() < 0.2 and cfg.storage == STORAGE_FP32 and npml > 0:
+            shape.update(fused=True, tile=pick([64, 256, 512]), dbuf=True, merged=False)
+        return cfg, shape
+
+
+def address_violations(cfg, sh):
+    """The CPU address model of the launches this case makes."""
+    if sh.get("merged"):  # the absorbing-layer boxes run with the interior shape
+        bad = violations(cfg, sh["block"], sh["vec"], sh["zseg_interior"], sh["zseg_slab"],
+                         block_slab=sh["block"], vec_slab=sh["vec"])
+    else:
+        bad = violations(cfg, sh["block"], sh["vec"], sh["zseg_interior"], sh["zseg_slab"],
+                         block_slab=sh["block_slab"], vec_slab=sh["vec_slab"])
+    if sh.get("fused"):
+        f = fused_violations(cfg, sh["tile"], sh["zseg_interior"])
+        bad = bad + (f or [])
+    return bad
+
+
+def test_random_cases_address_model():
+    for seed in range(NCASES):
+        cfg, sh = draw(seed)
+        assert address_violations(cfg, sh) == [], (seed, sh)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(NCASES))
+def test_random_case_one_step(gpu, seed):
+    cfg, sh = draw(seed)
+    assert address_violations(cfg, sh) == []
+    print(f"case {seed}: {cfg.nx}x{cfg.ny}x{cfg.nz} npml {cfg.npoints_pml} compat {cfg.compat_flags} "
+          f"src {cfg.src_kind} wavelet {cfg.wavelet} storage {cfg.storage} nrec {cfg.nrec} {sh}", flush=True)
+    set_tuning(cfg, **sh)
+    model = hetero_model(cfg, seed=seed)
+    st = random_state(cfg, seed=seed, dtype=DT[cfg.storage])
+    with Shot(cfg, *model) as shot:
+        fused = shot.info()["fused"]
+        push_state(shot, st)
+        shot.step(1)
+        shot.sync()
+        seis_o, en_o, st = oracle.run(cfg, model, 1, st=st, dtype=DT[cfg.storage])
+        compare_state(shot, st, TOL[cfg.storage])
+        seis_g, en_g = shot.seismograms()[:1], shot.energy()[:1]
+    if cfg.nrec:
+        assert max_rel(seis_g, seis_o[:1]) <= TOL[cfg.storage]
+    assert np.allclose(en_g, en_o[:1], rtol=10 * TOL[cfg.storage], atol=0)
+    assert fused == int(sh.get("fused", False)) or fused == 0  # refused fused shapes fall back to two passes

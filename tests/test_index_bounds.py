@@ -141,8 +141,9 @@ def fused_violations(cfg, tile, zseg, lds_bytes_limit=65536 - 1024, elem=4, thre
     w0 = lo[0] + nx - hi[0]
     memsize0 = nz * ny * w0
     pa, pb = (lo[1] * nx + 63) // 64 * 64, (hi[1] * nx) // 64 * 64
-    assert pb > pa
     zlo, zhi = lo[2], hi[2]
+    if pb <= pa or zhi <= zlo or lo[1] < 1 or lo[2] < 1 or hi[1] > ny - 1 or hi[2] > nz - 1:
+        return None  # no interior box, or a face without absorbing layer: refused (fused_unsupported)
     LR = tile + 2 * nx if ldsreg is None else ldsreg
     if nx > 2 * threads or tile > 4 * threads or 2 * 5 * LR * elem > lds_bytes_limit:
         return None  # the library refuses this tile (fused_lds_bytes, fused_unsupported)
