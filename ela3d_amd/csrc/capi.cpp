@@ -315,6 +315,10 @@ int ela3d_checkpoint_load(ela3d_shot* s, const char* path)
         if (h.fingerprint != fingerprint(p) || h.state_bytes != uint64_t(s->impl->device_bytes()) ||
             h.steps_done < 0 || h.steps_done > p.cfg.nstep)
             throw Error{ELA3D_E_INVALID, "checkpoint belongs to a different configuration"};
+        // a short file is refused before any state is overwritten, so the shot is left as it was
+        if (std::fseek(f.f, 0, SEEK_END) != 0 || std::ftell(f.f) != long(sizeof(h) + h.state_bytes) ||
+            std::fseek(f.f, long(sizeof(h)), SEEK_SET) != 0)
+            throw Error{ELA3D_E_IO, "checkpoint truncated or oversized"};
         std::vector<char> buf(std::min<size_t>(kChunk, h.state_bytes));
         for (size_t off = 0; off < h.state_bytes; off += buf.size()) {
             const size_t n = std::min<size_t>(buf.size(), h.state_bytes - off);

@@ -182,7 +182,8 @@ int ela3d_source_trace(ela3d_shot* shot, double* out, size_t cap);
 /* Checkpoint / resume (no reference counterpart; the reference snapshots cannot
  * resume).  Writes the shot's whole device state and step count to a file; a shot
  * created from the same config and model can load it and continue bit-identically.
- * ELA3D_E_INVALID if the file belongs to another config, ELA3D_E_IO on file errors. */
+ * ELA3D_E_INVALID if the file belongs to another config, ELA3D_E_IO on file errors;
+ * a file of the wrong length is refused before any state is overwritten. */
 int ela3d_checkpoint_save(ela3d_shot* shot, const char* path);
 int ela3d_checkpoint_load(ela3d_shot* shot, const char* path);
 /* Sizes, regions, launch shape and algorithmic bytes of a shot (no reference counterpart). */

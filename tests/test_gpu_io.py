@@ -61,6 +61,28 @@ def test_checkpoint_rejects_other_config(gpu, tmp_path):
         assert e.value.code == -11
 
 
+def test_checkpoint_truncated_leaves_shot_unchanged(gpu, tmp_path):
+    cfg = small_cfg(nstep=20)
+    ck = str(tmp_path / "a.ck")
+    with Shot(cfg, *homo(cfg)) as a:
+        a.step(9)
+        a.checkpoint(ck)
+    with open(ck, "rb") as f:
+        whole = f.read()
+    short = str(tmp_path / "short.ck")
+    with open(short, "wb") as f:
+        f.write(whole[:-4096])
+    with Shot(cfg, *homo(cfg)) as b:
+        b.step(4)
+        before = [b.snapshot(q) for q in (0, 3)]
+        with pytest.raises(ela3d_amd.Ela3dError) as e:
+            b.restore(short)
+        assert e.value.code == -11
+        assert b.info()["steps_done"] == 4
+        for q, x in zip((0, 3), before):
+            assert np.array_equal(b.snapshot(q), x)
+
+
 @pytest.mark.parametrize("src", [SRC_EXPLOSIVE, SRC_REFERENCE_SIGMA])
 def test_source_trace_vs_oracle(gpu, src):
     cfg = small_cfg(nstep=30, src_kind=src)
