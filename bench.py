@@ -278,7 +278,7 @@ def main():
     ap.add_argument("--calibrate", type=float, default=0.0,
                     help="only run the stream-copy kernel over this many GiB (HBM ceiling)")
     ap.add_argument("--copy-variant", type=int, default=-1,
-                    help="copy kernel for --calibrate (0-3); default: all four")
+                    help="copy kernel for --calibrate (0-8); default: all nine")
     ap.add_argument("--shape", default="",
                     help="fixed launch shape block,zseg,vec[,block_slab,zseg_slab,vec_slab"
                          "[,overlap,merged]] (interior, then absorbing layers; skips the tuner)")
@@ -295,8 +295,8 @@ def main():
     if args.calibrate > 0:
         import ela3d_amd
         nbytes = int(args.calibrate * (1 << 30))
-        variants = [args.copy_variant] if args.copy_variant >= 0 else [0, 1, 2, 3, 4, 5, 6, 7]
-        arrays = {4: (12, 6), 5: (10, 3), 6: (12, 9), 7: (12, 6)}
+        variants = [args.copy_variant] if args.copy_variant >= 0 else list(range(9))
+        arrays = {4: (12, 6), 5: (10, 3), 6: (12, 9), 7: (12, 6), 8: (12, 6)}
         for v in variants:
             nr, nw = arrays.get(v, (1, 1))
             size = nbytes if v < 4 else nbytes // 8  # multi-stream: every array this long

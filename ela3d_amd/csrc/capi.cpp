@@ -441,14 +441,17 @@ int ela3d_copy_bench(int32_t device, int64_t bytes, int32_t reps, int32_t varian
     return guarded([&] {
         int rc = need_device(device);
         if (rc) return rc;
-        if (bytes < 16 || reps < 1 || !ms_per_copy || variant < 0 || variant > 7 ||
+        if (bytes < (variant == 8 ? 4 * ela3d::kStreamBlock : 16) || reps < 1 || !ms_per_copy || variant < 0 ||
+            variant > 8 ||
             bytes > (int64_t(1) << 36))
             throw Error{ELA3D_E_INVALID, "bad argument"};
         HIP_CHECK(hipSetDevice(device));
         if (variant >= 4) {  // multi-stream: every array holds `bytes`
             int nr = 0, nw = 0;
             ela3d::stream_counts(variant, nr, nw);
-            const long n = long(bytes / 16) * 4;  // floats per array, a multiple of 4
+            // floats per array: a multiple of 4, and of the block for the interleaved variant
+            const long q = variant == 8 ? 4 * ela3d::kStreamBlock : 16;
+            const long n = long(bytes / q) * (q / 4);
             void* base = nullptr;
             HIP_CHECK(hipMalloc(&base, size_t(n) * 4 * size_t(nr + nw)));
             hipStream_t st;

@@ -118,6 +118,7 @@ constexpr int kEpartBlocks = 256;
 constexpr int kEpartDirect = 2048;  // up to this many partials the tail sums them itself
 void launch_epart(const double* in, int n, double* out, int nout, hipStream_t s);
 // multi-stream calibration (variants 4-7): arrays of n floats packed back to back at base
+constexpr long kStreamBlock = 1024;  // floats per block of the interleaved variant 8
 int stream_counts(int variant, int& nreads, int& nwrites);  // returns floats per lane
 void launch_streams(float* base, long n, int variant, hipStream_t s);
 

@@ -237,9 +237,27 @@ __global__ void __launch_bounds__(256) k_streams(const StreamSet S)
     }
 }
 
+// The same 12-read / 6-write pattern with the arrays interleaved in blocks of B
+// floats: the reads live in one region as [n/B][12][B], the writes as [n/B][6][B].
+// Requires n % B == 0, so every address stays below 12n (reads) and 6n (writes).
+template <int NR, int NW, int B>
+__global__ void __launch_bounds__(256) k_streams_blocked(const float* __restrict__ in, float* __restrict__ out,
+                                                         long n)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long blk = i / B, lane = i % B;
+    float acc = 0.f;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc += in[(blk * NR + r) * B + lane];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) out[(blk * NW + w) * B + lane] = acc + (float)w;
+}
+
 int stream_counts(int variant, int& nr, int& nw)
 {
     switch (variant) {
+    case 8: nr = 12; nw = 6; return 1;
     case 4: nr = 12; nw = 6; return 1;
     case 5: nr = 10; nw = 3; return 1;
     case 6: nr = 12; nw = 9; return 1;
@@ -257,6 +275,11 @@ void launch_streams(float* base, long n, int variant, hipStream_t s)
     for (int q = 0; q < 9; ++q) S.out[q] = base + long(nr + (q < nw ? q : 0)) * n;
     S.n = n;
     const unsigned blocks = (unsigned)((n / w + 255) / 256);
+    if (variant == 8) {
+        hipLaunchKernelGGL((k_streams_blocked<12, 6, kStreamBlock>), dim3(blocks), dim3(256), 0, s, base,
+                           base + 12 * n, n);
+        return;
+    }
     switch (variant) {
     case 4: hipLaunchKernelGGL((k_streams<12, 6, 1>), dim3(blocks), dim3(256), 0, s, S); break;
     case 5: hipLaunchKernelGGL((k_streams<10, 3, 1>), dim3(blocks), dim3(256), 0, s, S); break;

@@ -286,7 +286,9 @@ int ela3d_device_count(int32_t* out);
  * thread, nontemporal; 3: one float per thread.  Multi-stream variants, every
  * array `bytes` long, element-wise in one pass: 4: 12 arrays read + 6 written,
  * one float per lane (a stress pass); 5: 10 + 3 (a velocity pass); 6: 12 + 9
- * (one fused step); 7: 12 + 6 with float4 per lane. */
+ * (one fused step); 7: 12 + 6 with float4 per lane; 8: 12 + 6, one float per lane,
+ * the arrays interleaved in blocks of 1024 floats ([n/1024][12][1024] read,
+ * [n/1024][6][1024] written; `bytes` is rounded down to a multiple of 4096). */
 int ela3d_copy_bench(int32_t device, int64_t bytes, int32_t reps, int32_t variant, double* ms_per_copy);
 
 /* ---- seeded synthetic models (CPU:41-59 homogeneous; layered per the workload spec) ---- */
