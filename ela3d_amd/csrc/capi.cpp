@@ -111,13 +111,7 @@ int ela3d_create(const ela3d_config* cfg, const float* cp, const float* cs, cons
         if (!cfg || !cp || !cs || !rho || !out) throw Error{ELA3D_E_INVALID, "null argument"};
         *out = nullptr;
         ela3d::ShotPlan plan = ela3d::make_plan(*cfg);
-        const size_t cells = size_t(plan.cells);
-        for (size_t q = 0; q < cells; ++q) {
-            if (!(std::isfinite(cp[q]) && std::isfinite(cs[q]) && std::isfinite(rho[q])) ||
-                !(cs[q] > 0.f) || !(rho[q] > 0.f) || !(cp[q] > cs[q]))
-                throw Error{ELA3D_E_INVALID, "model cell " + std::to_string(q) +
-                                                 " invalid (need finite cp > cs > 0, rho > 0)"};
-        }
+        ela3d::check_model(plan.cells, cp, cs, rho);
         int rc = need_device(plan.cfg.device);
         if (rc) return rc;
         auto s = new ela3d_shot;

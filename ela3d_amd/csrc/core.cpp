@@ -258,6 +258,16 @@ ShotPlan make_plan(const ela3d_config& in)
     return P;
 }
 
+void check_model(int64_t cells, const float* cp, const float* cs, const float* rho)
+{
+    for (int64_t q = 0; q < cells; ++q) {
+        if (!(std::isfinite(cp[q]) && std::isfinite(cs[q]) && std::isfinite(rho[q])) || !(cs[q] > 0.f) ||
+            !(rho[q] > 0.f) || !(cp[q] > cs[q]))
+            throw Error{ELA3D_E_INVALID,
+                        "model cell " + std::to_string(q) + " invalid (need finite cp > cs > 0, rho > 0)"};
+    }
+}
+
 int64_t device_bytes_estimate(const ela3d_config& c, const ShotPlan* plan)
 {
     if (plan) return shot_device_bytes(*plan);

@@ -77,6 +77,8 @@ PmlAxis make_axis(const ela3d_config& cfg, int axis);
 double wavelet_value(const ela3d_config& cfg, double t0, int it);
 // Device bytes a shot of this config holds (used by admission control).
 int64_t device_bytes_estimate(const ela3d_config& cfg, const ShotPlan* plan);
+// Throws ELA3D_E_INVALID unless every cell has finite cp > cs > 0 and rho > 0.
+void check_model(int64_t cells, const float* cp, const float* cs, const float* rho);
 // exact size of the single device allocation a shot of this plan makes
 int64_t shot_device_bytes(const ShotPlan& p);
 

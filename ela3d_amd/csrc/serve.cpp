@@ -66,6 +66,7 @@ public:
     uint64_t submit(const ela3d_config& cfg, const float* cp, const float* cs, const float* rho)
     {
         ShotPlan plan = make_plan(cfg);  // validates; throws Error
+        check_model(plan.cells, cp, cs, rho);
         auto r = std::unique_ptr<Request>(new Request);
         r->cfg = plan.cfg;
         r->rec = plan.rec;

@@ -237,7 +237,8 @@ typedef struct ela3d_result {
 #define ELA3D_SHOT_DONE    2
 void ela3d_server_opts_default(ela3d_server_opts* o);
 int ela3d_server_create(const ela3d_server_opts* o, ela3d_server** out);
-/* Queue one shot; config and model are copied.  ELA3D_E_BUDGET if it can never fit. */
+/* Queue one shot; config and model are copied.  ELA3D_E_INVALID for a config or model
+ * that ela3d_create would refuse, ELA3D_E_BUDGET if it can never fit. */
 int ela3d_submit(ela3d_server* s, const ela3d_config* cfg, const float* cp, const float* cs,
                  const float* rho, uint64_t* ticket);
 /* state: ELA3D_SHOT_QUEUED / RUNNING / DONE, or a negative ELA3D_E_* code if it failed */

@@ -111,6 +111,12 @@ def test_budget_rejection_and_tickets(gpu):
         srv.result(t)  # releases
         with pytest.raises(Ela3dError):
             srv.poll(t)
+        cp, cs, rho = (np.array(a, copy=True) for a in model)
+        cs.reshape(-1)[7] = cp.reshape(-1)[7]  # cs == cp: no valid Lame parameters
+        with pytest.raises(Ela3dError) as e:
+            srv.submit(cfg, cp, cs, rho)
+        assert e.value.code == -1 and "model cell 7" in str(e.value)
+        assert srv.stats()["submitted"] == 1
 
 
 def test_served_shot_vs_fp64_oracle(gpu):
