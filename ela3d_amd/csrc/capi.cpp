@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <new>
 #include <string>
 #include <vector>
@@ -440,61 +441,51 @@ int ela3d_copy_bench(int32_t device, int64_t bytes, int32_t reps, int32_t varian
             bytes > (int64_t(1) << 36))
             throw Error{ELA3D_E_INVALID, "bad argument"};
         HIP_CHECK(hipSetDevice(device));
+        // released on every path, errors included
+        struct Res {
+            void* a = nullptr;
+            void* b = nullptr;
+            hipStream_t st = nullptr;
+            hipEvent_t t0 = nullptr, t1 = nullptr;
+            ~Res()
+            {
+                if (t0) hipEventDestroy(t0);
+                if (t1) hipEventDestroy(t1);
+                if (st) hipStreamDestroy(st);
+                if (a) hipFree(a);
+                if (b) hipFree(b);
+            }
+        } R;
+        HIP_CHECK(hipStreamCreateWithFlags(&R.st, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreate(&R.t0));
+        HIP_CHECK(hipEventCreate(&R.t1));
+        std::function<void()> launch;
         if (variant >= 4) {  // multi-stream: every array holds `bytes`
             int nr = 0, nw = 0;
             ela3d::stream_counts(variant, nr, nw);
             // floats per array: a multiple of 4, and of the block for the interleaved variant
             const long q = variant == 8 ? 4 * ela3d::kStreamBlock : 16;
             const long n = long(bytes / q) * (q / 4);
-            void* base = nullptr;
-            HIP_CHECK(hipMalloc(&base, size_t(n) * 4 * size_t(nr + nw)));
-            hipStream_t st;
-            hipEvent_t t0, t1;
-            HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-            HIP_CHECK(hipEventCreate(&t0));
-            HIP_CHECK(hipEventCreate(&t1));
-            HIP_CHECK(hipMemsetAsync(base, 0, size_t(n) * 4 * size_t(nr + nw), st));
-            ela3d::launch_streams(static_cast<float*>(base), n, variant, st);  // warm-up
-            HIP_CHECK(hipEventRecord(t0, st));
-            for (int r = 0; r < reps; ++r) ela3d::launch_streams(static_cast<float*>(base), n, variant, st);
-            HIP_CHECK(hipEventRecord(t1, st));
-            HIP_CHECK(hipEventSynchronize(t1));
-            float ms = 0.f;
-            HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
-            *ms_per_copy = ms / reps;
-            hipEventDestroy(t0);
-            hipEventDestroy(t1);
-            hipStreamDestroy(st);
-            hipFree(base);
-            return ELA3D_OK;
+            const size_t total = size_t(n) * 4 * size_t(nr + nw);
+            HIP_CHECK(hipMalloc(&R.a, total));
+            HIP_CHECK(hipMemsetAsync(R.a, 0, total, R.st));
+            launch = [&, n] { ela3d::launch_streams(static_cast<float*>(R.a), n, variant, R.st); };
+        } else {
+            const long n4 = long(bytes / 16);
+            HIP_CHECK(hipMalloc(&R.a, size_t(n4) * 16));
+            HIP_CHECK(hipMalloc(&R.b, size_t(n4) * 16));
+            HIP_CHECK(hipMemsetAsync(R.a, 0, size_t(n4) * 16, R.st));
+            launch = [&, n4] { ela3d::launch_copy((const float4*)R.a, (float4*)R.b, n4, variant, R.st); };
         }
-        const long n4 = long(bytes / 16);
-        void *a = nullptr, *b = nullptr;
-        HIP_CHECK(hipMalloc(&a, size_t(n4) * 16));
-        hipError_t e = hipMalloc(&b, size_t(n4) * 16);
-        if (e != hipSuccess) {
-            hipFree(a);
-            HIP_CHECK(e);
-        }
-        hipStream_t st;
-        hipEvent_t t0, t1;
-        HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        HIP_CHECK(hipEventCreate(&t0));
-        HIP_CHECK(hipEventCreate(&t1));
-        HIP_CHECK(hipMemsetAsync(a, 0, size_t(n4) * 16, st));
-        ela3d::launch_copy((const float4*)a, (float4*)b, n4, variant, st);  // warm-up
-        HIP_CHECK(hipEventRecord(t0, st));
-        for (int r = 0; r < reps; ++r) ela3d::launch_copy((const float4*)a, (float4*)b, n4, variant, st);
-        HIP_CHECK(hipEventRecord(t1, st));
-        HIP_CHECK(hipEventSynchronize(t1));
+        launch();  // warm-up
+        HIP_CHECK(hipEventRecord(R.t0, R.st));
+        for (int r = 0; r < reps; ++r) launch();
+        HIP_CHECK(hipEventRecord(R.t1, R.st));
+        HIP_CHECK(hipEventSynchronize(R.t1));
+        HIP_CHECK(hipGetLastError());
         float ms = 0.f;
-        HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+        HIP_CHECK(hipEventElapsedTime(&ms, R.t0, R.t1));
         *ms_per_copy = ms / reps;
-        hipEventDestroy(t0);
-        hipEventDestroy(t1);
-        hipStreamDestroy(st);
-        hipFree(a);
-        hipFree(b);
         return ELA3D_OK;
     });
 }

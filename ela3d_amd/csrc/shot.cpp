@@ -187,6 +187,19 @@ public:
         const ela3d_config& c = plan_.cfg;
         HIP_CHECK(hipSetDevice(c.device));
         device_ = c.device;
+        try {
+            init(cp, cs, rho, ext, dbuf);
+        } catch (...) {  // a throwing constructor runs no destructor: free what was made
+            release();
+            throw;
+        }
+    }
+
+    ~ShotImpl() override { release(); }
+
+private:
+    void init(const float* cp, const float* cs, const float* rho, hipStream_t ext, bool dbuf)
+    {
         if (ext) {
             stream_ = ext;
             own_stream_ = false;
@@ -211,18 +224,16 @@ public:
         set_args();
     }
 
-    ~ShotImpl() override
+    void release()
     {
         hipSetDevice(device_);
         if (stream_) hipStreamSynchronize(stream_);
+        if (aux_) hipStreamSynchronize(aux_);
         for (auto& g : graphs_)
             if (g) hipGraphExecDestroy(g);
         if (arena_) hipFree(arena_);
         if (h_flag_) hipHostFree(h_flag_);
-        if (aux_) {
-            hipStreamSynchronize(aux_);
-            hipStreamDestroy(aux_);
-        }
+        if (aux_) hipStreamDestroy(aux_);
         for (auto& e : fork_)
             if (e) hipEventDestroy(e);
         for (auto& e : join_)
@@ -230,6 +241,7 @@ public:
         if (own_stream_ && stream_) hipStreamDestroy(stream_);
     }
 
+public:
     hipStream_t stream() const override { return stream_; }
     void set_overlap(bool on) override
     {
