@@ -383,7 +383,8 @@ def set_tuning(cfg: Config, block: int, zseg_interior: int, zseg_slab: int, vec:
     ``merged`` runs one mixed launch per pass with the interior shape instead;
     ``overlap`` runs the absorbing-layer launches on a second stream beside the
     interior ones; ``dbuf`` keeps two field sets (each step reads one, writes the
-    other); ``fused`` (implies ``dbuf``) runs the fused interior step with ``tile``
+    other; FP16 storage always updates in place and ignores it); ``fused`` (implies
+    ``dbuf``) runs the fused interior step with ``tile``
     cells per workgroup and ``zseg_interior`` planes per segment.  A shape a grid
     cannot run (a fused tile that does not fit, FP16 storage with ``fused``) falls
     back to the two-pass schedule when the shot is created."""

@@ -164,6 +164,28 @@ __device__ __forceinline__ void flush_maxima(const StepArgs<T>& A, Scales<T>& S,
     }
 }
 
+// Output arrays of a pass.  FP16 storage always updates in place (it has no
+// double-buffered schedule), so its kernels store through the input pointers:
+// the separate output pointers would take up to 30 more scalar registers, which
+// the FP16 kernels' storage scales have already used up (spilled scalar
+// registers are read back with one VALU instruction each, every iteration).
+template <typename T>
+__device__ __forceinline__ T* fout(const StepArgs<T>& A, int q)
+{
+    if constexpr (std::is_same<T, half_t>::value)
+        return const_cast<T*>(A.fi[q]);
+    else
+        return A.fo[q];
+}
+template <typename T>
+__device__ __forceinline__ T* mout(const StepArgs<T>& A, int q)
+{
+    if constexpr (std::is_same<T, half_t>::value)
+        return const_cast<T*>(A.mi[q]);
+    else
+        return A.mo[q];
+}
+
 // store of one memory-variable element, tracked
 template <typename T>
 __device__ __forceinline__ void stm(T* p, unsigned o, typename Arith<T>::R v, Scales<T>& S, int q)
@@ -463,16 +485,16 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvz_dz = (P.vz_c[e] - Q.vz_m[e]) * idz;
             if (BFX || cx >= 0) {
                 const R m = cpml(P.mx[0][e], (R)LT.x[3][e], (R)LT.x[4][e], (R)LT.x[5][e], dvx_dx);
-                if (store && onN[e] && (!BFX || cx >= 0)) stm(A.mo[0], omx, m, S, 9);
+                if (store && onN[e] && (!BFX || cx >= 0)) stm(mout(A, 0), omx, m, S, 9);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[0][e], (R)LT.y[0][e], (R)LT.y[1][e], (R)LT.y[2][e], dvy_dy);
-                    if (store && onN[e]) stm(A.mo[1], omy, m, S, 10);
+                    if (store && onN[e]) stm(mout(A, 1), omy, m, S, 10);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[0][e], Z.az, Z.bz, Z.Kz, dvz_dz);
-                    if (store && onN[e]) stm(A.mo[2], omz, m, S, 11);
+                    if (store && onN[e]) stm(mout(A, 2), omz, m, S, 11);
                 }
             }
             nsxx[e] = DT_l2m * dvx_dx + DT_l * (dvy_dy + dvz_dz) + P.sxx[e];
@@ -489,11 +511,11 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvx_dy = (P.vxN[e] - vx_c) * idy;
             if (BFX || cx >= 0) {
                 const R m = cpml(P.mx[1][e], (R)LT.x[0][e], (R)LT.x[1][e], (R)LT.x[2][e], dvy_dx);
-                if (store && onXY[e] && (!BFX || cx >= 0)) stm(A.mo[3], omx, m, S, 12);
+                if (store && onXY[e] && (!BFX || cx >= 0)) stm(mout(A, 3), omx, m, S, 12);
             }
             if (MODE == kSlab && cy >= 0) {
                 const R m = cpml(P.my[1][e], (R)LT.y[3][e], (R)LT.y[4][e], (R)LT.y[5][e], dvx_dy);
-                if (store && onXY[e]) stm(A.mo[4], omy, m, S, 13);
+                if (store && onXY[e]) stm(mout(A, 4), omy, m, S, 13);
             }
             nsxy[e] = DT_mu * (dvy_dx + dvx_dy) + P.sxy[e];
         }
@@ -507,11 +529,11 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvx_dz = (P.vx_u[e] - vx_c) * idz;
             if (BFX || cx >= 0) {
                 const R m = cpml(P.mx[2][e], (R)LT.x[0][e], (R)LT.x[1][e], (R)LT.x[2][e], dvz_dx);
-                if (store && onXZ[e] && (!BFX || cx >= 0)) stm(A.mo[5], omx, m, S, 14);
+                if (store && onXZ[e] && (!BFX || cx >= 0)) stm(mout(A, 5), omx, m, S, 14);
             }
             if (MODE == kSlab && Z.cz >= 0) {
                 const R m = cpml(P.mz[1][e], Z.azh, Z.bzh, Z.Kzh, dvx_dz);
-                if (store && onXZ[e]) stm(A.mo[6], omz, m, S, 15);
+                if (store && onXZ[e]) stm(mout(A, 6), omz, m, S, 15);
             }
             nsxz[e] = DT_mu * (dvz_dx + dvx_dz) + P.sxz[e];
         }
@@ -526,11 +548,11 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[2][e], (R)LT.y[3][e], (R)LT.y[4][e], (R)LT.y[5][e], dvz_dy);
-                    if (store && onYZ[e]) stm(A.mo[7], omy, m, S, 16);
+                    if (store && onYZ[e]) stm(mout(A, 7), omy, m, S, 16);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[2][e], Z.azh, Z.bzh, Z.Kzh, dvy_dz);
-                    if (store && onYZ[e]) stm(A.mo[8], omz, m, S, 17);
+                    if (store && onYZ[e]) stm(mout(A, 8), omz, m, S, 17);
                 }
             }
             nsyz[e] = DT_mu * (dvz_dy + dvy_dz) + P.syz[e];
@@ -548,12 +570,12 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
         }
     }
     if (!store) return;
-    stv<T, V>(A.fo[3], o, nsxx, onN, S, 3);
-    stv<T, V>(A.fo[4], o, nsyy, onN, S, 4);
-    stv<T, V>(A.fo[5], o, nszz, onN, S, 5);
-    stv<T, V>(A.fo[6], o, nsxy, onXY, S, 6);
-    stv<T, V>(A.fo[7], o, nsxz, onXZ, S, 7);
-    stv<T, V>(A.fo[8], o, nsyz, onYZ, S, 8);
+    stv<T, V>(fout(A, 3), o, nsxx, onN, S, 3);
+    stv<T, V>(fout(A, 4), o, nsyy, onN, S, 4);
+    stv<T, V>(fout(A, 5), o, nszz, onN, S, 5);
+    stv<T, V>(fout(A, 6), o, nsxy, onXY, S, 6);
+    stv<T, V>(fout(A, 7), o, nsxz, onXZ, S, 7);
+    stv<T, V>(fout(A, 8), o, nsyz, onYZ, S, 8);
 }
 
 template <typename T, int MODE, int V>
@@ -707,16 +729,16 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d3 = (P.sxz[e] - Q.sxz_m[e]) * idz;
             if (BFX || cx >= 0) {
                 const R m = cpml(P.mx[0][e], (R)LT.x[0][e], (R)LT.x[1][e], (R)LT.x[2][e], d1);
-                if (onx && (!BFX || cx >= 0)) stm(A.mo[9], omx, m, S, 18);
+                if (onx && (!BFX || cx >= 0)) stm(mout(A, 9), omx, m, S, 18);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[0][e], (R)LT.y[0][e], (R)LT.y[1][e], (R)LT.y[2][e], d2);
-                    if (onx) stm(A.mo[10], omy, m, S, 19);
+                    if (onx) stm(mout(A, 10), omy, m, S, 19);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[0][e], Z.az, Z.bz, Z.Kz, d3);
-                    if (onx) stm(A.mo[11], omz, m, S, 20);
+                    if (onx) stm(mout(A, 11), omz, m, S, 20);
                 }
             }
             if (BFX)
@@ -735,16 +757,16 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d3 = (P.syz[e] - Q.syz_m[e]) * idz;
             if (BFX || cx >= 0) {
                 const R m = cpml(P.mx[1][e], (R)LT.x[3][e], (R)LT.x[4][e], (R)LT.x[5][e], d1);
-                if (ony && (!BFX || cx >= 0)) stm(A.mo[12], omx, m, S, 21);
+                if (ony && (!BFX || cx >= 0)) stm(mout(A, 12), omx, m, S, 21);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[1][e], (R)LT.y[3][e], (R)LT.y[4][e], (R)LT.y[5][e], d2);
-                    if (ony) stm(A.mo[13], omy, m, S, 22);
+                    if (ony) stm(mout(A, 13), omy, m, S, 22);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[1][e], Z.az, Z.bz, Z.Kz, d3);
-                    if (ony) stm(A.mo[14], omz, m, S, 23);
+                    if (ony) stm(mout(A, 14), omz, m, S, 23);
                 }
             }
             if (BFX)
@@ -763,16 +785,16 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d3 = (P.szz_u[e] - Q.szz_c[e]) * idz;
             if (BFX || cx >= 0) {
                 const R m = cpml(P.mx[2][e], (R)LT.x[3][e], (R)LT.x[4][e], (R)LT.x[5][e], d1);
-                if (onz && (!BFX || cx >= 0)) stm(A.mo[15], omx, m, S, 24);
+                if (onz && (!BFX || cx >= 0)) stm(mout(A, 15), omx, m, S, 24);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[2][e], (R)LT.y[0][e], (R)LT.y[1][e], (R)LT.y[2][e], d2);
-                    if (onz) stm(A.mo[16], omy, m, S, 25);
+                    if (onz) stm(mout(A, 16), omy, m, S, 25);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[2][e], Z.azh, Z.bzh, Z.Kzh, d3);
-                    if (onz) stm(A.mo[17], omz, m, S, 26);
+                    if (onz) stm(mout(A, 17), omz, m, S, 26);
                 }
             }
             if (BFX)
@@ -810,9 +832,9 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             }
         }
     }
-    stv<T, V>(A.fo[0], o, nvx, wx, S, 0);
-    stv<T, V>(A.fo[1], o, nvy, wy, S, 1);
-    stv<T, V>(A.fo[2], o, nvz, wz, S, 2);
+    stv<T, V>(fout(A, 0), o, nvx, wx, S, 0);
+    stv<T, V>(fout(A, 1), o, nvy, wy, S, 1);
+    stv<T, V>(fout(A, 2), o, nvz, wz, S, 2);
 }
 
 template <typename T, int MODE, bool ENERGY, int V>

@@ -17,10 +17,11 @@ constexpr int kFused = 3;     // launch mode only: one fused stress+velocity ste
 // FP16 storage keeps one power-of-two scale per field (stored = value * scale)
 // in a device table of kScaleTable floats, so replayed graphs see live values:
 //   [0,27) current scale of the data in memory, [27,54) its inverse,
-//   [54,81) scale of the data being written this step, [81,108) its inverse.
+//   [54,81) scale of the data being written this step, [81,108) its inverse,
+//   [108,135) each field's largest |value| of the previous step (0 at first).
 // The tail kernel makes next current at the end of a step and picks new next
 // scales from the maxima the stencil kernels fold into kMaxSlots spread slots.
-constexpr int kScaleTable = 4 * 27;
+constexpr int kScaleTable = 5 * 27;
 constexpr int kMaxSlots = 64;
 
 // A launch covers up to kMaxBox boxes.  A box is a flat xy range [p0,p1)

@@ -173,9 +173,17 @@ ArenaLayout arena_layout(const ShotPlan& p, int elem, int nepart, bool dbuf)
     return L;
 }
 
+// FP16 storage always updates in place: its kernels store through the input
+// pointers, and it has no fused schedule, the one that needs two field sets.
+static bool double_buffers(const ShotPlan& p, bool asked)
+{
+    return asked && p.cfg.storage != ELA3D_STORAGE_FP16;
+}
+
 int64_t shot_device_bytes(const ShotPlan& p)
 {
-    return int64_t(arena_layout(p, storage_bytes(p.cfg.storage), 0, choose_tuning(p).dbuf != 0).total);
+    const Tuning t = choose_tuning(p);
+    return int64_t(arena_layout(p, storage_bytes(p.cfg.storage), 0, double_buffers(p, t.dbuf != 0)).total);
 }
 
 template <typename T>
@@ -214,7 +222,7 @@ private:
         for (auto& e : join_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         tune_ = choose_tuning(plan_);
         if (tune_.fused && !fused_unsupported(plan_, tune_.tile).empty()) tune_.fused = 0;
-        dbuf_ = dbuf || tune_.dbuf != 0 || tune_.fused != 0;
+        dbuf_ = double_buffers(plan_, dbuf || tune_.dbuf != 0 || tune_.fused != 0);
         tune_.dbuf = dbuf_ ? 1 : 0;
         overlap_ = tune_.overlap != 0;
         geometry(plan_, tune_, geo_[1], geo_[0]);
@@ -676,6 +684,7 @@ private:
             }
             tab[q] = tab[54 + q] = sc;
             tab[27 + q] = tab[81 + q] = 1.0f / sc;
+            tab[108 + q] = 0.0f;  // no previous maximum yet
         }
         HIP_CHECK(hipMemcpy(d_scl_, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
     }
@@ -739,6 +748,8 @@ private:
             a.mi[q] = d_m_[in][q];
             a.mo[q] = d_m_[out][q];
         }
+        if (std::is_same<T, half_t>::value && in != out)  // FP16 kernels store through fi/mi
+            throw Error{ELA3D_E_INVALID, "FP16 storage updates in place"};
         a.cp = d_model_[0] - guard_;
         a.cs = d_model_[1] - guard_;
         a.rho = d_model_[2] - guard_;

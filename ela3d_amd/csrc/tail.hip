@@ -139,8 +139,9 @@ __global__ void __launch_bounds__(kTailThreads) k_tail(const TailArgs<T> A)
     if constexpr (std::is_same<T, half_t>::value) {
         // FP16 storage: the data now sits at the "next" scales; make them current
         // and choose the scales of the next step from this step's maxima, every
-        // rescale_every steps, or early when a stored maximum leaves [2^8, 2^13]
-        // (headroom below 65504, precision above the subnormals).
+        // rescale_every steps, or early when the stored maximum, grown once more
+        // by its last step's growth g, would pass 2^13 (headroom below 65504), or
+        // when it is below 2^8 (precision above the subnormals).
         __syncthreads();
         if (tid < 27) {
             const int q = tid;
@@ -153,13 +154,24 @@ __global__ void __launch_bounds__(kTailThreads) k_tail(const TailArgs<T> A)
             float sc = A.scl[54 + q];
             A.scl[q] = sc;
             A.scl[27 + q] = A.scl[81 + q];
+            const float mp = A.scl[108 + q];  // the field's maximum one step earlier
+            A.scl[108 + q] = m;
+            const float g = (mp > 0.f && m > mp) ? fminf(m / mp, 256.f) : 1.f;
             const float stored = m * sc;
-            const bool due = (it % A.rescale_every == 0) || stored > 8192.f || stored < 256.f;
+            const bool due = (it % A.rescale_every == 0) || stored * g > 8192.f || stored < 256.f;
             if (due && m > 0.f && isfinite(m)) {
-                int e;
-                frexpf(m, &e);  // m <= 2^e
-                const int x = min(100, max(-100, 12 - e));
-                sc = ldexpf(1.f, x);  // stored maximum in (2^11, 2^12]
+                int e, es;
+                frexpf(m * g, &e);  // m * g <= 2^e
+                frexpf(sc, &es);    // sc = 2^(es - 1)
+                // stored maximum in (2^11, 2^12] / g, so one more step of growth g
+                // lands it in (2^11, 2^12].  Where the growth is unknown or extreme
+                // (a field that has just turned non-zero: a wavefront's leading edge
+                // reaching a C-PML slab, whose maximum can grow a thousandfold in one
+                // step), a scale rises at most 4x per step.
+                const bool blind = !(mp > 0.f) || m > 256.f * mp;
+                int x = min(100, max(-100, 12 - e));
+                if (blind) x = min(x, es + 1);
+                sc = ldexpf(1.f, x);
             }
             A.scl[54 + q] = sc;
             A.scl[81 + q] = 1.f / sc;

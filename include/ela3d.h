@@ -267,7 +267,8 @@ typedef struct ela3d_tuning {
     int32_t vec_slab;     /* absorbing-layer launch: xy positions per lane, 1 or 2 (0 = 1) */
     int32_t overlap;      /* 1: absorbing-layer launches run concurrently with the interior ones */
     int32_t merged;       /* 1: one mixed launch per pass over every box (interior shape) */
-    int32_t dbuf;         /* 1: two field sets; each step reads one and writes the other */
+    int32_t dbuf;         /* 1: two field sets; each step reads one and writes the other
+                           * (FP32 and FP64 storage; FP16 storage always updates in place) */
     int32_t fused;        /* 1: one fused stress+velocity launch over the interior box (needs dbuf) */
     int32_t tile;         /* fused: interior cells per workgroup (multiple of 64, <= 1024) */
 } ela3d_tuning;
