@@ -99,8 +99,9 @@ def test_random_case_one_step(gpu, seed):
 NFP16 = 8
 
 
-def draw_fp16(seed):
-    """One random multi-step FP16-storage case: (config, launch shape).
+def draw_fp16(seed, storage=STORAGE_FP16):
+    """One random multi-step case from rest: (config, launch shape), FP16 storage
+    unless `storage` says otherwise.
 
     Stress sources (the reference's sigma source, the explosive source) are drawn
     with zero-mean wavelets only.  A Gaussian injected into the stresses leaves a
@@ -108,7 +109,7 @@ def draw_fp16(seed):
     stresses; FP16's relative rounding of it (2^-11) then kicks the neighbouring
     velocities by more than the wave itself (measured: seismogram rel L2 4.7e-2 on
     a 24x26x46 grid, the same before and after the growth-aware rescale)."""
-    rng = np.random.default_rng(5000 + seed)
+    rng = np.random.default_rng((5000 if storage == STORAGE_FP16 else 6000) + seed)
     pick = lambda xs: xs[int(rng.integers(len(xs)))]
     while True:
         nx, ny, nz = (int(rng.integers(24, 57)) for _ in range(3))
@@ -116,9 +117,9 @@ def draw_fp16(seed):
         lo = npml + 1
         rec = np.stack([rng.integers(1, nx - 1, 8), rng.integers(1, ny - 1, 8), rng.integers(1, nz - 1, 8)], 1)
         src = pick([SRC_REFERENCE_SIGMA, SRC_FORCE, SRC_EXPLOSIVE, SRC_FORCE3])
-        zero_mean = src in (SRC_REFERENCE_SIGMA, SRC_EXPLOSIVE)
+        zero_mean = storage == STORAGE_FP16 and src in (SRC_REFERENCE_SIGMA, SRC_EXPLOSIVE)
         cfg = small_cfg(nx=nx, ny=ny, nz=nz, npml=npml, nstep=int(rng.integers(150, 251)),
-                        compat=pick([0, COMPAT_REFERENCE]), storage=STORAGE_FP16, src_kind=src,
+                        compat=pick([0, COMPAT_REFERENCE]), storage=storage, src_kind=src,
                         wavelet=pick([WAVELET_DGAUSSIAN, WAVELET_RICKER] if zero_mean else
                                      [WAVELET_GAUSSIAN, WAVELET_DGAUSSIAN, WAVELET_RICKER]),
                         isrc=int(rng.integers(lo, nx - lo)), jsrc=int(rng.integers(lo, ny - lo)),
@@ -136,8 +137,9 @@ def draw_fp16(seed):
 
 def test_random_fp16_cases_address_model():
     for seed in range(NFP16):
-        cfg, sh = draw_fp16(seed)
-        assert address_violations(cfg, sh) == [], (seed, sh)
+        for storage in (STORAGE_FP16, STORAGE_FP32):
+            cfg, sh = draw_fp16(seed, storage)
+            assert address_violations(cfg, sh) == [], (seed, storage, sh)
 
 
 @pytest.mark.gpu
@@ -160,3 +162,24 @@ def test_random_fp16_case_many_steps(gpu, seed):
     print(f"fp16 case {seed}: seismogram rel L2 {err:.3e}, max energy deviation {eerr:.3e}")
     assert np.all(np.isfinite(s16)) and np.all(np.isfinite(e16))
     assert err < 3e-2 and eerr < 3e-2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(NFP16))
+def test_random_fp32_case_many_steps(gpu, seed):
+    """The same kind of whole shot from rest in FP32 storage: seismograms within the
+    1e-4 relative L2 bar of the FP64 oracle (every wavelet with every source)."""
+    cfg, sh = draw_fp16(seed, STORAGE_FP32)
+    assert address_violations(cfg, sh) == []
+    print(f"fp32 case {seed}: {cfg.nx}x{cfg.ny}x{cfg.nz} npml {cfg.npoints_pml} compat {cfg.compat_flags} "
+          f"src {cfg.src_kind} wavelet {cfg.wavelet} nstep {cfg.nstep} {sh}", flush=True)
+    set_tuning(cfg, **sh)
+    model = hetero_model(cfg, seed=200 + seed)
+    with Shot(cfg, *model) as shot:
+        shot.run()
+        s32, e32 = shot.seismograms(), shot.energy()
+    s64, e64, _ = oracle.run(cfg, model, dtype=np.float64)
+    err = float(np.linalg.norm(s32 - s64) / np.linalg.norm(s64))
+    eerr = float(np.max(np.abs(e32.sum(1) - e64.sum(1))) / np.max(e64.sum(1)))
+    print(f"fp32 case {seed}: seismogram rel L2 {err:.3e}, max energy deviation {eerr:.3e}")
+    assert err <= 1e-4 and eerr <= 1e-4
