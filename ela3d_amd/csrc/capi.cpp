@@ -449,11 +449,11 @@ int ela3d_copy_bench(int32_t device, int64_t bytes, int32_t reps, int32_t varian
             hipEvent_t t0 = nullptr, t1 = nullptr;
             ~Res()
             {
-                if (t0) hipEventDestroy(t0);
-                if (t1) hipEventDestroy(t1);
-                if (st) hipStreamDestroy(st);
-                if (a) hipFree(a);
-                if (b) hipFree(b);
+                if (t0) (void)hipEventDestroy(t0);
+                if (t1) (void)hipEventDestroy(t1);
+                if (st) (void)hipStreamDestroy(st);
+                if (a) (void)hipFree(a);
+                if (b) (void)hipFree(b);
             }
         } R;
         HIP_CHECK(hipStreamCreateWithFlags(&R.st, hipStreamNonBlocking));

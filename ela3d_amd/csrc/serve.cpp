@@ -163,7 +163,7 @@ private:
     // ---- device thread ----
     void loop()
     {
-        hipSetDevice(opts_.device);
+        (void)hipSetDevice(opts_.device);
         for (;;) {
             std::vector<Request*> admitted;
             {
@@ -317,7 +317,7 @@ private:
                 if (r->status == ELA3D_OK) r->status = e.code;
             }
             for (auto& e : r->ev)
-                if (e) hipEventDestroy(e);
+                if (e) (void)hipEventDestroy(e);
             r->shot.reset();
         }
         std::lock_guard<std::mutex> g(mu_);

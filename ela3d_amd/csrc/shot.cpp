@@ -234,19 +234,19 @@ private:
 
     void release()
     {
-        hipSetDevice(device_);
-        if (stream_) hipStreamSynchronize(stream_);
-        if (aux_) hipStreamSynchronize(aux_);
+        (void)hipSetDevice(device_);
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        if (aux_) (void)hipStreamSynchronize(aux_);
         for (auto& g : graphs_)
-            if (g) hipGraphExecDestroy(g);
-        if (arena_) hipFree(arena_);
-        if (h_flag_) hipHostFree(h_flag_);
-        if (aux_) hipStreamDestroy(aux_);
+            if (g) (void)hipGraphExecDestroy(g);
+        if (arena_) (void)hipFree(arena_);
+        if (h_flag_) (void)hipHostFree(h_flag_);
+        if (aux_) (void)hipStreamDestroy(aux_);
         for (auto& e : fork_)
-            if (e) hipEventDestroy(e);
+            if (e) (void)hipEventDestroy(e);
         for (auto& e : join_)
-            if (e) hipEventDestroy(e);
-        if (own_stream_ && stream_) hipStreamDestroy(stream_);
+            if (e) (void)hipEventDestroy(e);
+        if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
     }
 
 public:
@@ -493,9 +493,9 @@ public:
         float w = 0.f;
         HIP_CHECK(hipEventElapsedTime(&w, t0, t1));
         wall_ms = w;
-        for (auto& e : ev) hipEventDestroy(e);
-        hipEventDestroy(t0);
-        hipEventDestroy(t1);
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
     }
 
     void retune(const Tuning& t) override
@@ -526,7 +526,7 @@ public:
     {
         for (auto& x : graphs_)
             if (x) {
-                hipGraphExecDestroy(x);
+                (void)hipGraphExecDestroy(x);
                 x = nullptr;
             }
     }
@@ -913,7 +913,7 @@ private:
         launch_step(energy, nullptr, par);
         HIP_CHECK(hipStreamEndCapture(stream_, &graph));
         hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         HIP_CHECK(e);
         return exec;
     }
