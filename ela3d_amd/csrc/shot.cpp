@@ -403,7 +403,9 @@ public:
         const size_t cells = size_t(plan_.cells);
         if (field < 9) {
             std::vector<T> h(cells);
-            const double sc = current_scale(field);
+            double mx = 0.0;
+            for (size_t q = 0; q < cells; ++q) mx = std::max(mx, std::fabs(in[q]));
+            const double sc = host_scale(field, mx);
             for (size_t q = 0; q < cells; ++q) h[q] = T(in[q] * sc);
             for (int b = 0; b < (dbuf_ ? 2 : 1); ++b)  // both sets: cells a step never writes keep it
                 copy_planes(d_f_[b][field], h.data(), sizeof(T), hipMemcpyHostToDevice);
@@ -413,7 +415,9 @@ public:
         const size_t nm = mem_count(axis);
         std::vector<T> h(nm);
         std::vector<char> covered(cells, 0);
-        const double sc = current_scale(field);
+        double mx = 0.0;
+        for_each_compact(axis, [&](size_t full, size_t) { mx = std::max(mx, std::fabs(in[full])); });
+        const double sc = host_scale(field, mx);
         for_each_compact(axis, [&](size_t full, size_t comp) {
             h[comp] = T(in[full] * sc);
             covered[full] = 1;
@@ -957,6 +961,28 @@ private:
         float s = 1.f;
         HIP_CHECK(hipMemcpy(&s, d_scl_ + q, sizeof(float), hipMemcpyDeviceToHost));
         return s;
+    }
+    // FP16 storage: the scale field q is written at when the host sets it to values
+    // whose largest magnitude is mx.  The current scale is kept while it holds mx in
+    // [2^8, 2^13] (stored); otherwise the field moves to a scale that puts mx in
+    // (2^11, 2^12], so no value overflows FP16, and the device table takes it as
+    // both the current and the next scale.  mx becomes the field's previous maximum.
+    double host_scale(int q, double mx)
+    {
+        if (!std::is_same<T, half_t>::value) return 1.0;
+        double sc = current_scale(q);
+        const double stored = mx * sc;
+        if (mx > 0.0 && std::isfinite(mx) && (stored > 8192.0 || stored < 256.0)) {
+            int e = 0;
+            std::frexp(mx, &e);  // mx <= 2^e
+            sc = std::ldexp(1.0, std::min(100, std::max(-100, 12 - e)));
+            const float s = float(sc), inv = float(1.0 / sc);
+            for (int row : {0, 54}) HIP_CHECK(hipMemcpy(d_scl_ + row + q, &s, sizeof(float), hipMemcpyHostToDevice));
+            for (int row : {27, 81}) HIP_CHECK(hipMemcpy(d_scl_ + row + q, &inv, sizeof(float), hipMemcpyHostToDevice));
+        }
+        const float m = std::isfinite(mx) ? float(mx) : 0.f;
+        HIP_CHECK(hipMemcpy(d_scl_ + 108 + q, &m, sizeof(float), hipMemcpyHostToDevice));
+        return sc;
     }
     hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};  // [set][energy]
 };

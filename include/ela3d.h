@@ -174,7 +174,8 @@ int ela3d_energy(ela3d_shot* shot, double* out, size_t cap);
 /* Full-grid copy of one field (ids above) as FP64, nx*ny*nz values (CPU:685-722 wrote vx,vy,vz). */
 int ela3d_snapshot(ela3d_shot* shot, int32_t field, double* out, size_t cap);
 /* Overwrite one field from FP64 values (rounded to the storage type); the reference
- * starts from zero fields only (CPU:107-140). */
+ * starts from zero fields only (CPU:107-140).  In FP16 storage the field's scale is
+ * re-fitted to the values when the current one would overflow or lose them. */
 int ela3d_set_field(ela3d_shot* shot, int32_t field, const double* in, size_t n);
 /* sigma_xx, sigma_yy, sigma_zz at the source cell right after each step's injection,
  * [nstep+1][3] FP64 with row 0 zero (the lesrcx/y/z arrays, UNI:463-465, 1202-1223). */

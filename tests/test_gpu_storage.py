@@ -151,3 +151,28 @@ def test_fp16_storage_where_memory_variables_turn_on(gpu):
     print(f"FP16 storage, 37x33x29 seed 21: seismogram rel L2 {err:.3e}, max energy deviation {eerr:.3e}")
     assert np.all(np.isfinite(s16)) and np.all(np.isfinite(e16))
     assert err < 3e-2 and eerr < 3e-2
+
+
+@pytest.mark.parametrize("magnitude", [1e12, 1e-12])
+def test_fp16_set_field_fits_its_scale(gpu, magnitude):
+    """set_field in FP16 storage re-fits the field's scale to the data it is given:
+    values far above or below what the current scale holds come back within FP16
+    rounding, not as inf and not flushed to zero.  (Stepping from such a state is
+    another matter: the fields the host did not set keep scales fitted to the
+    configured source, and the first step writes them at 10^12 times that.)"""
+    cfg = small_cfg(nstep=20, storage=STORAGE_FP16)
+    model = hetero_model(cfg, seed=4)
+    rng = np.random.default_rng(8)
+    shape = (cfg.nz, cfg.ny, cfg.nx)
+    with Shot(cfg, *model) as shot:
+        for q in (0, 4, 9):  # a velocity, a stress, an x memory variable
+            x = magnitude * rng.standard_normal(shape)
+            if q == 9:  # memory variables exist only inside their C-PML layers
+                axis, half = oracle.MEM_TABLE[0]
+                a = oracle.tables(cfg, axis)["a_half" if half else "a"]
+                x = np.where((a != 0.0)[None, None, :], x, 0.0)
+            shot.set_field(q, x)
+            y = shot.snapshot(q)
+            assert np.all(np.isfinite(y))
+            big = np.abs(x) > 1e-3 * np.abs(x).max()
+            assert np.max(np.abs(y[big] - x[big]) / np.abs(x[big])) <= 2.0 ** -11
