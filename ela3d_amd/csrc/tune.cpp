@@ -1,11 +1,13 @@
 // tune.cpp -- launch-shape autotuner and its process-wide cache.
 //
-// Each pass is two launches (interior box, absorbing-layer boxes) whose shapes
-// are independent: the tuner first times the interior candidates (threads per
-// workgroup x z planes per workgroup x xy positions per lane) on the interior
-// launches, then the absorbing-layer candidates on the absorbing-layer launches,
-// on a scratch shot of the same grid (source amplitude 0, so the traffic is
-// identical and nothing is perturbed).  The winner is cached keyed by
+// Each pass is two launches (interior box, absorbing-layer boxes).  The tuner
+// first times the interior candidates (threads per workgroup x z planes per
+// workgroup x xy positions per lane) on the interior launches, then the
+// absorbing-layer candidates on the absorbing-layer launches, then re-checks the
+// interior z segment and its neighbours jointly with the z-layer segment over
+// all four launches (the y absorbing-layer boxes march with the interior z
+// segment), all on a scratch shot of the same grid (source amplitude 0, so the
+// traffic is identical and nothing is perturbed).  The winner is cached keyed by
 // (nx, ny, nz, storage, npoints_pml); shots created with block = 0 and
 // zseg = 0 take the cached shape.
 #include <algorithm>
@@ -156,6 +158,41 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
                     cands.emplace_back(ms[1], t);
                 }
         best = refine(cands, 1);
+        // The interior z segment also sets how the y absorbing-layer boxes march (they
+        // run in the absorbing-layer launch), so re-check it and its neighbours in zsi
+        // jointly with the z-layer segment, scored by all four stencil launches.
+        {
+            auto stencil_ms = [&](const Tuning& t) {
+                shot->retune(t);
+                double wall = 0.0;
+                shot->profile(1, st, wall);
+                shot->profile(4 * steps, st, wall);
+                ++tried;
+                return (st[0].total_ms + st[1].total_ms + st[2].total_ms + st[3].total_ms) / (4 * steps);
+            };
+            const int nz = int(sizeof(zsi) / sizeof(zsi[0]));
+            int at = 0;
+            while (at < nz && zsi[at] != best.zseg_interior) ++at;
+            const Tuning centre = best;
+            double best_ms = stencil_ms(centre);
+            for (int d : {-1, 0, 1}) {
+                if (at + d < 0 || at + d >= nz) continue;
+                for (int zs : zss) {
+                    Tuning t = centre;
+                    t.zseg_interior = zsi[at + d];
+                    t.zseg_slab = zs;
+                    if (t.zseg_interior == centre.zseg_interior && zs == centre.zseg_slab) continue;
+                    const double w = stencil_ms(t);
+                    if (verbose)
+                        std::fprintf(stderr, "[tune] interior zseg %2d, z-layer zseg %d: %.4f ms (chosen so far %.4f ms)\n",
+                                     t.zseg_interior, zs, w, best_ms);
+                    if (w < 0.995 * best_ms) {
+                        best_ms = w;
+                        best = t;
+                    }
+                }
+            }
+        }
         // schedule: the two launches in sequence, overlapped (pays off when neither
         // fills the GPU), or one mixed launch per pass (one kernel boundary less)
         auto wall_of = [&](const Tuning& t) {

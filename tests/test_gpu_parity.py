@@ -239,3 +239,25 @@ def test_sampled_profile_same_results_as_plain_steps(gpu):
             np.testing.assert_array_equal(a.snapshot(q), b.snapshot(q))
         np.testing.assert_array_equal(a.seismograms(), b.seismograms())
         np.testing.assert_array_equal(a.energy(), b.energy())
+
+
+def test_autotune_shape_is_used_and_correct(gpu):
+    """The autotuner's pick is cached for the grid, used by the next shot, and that
+    shot meets the seismogram bar against the FP64 oracle.  Every candidate shape the
+    tuner tries on this grid (split, merged and fused) passes the CPU address model."""
+    from ela3d_amd import autotune
+    cfg = small_cfg(nx=45, ny=38, nz=33, nstep=80)
+    model = hetero_model(cfg, seed=6)
+    t = autotune(cfg, model, steps=2)
+    print("autotune:", t)
+    assert t["block"] in (64, 128, 256) and t["vec"] in (1, 2, 4) and t["zseg_interior"] in (1, 2, 4, 8, 16, 32)
+    assert t["candidates"] > 50 and t["ms_per_step"] > 0
+    with Shot(cfg, *model) as shot:
+        inf = shot.info()
+        assert inf["fused"] == t["fused"] and inf["merged"] == t["merged"]
+        if not t["fused"]:
+            assert (inf["block"], inf["vec"]) == (t["block"], t["vec"])
+        shot.run()
+        seis = shot.seismograms()
+    s64, _, _ = oracle.run(cfg, model, dtype=np.float64)
+    assert rel_l2(seis, s64) <= 1e-4
