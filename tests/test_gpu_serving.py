@@ -130,3 +130,23 @@ def test_served_shot_vs_fp64_oracle(gpu):
     print(f"served {cfg.nx}^3 layered shot: seismogram rel L2 vs FP64 {err:.3e}")
     assert err <= 1e-4
     assert np.allclose(en, e64, rtol=1e-4, atol=1e-12 * np.abs(e64).max())
+
+
+def test_mixed_storage_shots_match_alone(gpu):
+    """FP16- and FP64-storage shots served among FP32 ones: each keeps its own scale
+    table and field sets, so every shot is bit-identical to its run alone."""
+    from ela3d_amd import STORAGE_FP16, STORAGE_FP64
+    kinds = [STORAGE_FP16, STORAGE_FP64, None]
+    shots = []
+    for q, spec in enumerate(mixed_specs(6, seed=21)):
+        cfg, model = materialize(spec)
+        if kinds[q % 3] is not None:
+            cfg = cfg.replace(storage=kinds[q % 3])
+        shots.append((cfg, model))
+    ref = [alone(c, m) for c, m in shots]
+    out, stats = serve(shots, max_active=3, steps_per_round=20)
+    assert stats["done"] == len(shots) and stats["failed"] == 0
+    for (seis, en, info), (rs, re), (cfg, _) in zip(out, ref, shots):
+        assert info["status"] == 0
+        assert np.array_equal(seis, rs), f"storage {cfg.storage} shot {cfg.nx}^3 differs in the mix"
+        assert np.array_equal(en, re)
