@@ -18,9 +18,12 @@
  *   wavelets : closed forms and derivative relations between them.
  *   stencils : discrete plane-wave eigenmodes (P and S, along x, y and z)
  *              matched to the exact staggered-grid dispersion relation; 1-D
- *              heterogeneous reduction; zero-in/zero-out; power-of-two
- *              linearity; translation invariance; energy conservation with
- *              the C-PML off; C-PML absorption against an enlarged domain.
+ *              heterogeneous reductions for all nine (axis, polarisation)
+ *              pairs, which pin the neighbour pair of every material average;
+ *              a one-step closed form of the ghost-plane sigma_xz/sigma_yz
+ *              update (CPU:445); zero-in/zero-out; power-of-two linearity;
+ *              translation invariance; energy conservation with the C-PML
+ *              off; C-PML absorption against an enlarged domain.
  *   source / receivers : injection formulas checked on one step from rest.
  */
 #include "ela3d_oracle.h"

@@ -22,7 +22,7 @@ import numpy as np
 import pytest
 
 import oracle
-from ela3d_amd import (COMPAT_REFERENCE, SRC_EXPLOSIVE, SRC_FORCE, SRC_FORCE3,
+from ela3d_amd import (COMPAT_GHOST, COMPAT_REFERENCE, SRC_EXPLOSIVE, SRC_FORCE, SRC_FORCE3,
                        SRC_REFERENCE_SIGMA, WAVELET_GAUSSIAN, WAVELET_RICKER, Config)
 
 VP, VS, RHO = np.float32(3300.0), np.float32(3300.0 / 1.732), np.float32(2800.0)
@@ -98,8 +98,14 @@ def test_discrete_plane_wave(axis, comp, dtype, tol):
             assert np.max(np.abs(st.f[q][inner])) <= tol * max(abs(A), abs(B))
 
 
-def _reduce_1d(c, axis, comp, model, steps, v0):
-    """Independent 1-D staggered scheme of the 3-D loop in a medium varying along `axis`."""
+def _reduce_1d(c, axis, comp, model, steps, v0, shift=False):
+    """Independent 1-D staggered scheme of the 3-D loop in a medium varying along `axis`,
+    for the wave polarised along `comp`.  Each branch is read off the reference loop for
+    the one stress and one velocity the reduction keeps (1-D index x along `axis`):
+    which neighbours the modulus and the density are averaged over, which cells are
+    updated, and the sign of each difference.  `mu[x]` is the average over cells x, x+1
+    (both its factors averaged, as the reference does: rho_bar * cs_bar**2).  With
+    `shift`, every material factor is taken one cell over (a deliberate mistake)."""
     cp, cs, rh = (np.asarray(m, np.float64) for m in model)
     sl = [c.nz // 2, c.ny // 2, c.nx // 2]
     sl[2 - axis] = slice(None)
@@ -107,35 +113,84 @@ def _reduce_1d(c, axis, comp, model, steps, v0):
     n = len(cp1)
     h = (c.dx, c.dy, c.dz)[axis]
     dt = c.dt
+    M = r1 * cp1 * cp1                              # lambda + 2 mu at the cell (CPU:385-387)
+    mu = ((cs1[:-1] + cs1[1:]) / 2) ** 2 * ((r1[:-1] + r1[1:]) / 2)
+    rbar = (r1[:-1] + r1[1:]) / 2                   # rbar[x]: density averaged over x, x+1
+    if shift:  # the mistake the test must catch: every material factor one cell over
+        M, mu, rbar, r1 = np.roll(M, 1), np.roll(mu, 1), np.roll(rbar, 1), np.roll(r1, 1)
     v = v0.astype(np.float64).copy()
     s = np.zeros(n)
     for _ in range(steps):
-        if axis == 0 and comp == 1:  # sigma_xy at i-1/2 over i in [1,n-1]; vy at i over [0,n-2]
-            mu = ((cs1[:-1] + cs1[1:]) / 2) ** 2 * ((r1[:-1] + r1[1:]) / 2)
+        if (axis, comp) == (0, 0):
+            # sigma_xx(i), i in [0,n-2], from vx(i+1) - vx(i), modulus at i (CPU:384-388)
+            s[:-1] += dt * M[:-1] * (v[1:] - v[:-1]) / h
+            # vx(i), i in [1,n-1], from sxx(i) - sxx(i-1), rho over (i, i-1) (CPU:512-516)
+            v[1:] += dt / rbar * (s[1:] - s[:-1]) / h
+        elif (axis, comp) == (1, 1):
+            # sigma_yy(j), j in [1,n-1], from vy(j) - vy(j-1) (CPU:389)
+            s[1:] += dt * M[1:] * (v[1:] - v[:-1]) / h
+            # vy(j), j in [0,n-2], from syy(j+1) - syy(j), rho over (j, j+1) (CPU:541-545)
+            v[:-1] += dt / rbar * (s[1:] - s[:-1]) / h
+        elif (axis, comp) == (2, 2):
+            # sigma_zz(k), k in [1,n-1], from vz(k) - vz(k-1) (CPU:390)
+            s[1:] += dt * M[1:] * (v[1:] - v[:-1]) / h
+            # vz(k), k in [0,n-2], from szz(k+1) - szz(k), rho over (k, k+1) (CPU:571-576)
+            v[:-1] += dt / rbar * (s[1:] - s[:-1]) / h
+        elif (axis, comp) == (0, 1):
+            # sigma_xy(i), i in [1,n-1], from vy(i) - vy(i-1); mu over (i-1, j), (i, j+1)
+            # = cells i-1, i along x (CPU:423-431)
             s[1:] += dt * mu * (v[1:] - v[:-1]) / h
+            # vy(i), i in [0,n-2], from sxy(i+1) - sxy(i); rho over (j, j+1) = rho(i) (CPU:541-544)
             v[:-1] += dt / r1[:-1] * (s[1:] - s[:-1]) / h
-        elif axis == 2 and comp == 2:  # sigma_zz at k over [1,n-1]; vz at k+1/2 over [0,n-2]
-            m = r1 * cp1 * cp1
-            s[1:] += dt * m[1:] * (v[1:] - v[:-1]) / h
-            v[:-1] += dt / ((r1[:-1] + r1[1:]) / 2) * (s[1:] - s[:-1]) / h
-        elif axis == 1 and comp == 2:  # sigma_yz at j+1/2 over [0,n-2]; vz at j over [1,n-1]
-            mu = ((cs1[1:] + cs1[:-1]) / 2) ** 2 * ((r1[1:] + r1[:-1]) / 2)
+        elif (axis, comp) == (0, 2):
+            # sigma_xz(i), i in [1,n-1], from vz(i) - vz(i-1); mu over (i-1, k), (i, k+1)
+            # = cells i-1, i along x (CPU:453-460)
+            s[1:] += dt * mu * (v[1:] - v[:-1]) / h
+            # vz(i), i in [0,n-2], from sxz(i+1) - sxz(i); rho over (k, k+1) = rho(i) (CPU:571-574)
+            v[:-1] += dt / r1[:-1] * (s[1:] - s[:-1]) / h
+        elif (axis, comp) == (1, 0):
+            # sigma_xy(j), j in [0,n-2], from vx(j+1) - vx(j); mu over (i-1, j), (i, j+1)
+            # = cells j, j+1 along y (CPU:423-432)
             s[:-1] += dt * mu * (v[1:] - v[:-1]) / h
+            # vx(j), j in [1,n-1], from sxy(j) - sxy(j-1); rho over (i, i-1) = rho(j) (CPU:512-515)
+            v[1:] += dt / r1[1:] * (s[1:] - s[:-1]) / h
+        elif (axis, comp) == (1, 2):
+            # sigma_yz(j), j in [0,n-2], from vz(j+1) - vz(j); mu over (j+1, k), (j, k+1)
+            # = cells j+1, j along y (CPU:480-487)
+            s[:-1] += dt * mu * (v[1:] - v[:-1]) / h
+            # vz(j), j in [1,n-1], from syz(j) - syz(j-1); rho over (k, k+1) = rho(j) (CPU:571-575)
+            v[1:] += dt / r1[1:] * (s[1:] - s[:-1]) / h
+        elif (axis, comp) == (2, 0):
+            # sigma_xz(k), k in [0,n-2], from vx(k+1) - vx(k); mu over (i-1, k), (i, k+1)
+            # = cells k, k+1 along z (CPU:453-461)
+            s[:-1] += dt * mu * (v[1:] - v[:-1]) / h
+            # vx(k), k in [1,n-1], from sxz(k) - sxz(k-1); rho over (i, i-1) = rho(k) (CPU:512-516)
+            v[1:] += dt / r1[1:] * (s[1:] - s[:-1]) / h
+        elif (axis, comp) == (2, 1):
+            # sigma_yz(k), k in [0,n-2], from vy(k+1) - vy(k); mu over (j+1, k), (j, k+1)
+            # = cells k, k+1 along z (CPU:480-488)
+            s[:-1] += dt * mu * (v[1:] - v[:-1]) / h
+            # vy(k), k in [1,n-1], from syz(k) - syz(k-1); rho over (j, j+1) = rho(k) (CPU:541-545)
             v[1:] += dt / r1[1:] * (s[1:] - s[:-1]) / h
         else:
-            raise NotImplementedError
+            raise ValueError((axis, comp))
     return v
 
 
-@pytest.mark.parametrize("axis,comp", [(0, 1), (2, 2), (1, 2)])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("comp", [0, 1, 2])
 def test_heterogeneous_1d_reduction(axis, comp):
+    """Every (axis, polarisation) pair in a medium whose cp, cs and rho all vary along
+    one axis: the 3-D oracle's middle line equals the independent 1-D scheme, which
+    pins the neighbour pair of each material average (a shift of any average to the
+    other side changes the result by far more than the 1e-12 bar)."""
     T = 30
     nline, nside = 64, 2 * T + 9
     dims = [nside, nside, nside]
     dims[axis] = nline
     c = base_cfg(n=8, nx=dims[0], ny=dims[1], nz=dims[2], isrc=dims[0] // 2, jsrc=dims[1] // 2,
                  ksrc=dims[2] // 2)
-    rng = np.random.default_rng(5)
+    rng = np.random.default_rng(5 + 3 * axis + comp)
     prof = lambda lo, hi: (lo + (hi - lo) * rng.random(nline)).astype(np.float32)
     vp1, r1 = prof(3000, 4200), prof(2000, 2900)
     vs1 = (vp1 / prof(1.6, 1.9)).astype(np.float32)
@@ -153,6 +208,51 @@ def test_heterogeneous_1d_reduction(axis, comp):
     sl[2 - axis] = slice(None)
     got = st.f[comp][tuple(sl)]
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+    # power of the check: the same scheme with every average and cell factor taken one
+    # cell over (the neighbour-shift mistake) misses by orders of magnitude more
+    bad = _reduce_1d(c, axis, comp, model, T, v0, shift=True)
+    assert np.max(np.abs(got - bad)) > 1e-4 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("comp", [0, 1])
+def test_ghost_top_plane_closed_form(comp):
+    """With the ghost quirk (CPU:445: k up to NIMZ, so ztop reads a plane past the
+    grid that is never written), sigma_xz / sigma_yz on the top plane k = nz-1 are
+    updated against zero velocity, zero cs and zero rho above.  One step from a single
+    excited vx (or vy) cell on the top plane, no C-PML, in a random medium:
+      sigma_xz(i, j, top) = dt * ((rho(i-1)+0)/2) * ((cs(i-1)+0)/2)**2 * (0 - vx)/dz
+      sigma_yz(i, j, top) = dt * ((rho(j+1)+0)/2) * ((cs(j+1)+0)/2)**2 * (0 - vy)/dz
+    and without the quirk the top plane is not updated at all."""
+    n = 12
+    rng = np.random.default_rng(11 + comp)
+    shape = (n, n, n)
+    model = ((3000 + 500 * rng.random(shape)).astype(np.float32),
+             (1600 + 300 * rng.random(shape)).astype(np.float32),
+             (2000 + 800 * rng.random(shape)).astype(np.float32))
+    cs, rho = (np.asarray(m, np.float64) for m in model[1:])
+    i, j, k = 5, 6, n - 1
+    amp = 0.37
+    for flags, on in ((COMPAT_GHOST, True), (0, False)):
+        c = base_cfg(n=n, compat_flags=flags)
+        st = oracle.State(c, np.float64)
+        st.f[comp][k, j, i] = amp
+        oracle.stress(c, model, st)
+        q = 7 if comp == 0 else 8
+        got = st.f[q][k]
+        exp = np.zeros((n, n))
+        if on:
+            if comp == 0:
+                exp[j, i] = c.dt * (rho[k, j, i - 1] / 2) * (cs[k, j, i - 1] / 2) ** 2 * (-amp) / c.dz
+            else:
+                exp[j, i] = c.dt * (rho[k, j + 1, i] / 2) * (cs[k, j + 1, i] / 2) ** 2 * (-amp) / c.dz
+        assert np.allclose(got, exp, rtol=1e-13, atol=0.0), (flags, got[j, i], exp[j, i])
+        # the plane below sees the same cell as its ztop neighbour: a regular update
+        below = st.f[q][k - 1]
+        if comp == 0:
+            mu = ((rho[k - 1, j, i - 1] + rho[k, j, i]) / 2) * ((cs[k - 1, j, i - 1] + cs[k, j, i]) / 2) ** 2
+        else:
+            mu = ((rho[k - 1, j + 1, i] + rho[k, j, i]) / 2) * ((cs[k - 1, j + 1, i] + cs[k, j, i]) / 2) ** 2
+        assert below[j, i] == pytest.approx(c.dt * mu * amp / c.dz, rel=1e-13)
 
 
 def _point_source_cfg(n, src, rec, npml, dirichlet=True):
