@@ -26,7 +26,7 @@ __all__ = [
     "WAVELET_GAUSSIAN", "WAVELET_DGAUSSIAN", "WAVELET_RICKER", "STORAGE_FP32", "STORAGE_FP16",
     "STORAGE_FP64", "COMPAT_XHALF_A", "COMPAT_ZHALF_MIN", "COMPAT_GHOST", "COMPAT_REFERENCE",
     "FIELD_NAMES", "Server", "SHOT_QUEUED", "SHOT_RUNNING", "SHOT_DONE", "anticline_model",
-    "read_raw", "write_raw", "write_traces", "write_rechor",
+    "read_raw", "write_raw", "write_traces", "write_rechor", "run_noalloc", "NOALLOC_BYTES",
 ]
 
 SRC_REFERENCE_SIGMA, SRC_FORCE, SRC_EXPLOSIVE, SRC_FORCE3 = 0, 1, 2, 3
@@ -264,6 +264,23 @@ class Shot:
                    alg_bytes_per_launch=st[q].alg_bytes_per_launch) for q in range(nk.value)]
         return ks, wall.value
 
+
+def run_noalloc(cfg: Config, cp, cs, rho):
+    """A whole shot in one call without device allocation (ela3d_run_noalloc): the
+    device state lives in the library's static device buffer (NOALLOC_BYTES per GPU).
+    Returns (seismograms [nsamples][nrec][3], energy [nenergy][2])."""
+    shape = (cfg.nz, cfg.ny, cfg.nx)
+    model = [np.ascontiguousarray(np.asarray(a, dtype=np.float32)).reshape(shape) for a in (cp, cs, rho)]
+    p = cfg.plan()
+    seis = np.zeros((p["nsamples"], cfg.nrec, 3), dtype=np.float64)
+    en = np.zeros((p["nenergy"], 2), dtype=np.float64)
+    s, keep = cfg.to_struct()
+    check(lib().ela3d_run_noalloc(C.byref(s), _fptr(model[0]), _fptr(model[1]), _fptr(model[2]),
+                                  _dptr(seis), seis.size, _dptr(en), en.size, None))
+    return seis, en
+
+
+NOALLOC_BYTES = 128 * 1024 * 1024  # ELA3D_NOALLOC_BYTES
 
 SHOT_QUEUED, SHOT_RUNNING, SHOT_DONE = 0, 1, 2
 

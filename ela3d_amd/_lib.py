@@ -119,6 +119,8 @@ def lib() -> C.CDLL:
                                 P(C.c_double)]
     L.ela3d_profile_sampled.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(KernelStats), C.c_int32,
                                         P(C.c_int32), P(C.c_double)]
+    L.ela3d_run_noalloc.argtypes = [P(ConfigStruct), P(f), P(f), P(f), P(C.c_double), C.c_size_t,
+                                    P(C.c_double), C.c_size_t, C.c_void_p]
     L.ela3d_destroy.argtypes = [C.c_void_p]
     L.ela3d_destroy.restype = None
     L.ela3d_plan.argtypes = [P(ConfigStruct), P(InfoStruct)]

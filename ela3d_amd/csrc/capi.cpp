@@ -127,6 +127,35 @@ int ela3d_create(const ela3d_config* cfg, const float* cp, const float* cs, cons
     });
 }
 
+int ela3d_run_noalloc(const ela3d_config* cfg, const float* cp, const float* cs, const float* rho, double* seis,
+                      size_t seis_cap, double* energy, size_t energy_cap, void* stream)
+{
+    return guarded([&] {
+        if (!cfg || !cp || !cs || !rho) throw Error{ELA3D_E_INVALID, "null argument"};
+        ela3d_config c = *cfg;
+        c.use_graph = 0;  // graph instantiation may allocate device memory for kernel arguments
+        ela3d::ShotPlan plan = ela3d::make_plan(c);
+        ela3d::check_model(plan.cells, cp, cs, rho);
+        const size_t nseis = size_t(plan.nsamples) * plan.cfg.nrec * 3, nen = size_t(plan.nenergy) * 2;
+        if (seis_cap < nseis || (nseis && !seis)) throw Error{ELA3D_E_RANGE, "seismogram buffer too small"};
+        if (energy_cap < nen || (nen && !energy)) throw Error{ELA3D_E_RANGE, "energy buffer too small"};
+        int rc = need_device(plan.cfg.device);
+        if (rc) return rc;
+        HIP_CHECK(hipSetDevice(plan.cfg.device));
+        ela3d::ShotMem mem;
+        mem.arena = ela3d::noalloc_scratch(plan.cfg.device);
+        mem.bytes = size_t(ELA3D_NOALLOC_BYTES);
+        mem.inplace = true;
+        auto shot = ela3d::make_shot(std::move(plan), cp, cs, rho, static_cast<hipStream_t>(stream), false, mem);
+        shot->enqueue_steps(shot->plan().cfg.nstep);
+        rc = shot->sync();
+        if (nseis) shot->seismograms(seis);
+        if (nen) shot->energy(energy);
+        if (rc == ELA3D_E_UNSTABLE) ela3d::set_last_error("non-finite value reached a receiver or the energy");
+        return rc;
+    });
+}
+
 int ela3d_step(ela3d_shot* s, int32_t n)
 {
     return guarded([&] {

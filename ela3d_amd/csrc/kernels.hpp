@@ -112,6 +112,8 @@ size_t fused_lds_bytes(int nx, int tile);
 template <typename T>
 void launch_tail(const TailArgs<T>& a, hipStream_t s);
 void launch_copy(const float4* src, float4* dst, long n4, int variant, hipStream_t s);
+// zero `bytes` bytes at p (p 16-byte aligned) with a kernel on stream s
+void launch_zero(void* p, size_t bytes, hipStream_t s);
 // First level of the energy reduction on energy steps: the n per-block partial
 // pairs in `in` are summed in a fixed order by `nout` workgroups into
 // out[nout][2], which the tail kernel then sums (kEpartBlocks at most).

@@ -189,9 +189,11 @@ int64_t shot_device_bytes(const ShotPlan& p)
 template <typename T>
 class ShotImpl final : public Shot {
 public:
-    ShotImpl(ShotPlan plan, const float* cp, const float* cs, const float* rho, hipStream_t ext, bool dbuf)
+    ShotImpl(ShotPlan plan, const float* cp, const float* cs, const float* rho, hipStream_t ext, bool dbuf,
+             const ShotMem& mem)
     {
         plan_ = std::move(plan);
+        mem_ = mem;
         const ela3d_config& c = plan_.cfg;
         HIP_CHECK(hipSetDevice(c.device));
         device_ = c.device;
@@ -208,23 +210,25 @@ public:
 private:
     void init(const float* cp, const float* cs, const float* rho, hipStream_t ext, bool dbuf)
     {
-        if (ext) {
+        if (ext || mem_.arena) {  // a shot in a caller's arena runs on the caller's stream (0: default)
             stream_ = ext;
             own_stream_ = false;
         } else {
             HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
             own_stream_ = true;
         }
-        // second stream for the absorbing-layer launches, forked from and joined
-        // back into the main stream every pass (default priority, like the main one)
-        HIP_CHECK(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
-        for (auto& e : fork_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        for (auto& e : join_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         tune_ = choose_tuning(plan_);
         if (tune_.fused && !fused_unsupported(plan_, tune_.tile).empty()) tune_.fused = 0;
+        if (mem_.inplace) {
+            tune_.fused = 0;
+            tune_.dbuf = 0;
+            tune_.overlap = 0;
+            dbuf = false;
+        }
         dbuf_ = double_buffers(plan_, dbuf || tune_.dbuf != 0 || tune_.fused != 0);
         tune_.dbuf = dbuf_ ? 1 : 0;
         overlap_ = tune_.overlap != 0;
+        if (overlap_) make_aux();
         geometry(plan_, tune_, geo_[1], geo_[0]);
         allocate();
         upload(cp, cs, rho);
@@ -235,11 +239,11 @@ private:
     void release()
     {
         (void)hipSetDevice(device_);
-        if (stream_) (void)hipStreamSynchronize(stream_);
+        if (stream_ || mem_.arena) (void)hipStreamSynchronize(stream_);
         if (aux_) (void)hipStreamSynchronize(aux_);
         for (auto& g : graphs_)
             if (g) (void)hipGraphExecDestroy(g);
-        if (arena_) (void)hipFree(arena_);
+        if (arena_ && !mem_.arena) (void)hipFree(arena_);
         if (h_flag_) (void)hipHostFree(h_flag_);
         if (aux_) (void)hipStreamDestroy(aux_);
         for (auto& e : fork_)
@@ -254,7 +258,9 @@ public:
     void set_overlap(bool on) override
     {
         if (on == overlap_) return;
+        if (on && mem_.inplace) throw Error{ELA3D_E_INVALID, "this shot runs its launches in sequence"};
         HIP_CHECK(hipStreamSynchronize(stream_));
+        if (on) make_aux();
         overlap_ = on;
         tune_.overlap = on ? 1 : 0;
         drop_graphs();
@@ -508,6 +514,7 @@ public:
         HIP_CHECK(hipStreamSynchronize(stream_));
         if ((t.dbuf != 0 || t.fused != 0) && !dbuf_)
             throw Error{ELA3D_E_INVALID, "double buffering is fixed when the shot is created"};
+        if (t.overlap != 0 && mem_.inplace) throw Error{ELA3D_E_INVALID, "this shot runs its launches in sequence"};
         if (t.fused) {
             const std::string why = fused_unsupported(plan_, t.tile);
             if (!why.empty()) throw Error{ELA3D_E_INVALID, why};
@@ -519,6 +526,7 @@ public:
         tune_ = t;
         tune_.dbuf = dbuf_ ? 1 : 0;
         overlap_ = t.overlap != 0;
+        if (overlap_) make_aux();
         geo_[0] = g[0];
         geo_[1] = g[1];
         nepart_ = g[0].nblocks + g[1].nblocks;
@@ -623,7 +631,14 @@ private:
         const size_t o_wav = L.wav, o_rec = L.rec, o_seis = L.seis, o_en = L.energy, o_ep = L.epart;
         const size_t o_ctl = L.ctl, o_scl = L.scl, o_slot = L.slot, o_src = L.srctrace;
         arena_bytes_ = int64_t(off);
-        HIP_CHECK(hipMalloc(&arena_, off));
+        if (mem_.arena) {
+            if (off > mem_.bytes)
+                throw Error{ELA3D_E_RANGE, "shot needs " + std::to_string(off) + " bytes of device memory, " +
+                                               std::to_string(mem_.bytes) + " available"};
+            arena_ = mem_.arena;
+        } else {
+            HIP_CHECK(hipMalloc(&arena_, off));
+        }
         char* b = static_cast<char*>(arena_);
         for (int q = 0; q < 9; ++q) {
             d_f_[0][q] = reinterpret_cast<T*>(b + o_f[q]) + guard_;
@@ -649,7 +664,12 @@ private:
         d_tabf_ = reinterpret_cast<float*>(b + L.tabf);
         // zero everything: wavefields and memory variables (CPU:107-140), the guards
         // and plane padding (read as neighbours, never written), outputs
-        HIP_CHECK(hipMemsetAsync(arena_, 0, off, stream_));
+        if (mem_.arena) {  // memory the shot does not own may hold a previous user's data
+            launch_zero(arena_, off, stream_);
+            HIP_CHECK(hipGetLastError());
+        } else {
+            HIP_CHECK(hipMemsetAsync(arena_, 0, off, stream_));
+        }
         HIP_CHECK(hipStreamSynchronize(stream_));
         const std::vector<float> ones(kScaleTable, 1.0f);
         HIP_CHECK(hipMemcpy(d_scl_, ones.data(), ones.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -922,6 +942,18 @@ private:
         return exec;
     }
 
+    // second stream for the absorbing-layer launches, forked from and joined back
+    // into the main stream every pass (default priority, like the main one);
+    // made when an overlapped schedule is first asked for
+    void make_aux()
+    {
+        if (aux_) return;
+        HIP_CHECK(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
+        for (auto& e : fork_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        for (auto& e : join_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+
+    ShotMem mem_;
     int device_ = 0;
     hipStream_t stream_ = nullptr;
     bool own_stream_ = true;
@@ -988,15 +1020,15 @@ private:
 };
 
 std::unique_ptr<Shot> make_shot(ShotPlan plan, const float* cp, const float* cs, const float* rho,
-                                hipStream_t ext, bool dbuf)
+                                hipStream_t ext, bool dbuf, const ShotMem& mem)
 {
     switch (plan.cfg.storage) {
     case ELA3D_STORAGE_FP64:
-        return std::unique_ptr<Shot>(new ShotImpl<double>(std::move(plan), cp, cs, rho, ext, dbuf));
+        return std::unique_ptr<Shot>(new ShotImpl<double>(std::move(plan), cp, cs, rho, ext, dbuf, mem));
     case ELA3D_STORAGE_FP16:
-        return std::unique_ptr<Shot>(new ShotImpl<half_t>(std::move(plan), cp, cs, rho, ext, dbuf));
+        return std::unique_ptr<Shot>(new ShotImpl<half_t>(std::move(plan), cp, cs, rho, ext, dbuf, mem));
     default:
-        return std::unique_ptr<Shot>(new ShotImpl<float>(std::move(plan), cp, cs, rho, ext, dbuf));
+        return std::unique_ptr<Shot>(new ShotImpl<float>(std::move(plan), cp, cs, rho, ext, dbuf, mem));
     }
 }
 

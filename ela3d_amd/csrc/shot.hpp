@@ -99,10 +99,23 @@ protected:
     ShotPlan plan_;
 };
 
+// Device memory a shot is placed in instead of an allocation of its own: an
+// arena the caller owns (the allocation-free entry point's static buffer).  The
+// shot zeroes the part it uses with a kernel and never frees it.
+struct ShotMem {
+    void* arena = nullptr;  // 256-byte aligned; nullptr: the shot allocates its own
+    size_t bytes = 0;       // usable bytes at arena
+    bool inplace = false;   // one field set: no fused, double-buffered or overlapped schedule
+};
+
 // double_buffer: allocate a second field set even if the chosen tuning does not
 // need one (the tuner's scratch shot, which tries every schedule)
 std::unique_ptr<Shot> make_shot(ShotPlan plan, const float* cp, const float* cs, const float* rho,
-                                hipStream_t external_stream = nullptr, bool double_buffer = false);
+                                hipStream_t external_stream = nullptr, bool double_buffer = false,
+                                const ShotMem& mem = ShotMem());
+// device address of the allocation-free entry point's static buffer on `device`
+// (ELA3D_NOALLOC_BYTES long; looked up once per device)
+void* noalloc_scratch(int device);
 Tuning choose_tuning(const ShotPlan& p);
 bool cached_tuning(const ela3d_config& c, Tuning& out);
 void store_tuning(const ela3d_config& c, const Tuning& t, double ms);

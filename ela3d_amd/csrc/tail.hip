@@ -348,6 +348,28 @@ void launch_tail(const TailArgs<T>& a, hipStream_t s)
     hipLaunchKernelGGL((k_tail<T>), dim3(1), dim3(kTailThreads), 0, s, a);
 }
 
+// Zero fill of a device range (256-byte aligned start, any length): one uint4 per
+// lane, then the tail bytes.  A kernel rather than hipMemsetAsync so that a caller
+// may capture it in a graph.
+__global__ void __launch_bounds__(256) k_zero(uint4* __restrict__ d, long n16, unsigned char* __restrict__ tail,
+                                              int ntail)
+{
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n16) d[i] = make_uint4(0u, 0u, 0u, 0u);
+    if (i < ntail) tail[i] = 0;
+}
+
+void launch_zero(void* p, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0) return;
+    const long n16 = long(bytes / 16);
+    const int ntail = int(bytes % 16);
+    const long n = std::max<long>(n16, ntail);
+    const long blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_zero, dim3(unsigned(blocks)), dim3(256), 0, s, static_cast<uint4*>(p), n16,
+                       static_cast<unsigned char*>(p) + n16 * 16, ntail);
+}
+
 void launch_copy(const float4* src, float4* dst, long n4, int variant, hipStream_t s)
 {
     if (variant == 1) {

@@ -187,6 +187,20 @@ int ela3d_source_trace(ela3d_shot* shot, double* out, size_t cap);
  * a file of the wrong length is refused before any state is overwritten. */
 int ela3d_checkpoint_save(ela3d_shot* shot, const char* path);
 int ela3d_checkpoint_load(ela3d_shot* shot, const char* path);
+/* Allocation-free run (no counterpart in the reference).  For callers that cannot
+ * allocate device memory (no hipMalloc allowed in their loop, harnesses that forbid
+ * it): ela3d_create + ela3d_run + ela3d_seismograms + ela3d_energy + ela3d_destroy in
+ * one synchronous call, with the shot's whole device state in a static device buffer
+ * of ELA3D_NOALLOC_BYTES that each GPU the library runs on holds until the process
+ * ends.  It calls no hipMalloc or hipFree: the buffer is zeroed by a kernel, the
+ * launches are eager (no graph), in place and in sequence on `stream` (a
+ * hipStream_t; NULL is the default stream).  Results equal ela3d_create/ela3d_run's
+ * for the same config with the same launch shape.  A config whose device state
+ * exceeds the buffer is refused with ELA3D_E_RANGE before anything runs.  Every call
+ * on one GPU uses that GPU's one buffer: calls that overlap on one GPU race. */
+#define ELA3D_NOALLOC_BYTES (128ll * 1024 * 1024)
+int ela3d_run_noalloc(const ela3d_config* cfg, const float* cp, const float* cs, const float* rho,
+                      double* seis, size_t seis_cap, double* energy, size_t energy_cap, void* stream);
 /* Sizes, regions, launch shape and algorithmic bytes of a shot (no reference counterpart). */
 int ela3d_info_get(ela3d_shot* shot, ela3d_info* out);
 /* Instrumentation (no reference counterpart).
