@@ -26,7 +26,7 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{INCLUDE}"]
 
 LIB_SOURCES = ["core.cpp", "models.cpp", "capi.cpp", "shot.cpp", "kernels.hip", "tail.hip",
-               "serve.cpp", "tune.cpp", "formats.cpp", "noalloc.hip"]
+               "serve.cpp", "tune.cpp", "formats.cpp", "noalloc.hip", "fused.hip"]
 
 
 def _run(cmd):

@@ -28,41 +28,9 @@
 // normal stress CPU:377-413, sigma_xy CPU:415-442, sigma_xz/yz CPU:444-499,
 // vx/vy CPU:501-560, vz CPU:562-590.
 #include "device_util.hpp"
+#include "geom.hpp"
 
 namespace ela3d {
-
-// Blocks are dispatched round-robin over the 8 XCDs; renumber so XCD x owns
-// the contiguous logical range [x*q + min(x,r), ...).  A bijection for any n.
-__device__ __forceinline__ int xcd_block(int b, int n)
-{
-    const int X = 8;
-    const int q = n / X, r = n % X;
-    const int x = b % X, l = b / X;
-    return x * q + min(x, r) + l;
-}
-
-// Tile (xy chunk, z segment) of a box's local block index, strip-major: the
-// chunks are cut into up to 8 strips of at least 8 chunks, and each strip's
-// chunks are enumerated for every z segment before the next strip starts.
-// With the contiguous per-XCD block ranges of xcd_block, each XCD then marches
-// one xy strip through z (a narrow box, such as a y absorbing layer, is one
-// strip: each XCD takes a run of its whole planes), so the plane above a z
-// segment and the rows beside a chunk are re-read from that XCD's L2 rather
-// than from HBM.  A bijection onto [0, nchunk) x [0, nzs) for local in
-// [0, nchunk*nzs).
-__device__ __forceinline__ void tile_of(const Box& B, int local, int& chunk, int& zs)
-{
-    const int nzs = (B.k1 - B.k0 + B.zseg - 1) / B.zseg;
-    const int ns = min(8, max(1, B.nchunk / 8));
-    const int w = B.nchunk / ns, r = B.nchunk % ns;  // strips 0..r-1 hold w+1 chunks
-    const int cp = local / nzs;
-    const int s = cp < r * (w + 1) ? cp / (w + 1) : r + (cp - r * (w + 1)) / w;
-    const int ws = w + (s < r ? 1 : 0);
-    const int c0 = s * w + min(s, r);
-    const int off = local - c0 * nzs;
-    chunk = c0 + off % ws;
-    zs = off / ws;
-}
 
 __device__ __forceinline__ Box pick_box(const Box* bx, int nbox, int lb)
 {
@@ -81,7 +49,7 @@ __device__ __forceinline__ Box pick_box(const Box* bx, int nbox, int lb)
 template <typename T>
 __device__ __forceinline__ unsigned cell(const StepArgs<T>& A, int k, int p)
 {
-    return unsigned(A.g0) + unsigned(k) * unsigned(A.nxyp) + unsigned(p);
+    return cell_offset(A.g0, A.nxyp, k, p);
 }
 
 // VEC-wide load/store of storage T at element offset o (element-aligned only)
@@ -344,20 +312,6 @@ __device__ __forceinline__ ZTab<R> ztab(const StepArgs<T>& A, int k)
     return z;
 }
 
-// Offset of element e's x memory variables at plane k, as the fused kernel reads
-// them: for every element, so that a plane's loads form one batch and no load is
-// left under a branch (a load under a branch is issued only there, and waiting
-// for it then also waits for every access issued before it).  Elements outside
-// the x C-PML read offset 0 of the slab (their recurrence has a = b = 0),
-// elements past the box end offset 0, and a grid without x C-PML (w[0] = 0)
-// reads guard element 0 of a field instead.
-template <typename T, int V>
-__device__ __forceinline__ unsigned xmem_offset(const StepArgs<T>& A, const Lane<V>& L, int e, int k)
-{
-    if (A.w[0] <= 0 || !L.act[e]) return 0u;
-    return unsigned((k * A.ny + L.j[e]) * A.w[0] + max(L.cx[e], 0));
-}
-
 // ===========================================================================
 // Stress pass: sigma_xx/yy/zz, sigma_xy, sigma_xz, sigma_yz + 9 memory variables
 // ===========================================================================
@@ -440,24 +394,9 @@ __device__ __forceinline__ void stress_load(const StepArgs<T>& A, const Lane<V>&
     }
 }
 
-// Stresses after one stress update of a cell: the new value where the cell is
-// in the update range, the old one elsewhere (what memory holds after the pass).
-template <typename R, int V>
-struct SOut {
-    R sxx[V], syy[V], szz[V], sxy[V], sxz[V], syz[V];
-};
-
-// `store` = false computes without writing anything (a halo cell recomputed by
-// a neighbouring fused workgroup); `out`, if given, receives the stresses.
-// BFX (branch-free x C-PML, fused kernel): every lane runs the x recurrences,
-// lanes outside the x C-PML with a = b = 0, K = 1 (which leaves the derivative
-// unchanged), and only the memory-variable stores are conditional.  No loaded
-// value is then used under a branch only, so the compiler cannot sink a load
-// into one (where its wait would drain every access in flight).
-template <typename T, int MODE, int V, typename R, bool BFX = false>
+template <typename T, int MODE, int V, typename R>
 __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V>& L, const LaneTabs<TabP<T>, V>& LT, int k, const SPlane<R, V>& P,
-                                              const SQueue<R, V>& Q, Scales<T>& S, bool store = true,
-                                              SOut<R, V>* out = nullptr)
+                                              const SQueue<R, V>& Q, Scales<T>& S)
 {
     const unsigned o = cell(A, k, L.p0);
     const R dt = (R)A.dt, idx = (R)A.idx, idy = (R)A.idy, idz = (R)A.idz;
@@ -483,18 +422,18 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             R dvx_dx = (P.vxE[e] - vx_c) * idx;
             R dvy_dy = (vy_c - P.vyS[e]) * idy;
             R dvz_dz = (P.vz_c[e] - Q.vz_m[e]) * idz;
-            if (BFX || cx >= 0) {
+            if (cx >= 0) {
                 const R m = cpml(P.mx[0][e], (R)LT.x[3][e], (R)LT.x[4][e], (R)LT.x[5][e], dvx_dx);
-                if (store && onN[e] && (!BFX || cx >= 0)) stm(mout(A, 0), omx, m, S, 9);
+                if (onN[e]) stm(mout(A, 0), omx, m, S, 9);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[0][e], (R)LT.y[0][e], (R)LT.y[1][e], (R)LT.y[2][e], dvy_dy);
-                    if (store && onN[e]) stm(mout(A, 1), omy, m, S, 10);
+                    if (onN[e]) stm(mout(A, 1), omy, m, S, 10);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[0][e], Z.az, Z.bz, Z.Kz, dvz_dz);
-                    if (store && onN[e]) stm(mout(A, 2), omz, m, S, 11);
+                    if (onN[e]) stm(mout(A, 2), omz, m, S, 11);
                 }
             }
             nsxx[e] = DT_l2m * dvx_dx + DT_l * (dvy_dy + dvz_dz) + P.sxx[e];
@@ -509,13 +448,13 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             const R DT_mu = dt * (rh * vs * vs);
             R dvy_dx = (vy_c - P.vyW[e]) * idx;
             R dvx_dy = (P.vxN[e] - vx_c) * idy;
-            if (BFX || cx >= 0) {
+            if (cx >= 0) {
                 const R m = cpml(P.mx[1][e], (R)LT.x[0][e], (R)LT.x[1][e], (R)LT.x[2][e], dvy_dx);
-                if (store && onXY[e] && (!BFX || cx >= 0)) stm(mout(A, 3), omx, m, S, 12);
+                if (onXY[e]) stm(mout(A, 3), omx, m, S, 12);
             }
             if (MODE == kSlab && cy >= 0) {
                 const R m = cpml(P.my[1][e], (R)LT.y[3][e], (R)LT.y[4][e], (R)LT.y[5][e], dvx_dy);
-                if (store && onXY[e]) stm(mout(A, 4), omy, m, S, 13);
+                if (onXY[e]) stm(mout(A, 4), omy, m, S, 13);
             }
             nsxy[e] = DT_mu * (dvy_dx + dvx_dy) + P.sxy[e];
         }
@@ -527,13 +466,13 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             const R DT_mu = dt * (rh * vs * vs);
             R dvz_dx = (P.vz_c[e] - P.vzW[e]) * idx;
             R dvx_dz = (P.vx_u[e] - vx_c) * idz;
-            if (BFX || cx >= 0) {
+            if (cx >= 0) {
                 const R m = cpml(P.mx[2][e], (R)LT.x[0][e], (R)LT.x[1][e], (R)LT.x[2][e], dvz_dx);
-                if (store && onXZ[e] && (!BFX || cx >= 0)) stm(mout(A, 5), omx, m, S, 14);
+                if (onXZ[e]) stm(mout(A, 5), omx, m, S, 14);
             }
             if (MODE == kSlab && Z.cz >= 0) {
                 const R m = cpml(P.mz[1][e], Z.azh, Z.bzh, Z.Kzh, dvx_dz);
-                if (store && onXZ[e]) stm(mout(A, 6), omz, m, S, 15);
+                if (onXZ[e]) stm(mout(A, 6), omz, m, S, 15);
             }
             nsxz[e] = DT_mu * (dvz_dx + dvx_dz) + P.sxz[e];
         }
@@ -548,28 +487,16 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
             if (MODE == kSlab) {
                 if (cy >= 0) {
                     const R m = cpml(P.my[2][e], (R)LT.y[3][e], (R)LT.y[4][e], (R)LT.y[5][e], dvz_dy);
-                    if (store && onYZ[e]) stm(mout(A, 7), omy, m, S, 16);
+                    if (onYZ[e]) stm(mout(A, 7), omy, m, S, 16);
                 }
                 if (Z.cz >= 0) {
                     const R m = cpml(P.mz[2][e], Z.azh, Z.bzh, Z.Kzh, dvy_dz);
-                    if (store && onYZ[e]) stm(mout(A, 8), omz, m, S, 17);
+                    if (onYZ[e]) stm(mout(A, 8), omz, m, S, 17);
                 }
             }
             nsyz[e] = DT_mu * (dvz_dy + dvy_dz) + P.syz[e];
         }
     }
-    if (out) {
-#pragma unroll
-        for (int e = 0; e < V; ++e) {
-            out->sxx[e] = onN[e] ? nsxx[e] : P.sxx[e];
-            out->syy[e] = onN[e] ? nsyy[e] : P.syy[e];
-            out->szz[e] = onN[e] ? nszz[e] : P.szz[e];
-            out->sxy[e] = onXY[e] ? nsxy[e] : P.sxy[e];
-            out->sxz[e] = onXZ[e] ? nsxz[e] : P.sxz[e];
-            out->syz[e] = onYZ[e] ? nsyz[e] : P.syz[e];
-        }
-    }
-    if (!store) return;
     stv<T, V>(fout(A, 3), o, nsxx, onN, S, 3);
     stv<T, V>(fout(A, 4), o, nsyy, onN, S, 4);
     stv<T, V>(fout(A, 5), o, nszz, onN, S, 5);
@@ -699,8 +626,7 @@ __device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V
     }
 }
 
-// BFX: branch-free x C-PML and value selects, as for stress_update.
-template <typename T, int MODE, bool ENERGY, int V, typename R, bool BFX = false>
+template <typename T, int MODE, bool ENERGY, int V, typename R>
 __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane<V>& L, const LaneTabs<TabP<T>, V>& LT, int k, const VPlane<R, V>& P,
                                                 const VQueue<R, V>& Q, double& ek, double& ep,
                                                 Scales<T>& S)
@@ -727,9 +653,9 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d1 = (P.sxx[e] - P.sxxW[e]) * idx;
             R d2 = (P.sxy[e] - P.sxyS[e]) * idy;
             R d3 = (P.sxz[e] - Q.sxz_m[e]) * idz;
-            if (BFX || cx >= 0) {
+            if (cx >= 0) {
                 const R m = cpml(P.mx[0][e], (R)LT.x[0][e], (R)LT.x[1][e], (R)LT.x[2][e], d1);
-                if (onx && (!BFX || cx >= 0)) stm(mout(A, 9), omx, m, S, 18);
+                if (onx) stm(mout(A, 9), omx, m, S, 18);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
@@ -741,9 +667,7 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
                     if (onx) stm(mout(A, 11), omz, m, S, 20);
                 }
             }
-            if (BFX)
-                nvx[e] = onx ? DT_over_rho * (d1 + d2 + d3) + P.vx[e] : P.vx[e];
-            else if (onx)
+            if (onx)
                 nvx[e] = DT_over_rho * (d1 + d2 + d3) + P.vx[e];
         }
         // vy: i in [0,nx-2], j in [0,ny-2], k in [1,nz-1] (CPU:531-558)
@@ -755,9 +679,9 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d1 = (P.sxyE[e] - P.sxy[e]) * idx;
             R d2 = (P.syyN[e] - P.syy[e]) * idy;
             R d3 = (P.syz[e] - Q.syz_m[e]) * idz;
-            if (BFX || cx >= 0) {
+            if (cx >= 0) {
                 const R m = cpml(P.mx[1][e], (R)LT.x[3][e], (R)LT.x[4][e], (R)LT.x[5][e], d1);
-                if (ony && (!BFX || cx >= 0)) stm(mout(A, 12), omx, m, S, 21);
+                if (ony) stm(mout(A, 12), omx, m, S, 21);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
@@ -769,9 +693,7 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
                     if (ony) stm(mout(A, 14), omz, m, S, 23);
                 }
             }
-            if (BFX)
-                nvy[e] = ony ? DT_over_rho * (d1 + d2 + d3) + P.vy[e] : P.vy[e];
-            else if (ony)
+            if (ony)
                 nvy[e] = DT_over_rho * (d1 + d2 + d3) + P.vy[e];
         }
         // vz: i in [0,nx-2], j in [1,ny-1], k in [0,nz-2] (CPU:562-590)
@@ -783,9 +705,9 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             R d1 = (P.sxzE[e] - P.sxz[e]) * idx;
             R d2 = (P.syz[e] - P.syzS[e]) * idy;
             R d3 = (P.szz_u[e] - Q.szz_c[e]) * idz;
-            if (BFX || cx >= 0) {
+            if (cx >= 0) {
                 const R m = cpml(P.mx[2][e], (R)LT.x[3][e], (R)LT.x[4][e], (R)LT.x[5][e], d1);
-                if (onz && (!BFX || cx >= 0)) stm(mout(A, 15), omx, m, S, 24);
+                if (onz) stm(mout(A, 15), omx, m, S, 24);
             }
             if (MODE == kSlab) {
                 if (cy >= 0) {
@@ -797,9 +719,7 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
                     if (onz) stm(mout(A, 17), omz, m, S, 26);
                 }
             }
-            if (BFX)
-                nvz[e] = onz ? DT_over_rho * (d1 + d2 + d3) + P.vz[e] : P.vz[e];
-            else if (onz)
+            if (onz)
                 nvz[e] = DT_over_rho * (d1 + d2 + d3) + P.vz[e];
         }
         // rigid faces: the memory variables above are still advanced, then v = 0
@@ -816,7 +736,7 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             const int em = A.emargin;
             const bool in_e = L.act[e] && i >= em && i < nx - 1 - em && j >= em && j < ny - 1 - em && k >= em &&
                               k < nz - 1 - em;
-            if (BFX || in_e) {
+            if (in_e) {
                 // state after this velocity update: cell-local values, FP64 terms
                 const double r = Q.rho_c[e], vp = P.cp[e], vs = P.cs[e];
                 const double mu = r * vs * vs, lam = r * (vp * vp - 2.0 * vs * vs);
@@ -827,8 +747,8 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
                 const double dvx = nvx[e], dvy = nvy[e], dvz = nvz[e];
                 const double tk = 0.5 * r * (dvx * dvx + dvy * dvy + dvz * dvz);
                 const double tp = 0.5 * (ss / (2.0 * mu) - lam * tr * tr / (2.0 * mu * (3.0 * lam + 2.0 * mu)));
-                ek += (!BFX || in_e) ? tk : 0.0;
-                ep += (!BFX || in_e) ? tp : 0.0;
+                ek += tk;
+                ep += tp;
             }
         }
     }
@@ -1012,353 +932,6 @@ void launch_velocity(const StepArgs<T>& a, bool energy, hipStream_t s)
     else
         launch_velocity_e<T, false>(a, s);
 }
-
-// ===========================================================================
-// Fused step over the interior box (double-buffered fields, FP32 or FP64).
-//
-// A workgroup of 256 threads owns a tile [q0, q0+n) of the flattened interior
-// box (n <= tile) and marches its z-segment [kb, ke).  The region of a plane is
-// the tile plus one row on each side: c in [-nx, n+nx), relative to q0.  Thread
-// t holds the region cells c = t + 256*(s - NH), s in [0, NH + NU + NH) (its
-// slots); slots NH .. NH+NU-1 hold its own cells.  Each slot keeps its z queue
-// in registers, so one plane costs a slot one batch of loads.  Per plane k:
-//  * the new stresses of every region cell at plane k+1.  Own cells are stored;
-//    halo cells inside the interior box are recomputed from the previous set
-//    (bit-identical to what their owner computes, never written); halo cells
-//    outside it, and the plane above the box, are read from the set the
-//    absorbing-layer stress launch wrote, which runs first.  sxx, sxy, syy, sxz
-//    and syz go to the LDS plane buffer (k+1) & 1; the own cells' szz stays in
-//    registers.
-//  * the new velocities of the own cells at plane k: in-plane stress
-//    neighbours from the LDS buffer k & 1, the own column's z neighbours from
-//    registers, old velocities and density from the previous set.
-//  * one barrier.
-// Every load of a plane is issued before any of its arithmetic.  The update
-// arithmetic is the two-pass kernels' own (stress_update, velocity_update).
-// ===========================================================================
-constexpr int kFusedThreads = 256;
-
-template <typename T>
-size_t fused_lds_bytes(int nx, int tile)
-{
-    typedef typename Arith<T>::R R;
-    if (nx < 1 || nx > 2 * kFusedThreads || tile < 1 || tile > 4 * kFusedThreads) return 0;
-    const size_t bytes = size_t(2) * 5 * size_t(tile + 2 * nx) * sizeof(R);
-    return bytes <= 65536 - 1024 ? bytes : 0;  // room for the kernel's static LDS
-}
-
-// Loads of one interior cell's stress update at plane k: stress_load of an
-// interior lane, except that the x memory variables are read unconditionally
-// (at offset 0 of their slab for cells outside the x C-PML; the field sxx when
-// the grid has no x C-PML) so that a plane's loads form one branch-free batch.
-template <typename T, typename R>
-__device__ __forceinline__ void fused_sload(const StepArgs<T>& A, const Lane<1>& L, int k, SPlane<R, 1>& P)
-{
-    const unsigned nxyp = A.nxyp, nx = A.nx;
-    const unsigned o = cell(A, k, L.p0);
-    const T *vx = A.fi[0], *vy = A.fi[1], *vz = A.fi[2];
-    P.vz_c[0] = el(vz, o);
-    P.cp_c[0] = (R)el(A.cp, o);
-    P.vx_u[0] = el(vx, o + nxyp);
-    P.vy_u[0] = el(vy, o + nxyp);
-    P.cs_u[0] = (R)el(A.cs, o + nxyp);
-    P.rho_u[0] = (R)el(A.rho, o + nxyp);
-    P.vxE[0] = el(vx, o + 1);
-    P.vyS[0] = el(vy, o - nx);
-    P.vyW[0] = el(vy, o - 1);
-    P.vxN[0] = el(vx, o + nx);
-    P.vzW[0] = el(vz, o - 1);
-    P.vzN[0] = el(vz, o + nx);
-    P.csW[0] = (R)el(A.cs, o - 1);
-    P.csN[0] = (R)el(A.cs, o + nx);
-    P.rhoW[0] = (R)el(A.rho, o - 1);
-    P.rhoN[0] = (R)el(A.rho, o + nx);
-    P.sxx[0] = el(A.fi[3], o);
-    P.syy[0] = el(A.fi[4], o);
-    P.szz[0] = el(A.fi[5], o);
-    P.sxy[0] = el(A.fi[6], o);
-    P.sxz[0] = el(A.fi[7], o);
-    P.syz[0] = el(A.fi[8], o);
-    const bool mx = A.w[0] > 0;
-    const unsigned om = xmem_offset(A, L, 0, k);
-    P.mx[0][0] = el((mx ? A.mi[0] : A.fi[3]), om);
-    P.mx[1][0] = el((mx ? A.mi[3] : A.fi[3]), om);
-    P.mx[2][0] = el((mx ? A.mi[5] : A.fi[3]), om);
-}
-
-// Per-thread slot state of the fused kernel.
-template <typename T, int NS>
-struct FSlots {
-    typedef typename Arith<T>::R R;
-    int c[NS];                      // region offset of the slot's cell (relative to q0)
-    bool act[NS];                   // the slot holds a region cell
-    bool wact[NS];                  // some lane of this wave holds one (wave-uniform)
-    bool own[NS];                   // ... one of the tile's own cells
-    bool inb[NS];                   // ... inside the interior box (recomputed, not read)
-    Lane<1> L[NS];                  // geometry (inactive slots alias cell q0)
-    LaneTabs<TabP<T>, 1> LT[NS];    // x C-PML coefficients
-    SQueue<R, 1> Q[NS];             // z queue at the next plane the slot computes
-};
-
-// z queues of slots [S0, S1) at plane m (velocity and density of plane m, vz of m-1)
-template <typename T, int NS, int S0, int S1>
-__device__ __forceinline__ void fused_qinit(const StepArgs<T>& A, FSlots<T, NS>& F, int m)
-{
-#pragma unroll
-    for (int s = S0; s < S1; ++s) {
-        const unsigned o = cell(A, m, F.L[s].p0);
-        F.Q[s].vx_c[0] = el(A.fi[0], o);
-        F.Q[s].vy_c[0] = el(A.fi[1], o);
-        F.Q[s].cs_c[0] = el(A.cs, o);
-        F.Q[s].rho_c[0] = el(A.rho, o);
-        F.Q[s].vz_m[0] = el(A.fi[2], o - unsigned(A.nxyp));
-    }
-}
-
-// new stresses of slots [S0, S1) at plane m into O; own cells stored when
-// `store`; with LDSW, the region cells' sxx, sxy, syy, sxz, syz into the LDS
-// plane buffer `lb` (component stride LR, cell c at c + nx).  Advances the queues.
-template <typename T, int NS, int S0, int S1, bool EDGE, bool LDSW>
-__device__ __forceinline__ void fused_plane(const StepArgs<T>& A, FSlots<T, NS>& F, int m, bool store,
-                                            typename Arith<T>::R* lb, int LR,
-                                            SOut<typename Arith<T>::R, 1> (&O)[NS], Scales<T>& S)
-{
-    typedef typename Arith<T>::R R;
-    SPlane<R, 1> P[NS];
-#pragma unroll
-    for (int s = S0; s < S1; ++s)
-        if (F.wact[s]) fused_sload<T, R>(A, F.L[s], m, P[s]);
-#pragma unroll
-    for (int s = S0; s < S1; ++s) {
-        if (!F.wact[s]) continue;  // a wave with no region cell in this slot skips it
-        if (!EDGE || F.inb[s]) {
-            stress_update<T, kInterior, 1, R, true>(A, F.L[s], F.LT[s], m, P[s], F.Q[s], S, store && F.own[s], &O[s]);
-        } else {  // outside the interior box: this step's values, from the absorbing-layer launch
-            const unsigned o = cell(A, m, F.L[s].p0);
-            O[s].sxx[0] = el(A.fo[3], o);
-            O[s].syy[0] = el(A.fo[4], o);
-            O[s].szz[0] = el(A.fo[5], o);
-            O[s].sxy[0] = el(A.fo[6], o);
-            O[s].sxz[0] = el(A.fo[7], o);
-            O[s].syz[0] = el(A.fo[8], o);
-        }
-        F.Q[s].vx_c[0] = P[s].vx_u[0];
-        F.Q[s].vy_c[0] = P[s].vy_u[0];
-        F.Q[s].cs_c[0] = P[s].cs_u[0];
-        F.Q[s].rho_c[0] = P[s].rho_u[0];
-        F.Q[s].vz_m[0] = P[s].vz_c[0];
-        if (LDSW && F.act[s]) {
-            R* d = lb + (F.c[s] + A.nx);
-            d[0] = O[s].sxx[0];
-            d[LR] = O[s].sxy[0];
-            d[2 * LR] = O[s].syy[0];
-            d[3 * LR] = O[s].sxz[0];
-            d[4 * LR] = O[s].syz[0];
-        }
-    }
-}
-
-// this step's stresses of slots [S0, S1) at plane m, as written by the
-// absorbing-layer stress launch (planes outside the interior box)
-template <typename T, int NS, int S0, int S1>
-__device__ __forceinline__ void fused_fresh(const StepArgs<T>& A, const FSlots<T, NS>& F, int m,
-                                            SOut<typename Arith<T>::R, 1> (&O)[NS])
-{
-#pragma unroll
-    for (int s = S0; s < S1; ++s) {
-        const unsigned o = cell(A, m, F.L[s].p0);
-        O[s].sxx[0] = el(A.fo[3], o);
-        O[s].syy[0] = el(A.fo[4], o);
-        O[s].szz[0] = el(A.fo[5], o);
-        O[s].sxy[0] = el(A.fo[6], o);
-        O[s].sxz[0] = el(A.fo[7], o);
-        O[s].syz[0] = el(A.fo[8], o);
-    }
-}
-
-template <typename T, bool ENERGY, int NU, int NH, bool EDGE>
-__device__ __forceinline__ void fused_body(const StepArgs<T>& A, const Box& B, int q0, int n, int kb, int ke,
-                                           typename Arith<T>::R* lds, double& ek, double& ep, Scales<T>& S)
-{
-    typedef typename Arith<T>::R R;
-    constexpr int NS = 2 * NH + NU;
-    const int tid = threadIdx.x, nx = A.nx, LR = A.ldsreg;
-    FSlots<T, NS> F;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        const int c = tid + kFusedThreads * (s - NH);
-        F.c[s] = c;
-        F.act[s] = c >= -nx && c < n + nx;
-        F.wact[s] = __ballot(F.act[s]) != 0;
-        F.own[s] = c >= 0 && c < n;
-        const int p = q0 + (F.act[s] ? c : 0);
-        F.inb[s] = p >= B.p0 && p < B.p1;
-        make_lane<T, kInterior, 1>(A, p, B.p1, F.L[s]);
-        lane_tabs<T, kInterior, 1>(A, F.L[s], F.LT[s]);
-    }
-    SOut<R, 1> O[NS];
-    VQueue<R, 1> VQ[NU];
-    // own sxz, syz of the plane below the segment
-    if (kb - 1 >= B.k0) {
-        fused_qinit<T, NS, NH, NH + NU>(A, F, kb - 1);
-        fused_plane<T, NS, NH, NH + NU, EDGE, false>(A, F, kb - 1, false, nullptr, LR, O, S);
-        fused_qinit<T, NS, 0, NH>(A, F, kb);
-        fused_qinit<T, NS, NH + NU, NS>(A, F, kb);
-    } else {
-        fused_fresh<T, NS, NH, NH + NU>(A, F, kb - 1, O);
-        fused_qinit<T, NS, 0, NS>(A, F, kb);
-    }
-#pragma unroll
-    for (int u = 0; u < NU; ++u) {
-        VQ[u].sxz_m[0] = O[NH + u].sxz[0];
-        VQ[u].syz_m[0] = O[NH + u].syz[0];
-        VQ[u].rho_c[0] = F.Q[NH + u].rho_c[0];
-    }
-    // plane kb: the region into LDS, own cells stored
-    fused_plane<T, NS, 0, NS, EDGE, true>(A, F, kb, true, lds + (kb & 1) * 5 * LR, LR, O, S);
-#pragma unroll
-    for (int u = 0; u < NU; ++u) VQ[u].szz_c[0] = O[NH + u].szz[0];
-    __syncthreads();
-    for (int k = kb; k < ke; ++k) {
-        const int m = k + 1;
-        // own cells' velocity-pass loads of plane k (old v, density, x memory variables)
-        VPlane<R, 1> V[NU];
-        const bool mx = A.w[0] > 0;
-#pragma unroll
-        for (int u = 0; u < NU; ++u) {
-            const Lane<1>& L = F.L[NH + u];
-            const unsigned o = cell(A, k, L.p0);
-            V[u].vx[0] = el(A.fi[0], o);
-            V[u].vy[0] = el(A.fi[1], o);
-            V[u].vz[0] = F.Q[NH + u].vz_m[0];
-            V[u].rhoW[0] = el(A.rho, o - 1);
-            V[u].rhoN[0] = el(A.rho, o + unsigned(nx));
-            V[u].rho_u[0] = F.Q[NH + u].rho_c[0];
-            if (ENERGY) {
-                V[u].cp[0] = el(A.cp, o);
-                V[u].cs[0] = el(A.cs, o);
-            }
-            const unsigned om = xmem_offset(A, L, 0, k);
-            V[u].mx[0][0] = el((mx ? A.mi[9] : A.fi[0]), om);
-            V[u].mx[1][0] = el((mx ? A.mi[12] : A.fi[0]), om);
-            V[u].mx[2][0] = el((mx ? A.mi[15] : A.fi[0]), om);
-        }
-        // stresses of plane k+1
-        if (m < ke)  // the region, own cells stored
-            fused_plane<T, NS, 0, NS, EDGE, true>(A, F, m, true, lds + (m & 1) * 5 * LR, LR, O, S);
-        else if (m < B.k1)  // above the segment: the own cells' szz only, not stored
-            fused_plane<T, NS, NH, NH + NU, EDGE, false>(A, F, m, false, nullptr, LR, O, S);
-        else  // above the interior box
-            fused_fresh<T, NS, NH, NH + NU>(A, F, m, O);
-        // velocities of plane k
-        const R* lb = lds + (k & 1) * 5 * LR + nx;
-#pragma unroll
-        for (int u = 0; u < NU; ++u) {
-            const int s = NH + u;
-            if (!F.own[s]) continue;
-            const int c = F.c[s];
-            VPlane<R, 1>& P = V[u];
-            P.sxx[0] = lb[c];
-            P.sxxW[0] = lb[c - 1];
-            P.sxy[0] = lb[LR + c];
-            P.sxyS[0] = lb[LR + c - nx];
-            P.sxyE[0] = lb[LR + c + 1];
-            P.syy[0] = lb[2 * LR + c];
-            P.syyN[0] = lb[2 * LR + c + nx];
-            P.sxz[0] = lb[3 * LR + c];
-            P.sxzE[0] = lb[3 * LR + c + 1];
-            P.syz[0] = lb[4 * LR + c];
-            P.syzS[0] = lb[4 * LR + c - nx];
-            P.szz_u[0] = O[s].szz[0];
-            velocity_update<T, kInterior, ENERGY, 1, R, true>(A, F.L[s], F.LT[s], k, P, VQ[u], ek, ep, S);
-            VQ[u].sxz_m[0] = P.sxz[0];
-            VQ[u].syz_m[0] = P.syz[0];
-            VQ[u].szz_c[0] = O[s].szz[0];
-            VQ[u].rho_c[0] = P.rho_u[0];
-        }
-        __syncthreads();
-    }
-}
-
-template <typename T, bool ENERGY, int NU, int NH>
-__global__ void __launch_bounds__(kFusedThreads) k_fused(const StepArgs<T> A)
-{
-    typedef typename Arith<T>::R R;
-    extern __shared__ float smem[];
-    const int tid = threadIdx.x, nx = A.nx;
-    const Box B = A.box[0];
-    int chunk, zs;
-    tile_of(B, xcd_block(blockIdx.x, gridDim.x), chunk, zs);
-    const int q0 = B.p0 + chunk * A.tile, n = min(A.tile, B.p1 - q0);
-    const int kb = B.k0 + zs * B.zseg, ke = min(kb + B.zseg, B.k1);
-    Scales<T> S;
-    load_scales(A, S, true);
-    R* lds = reinterpret_cast<R*>(smem);
-    double ek = 0.0, ep = 0.0;
-    // tiles whose halo rows leave the interior box take the slow per-slot path
-    if (q0 - nx < B.p0 || q0 + n + nx > B.p1)
-        fused_body<T, ENERGY, NU, NH, true>(A, B, q0, n, kb, ke, lds, ek, ep, S);
-    else
-        fused_body<T, ENERGY, NU, NH, false>(A, B, q0, n, kb, ke, lds, ek, ep, S);
-    if (ENERGY) {  // per-block FP64 partials, summed in block order by the tail kernel
-        __shared__ double red[2][kFusedThreads / 64];
-        ek = wave_sum(ek);
-        ep = wave_sum(ep);
-        const int lane = tid & 63, wid = tid >> 6;
-        if (lane == 0) {
-            red[0][wid] = ek;
-            red[1][wid] = ep;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            double a = 0.0, bsum = 0.0;
-            for (int w = 0; w < kFusedThreads / 64; ++w) {
-                a += red[0][w];
-                bsum += red[1][w];
-            }
-            A.epart[2 * blockIdx.x] = a;
-            A.epart[2 * blockIdx.x + 1] = bsum;
-        }
-    }
-}
-
-template <typename T, bool E, int NH>
-static void launch_fused_h(const StepArgs<T>& a, size_t lds, hipStream_t s)
-{
-    if (a.tile <= kFusedThreads)
-        hipLaunchKernelGGL((k_fused<T, E, 1, NH>), dim3(a.nblocks), dim3(kFusedThreads), lds, s, a);
-    else if (a.tile <= 2 * kFusedThreads)
-        hipLaunchKernelGGL((k_fused<T, E, 2, NH>), dim3(a.nblocks), dim3(kFusedThreads), lds, s, a);
-    else
-        hipLaunchKernelGGL((k_fused<T, E, 4, NH>), dim3(a.nblocks), dim3(kFusedThreads), lds, s, a);
-}
-
-template <typename T, bool E>
-static void launch_fused_e(const StepArgs<T>& a, hipStream_t s)
-{
-    const size_t lds = fused_lds_bytes<T>(a.nx, a.tile);
-    if (lds == 0 || a.ldsreg < a.tile + 2 * a.nx) return;  // refused on the host first
-    if (a.nx <= kFusedThreads)
-        launch_fused_h<T, E, 1>(a, lds, s);
-    else
-        launch_fused_h<T, E, 2>(a, lds, s);
-}
-
-template <typename T>
-void launch_fused(const StepArgs<T>& a, bool energy, hipStream_t s)
-{
-    if (a.nblocks <= 0) return;
-    if (energy)
-        launch_fused_e<T, true>(a, s);
-    else
-        launch_fused_e<T, false>(a, s);
-}
-
-template void launch_fused<float>(const StepArgs<float>&, bool, hipStream_t);
-template void launch_fused<double>(const StepArgs<double>&, bool, hipStream_t);
-template size_t fused_lds_bytes<float>(int, int);
-template size_t fused_lds_bytes<double>(int, int);
-template size_t fused_lds_bytes<half_t>(int, int);
 
 template void launch_stress<float>(const StepArgs<float>&, hipStream_t);
 template void launch_stress<double>(const StepArgs<double>&, hipStream_t);

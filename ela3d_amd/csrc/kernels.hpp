@@ -67,8 +67,13 @@ struct StepArgs {
     int nbox, nblocks, block;
     int vec;                 // consecutive xy positions per lane: 1, 2 or 4
     int mode;                // kInterior or kSlab (every box of the launch), kMixed or kFused
-    int tile;                // kFused: cells of the flattened interior box per workgroup
-    int ldsreg;              // kFused: cells of one LDS stress plane (tile + 2*nx)
+    // kFused launches address both field sets from one pointer per array: fi[q]
+    // and mi[q] hold the first copy of each array (the second copy follows it),
+    // a step reads at element offset fdin (fields) / mdin (memory variables) from
+    // them and writes at fdout / mdout; fo and mo are unused.
+    unsigned fdin, fdout, mdin, mdout;
+    int tile;                // kFused: threads per workgroup = region cells (own cells + 2 halo rows)
+    int ldsreg;              // kFused: cells of one LDS stress plane (= tile)
 };
 
 template <typename T>
@@ -103,10 +108,15 @@ void launch_stress(const StepArgs<T>& a, hipStream_t s);
 template <typename T>
 void launch_velocity(const StepArgs<T>& a, bool energy, hipStream_t s);
 // fused step over the interior box; needs double-buffered fields and the
-// absorbing-layer stress launch of the same step to have run
+// absorbing-layer stress launch of the same step to have run.  A workgroup has
+// `tile` threads (at most kFusedMaxThreads, kFusedMaxThreadsF64 in FP64), one per
+// cell of its region: tile - 2*nx own cells and one halo row below and above.
+constexpr int kFusedMaxThreads = 1024;
+constexpr int kFusedMaxThreadsF64 = 512;
 template <typename T>
 void launch_fused(const StepArgs<T>& a, bool energy, hipStream_t s);
-// LDS bytes of a fused launch (0: not launchable with these sizes)
+// LDS bytes of a fused launch (0: not launchable with these sizes: tile not a
+// multiple of 64 in [64, max], or fewer than 64 own cells)
 template <typename T>
 size_t fused_lds_bytes(int nx, int tile);
 template <typename T>

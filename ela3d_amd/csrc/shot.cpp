@@ -11,6 +11,7 @@
 //   FP64: C-PML a/b/K tables [3 axes][6][n], wavelet [nstep], seismograms
 //   [nsamples][nrec][3], energy [nenergy][2], energy partials.
 #include "shot.hpp"
+#include "geom.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -79,8 +80,8 @@ void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& slab, LaunchGeom& 
     interior.block = t.block;
     interior.vec = t.vec;
     interior.mode = t.fused ? kFused : t.merged ? kMixed : kInterior;
-    if (t.fused) {  // one workgroup per (tile, z segment): 256 threads over `tile` cells
-        interior.block = 256;
+    if (t.fused) {  // one workgroup per (tile, z segment): `tile` threads, one per region cell
+        interior.block = t.tile;
         interior.vec = 1;
         interior.tile = t.tile;
     }
@@ -92,7 +93,8 @@ void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& slab, LaunchGeom& 
             if (t.fused) {
                 Box& b = interior.box[interior.nbox++];
                 b.p0 = r[q].p0; b.p1 = r[q].p1; b.k0 = r[q].k0; b.k1 = r[q].k1;
-                b.nchunk = (r[q].p1 - r[q].p0 + t.tile - 1) / t.tile;
+                const int own = fused_own(t.tile, p.cfg.nx);  // own cells per workgroup
+                b.nchunk = (r[q].p1 - r[q].p0 + own - 1) / own;
                 b.zseg = std::max(1, t.zseg_interior);
                 b.blk0 = 0;
                 b.mode = kInterior;
@@ -112,10 +114,16 @@ std::string fused_unsupported(const ShotPlan& p, int tile)
     if (c.storage == ELA3D_STORAGE_FP16) return "the fused schedule runs FP32 and FP64 storage only";
     if (p.ax[1].lo < 1 || p.ax[2].lo < 1 || p.ax[1].hi > c.ny - 1 || p.ax[2].hi > c.nz - 1)
         return "the fused schedule needs absorbing layers on every y and z face";
-    if (tile < 64 || tile > 1024 || tile % 64) return "fused tile must be a multiple of 64 in [64, 1024]";
+    {  // both copies of an array must lie within 4 GiB of its first byte (32-bit offsets)
+        const size_t elem = storage_bytes(c.storage);
+        const size_t arr = size_t(plane_stride(c.nx, c.ny)) * c.nz + 2 * size_t(guard_elems(c.nx));
+        if (2 * (arr * elem + 256) >= (size_t(1) << 32)) return "the fused schedule needs field arrays below 2 GiB";
+    }
     const size_t lds = c.storage == ELA3D_STORAGE_FP64 ? fused_lds_bytes<double>(c.nx, tile)
                                                        : fused_lds_bytes<float>(c.nx, tile);
-    if (lds == 0) return "fused tile and nx need more LDS than one workgroup has";
+    if (lds == 0)
+        return "fused tile (threads per workgroup) must be a multiple of 64, at most 1024 (512 in FP64), "
+               "and leave at least 64 own cells after two halo rows of nx";
     BoxRange r[5];
     const int n = split_boxes(p, r);
     for (int q = 0; q < n; ++q)
@@ -142,14 +150,19 @@ ArenaLayout arena_layout(const ShotPlan& p, int elem, int nepart, bool dbuf)
     ArenaLayout L;
     const int64_t nxyp = plane_stride(c.nx, c.ny);
     const size_t padded = size_t(nxyp) * c.nz + 2 * size_t(guard_elems(c.nx));  // guard | planes | guard
+    // The second set of a double-buffered shot sits right after the first copy of
+    // each array, one array size above it: the fused kernel addresses both sets
+    // from one base pointer per array with a 32-bit offset.
     size_t off = 0;
-    for (int q = 0; q < 9; ++q) { L.f[q] = off; off += al(padded * elem); }
-    for (int q = 0; q < 18; ++q) { L.m[q] = off; off += al(mem_elems(p, kMemAxis[q]) * elem); }
-    for (int q = 0; q < 9; ++q) {  // second set of a double-buffered shot
+    for (int q = 0; q < 9; ++q) {
+        L.f[q] = off;
+        off += al(padded * elem);
         L.f2[q] = L.f[q];
         if (dbuf) { L.f2[q] = off; off += al(padded * elem); }
     }
     for (int q = 0; q < 18; ++q) {
+        L.m[q] = off;
+        off += al(mem_elems(p, kMemAxis[q]) * elem);
         L.m2[q] = L.m[q];
         if (dbuf) { L.m2[q] = off; off += al(mem_elems(p, kMemAxis[q]) * elem); }
     }
@@ -772,6 +785,21 @@ private:
             a.mi[q] = d_m_[in][q];
             a.mo[q] = d_m_[out][q];
         }
+        if (g.mode == kFused) {  // one base per array, the sets at 32-bit element offsets
+            const unsigned df = unsigned(d_f_[1][0] - d_f_[0][0]), dm = unsigned(d_m_[1][0] - d_m_[0][0]);
+            for (int q = 0; q < 9; ++q) {
+                a.fi[q] = d_f_[0][q] - guard_;
+                a.fo[q] = nullptr;
+            }
+            for (int q = 0; q < 18; ++q) {
+                a.mi[q] = d_m_[0][q];
+                a.mo[q] = nullptr;
+            }
+            a.fdin = in ? df : 0u;
+            a.fdout = out ? df : 0u;
+            a.mdin = in ? dm : 0u;
+            a.mdout = out ? dm : 0u;
+        }
         if (std::is_same<T, half_t>::value && in != out)  // FP16 kernels store through fi/mi
             throw Error{ELA3D_E_INVALID, "FP16 storage updates in place"};
         a.cp = d_model_[0] - guard_;
@@ -804,7 +832,7 @@ private:
         a.block = g.block;
         a.mode = g.mode;
         a.tile = g.tile;
-        a.ldsreg = g.tile + 2 * c.nx;
+        a.ldsreg = g.tile;
         return a;
     }
 

@@ -237,7 +237,7 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
         best = cand[pick];
         double best_wall = wall3[pick];
         if (fusable) {  // fused interior step: tile x z segment
-            static const int tiles[] = {512, 1024};
+            static const int tiles[] = {512, 768, 1024};
             static const int fzs[] = {4, 8, 16, 32};
             for (int tl : tiles) {
                 if (!ela3d::fused_unsupported(plan, tl).empty()) continue;

@@ -48,7 +48,7 @@ def draw(seed):
                      overlap=bool(rng.random() < 0.3), merged=bool(rng.random() < 0.25),
                      dbuf=bool(rng.random() < 0.3))
         if rng.random() < 0.2 and cfg.storage == STORAGE_FP32 and npml > 0:
-            shape.update(fused=True, tile=pick([64, 256, 512]), dbuf=True, merged=False)
+            shape.update(fused=True, tile=pick([256, 512, 1024]), dbuf=True, merged=False)
         return cfg, shape
 
 
