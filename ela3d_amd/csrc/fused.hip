@@ -18,10 +18,13 @@
 //    column's z neighbours and the values loaded at plane m-1).
 //  * two LDS plane buffers alternate, so one barrier per plane suffices.
 // Every thread keeps one cell, so the register state is one stress z queue and
-// one plane of velocity inputs: about 64 VGPRs, 8 waves per SIMD.  Every load
-// of a plane is issued before its arithmetic.  The x C-PML recurrences run
-// branch-free (a = b = 0, 1/K = 1 outside the layer leave the derivative
-// unchanged exactly); only their stores are conditional.
+// one plane of velocity inputs (74 VGPRs in FP32: a 1024-thread workgroup per
+// CU, or two of 768 threads).  Every thread issues the same loads every plane,
+// at a safe offset where it needs none, so that no wait for one branch's loads
+// is placed where it would also drain another's.  The x C-PML coefficients come
+// from an LDS copy of the x tables (a = 0, b = 1, 1/K = 1 outside the layer,
+// where the recurrence leaves the derivative unchanged exactly); only the
+// memory-variable stores are conditional.
 //
 // Updates follow the reference CPU loops (seismicCPML_3d_isotropic_simpelmodel.cpp):
 // normal stress CPU:377-413, sigma_xy CPU:415-442, sigma_xz/yz CPU:444-499,
@@ -56,10 +59,20 @@ struct FQueue {
     R vx, vy, cs, rho, vz_m;
 };
 
-// A cell's x C-PML coefficients (a, b, 1/K on the full and the half grid).
+// A cell's x C-PML coefficients, read from the workgroup's LDS copy of the x
+// tables ([6][nx]: a, b, 1/K on the full grid, then on the half grid; a = 0,
+// b = 1, 1/K = 1 outside the layer, where the recurrence then leaves the
+// derivative unchanged exactly).  t points at column i of row 0.
 template <typename R>
 struct FTab {
-    R a, b, K, ah, bh, Kh;
+    const R* t;
+    int n;
+    __device__ __forceinline__ R a() const { return t[0]; }
+    __device__ __forceinline__ R b() const { return t[n]; }
+    __device__ __forceinline__ R K() const { return t[2 * n]; }
+    __device__ __forceinline__ R ah() const { return t[3 * n]; }
+    __device__ __forceinline__ R bh() const { return t[4 * n]; }
+    __device__ __forceinline__ R Kh() const { return t[5 * n]; }
 };
 
 // What the velocity update of plane k needs from the stress pass of plane k
@@ -71,44 +84,44 @@ struct FSaved {
     R cp, cs;                              // energy steps only
 };
 
-// o: cell offset (model arrays); oi = o + fdin (fields of the set read);
-// omi: x memory-variable offset in the set read
+// Loads of one plane of a cell.  om: model arrays and velocities at a cell
+// offset (fields at om + fdin); os: the six stresses (the set read, or the set
+// being written where another launch computed them); omi: x memory variables.
+// Every thread issues the same loads every plane (at a safe offset where it
+// needs none), so the wait for one plane's loads never drains the next plane's.
 template <typename T>
-__device__ __forceinline__ void fload(const StepArgs<T>& A, unsigned o, unsigned oi, unsigned omi, bool pml,
+__device__ __forceinline__ void fload(const StepArgs<T>& A, unsigned om, unsigned os, unsigned omi,
                                       FLoad<typename Arith<T>::R>& L)
 {
     typedef typename Arith<T>::R R;
     const unsigned nxyp = A.nxyp, nx = A.nx;
+    const unsigned oi = om + A.fdin;
     const T *vx = A.fi[0], *vy = A.fi[1], *vz = A.fi[2];
     L.vz_c = el(vz, oi);
-    L.cp_c = (R)el(A.cp, o);
+    L.cp_c = (R)el(A.cp, om);
     L.vx_u = el(vx, oi + nxyp);
     L.vy_u = el(vy, oi + nxyp);
-    L.cs_u = (R)el(A.cs, o + nxyp);
-    L.rho_u = (R)el(A.rho, o + nxyp);
+    L.cs_u = (R)el(A.cs, om + nxyp);
+    L.rho_u = (R)el(A.rho, om + nxyp);
     L.vxE = el(vx, oi + 1);
     L.vyS = el(vy, oi - nx);
     L.vyW = el(vy, oi - 1);
     L.vxN = el(vx, oi + nx);
     L.vzW = el(vz, oi - 1);
     L.vzN = el(vz, oi + nx);
-    L.csW = (R)el(A.cs, o - 1);
-    L.csN = (R)el(A.cs, o + nx);
-    L.rhoW = (R)el(A.rho, o - 1);
-    L.rhoN = (R)el(A.rho, o + nx);
-    L.sxx = el(A.fi[3], oi);
-    L.syy = el(A.fi[4], oi);
-    L.szz = el(A.fi[5], oi);
-    L.sxy = el(A.fi[6], oi);
-    L.sxz = el(A.fi[7], oi);
-    L.syz = el(A.fi[8], oi);
-    if (pml) {  // uniform: the grid has an x C-PML
-        L.m0 = el(A.mi[0], omi);
-        L.m1 = el(A.mi[3], omi);
-        L.m2 = el(A.mi[5], omi);
-    } else {
-        L.m0 = L.m1 = L.m2 = 0;
-    }
+    L.csW = (R)el(A.cs, om - 1);
+    L.csN = (R)el(A.cs, om + nx);
+    L.rhoW = (R)el(A.rho, om - 1);
+    L.rhoN = (R)el(A.rho, om + nx);
+    L.sxx = el(A.fi[3], os);
+    L.syy = el(A.fi[4], os);
+    L.szz = el(A.fi[5], os);
+    L.sxy = el(A.fi[6], os);
+    L.sxz = el(A.fi[7], os);
+    L.syz = el(A.fi[8], os);
+    L.m0 = el(A.mi[0], omi);
+    L.m1 = el(A.mi[3], omi);
+    L.m2 = el(A.mi[5], omi);
 }
 
 // New stresses of one cell from its loads and queue.  iN: i <= nx-2 (normal
@@ -132,7 +145,7 @@ __device__ __forceinline__ void fstress(const StepArgs<T>& A, const FLoad<typena
         R dvx_dx = (L.vxE - Q.vx) * idx;
         const R dvy_dy = (Q.vy - L.vyS) * idy;
         const R dvz_dz = (L.vz_c - Q.vz_m) * idz;
-        const R m = cpml_step(L.m0, X.ah, X.bh, X.Kh, dvx_dx);
+        const R m = cpml_step(inx ? L.m0 : (R)0, X.ah(), X.bh(), X.Kh(), dvx_dx);
         if (store && inx && iN) el(const_cast<T*>(A.mi[0]), omo) = m;
         s[0] = iN ? DT_l2m * dvx_dx + DT_l * (dvy_dy + dvz_dz) + L.sxx : L.sxx;
         s[1] = iN ? DT_l * (dvx_dx + dvz_dz) + DT_l2m * dvy_dy + L.syy : L.syy;
@@ -145,7 +158,7 @@ __device__ __forceinline__ void fstress(const StepArgs<T>& A, const FLoad<typena
         const R DT_mu = dt * (rh * vs * vs);
         R dvy_dx = (Q.vy - L.vyW) * idx;
         const R dvx_dy = (L.vxN - Q.vx) * idy;
-        const R m = cpml_step(L.m1, X.a, X.b, X.K, dvy_dx);
+        const R m = cpml_step(inx ? L.m1 : (R)0, X.a(), X.b(), X.K(), dvy_dx);
         if (store && inx && iS) el(const_cast<T*>(A.mi[3]), omo) = m;
         s[3] = iS ? DT_mu * (dvy_dx + dvx_dy) + L.sxy : L.sxy;
     }
@@ -156,7 +169,7 @@ __device__ __forceinline__ void fstress(const StepArgs<T>& A, const FLoad<typena
         const R DT_mu = dt * (rh * vs * vs);
         R dvz_dx = (L.vz_c - L.vzW) * idx;
         const R dvx_dz = (L.vx_u - Q.vx) * idz;
-        const R m = cpml_step(L.m2, X.a, X.b, X.K, dvz_dx);
+        const R m = cpml_step(inx ? L.m2 : (R)0, X.a(), X.b(), X.K(), dvz_dx);
         if (store && inx && iS) el(const_cast<T*>(A.mi[5]), omo) = m;
         s[4] = iS ? DT_mu * (dvz_dx + dvx_dz) + L.sxz : L.sxz;
     }
@@ -199,19 +212,17 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
     const int cx = i < A.lo[0] ? i : (i >= A.hi[0] ? A.lo[0] + (i - A.hi[0]) : -1);
     const bool pml = A.w[0] > 0;
     const bool inx = cx >= 0;
-    FTab<R> X;
-    X.a = 0; X.b = 0; X.K = 1; X.ah = 0; X.bh = 0; X.Kh = 1;
-    if (inx) {
-        if constexpr (std::is_same<T, double>::value) {
-            const double* tx = A.tab[0];
-            X.a = tx[i]; X.b = tx[nx + i]; X.K = tx[2 * nx + i];
-            X.ah = tx[3 * nx + i]; X.bh = tx[4 * nx + i]; X.Kh = tx[5 * nx + i];
-        } else {
-            const float* tx = A.tabf;
-            X.a = tx[i]; X.b = tx[nx + i]; X.K = tx[2 * nx + i];
-            X.ah = tx[3 * nx + i]; X.bh = tx[4 * nx + i]; X.Kh = tx[5 * nx + i];
-        }
+    R* xtab = lds + 2 * 5 * LR;  // the x tables, after the two plane buffers
+    for (int q = t; q < 6 * nx; q += blockDim.x) {
+        if constexpr (std::is_same<T, double>::value)
+            xtab[q] = A.tab[0][q];
+        else
+            xtab[q] = A.tabf[q];
     }
+    FTab<R> X;
+    X.t = xtab + i;
+    X.n = nx;
+    __syncthreads();
     // x memory-variable offset of this cell at plane k (offset 0 outside the layer)
     const unsigned wx = unsigned(A.w[0]);
     const unsigned om0 = inx ? unsigned(j) * wx + unsigned(cx) : 0u;
@@ -228,6 +239,10 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
         if (m < kb - 1 || m > ke || m < B.k0 || m >= B.k1) return false;
         return (m == kb - 1 || m == ke) ? own : inb;
     };
+    // whether this thread needs plane m's new stresses at all
+    auto needs = [&](int m) {
+        return m >= kb - 1 && m <= ke && (own || (m >= kb && m < ke && act));
+    };
 
     FQueue<R> Q;
     auto qinit = [&](int m) {
@@ -238,9 +253,18 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
         Q.rho = (R)el(A.rho, o);
         Q.vz_m = el(A.fi[2], oi - unsigned(A.nxyp));
     };
+    // plane m's loads: a computed plane's stress inputs, or this step's stresses
+    // where the layer launch computed them (halo rows outside the box, own cells
+    // above or below it) into the old-stress slots; other threads load at the
+    // in-box cell of plane kb, which is never read
+    const unsigned osafe = cell_offset(A.g0, A.nxyp, kb, p);
     auto load = [&](int m, FLoad<R>& L) {
+        const bool cm = computes(m);
         const unsigned o = cell_offset(A.g0, A.nxyp, m, p);
-        fload<T>(A, o, o + A.fdin, om0 + unsigned(m) * omk + A.mdin, pml, L);
+        const unsigned om = cm ? o : osafe;
+        const unsigned os = cm ? o + A.fdin : (needs(m) ? o + A.fdout : osafe + A.fdin);
+        const unsigned omi = (cm && inx ? om0 + unsigned(m) * omk : 0u) + A.mdin;
+        fload<T>(A, om, os, omi, L);
     };
     // stress of plane m from its loads: new values into s; stored when `store`;
     // the velocity inputs of plane m into sv; the queue advances to m+1
@@ -261,12 +285,6 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
         if (ENERGY) { sv.cp = L.cp_c; sv.cs = Q.cs; }
         Q.vx = L.vx_u; Q.vy = L.vy_u; Q.cs = L.cs_u; Q.rho = L.rho_u; Q.vz_m = L.vz_c;
     };
-    // this step's stresses of a cell outside the interior box (the layer launch wrote them)
-    auto fresh = [&](int m, R (&s)[6]) {
-        const unsigned oo = cell_offset(A.g0, A.nxyp, m, p) + A.fdout;
-        s[0] = el(A.fi[3], oo); s[1] = el(A.fi[4], oo); s[2] = el(A.fi[5], oo);
-        s[3] = el(A.fi[6], oo); s[4] = el(A.fi[7], oo); s[5] = el(A.fi[8], oo);
-    };
     auto to_lds = [&](int m, const R (&s)[6]) {
         R* b = lds + (m & 1) * 5 * LR + li;
         b[kLxx * LR] = s[0];
@@ -278,38 +296,36 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
 
     // The first plane this thread computes is kb-1 (own cells inside the box) or kb.
     const int m0 = computes(kb - 1) ? kb - 1 : kb;
-    FLoad<R> Lc;  // loads of the plane being computed; the next plane's are issued first
-    if (computes(m0)) {
-        qinit(m0);
-        load(m0, Lc);
-    }
+    if (computes(m0)) qinit(m0);
     R pv_sxz = 0, pv_syz = 0;  // new sxz, syz of the own cell one plane below
     FSaved<R> S, N;            // velocity inputs of the plane whose velocity is next
     double ek = 0.0, ep = 0.0;
     const bool face = A.dirichlet && (i == 0 || i == nx - 1);  // the interior box has no j or k face
     const int em = A.emargin;
     const bool in_e_xy = i >= em && i < nx - 1 - em && j >= em && j < ny - 1 - em;
-    for (int m = kb - 1; m <= ke; ++m) {
+    // one plane: the next plane's loads first, then plane m's stresses from Lc,
+    // then the velocities of plane m-1
+    auto plane = [&](const int m) {
         const int k = m - 1;  // velocity plane of this iteration (m > kb)
-        FLoad<R> Ln;
-        if (m + 1 <= ke && computes(m + 1)) load(m + 1, Ln);
-        // velocity-pass loads of the own cell at plane k: its x memory variables
+        // velocity-pass loads of plane k (the own cell's x memory variables), then
+        // the next plane's loads: the counter of outstanding loads is in order,
+        // so these are waited for without waiting for the next plane's
         const unsigned omv = om0 + unsigned(k) * omk;
-        R mv0 = 0, mv1 = 0, mv2 = 0;
-        if (m > kb && own && pml) {
-            mv0 = el(A.mi[9], omv + A.mdin);
-            mv1 = el(A.mi[12], omv + A.mdin);
-            mv2 = el(A.mi[15], omv + A.mdin);
-        }
+        const unsigned omvi = (m > kb && own && inx ? omv : 0u) + A.mdin;
+        R mv0 = el(A.mi[9], omvi), mv1 = el(A.mi[12], omvi), mv2 = el(A.mi[15], omvi);
+        FLoad<R> Lc;
+        load(m, Lc);
+        if (!inx) mv0 = mv1 = mv2 = 0;
         // stresses of plane m: computed, or this step's values where the layer
-        // launch computed them (halo rows outside the box, own cells above or
-        // below it)
+        // launch computed them
         const bool region = m >= kb && m < ke;
-        R sn[6] = {0, 0, 0, 0, 0, 0};
-        if (computes(m))
+        R sn[6];
+        if (computes(m)) {
             stress(m, Lc, own && region, sn, N);
-        else if (own || (region && act))
-            fresh(m, sn);
+        } else {
+            sn[0] = Lc.sxx; sn[1] = Lc.syy; sn[2] = Lc.szz;
+            sn[3] = Lc.sxy; sn[4] = Lc.sxz; sn[5] = Lc.syz;
+        }
         if (region && act) to_lds(m, sn);
         if (m == kb - 1) {  // sxz, syz below the segment
             pv_sxz = sn[4];
@@ -336,7 +352,7 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
                 R d1 = (sxx - sxxW) * idx;
                 const R d2 = (sxy - sxyS) * idy;
                 const R d3 = (sxz - pv_sxz) * idz;
-                const R mm = cpml_step(mv0, X.a, X.b, X.K, d1);
+                const R mm = cpml_step(mv0, X.a(), X.b(), X.K(), d1);
                 if (inx && iS) el(mw[9], omo) = mm;
                 if (iS) nvx = DT_over_rho * (d1 + d2 + d3) + S.vx;
             }
@@ -346,7 +362,7 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
                 R d1 = (sxyE - sxy) * idx;
                 const R d2 = (syyN - syy) * idy;
                 const R d3 = (syz - pv_syz) * idz;
-                const R mm = cpml_step(mv1, X.ah, X.bh, X.Kh, d1);
+                const R mm = cpml_step(mv1, X.ah(), X.bh(), X.Kh(), d1);
                 if (inx && iN) el(mw[12], omo) = mm;
                 if (iN) nvy = DT_over_rho * (d1 + d2 + d3) + S.vy;
             }
@@ -356,7 +372,7 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
                 R d1 = (sxzE - sxz) * idx;
                 const R d2 = (syz - syzS) * idy;
                 const R d3 = (szz_u - S.szz) * idz;
-                const R mm = cpml_step(mv2, X.ah, X.bh, X.Kh, d1);
+                const R mm = cpml_step(mv2, X.ah(), X.bh(), X.Kh(), d1);
                 if (inx && iN) el(mw[15], omo) = mm;
                 if (iN) nvz = DT_over_rho * (d1 + d2 + d3) + S.vz;
             }
@@ -385,11 +401,14 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
             pv_syz = syz;
         }
         if (m >= kb) S = N;
-        Lc = Ln;
         // plane m's LDS buffer is read by the velocity update of the next
         // iteration; the buffer that iteration writes was last read by this one
         if (m >= kb && m < ke) __syncthreads();
-    }
+    };
+    // Planes kb-1 .. ke, two per trip so that the two load buffers alternate
+    // without a copy (a copy of a register a load is still filling waits for
+    // that load); an odd count starts with one idle plane kb-2.
+    for (int m = kb - 1; m <= ke; ++m) plane(m);
     if (ENERGY) {  // per-block FP64 partials, summed in block order by the tail kernel
         __shared__ double red[2][NT / 64];
         ek = wave_sum(ek);
@@ -421,7 +440,7 @@ size_t fused_lds_bytes(int nx, int tile)
     typedef typename Arith<T>::R R;
     const int maxt = std::is_same<T, double>::value ? kFusedMaxThreadsF64 : kFusedMaxThreads;
     if (nx < 1 || tile < 64 || tile > maxt || tile % 64 != 0 || fused_own(tile, nx) < 64) return 0;
-    return size_t(2) * 5 * size_t(tile) * sizeof(R);
+    return (size_t(2) * 5 * size_t(tile) + 6 * size_t(nx)) * sizeof(R);  // plane buffers, x tables
 }
 
 template <typename T, bool E>

@@ -102,8 +102,9 @@ void geometry(const ShotPlan& p, const Tuning& t, LaunchGeom& slab, LaunchGeom& 
             } else {
                 add_box(interior, r[q], t.zseg_interior);
             }
-        } else {  // z layers march in short segments; the y layers as the interior
-            add_box(t.merged ? interior : slab, r[q], zlayer ? t.zseg_slab : t.zseg_interior);
+        } else {  // z layers march in short segments; the y layers as the interior (with the
+                  // fused interior step, whose segments are long, as the z layers)
+            add_box(t.merged ? interior : slab, r[q], (zlayer || t.fused) ? t.zseg_slab : t.zseg_interior);
         }
     }
 }
