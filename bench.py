@@ -62,9 +62,13 @@ def make_workload(name, nstep=None):
 
 
 def cpu_baseline(cfg, model, budget_s=20.0):
-    """Time the oracle's FP32 restatement of the reference loop on this host's cores."""
+    """Time the oracle's FP32 restatement of the reference loop on all of this host's
+    cores (OpenMP), and on one core (the reference's own execution model)."""
     import oracle
-    threads = min(os.cpu_count() or 1, 16)
+    nproc = os.cpu_count() or 1
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(avail, omp) if omp > 0 else avail  # the cores this job may use
     sub = cfg.replace(nstep=1000000)
     st = oracle.State(sub, np.float32)
     t = time.perf_counter()
@@ -81,8 +85,10 @@ def cpu_baseline(cfg, model, budget_s=20.0):
     w1 = time.perf_counter() - t
     return {
         "value": round(rate, 2), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+        "nproc": nproc, "cores_available": avail, "omp_num_threads": omp or None,
         "sample": (f"oracle FP32 restatement of the reference loop, same grid, {nsteps} steps "
-                   f"({wall:.1f} s) on {threads} threads; 1 thread: "
+                   f"({wall:.1f} s) on {threads} OpenMP threads (every core this job may use: "
+                   f"{avail} in its affinity mask, OMP_NUM_THREADS {omp or 'unset'}; nproc {nproc}); 1 thread: "
                    f"{cfg.cells / w1 / 1e6:.1f} Mcell-updates/s over 1 step"),
         "value_1thread": round(cfg.cells / w1 / 1e6, 2),
     }
@@ -105,12 +111,13 @@ def load_traffic(workload, kernel):
     return (None if k is None else k.get("hbm_bytes_per_dispatch")), d.get("launch_shape")
 
 
-def accuracy_run(workload):
-    """The workload's full NSTEP in its own storage and in FP64 storage (the GPU FP64
-    path, itself pinned to the CPU FP64 oracle by the parity tests): seismogram
-    relative L2 and the largest energy deviation relative to the FP64 energy peak."""
+def accuracy_run(workload, nstep=None):
+    """The workload (its full NSTEP, or the first `nstep` steps) in its own storage and
+    in FP64 storage (the GPU FP64 path, itself pinned to the CPU FP64 oracle by the
+    parity tests): seismogram relative L2 and the largest energy deviation relative to
+    the FP64 energy peak."""
     from ela3d_amd import STORAGE_FP64, Shot
-    cfg, model, _ = make_workload(workload)
+    cfg, model, _ = make_workload(workload, nstep)
     out = {}
     for name, c in (("run", cfg), ("fp64", cfg.replace(storage=STORAGE_FP64))):
         t = time.perf_counter()
@@ -125,6 +132,7 @@ def accuracy_run(workload):
     edev = float(np.max(np.abs(tot32 - tot64)) / max(np.max(tot64), 1e-300)) if tot64.size else None
     log(f"[bench] accuracy: seismogram rel L2 vs FP64 {rel:.3e}, max energy deviation {edev}")
     return {"nstep": cfg.nstep, "seismogram_rel_l2_vs_fp64": rel, "max_energy_dev_vs_fp64": edev,
+            "bar": 1e-4, "within_bar": bool(rel <= 1e-4),
             "reference": "same workload with FP64 storage on the GPU"}
 
 
@@ -163,6 +171,53 @@ def run_server(shots, arrivals=None, label="", **opts):
         for k in ("t_submit", "t_admit", "t_done"):
             r[k] -= base
     return makespan, res, stats
+
+
+def serving_summary(nshots=40, nstep_range=(100, 400), seed=11, max_active=8, steps_per_round=50,
+                    budget_gb=200.0, rate=4.0, edges=None):
+    """A shortened seeded serving stream with the full stream's edge mix: closed-loop
+    shots/s of the batched server, of one-at-a-time graph replay and of the plain
+    per-step-synchronous loop, and the batched server's p50/p99 latency under Poisson
+    arrivals (open loop)."""
+    import ela3d_amd
+    from ela3d_amd import workloads
+    edges = workloads.SERVING_EDGES if edges is None else tuple(edges)
+    specs = workloads.serving_stream(nshots=nshots, seed=seed, nstep_range=nstep_range, rate=rate,
+                                     edges=edges)
+    shots = [sp.materialize() for sp in specs]
+    for n in sorted({sp.edge for sp in specs}):
+        cfg, model = [sh for sh, sp in zip(shots, specs) if sp.edge == n][0]
+        ela3d_amd.autotune(cfg.replace(nstep=64), model, steps=4)
+    cells = sum(sp.cell_updates for sp in specs)
+    common = dict(hbm_budget_bytes=int(budget_gb * 1e9), steps_per_round=steps_per_round)
+    out = {"workload": (f"{nshots} seeded shots, cubic edges {list(edges)}, "
+                        f"NSTEP {nstep_range[0]}-{nstep_range[1]}, Poisson {rate} shots/s"),
+           "cell_updates": cells, "max_active": max_active}
+    runs = (("batched", dict(max_active=max_active)), ("graph_one_at_a_time", dict(max_active=1)),
+            ("plain_loop", dict(plain_loop=True)))
+    for name, opts in runs:
+        mk, res, st = run_server(shots, label=name, **common, **opts)
+        service = [r["t_done"] - r["t_admit"] for r in res]
+        d = {"shots_per_s": round(len(res) / mk, 4), "makespan_s": round(mk, 3),
+             "gcell_updates_per_s": round(cells / mk / 1e9, 3),
+             "failed": sum(1 for r in res if r["status"] != 0)}
+        if name != "batched":
+            lat = workloads.fifo_latencies([sp.arrival for sp in specs], service)
+            d["poisson_p50_s"] = round(percentile(lat, 50), 4)
+            d["poisson_p99_s"] = round(percentile(lat, 99), 4)
+        out[name] = d
+    mk, res, st = run_server(shots, arrivals=[sp.arrival for sp in specs], label="batched Poisson",
+                             max_active=max_active, **common)
+    lat = [r["t_done"] - r["t_submit"] for r in res]
+    out["batched"]["poisson_p50_s"] = round(percentile(lat, 50), 4)
+    out["batched"]["poisson_p99_s"] = round(percentile(lat, 99), 4)
+    out["latency"] = ("batched: measured open loop at the Poisson arrival times; one-at-a-time: "
+                      "FIFO (Lindley) over each shot's measured service time")
+    b = out["batched"]["shots_per_s"]
+    out["speedup_vs_graph_one_at_a_time"] = round(b / out["graph_one_at_a_time"]["shots_per_s"], 3)
+    out["speedup_vs_plain_loop"] = round(b / out["plain_loop"]["shots_per_s"], 3)
+    log(f"[bench] serving: {json.dumps(out)}")
+    return out
 
 
 def serving_main(args):
@@ -289,8 +344,13 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serving: HBM budget (GB)")
     ap.add_argument("--no-plain", action="store_true", help="serving: skip the plain-loop leg")
     ap.add_argument("--accuracy", action="store_true",
-                    help="also run the whole NSTEP of the workload in its storage and in FP64 storage "
-                         "and report the seismogram relative L2 and the largest energy deviation")
+                    help="run the accuracy leg over the whole NSTEP of the workload")
+    ap.add_argument("--accuracy-steps", type=int, default=600,
+                    help="steps of the accuracy leg (the workload in its storage and in FP64 "
+                         "storage; seismogram relative L2 and largest energy deviation); 0: skip")
+    ap.add_argument("--serving-shots", type=int, default=40,
+                    help="shots of the shortened serving stream reported beside the main line "
+                         "(NSTEP 100-400, the full stream's edge mix); 0: skip")
     args = ap.parse_args()
     if args.calibrate > 0:
         import ela3d_amd
@@ -364,8 +424,9 @@ def main():
                 graph_ms = (time.perf_counter() - t) * 1e3 / g_steps
             log(f"[bench] graph replay: {graph_ms:.4f} ms/step")
     accuracy = None
-    if args.accuracy:
-        accuracy = accuracy_run(args.workload)
+    if args.accuracy or args.accuracy_steps > 0:
+        accuracy = accuracy_run(args.workload, None if args.accuracy else args.accuracy_steps)
+    serving = serving_summary(nshots=args.serving_shots) if args.serving_shots > 0 else None
     ms_per_step = host_s * 1e3 / K
     mcells = cfg.cells * K / host_s / 1e6
     for s in stats:
@@ -406,6 +467,7 @@ def main():
             "graph_ms_per_step": None if graph_ms is None else round(graph_ms, 5),
             "graph_mcell_updates_per_s": None if graph_ms is None else round(cfg.cells / (graph_ms * 1e-3) / 1e6, 2),
             "accuracy": accuracy,
+            "serving": serving,
             "launch_shape": {"block": info["block"], "zseg": info["zseg"], "vec": info["vec"],
                              "block_slab": info["block_slab"], "zseg_slab": info["zseg_slab"],
                              "vec_slab": info["vec_slab"], "overlap": info["overlap"],

@@ -1,14 +1,22 @@
 // serve.cpp -- continuous batching of forward shots on one GPU.
 //
-// One device thread owns every admitted shot.  Each shot has its own HIP stream
-// and keeps up to two rounds of steps_per_round graph replays in flight; after
-// each round it records an event.  The thread polls those events: a shot whose
-// last round completed is retired (results copied to host, device memory freed)
-// and the admission step then starts queued shots in arrival order while the
-// device-memory budget and the active-shot cap allow.  Shots of any grid size
-// and step count therefore overlap on the device, and arrivals join while
-// earlier shots are still running.
+// One device thread owns every admitted shot.  Each shot runs on one of
+// max_active stream slots and keeps up to two rounds of steps_per_round graph
+// replays in flight; after each round it records an event.  The thread polls
+// those events: a shot whose last round completed is retired (results copied to
+// host, its device memory returned to the pool) and the admission step then
+// starts queued shots in arrival order while the device-memory budget and the
+// active-shot cap allow.  Shots of any grid size and step count therefore
+// overlap on the device, and arrivals join while earlier shots are still running.
+//
+// Nothing per shot is allocated or freed through the HIP runtime in steady
+// state: shots are placed in a server-owned device pool (a few large hipMalloc
+// chunks, sub-allocated first-fit), and the streams, events and pinned flag
+// words are made once per slot.  hipFree waits for the whole device, so a
+// retirement that freed memory would drain every other shot's queued rounds.
+#include <algorithm>
 #include <atomic>
+#include <iterator>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -20,6 +28,92 @@
 #include "shot.hpp"
 
 namespace ela3d {
+
+// Server-owned device memory: chunks of at least kChunk bytes from hipMalloc,
+// sub-allocated first-fit in 256-byte units, freed ranges coalesced.  Chunks
+// are released only when the server is destroyed.
+class DevicePool {
+public:
+    static constexpr size_t kChunk = size_t(16) << 30;
+    explicit DevicePool(size_t limit) : limit_(limit) {}
+    ~DevicePool()
+    {
+        for (auto& c : chunks_) (void)hipFree(c.base);
+    }
+    // a 256-byte aligned range of `bytes`, or throws ELA3D_E_OOM
+    void* alloc(size_t bytes)
+    {
+        bytes = (bytes + 255) & ~size_t(255);
+        for (size_t q = 0; q < chunks_.size(); ++q)
+            if (void* p = take(q, bytes)) return p;
+        size_t want = std::max(bytes, std::min(kChunk, limit_));
+        void* base = nullptr;
+        if (hipMalloc(&base, want) != hipSuccess) {
+            (void)hipGetLastError();
+            want = bytes;  // no room for a whole chunk: just this shot
+            HIP_CHECK(hipMalloc(&base, want));
+        }
+        chunks_.push_back(Chunk{static_cast<char*>(base), want, {}});
+        chunks_.back().free[0] = want;
+        reserved_ += want;
+        return take(chunks_.size() - 1, bytes);
+    }
+    void release(void* p)
+    {
+        auto it = live_.find(p);
+        if (it == live_.end()) return;
+        Chunk& c = chunks_[it->second.first];
+        size_t off = size_t(static_cast<char*>(p) - c.base), len = it->second.second;
+        live_.erase(it);
+        auto nx = c.free.lower_bound(off);
+        if (nx != c.free.end() && off + len == nx->first) {  // merge with the range after
+            len += nx->second;
+            nx = c.free.erase(nx);
+        }
+        if (nx != c.free.begin()) {  // and with the range before
+            auto pv = std::prev(nx);
+            if (pv->first + pv->second == off) {
+                pv->second += len;
+                return;
+            }
+        }
+        c.free[off] = len;
+    }
+    size_t reserved() const { return reserved_; }
+
+private:
+    struct Chunk {
+        char* base;
+        size_t size;
+        std::map<size_t, size_t> free;  // offset -> length
+    };
+    void* take(size_t q, size_t bytes)
+    {
+        Chunk& c = chunks_[q];
+        for (auto it = c.free.begin(); it != c.free.end(); ++it) {
+            if (it->second < bytes) continue;
+            const size_t off = it->first, len = it->second;
+            c.free.erase(it);
+            if (len > bytes) c.free[off + bytes] = len - bytes;
+            void* p = c.base + off;
+            live_[p] = {int(q), bytes};
+            return p;
+        }
+        return nullptr;
+    }
+    size_t limit_;
+    size_t reserved_ = 0;
+    std::vector<Chunk> chunks_;
+    std::map<void*, std::pair<int, size_t>> live_;
+};
+
+// One stream slot: its stream, the two round events and a pinned flag word.
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int* flag = nullptr;
+    bool busy = false;
+};
 
 struct Request {
     uint64_t ticket = 0;
@@ -34,6 +128,8 @@ struct Request {
     std::vector<double> seis, energy;
     // device-thread only
     std::unique_ptr<Shot> shot;
+    void* mem = nullptr;         // its range of the device pool
+    int slot = -1;
     hipEvent_t ev[2] = {nullptr, nullptr};
     int ev_head = 0, ev_count = 0;
     int enqueued = 0;
@@ -41,9 +137,21 @@ struct Request {
 
 class Server {
 public:
-    explicit Server(const ela3d_server_opts& o) : opts_(o)
+    explicit Server(const ela3d_server_opts& o) : opts_(o), pool_(size_t(o.hbm_budget_bytes))
     {
         if (opts_.plain_loop) opts_.max_active = 1;
+        HIP_CHECK(hipSetDevice(opts_.device));
+        slots_.resize(size_t(opts_.max_active));
+        try {
+            for (Slot& sl : slots_) {  // default priority, like every stream of the library
+                HIP_CHECK(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+                for (auto& e : sl.ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&sl.flag), sizeof(int), hipHostMallocDefault));
+            }
+        } catch (...) {
+            free_slots();
+            throw;
+        }
         t0_ = std::chrono::steady_clock::now();
         th_ = std::thread([this] { loop(); });
     }
@@ -56,6 +164,20 @@ public:
         }
         cv_.notify_all();
         if (th_.joinable()) th_.join();
+        free_slots();
+    }
+
+    void free_slots()
+    {
+        (void)hipSetDevice(opts_.device);
+        for (Slot& sl : slots_) {
+            if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+            for (auto& e : sl.ev)
+                if (e) (void)hipEventDestroy(e);
+            if (sl.stream) (void)hipStreamDestroy(sl.stream);
+            if (sl.flag) (void)hipHostFree(sl.flag);
+            sl = Slot();
+        }
     }
 
     double now() const
@@ -233,7 +355,21 @@ private:
                 c.device = opts_.device;
                 return c;
             }());
-            r->shot = make_shot(std::move(plan), r->cp.data(), r->cs.data(), r->rho.data());
+            const size_t need = size_t(shot_device_bytes(plan));
+            for (size_t q = 0; q < slots_.size(); ++q)
+                if (!slots_[q].busy) {
+                    r->slot = int(q);
+                    break;
+                }
+            if (r->slot < 0) throw Error{ELA3D_E_STATE, "no free stream slot"};
+            Slot& sl = slots_[size_t(r->slot)];
+            sl.busy = true;
+            r->mem = pool_.alloc(need);
+            ShotMem mem;
+            mem.arena = r->mem;
+            mem.bytes = need;
+            mem.host_flag = sl.flag;
+            r->shot = make_shot(std::move(plan), r->cp.data(), r->cs.data(), r->rho.data(), sl.stream, false, mem);
             const int64_t actual = r->shot->device_bytes();
             {
                 std::lock_guard<std::mutex> g(mu_);
@@ -241,7 +377,8 @@ private:
                 r->bytes = actual;
                 peak_ = std::max(peak_, in_use_);
             }
-            for (auto& e : r->ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            r->ev[0] = sl.ev[0];
+            r->ev[1] = sl.ev[1];
             std::vector<float>().swap(r->cp);  // the model now lives on the device
             std::vector<float>().swap(r->cs);
             std::vector<float>().swap(r->rho);
@@ -316,9 +453,15 @@ private:
             } catch (const Error& e) {
                 if (r->status == ELA3D_OK) r->status = e.code;
             }
-            for (auto& e : r->ev)
-                if (e) (void)hipEventDestroy(e);
             r->shot.reset();
+        }
+        if (r->mem) {  // back to the pool: the shot's stream has finished with it
+            pool_.release(r->mem);
+            r->mem = nullptr;
+        }
+        if (r->slot >= 0) {
+            slots_[size_t(r->slot)].busy = false;
+            r->slot = -1;
         }
         std::lock_guard<std::mutex> g(mu_);
         in_use_ -= r->bytes;
@@ -330,6 +473,8 @@ private:
     }
 
     ela3d_server_opts opts_;
+    DevicePool pool_;             // device thread only (and the destructor)
+    std::vector<Slot> slots_;
     std::chrono::steady_clock::time_point t0_;
     std::thread th_;
     std::mutex mu_;

@@ -258,7 +258,7 @@ private:
         for (auto& g : graphs_)
             if (g) (void)hipGraphExecDestroy(g);
         if (arena_ && !mem_.arena) (void)hipFree(arena_);
-        if (h_flag_) (void)hipHostFree(h_flag_);
+        if (h_flag_ && h_flag_ != mem_.host_flag) (void)hipHostFree(h_flag_);
         if (aux_) (void)hipStreamDestroy(aux_);
         for (auto& e : fork_)
             if (e) (void)hipEventDestroy(e);
@@ -332,7 +332,10 @@ public:
     void enqueue_flag_readback() override
     {
         if (!h_flag_) {
-            HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_flag_), sizeof(int), hipHostMallocDefault));
+            if (mem_.host_flag)
+                h_flag_ = mem_.host_flag;
+            else
+                HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_flag_), sizeof(int), hipHostMallocDefault));
             *h_flag_ = 0;
         }
         HIP_CHECK(hipMemcpyAsync(h_flag_, d_flag_, sizeof(int), hipMemcpyDeviceToHost, stream_));

@@ -106,6 +106,8 @@ struct ShotMem {
     void* arena = nullptr;  // 256-byte aligned; nullptr: the shot allocates its own
     size_t bytes = 0;       // usable bytes at arena
     bool inplace = false;   // one field set: no fused, double-buffered or overlapped schedule
+    int* host_flag = nullptr;  // pinned host int for the instability-flag readback
+                               // (nullptr: the shot allocates one when first asked)
 };
 
 // double_buffer: allocate a second field set even if the chosen tuning does not
