@@ -171,15 +171,30 @@ def _dptr(a: np.ndarray):
 class Shot:
     """One forward shot resident on the GPU (ela3d_create ... ela3d_destroy)."""
 
-    def __init__(self, cfg: Config, cp, cs, rho):
+    def __init__(self, cfg: Config, cp, cs, rho, slab=None):
+        """slab=(k0, k1): a z-slab owning planes [k0, k1) of the grid cfg describes
+        (ela3d_create_slab); step it with step_phase and halo_exchange."""
         self.cfg = cfg
         shape = (cfg.nz, cfg.ny, cfg.nx)
         self._model = [np.ascontiguousarray(np.asarray(a, dtype=np.float32)).reshape(shape)
                        for a in (cp, cs, rho)]
         s, keep = cfg.to_struct()
         h = C.c_void_p()
-        check(lib().ela3d_create(C.byref(s), _fptr(self._model[0]), _fptr(self._model[1]),
-                                 _fptr(self._model[2]), C.byref(h)))
+        self._nrec, self._nz, self.receiver_index = cfg.nrec, cfg.nz, None
+        if slab is None:
+            check(lib().ela3d_create(C.byref(s), _fptr(self._model[0]), _fptr(self._model[1]),
+                                     _fptr(self._model[2]), C.byref(h)))
+        else:
+            k0, k1 = (int(v) for v in slab)
+            idx = np.zeros(max(1, cfg.nrec), dtype=np.int32)
+            nr = C.c_int32()
+            check(lib().ela3d_create_slab(C.byref(s), k0, k1, _fptr(self._model[0]), _fptr(self._model[1]),
+                                          _fptr(self._model[2]), C.byref(h),
+                                          idx.ctypes.data_as(C.POINTER(C.c_int32)), idx.size, C.byref(nr)))
+            self._nrec = nr.value
+            self.receiver_index = idx[:nr.value].copy()  # global indices of the kept receivers
+            self._nz = (k1 - k0) + (1 if k0 > 0 else 0) + (1 if k1 < cfg.nz else 0)
+            self.koff = k0 - (1 if k0 > 0 else 0)
         self._h = h
 
     def close(self):
@@ -218,7 +233,7 @@ class Shot:
 
     def seismograms(self) -> np.ndarray:
         inf = self.info()
-        out = np.zeros((inf["nsamples"], self.cfg.nrec, 3), dtype=np.float64)
+        out = np.zeros((inf["nsamples"], self._nrec, 3), dtype=np.float64)
         check(lib().ela3d_seismograms(self._h, _dptr(out), out.size))
         return out
 
@@ -229,7 +244,7 @@ class Shot:
         return out
 
     def snapshot(self, fid: int) -> np.ndarray:
-        out = np.zeros((self.cfg.nz, self.cfg.ny, self.cfg.nx), dtype=np.float64)
+        out = np.zeros((self._nz, self.cfg.ny, self.cfg.nx), dtype=np.float64)
         check(lib().ela3d_snapshot(self._h, int(fid), _dptr(out), out.size))
         return out
 
@@ -242,6 +257,19 @@ class Shot:
         out = np.zeros((self.cfg.nstep + 1, 3), dtype=np.float64)
         check(lib().ela3d_source_trace(self._h, _dptr(out), out.size))
         return out
+
+    def step_phase(self, phase: int):
+        """Phase 0: the stress pass; phase 1: the velocity pass and the tail (one step)."""
+        check(lib().ela3d_step_phase(self._h, int(phase)))
+
+    def halo_bytes(self, phase: int, side: int) -> int:
+        n = C.c_size_t()
+        check(lib().ela3d_halo_bytes(self._h, int(phase), int(side), C.byref(n)))
+        return n.value
+
+    def halo_exchange(self, upper: "Shot", phase: int):
+        """Both halos between this slab and the one above it after `phase` (same GPU)."""
+        check(lib().ela3d_halo_exchange(self._h, upper._h, int(phase)))
 
     def checkpoint(self, path: str):
         """Write the whole device state and step count to `path`."""

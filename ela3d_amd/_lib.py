@@ -121,6 +121,13 @@ def lib() -> C.CDLL:
                                         P(C.c_int32), P(C.c_double)]
     L.ela3d_run_noalloc.argtypes = [P(ConfigStruct), P(f), P(f), P(f), P(C.c_double), C.c_size_t,
                                     P(C.c_double), C.c_size_t, C.c_void_p]
+    L.ela3d_create_slab.argtypes = [P(ConfigStruct), C.c_int32, C.c_int32, P(f), P(f), P(f), P(C.c_void_p),
+                                    P(C.c_int32), C.c_size_t, P(C.c_int32)]
+    L.ela3d_step_phase.argtypes = [C.c_void_p, C.c_int32]
+    L.ela3d_halo_bytes.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(C.c_size_t)]
+    L.ela3d_halo_pack.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_size_t]
+    L.ela3d_halo_unpack.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_size_t]
+    L.ela3d_halo_exchange.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
     L.ela3d_destroy.argtypes = [C.c_void_p]
     L.ela3d_destroy.restype = None
     L.ela3d_plan.argtypes = [P(ConfigStruct), P(InfoStruct)]

@@ -12,6 +12,15 @@
 
 struct ela3d_shot {
     std::unique_ptr<ela3d::Shot> impl;
+    void* halo_buf = nullptr;  // staging of ela3d_halo_exchange (this shot as the lower slab)
+    size_t halo_cap = 0;
+    ~ela3d_shot()
+    {
+        if (halo_buf) {
+            impl.reset();  // its stream is drained first
+            (void)hipFree(halo_buf);
+        }
+    }
 };
 
 namespace ela3d {
@@ -153,6 +162,118 @@ int ela3d_run_noalloc(const ela3d_config* cfg, const float* cp, const float* cs,
         if (nen) shot->energy(energy);
         if (rc == ELA3D_E_UNSTABLE) ela3d::set_last_error("non-finite value reached a receiver or the energy");
         return rc;
+    });
+}
+
+int ela3d_create_slab(const ela3d_config* cfg, int32_t k0, int32_t k1, const float* cp, const float* cs,
+                      const float* rho, ela3d_shot** out, int32_t* receivers, size_t receivers_cap,
+                      int32_t* nreceivers)
+{
+    return guarded([&] {
+        if (!cfg || !cp || !cs || !rho || !out) throw Error{ELA3D_E_INVALID, "null argument"};
+        *out = nullptr;
+        std::vector<int> recs;
+        ela3d::ShotPlan plan = ela3d::make_slab_plan(*cfg, k0, k1, &recs);
+        const int64_t nxy = int64_t(plan.cfg.nx) * plan.cfg.ny;
+        ela3d::check_model(nxy * plan.nz_global, cp, cs, rho);
+        if (nreceivers) *nreceivers = int32_t(recs.size());
+        if (receivers) {
+            if (receivers_cap < recs.size()) throw Error{ELA3D_E_RANGE, "receiver index buffer too small"};
+            std::copy(recs.begin(), recs.end(), receivers);
+        }
+        int rc = need_device(plan.cfg.device);
+        if (rc) return rc;
+        const int64_t o = nxy * plan.koff;  // the slab's planes, ghosts included, of the global model
+        ela3d::ShotMem mem;
+        mem.inplace = true;
+        auto s = new ela3d_shot;
+        try {
+            s->impl = ela3d::make_shot(std::move(plan), cp + o, cs + o, rho + o, nullptr, false, mem);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+        return ELA3D_OK;
+    });
+}
+
+int ela3d_step_phase(ela3d_shot* s, int32_t phase)
+{
+    return guarded([&] {
+        if (!s) throw Error{ELA3D_E_INVALID, "null shot"};
+        s->impl->enqueue_phase(phase);
+        return ELA3D_OK;
+    });
+}
+
+int ela3d_halo_bytes(ela3d_shot* s, int32_t phase, int32_t side, size_t* bytes)
+{
+    return guarded([&] {
+        if (!s || !bytes) throw Error{ELA3D_E_INVALID, "null argument"};
+        if ((phase != 0 && phase != 1) || (side != 0 && side != 1)) throw Error{ELA3D_E_INVALID, "bad phase or side"};
+        *bytes = s->impl->halo_bytes(phase, side);
+        return ELA3D_OK;
+    });
+}
+
+int ela3d_halo_pack(ela3d_shot* s, int32_t phase, int32_t side, void* dev_buf, size_t cap)
+{
+    return guarded([&] {
+        if (!s || !dev_buf) throw Error{ELA3D_E_INVALID, "null argument"};
+        if ((phase != 0 && phase != 1) || (side != 0 && side != 1)) throw Error{ELA3D_E_INVALID, "bad phase or side"};
+        if (cap < s->impl->halo_bytes(phase, side)) throw Error{ELA3D_E_RANGE, "halo buffer too small"};
+        s->impl->halo_pack(phase, side, dev_buf);
+        return ELA3D_OK;
+    });
+}
+
+int ela3d_halo_unpack(ela3d_shot* s, int32_t phase, int32_t side, const void* dev_buf, size_t n)
+{
+    return guarded([&] {
+        if (!s || !dev_buf) throw Error{ELA3D_E_INVALID, "null argument"};
+        if ((phase != 0 && phase != 1) || (side != 0 && side != 1)) throw Error{ELA3D_E_INVALID, "bad phase or side"};
+        if (n < s->impl->halo_bytes(phase, 1 - side)) throw Error{ELA3D_E_RANGE, "halo buffer too small"};
+        s->impl->halo_unpack(phase, side, dev_buf);
+        return ELA3D_OK;
+    });
+}
+
+int ela3d_halo_exchange(ela3d_shot* lower, ela3d_shot* upper, int32_t phase)
+{
+    return guarded([&] {
+        if (!lower || !upper || lower == upper) throw Error{ELA3D_E_INVALID, "two distinct slabs needed"};
+        if (phase != 0 && phase != 1) throw Error{ELA3D_E_INVALID, "phase must be 0 or 1"};
+        const int dev = lower->impl->plan().cfg.device;
+        if (upper->impl->plan().cfg.device != dev) throw Error{ELA3D_E_INVALID, "slabs on different devices"};
+        const ela3d::ShotPlan &L = lower->impl->plan(), &U = upper->impl->plan();
+        if (L.koff + L.own_k1 != U.koff + U.own_k0 || L.cfg.nx != U.cfg.nx || L.cfg.ny != U.cfg.ny ||
+            L.cfg.storage != U.cfg.storage)
+            throw Error{ELA3D_E_INVALID, "the slabs are not neighbours (lower's last plane + 1 = upper's first)"};
+        HIP_CHECK(hipSetDevice(dev));
+        const size_t up = lower->impl->halo_bytes(phase, 1), dn = upper->impl->halo_bytes(phase, 0);
+        const size_t need = ((up + 255) & ~size_t(255)) + dn;
+        if (lower->halo_cap < need) {
+            if (lower->halo_buf) {
+                HIP_CHECK(hipStreamSynchronize(lower->impl->stream()));
+                HIP_CHECK(hipFree(lower->halo_buf));
+                lower->halo_buf = nullptr;
+                lower->halo_cap = 0;
+            }
+            HIP_CHECK(hipMalloc(&lower->halo_buf, need));
+            lower->halo_cap = need;
+        }
+        char* bup = static_cast<char*>(lower->halo_buf);
+        char* bdn = bup + ((up + 255) & ~size_t(255));
+        lower->impl->halo_pack(phase, 1, bup);  // lower's top plane, sent up
+        upper->impl->halo_pack(phase, 0, bdn);  // upper's bottom plane, sent down
+        HIP_CHECK(hipStreamSynchronize(lower->impl->stream()));
+        HIP_CHECK(hipStreamSynchronize(upper->impl->stream()));
+        upper->impl->halo_unpack(phase, 0, bup);
+        lower->impl->halo_unpack(phase, 1, bdn);
+        HIP_CHECK(hipStreamSynchronize(lower->impl->stream()));
+        HIP_CHECK(hipStreamSynchronize(upper->impl->stream()));
+        return ELA3D_OK;
     });
 }
 

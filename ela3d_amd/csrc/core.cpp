@@ -121,11 +121,14 @@ int split_boxes(const ShotPlan& p, BoxRange out[5])
     const int ylo = p.ax[1].lo, yhi = p.ax[1].hi, zlo = p.ax[2].lo, zhi = p.ax[2].hi;
     int pa = (ylo * nx + 63) / 64 * 64, pb = (yhi * nx) / 64 * 64;
     if (pb <= pa) pa = pb = nxy;  // too small for an interior box: all absorbing layer
-    const BoxRange all[5] = {{0, nxy, 0, zlo, 1}, {0, nxy, zhi, c.nz, 1}, {0, pa, zlo, zhi, 1},
-                             {pb, nxy, zlo, zhi, 1}, {pa, pb, zlo, zhi, 0}};
+    BoxRange all[5] = {{0, nxy, 0, zlo, 1}, {0, nxy, zhi, c.nz, 1}, {0, pa, zlo, zhi, 1},
+                       {pb, nxy, zlo, zhi, 1}, {pa, pb, zlo, zhi, 0}};
     int n = 0;
-    for (const BoxRange& b : all)
+    for (BoxRange& b : all) {  // a slab updates only the planes it owns
+        b.k0 = std::max(b.k0, p.own_k0);
+        b.k1 = std::min(b.k1, p.own_k1);
         if (b.p1 > b.p0 && b.k1 > b.k0) out[n++] = b;
+    }
     return n;
 }
 
@@ -171,6 +174,8 @@ double box_alg_bytes(const ShotPlan& P, const BoxRange& b, int pass)
     }
     return bytes + 2.0 * e * double(count);
 }
+
+static void plan_traffic(ShotPlan& P);
 
 ShotPlan make_plan(const ela3d_config& in)
 {
@@ -237,7 +242,23 @@ ShotPlan make_plan(const ela3d_config& in)
     P.nsamples = c.nstep / c.sample_every;
     P.nenergy = c.energy_every > 0 ? c.nstep / c.energy_every : 0;
     P.kmax_shear = (c.compat_flags & ELA3D_COMPAT_GHOST) ? c.nz - 1 : c.nz - 2;
+    P.koff = 0;
+    P.nz_global = c.nz;
+    P.own_k0 = 0;
+    P.own_k1 = c.nz;
+    P.ek0 = std::max(0, npml);
+    P.ek1 = c.nz - 1 - std::max(0, npml);
+    plan_traffic(P);
+    return P;
+}
 
+// Algorithmic traffic of a plan's boxes.
+static void plan_traffic(ShotPlan& P)
+{
+    const ela3d_config& c = P.cfg;
+    const int64_t cells = int64_t(c.nx) * c.ny * (P.own_k1 - P.own_k0);
+    P.alg_stress_interior = P.alg_stress_slab = P.alg_velocity_interior = P.alg_velocity_slab = 0;
+    P.interior_cells = 0;
     BoxRange boxes[5];
     const int nb = split_boxes(P, boxes);
     for (int q = 0; q < nb; ++q) {
@@ -255,6 +276,62 @@ ShotPlan make_plan(const ela3d_config& in)
     // two passes re-read v, sigma and rho: 27 fields + 4 model values per cell against
     // 18 fields + 3 model values in one fused pass
     P.fused_saving = stress_cell_bytes(c) + velocity_cell_bytes(c) - (18.0 * storage_bytes(c.storage) + 12.0);
+}
+
+ShotPlan make_slab_plan(const ela3d_config& cfg, int k0, int k1, std::vector<int>* slab_receivers)
+{
+    ShotPlan G = make_plan(cfg);
+    const ela3d_config& g = G.cfg;
+    if (k0 < 0 || k1 > g.nz || k1 - k0 < 2) fail("slab planes [k0, k1) must lie in the grid and hold >= 2 planes");
+    if (g.storage == ELA3D_STORAGE_FP16)
+        fail("slabs run FP32 or FP64 storage (FP16 scales are per shot, not shared across a halo)");
+    const int below = k0 > 0 ? 1 : 0, above = k1 < g.nz ? 1 : 0;
+    ShotPlan P = G;
+    P.koff = k0 - below;
+    P.nz_global = g.nz;
+    const int nzl = (k1 - k0) + below + above;
+    P.cfg.nz = nzl;
+    P.own_k0 = below;
+    P.own_k1 = below + (k1 - k0);
+    P.cells = int64_t(g.nx) * g.ny * nzl;
+    // z tables at global k, and the compact z region clipped to the slab
+    {
+        const PmlAxis& Z = G.ax[2];
+        PmlAxis L;
+        L.n = nzl;
+        auto cut = [&](const std::vector<double>& v) {
+            return std::vector<double>(v.begin() + P.koff, v.begin() + P.koff + nzl);
+        };
+        L.d = cut(Z.d); L.K = cut(Z.K); L.alpha = cut(Z.alpha); L.a = cut(Z.a); L.b = cut(Z.b);
+        L.d_h = cut(Z.d_h); L.K_h = cut(Z.K_h); L.alpha_h = cut(Z.alpha_h); L.a_h = cut(Z.a_h); L.b_h = cut(Z.b_h);
+        L.lo = std::min(nzl, std::max(0, Z.lo - P.koff));
+        L.hi = std::min(nzl, std::max(L.lo, Z.hi - P.koff));
+        P.ax[2] = L;
+    }
+    P.kmax_shear = G.kmax_shear - P.koff;
+    P.ek0 = G.ek0 - P.koff;
+    P.ek1 = G.ek1 - P.koff;
+    // receivers of the slab's own planes, in their global order
+    P.rec.clear();
+    if (slab_receivers) slab_receivers->clear();
+    for (int r = 0; r < g.nrec; ++r) {
+        const int k = G.rec[3 * r + 2];
+        if (k < k0 || k >= k1) continue;
+        P.rec.push_back(G.rec[3 * r]);
+        P.rec.push_back(G.rec[3 * r + 1]);
+        P.rec.push_back(k - P.koff);
+        if (slab_receivers) slab_receivers->push_back(r);
+    }
+    P.cfg.nrec = int(P.rec.size() / 3);
+    // the source: injected by the slab that owns its plane; elsewhere a zero
+    // wavelet at an owned interior cell adds nothing
+    if (g.ksrc >= k0 && g.ksrc < k1) {
+        P.cfg.ksrc = g.ksrc - P.koff;
+    } else {
+        std::fill(P.wavelet.begin(), P.wavelet.end(), 0.0);
+        P.cfg.ksrc = std::max(1, std::min(nzl - 2, P.own_k0));
+    }
+    plan_traffic(P);
     return P;
 }
 

@@ -39,6 +39,13 @@ struct ShotPlan {
     int nsamples = 0, nenergy = 0;
     int kmax_shear = 0;             // nz-2, or nz-1 with the ghost quirk
     double t0 = 0.0;
+    // z-slab of a larger grid (make_slab_plan): cfg.nz counts the slab's planes
+    // plus one ghost plane on each side that has a neighbour slab; local plane k
+    // is global plane k + koff.  Only planes [own_k0, own_k1) are updated; the
+    // ghost planes are written by halo exchange.  A whole grid has koff = 0,
+    // own = [0, nz), nz_global = nz.
+    int koff = 0, nz_global = 0, own_k0 = 0, own_k1 = 0;
+    int ek0 = 0, ek1 = 0;           // planes [ek0, ek1) of the energy box (local indices)
     // algorithmic traffic (bytes) of the two-pass schedule, split by launch
     // (interior box; absorbing-layer boxes)
     double alg_stress_interior = 0, alg_stress_slab = 0;
@@ -72,6 +79,11 @@ double box_alg_bytes(const ShotPlan& p, const BoxRange& b, int pass);
 int storage_bytes(int storage);
 // Validate and expand a config; throws Error.
 ShotPlan make_plan(const ela3d_config& cfg);
+// The plan of planes [k0, k1) of the grid `cfg` describes, as a slab with ghost
+// planes (see ShotPlan::koff).  C-PML z tables are the global ones at global k;
+// receivers outside the slab are dropped (slab_receivers lists the kept ones'
+// global indices); the source is injected only by the slab that owns its plane.
+ShotPlan make_slab_plan(const ela3d_config& cfg, int k0, int k1, std::vector<int>* slab_receivers = nullptr);
 // Tables of one axis (0 x, 1 y, 2 z) for a validated config.
 PmlAxis make_axis(const ela3d_config& cfg, int axis);
 double wavelet_value(const ela3d_config& cfg, double t0, int it);

@@ -385,7 +385,7 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
             if (iN || face) el(fw[1], oo) = nvy;
             if (iN || face) el(fw[2], oo) = nvz;
             if (ENERGY) {
-                if (in_e_xy && k >= em && k < A.nz - 1 - em) {
+                if (in_e_xy && k >= A.ek0 && k < A.ek1) {
                     const double r = S.rho, vp = S.cp, vs = S.cs;
                     const double mu = r * vs * vs, lam = r * (vp * vp - 2.0 * vs * vs);
                     const double dxx = sxx, dyy = syy, dzz = S.szz;

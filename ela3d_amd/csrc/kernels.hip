@@ -734,8 +734,8 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
         wz[e] = onz || f;
         if (ENERGY) {
             const int em = A.emargin;
-            const bool in_e = L.act[e] && i >= em && i < nx - 1 - em && j >= em && j < ny - 1 - em && k >= em &&
-                              k < nz - 1 - em;
+            const bool in_e = L.act[e] && i >= em && i < nx - 1 - em && j >= em && j < ny - 1 - em && k >= A.ek0 &&
+                              k < A.ek1;
             if (in_e) {
                 // state after this velocity update: cell-local values, FP64 terms
                 const double r = Q.rho_c[e], vp = P.cp[e], vs = P.cs[e];

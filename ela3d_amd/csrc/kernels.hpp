@@ -58,7 +58,8 @@ struct StepArgs {
     int lo[3], hi[3], w[3];  // compact C-PML regions [0,lo) u [hi,n), width w
     int kmax_shear;          // last plane of the sxz/syz update
     int dirichlet;
-    int emargin;             // energy box [emargin, n-1-emargin) per axis
+    int emargin;             // energy box [emargin, n-1-emargin) along x and y
+    int ek0, ek1;            // energy box planes [ek0, ek1) (a slab's local indices of the global box)
     double dt, idx, idy, idz;
     const float* scl;        // FP16 storage scales, kScaleTable floats (layout below)
     unsigned* maxslot;       // [kMaxSlots][27] |value| maxima as float bits (FP16 storage)
@@ -124,6 +125,10 @@ void launch_tail(const TailArgs<T>& a, hipStream_t s);
 void launch_copy(const float4* src, float4* dst, long n4, int variant, hipStream_t s);
 // zero `bytes` bytes at p (p 16-byte aligned) with a kernel on stream s
 void launch_zero(void* p, size_t bytes, hipStream_t s);
+// Halo of a z interface: nf planes, each the first n elements at fields[q],
+// copied to (pack) or from (unpack) the contiguous buffer buf[nf][n].
+template <typename T>
+void launch_halo(T* const* fields, int nf, long n, T* buf, bool pack, hipStream_t s);
 // First level of the energy reduction on energy steps: the n per-block partial
 // pairs in `in` are summed in a fixed order by `nout` workgroups into
 // out[nout][2], which the tail kernel then sums (kEpartBlocks at most).

@@ -280,6 +280,86 @@ public:
         drop_graphs();
     }
     int steps_done() const override { return host_step_; }
+
+    void enqueue_phase(int phase) override
+    {
+        HIP_CHECK(hipSetDevice(device_));
+        if (tune_.fused || dbuf_ || overlap_)
+            throw Error{ELA3D_E_STATE, "stepping by phases needs the in-place two-pass schedule"};
+        const int it = host_step_ + 1;
+        const bool e = energy_step(it);
+        if (phase == 0) {
+            launch_stress(args_[0][1][0], stream_);
+            launch_stress(args_[0][0][0], stream_);
+        } else if (phase == 1) {
+            launch_velocity(args_[0][1][1], e, stream_);
+            launch_velocity(args_[0][0][1], e, stream_);
+            if (e) launch_epart(d_epart_, nepart_, d_epart2_, epart_blocks(), stream_);
+            launch_tail(tail_args(0), stream_);
+            host_step_ = it;
+        } else {
+            throw Error{ELA3D_E_INVALID, "phase must be 0 (stress pass) or 1 (velocity pass and tail)"};
+        }
+        HIP_CHECK(hipGetLastError());
+    }
+
+    // fields that cross a z interface after `phase`, sent towards higher k (up) or lower k
+    static int halo_fields(int phase, bool up, int (&q)[3])
+    {
+        if (phase == 0) {  // sigma_xz, sigma_yz feed the plane above; sigma_zz the plane below
+            if (up) { q[0] = 7; q[1] = 8; return 2; }
+            q[0] = 5;
+            return 1;
+        }
+        if (up) { q[0] = 2; return 1; }  // vz feeds the plane above; vx, vy the plane below
+        q[0] = 0; q[1] = 1;
+        return 2;
+    }
+
+    size_t halo_bytes(int phase, int side) const override
+    {
+        int q[3];
+        const int nf = halo_fields(phase, side == 1, q);
+        return size_t(nf) * size_t(plan_.cfg.nx) * plan_.cfg.ny * sizeof(T);
+    }
+
+    // plane of the halo copy and its fields: pack sends from the boundary plane on
+    // `side`; unpack writes the ghost plane on `side`, with the fields the
+    // neighbour there sends towards this slab
+    int halo_plane(int phase, int side, bool pack, T* (&f)[3], int& nf)
+    {
+        if (phase != 0 && phase != 1) throw Error{ELA3D_E_INVALID, "phase must be 0 or 1"};
+        if (side != 0 && side != 1) throw Error{ELA3D_E_INVALID, "side must be 0 (below) or 1 (above)"};
+        const bool has = side == 0 ? plan_.own_k0 > 0 : plan_.own_k1 < plan_.cfg.nz;
+        if (!has) throw Error{ELA3D_E_RANGE, "no neighbour slab on that side"};
+        const bool up = pack ? side == 1 : side == 0;
+        int q[3];
+        nf = halo_fields(phase, up, q);
+        const int k = pack ? (side == 1 ? plan_.own_k1 - 1 : plan_.own_k0) : (side == 1 ? plan_.own_k1 : plan_.own_k0 - 1);
+        for (int e = 0; e < nf; ++e) f[e] = d_f_[cur()][q[e]] + size_t(k) * size_t(nxyp_);
+        return k;
+    }
+
+    void halo_pack(int phase, int side, void* buf) override
+    {
+        HIP_CHECK(hipSetDevice(device_));
+        T* f[3];
+        int nf = 0;
+        halo_plane(phase, side, true, f, nf);
+        launch_halo<T>(f, nf, long(plan_.cfg.nx) * plan_.cfg.ny, static_cast<T*>(buf), true, stream_);
+        HIP_CHECK(hipGetLastError());
+    }
+
+    void halo_unpack(int phase, int side, const void* buf) override
+    {
+        HIP_CHECK(hipSetDevice(device_));
+        T* f[3];
+        int nf = 0;
+        halo_plane(phase, side, false, f, nf);
+        launch_halo<T>(f, nf, long(plan_.cfg.nx) * plan_.cfg.ny, static_cast<T*>(const_cast<void*>(buf)), false,
+                       stream_);
+        HIP_CHECK(hipGetLastError());
+    }
     int64_t device_bytes() const override { return arena_bytes_; }
 
     // one step the production way: a replay of its captured graph (captured on
@@ -822,6 +902,8 @@ private:
         a.kmax_shear = plan_.kmax_shear;
         a.dirichlet = c.dirichlet ? 1 : 0;
         a.emargin = std::max(0, c.npoints_pml);
+        a.ek0 = plan_.ek0;
+        a.ek1 = plan_.ek1;
         a.dt = c.dt;
         a.idx = 1.0 / c.dx;
         a.idy = 1.0 / c.dy;

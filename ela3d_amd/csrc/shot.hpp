@@ -91,6 +91,17 @@ public:
     // run the absorbing-layer launches concurrently with the interior ones (default)
     // or in sequence on the main stream (drops captured graphs)
     virtual void set_overlap(bool on) = 0;
+    // One half of a step of the in-place two-pass schedule, eagerly: phase 0 the
+    // stress pass, phase 1 the velocity pass and the tail (receivers, energy,
+    // source).  A z-slab runs its steps this way, with halo exchanges between.
+    virtual void enqueue_phase(int phase) = 0;
+    // Halo of a z-slab after `phase`: the fields it sends to its neighbour on
+    // `side` (0 below, 1 above) from its boundary plane on that side, or (unpack)
+    // receives from that neighbour into its ghost plane there.  `buf` is device
+    // memory of halo_bytes(phase); the copy is enqueued on the shot's stream.
+    virtual size_t halo_bytes(int phase, int side) const = 0;
+    virtual void halo_pack(int phase, int side, void* buf) = 0;
+    virtual void halo_unpack(int phase, int side, const void* buf) = 0;
     virtual int steps_done() const = 0;
     virtual int64_t device_bytes() const = 0;
     const ShotPlan& plan() const { return plan_; }

@@ -359,6 +359,36 @@ __global__ void __launch_bounds__(256) k_zero(uint4* __restrict__ d, long n16, u
     if (i < ntail) tail[i] = 0;
 }
 
+// Halo planes of a z-slab: field plane <-> contiguous buffer, one element per lane.
+template <typename T>
+struct HaloPlanes {
+    T* f[3];
+};
+template <typename T>
+__global__ void __launch_bounds__(256) k_halo(const HaloPlanes<T> H, int nf, long n, T* __restrict__ buf, int pack)
+{
+    const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    if (e >= n || q >= nf) return;
+    if (pack)
+        buf[q * n + e] = H.f[q][e];
+    else
+        H.f[q][e] = buf[q * n + e];
+}
+
+template <typename T>
+void launch_halo(T* const* fields, int nf, long n, T* buf, bool pack, hipStream_t s)
+{
+    if (nf <= 0 || nf > 3 || n <= 0) return;
+    HaloPlanes<T> H{};
+    for (int q = 0; q < nf; ++q) H.f[q] = fields[q];
+    hipLaunchKernelGGL((k_halo<T>), dim3(unsigned((n + 255) / 256), unsigned(nf)), dim3(256), 0, s, H, nf, n, buf,
+                       pack ? 1 : 0);
+}
+template void launch_halo<float>(float* const*, int, long, float*, bool, hipStream_t);
+template void launch_halo<double>(double* const*, int, long, double*, bool, hipStream_t);
+template void launch_halo<half_t>(half_t* const*, int, long, half_t*, bool, hipStream_t);
+
 void launch_zero(void* p, size_t bytes, hipStream_t s)
 {
     if (bytes == 0) return;

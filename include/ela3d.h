@@ -187,6 +187,35 @@ int ela3d_source_trace(ela3d_shot* shot, double* out, size_t cap);
  * a file of the wrong length is refused before any state is overwritten. */
 int ela3d_checkpoint_save(ela3d_shot* shot, const char* path);
 int ela3d_checkpoint_load(ela3d_shot* shot, const char* path);
+/* ---- z-slabs: one GPU's share of a grid decomposed along z (no reference
+ * counterpart; the reference indexes its C-PML tables by global position at
+ * HOM:177-179, and the planes that cross a z interface are the ones its loops
+ * read at k-1 / k+1: CPU:384, 450, 477, 508, 569) ----
+ * A slab owns planes [k0, k1) of the grid `cfg` describes, plus one ghost plane
+ * on each side that has a neighbour.  Its C-PML z tables, update ranges, rigid
+ * faces and energy box are the global ones; receivers outside its planes are
+ * dropped (receivers[] gets the global indices of those it keeps, in order) and
+ * the source is injected only by the slab that owns its plane.  cp, cs, rho are
+ * the whole grid's model.  FP32 or FP64 storage.  A slab steps by phases; after
+ * each phase every pair of neighbours exchanges halos: the lower one packs side
+ * 1 and the upper one unpacks side 0, the upper one packs side 0 and the lower
+ * one unpacks side 1 (the buffers are device memory; copies run on the slab's
+ * stream).  Phase 0 sends sigma_xz, sigma_yz up and sigma_zz down; phase 1
+ * sends vz up and vx, vy down.  Stepped this way, every slab's fields equal the
+ * whole grid's on its planes. */
+int ela3d_create_slab(const ela3d_config* cfg, int32_t k0, int32_t k1, const float* cp, const float* cs,
+                      const float* rho, ela3d_shot** out, int32_t* receivers, size_t receivers_cap,
+                      int32_t* nreceivers);
+/* phase 0: the stress pass; phase 1: the velocity pass, receivers, energy, source (one step) */
+int ela3d_step_phase(ela3d_shot* shot, int32_t phase);
+/* bytes a halo_pack(phase, side) writes (what the neighbour's unpack on the other side reads) */
+int ela3d_halo_bytes(ela3d_shot* shot, int32_t phase, int32_t side, size_t* bytes);
+int ela3d_halo_pack(ela3d_shot* shot, int32_t phase, int32_t side, void* dev_buf, size_t cap);
+int ela3d_halo_unpack(ela3d_shot* shot, int32_t phase, int32_t side, const void* dev_buf, size_t n);
+/* Both halos of one interface between two slabs on the same GPU after `phase`,
+ * through a staging buffer the lower slab keeps (device-to-device; synchronous). */
+int ela3d_halo_exchange(ela3d_shot* lower, ela3d_shot* upper, int32_t phase);
+
 /* Allocation-free run (no counterpart in the reference).  For callers that cannot
  * allocate device memory (no hipMalloc allowed in their loop, harnesses that forbid
  * it): ela3d_create + ela3d_run + ela3d_seismograms + ela3d_energy + ela3d_destroy in
