@@ -26,7 +26,7 @@ __all__ = [
     "WAVELET_GAUSSIAN", "WAVELET_DGAUSSIAN", "WAVELET_RICKER", "STORAGE_FP32", "STORAGE_FP16",
     "STORAGE_FP64", "COMPAT_XHALF_A", "COMPAT_ZHALF_MIN", "COMPAT_GHOST", "COMPAT_REFERENCE",
     "FIELD_NAMES", "Server", "SHOT_QUEUED", "SHOT_RUNNING", "SHOT_DONE", "anticline_model",
-    "read_raw", "write_raw", "write_traces", "write_rechor", "run_noalloc", "NOALLOC_BYTES",
+    "read_raw", "write_raw", "write_traces", "write_rechor", "run_noalloc", "NOALLOC_BYTES", "set_device_limit",
 ]
 
 SRC_REFERENCE_SIGMA, SRC_FORCE, SRC_EXPLOSIVE, SRC_FORCE3 = 0, 1, 2, 3
@@ -440,6 +440,12 @@ def set_tuning(cfg: Config, block: int, zseg_interior: int, zseg_slab: int, vec:
                      overlap=1 if overlap else 0, merged=1 if merged else 0,
                      dbuf=1 if (dbuf or fused) else 0, fused=1 if fused else 0, tile=int(tile))
     check(lib().ela3d_tuning_set(C.byref(s), C.byref(t)))
+
+
+def set_device_limit(nbytes: int):
+    """Cap the device bytes the library holds at once (failure injection: allocations
+    past it fail with ELA3D_E_OOM); 0 removes the cap."""
+    check(lib().ela3d_set_device_limit(int(nbytes)))
 
 
 def device_count() -> int:

@@ -128,6 +128,7 @@ def lib() -> C.CDLL:
     L.ela3d_halo_pack.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_size_t]
     L.ela3d_halo_unpack.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_size_t]
     L.ela3d_halo_exchange.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
+    L.ela3d_set_device_limit.argtypes = [C.c_int64]
     L.ela3d_destroy.argtypes = [C.c_void_p]
     L.ela3d_destroy.restype = None
     L.ela3d_plan.argtypes = [P(ConfigStruct), P(InfoStruct)]

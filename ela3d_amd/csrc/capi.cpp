@@ -18,7 +18,7 @@ struct ela3d_shot {
     {
         if (halo_buf) {
             impl.reset();  // its stream is drained first
-            (void)hipFree(halo_buf);
+            ela3d::device_free(halo_buf, halo_cap);
         }
     }
 };
@@ -73,7 +73,9 @@ void ela3d_config_default(ela3d_config* c)
     std::memset(c, 0, sizeof(*c));
     c->struct_size = sizeof(ela3d_config);
     c->abi_version = ELA3D_ABI_VERSION;
-    // reference literals (CPU:19-78, 151-153), shrunk to a 101^3 grid
+    // the homogeneous 101^3 cube workload: the reference's medium, spacing and C-PML
+    // (CPU:41-78, 151-153) with a Ricker force source at f0 = 7 Hz; the reference's
+    // own literals are Config.from_reference (Python) and reference=1 (CLI)
     c->nx = c->ny = c->nz = 101;
     c->dx = c->dy = c->dz = 10.0;
     c->dt = 1.6e-3;
@@ -256,11 +258,11 @@ int ela3d_halo_exchange(ela3d_shot* lower, ela3d_shot* upper, int32_t phase)
         if (lower->halo_cap < need) {
             if (lower->halo_buf) {
                 HIP_CHECK(hipStreamSynchronize(lower->impl->stream()));
-                HIP_CHECK(hipFree(lower->halo_buf));
+                ela3d::device_free(lower->halo_buf, lower->halo_cap);
                 lower->halo_buf = nullptr;
                 lower->halo_cap = 0;
             }
-            HIP_CHECK(hipMalloc(&lower->halo_buf, need));
+            lower->halo_buf = ela3d::device_alloc(need);
             lower->halo_cap = need;
         }
         char* bup = static_cast<char*>(lower->halo_buf);
@@ -275,6 +277,12 @@ int ela3d_halo_exchange(ela3d_shot* lower, ela3d_shot* upper, int32_t phase)
         HIP_CHECK(hipStreamSynchronize(upper->impl->stream()));
         return ELA3D_OK;
     });
+}
+
+int ela3d_set_device_limit(int64_t bytes)
+{
+    ela3d::set_device_limit(bytes);
+    return ELA3D_OK;
 }
 
 int ela3d_step(ela3d_shot* s, int32_t n)

@@ -38,7 +38,7 @@ public:
     explicit DevicePool(size_t limit) : limit_(limit) {}
     ~DevicePool()
     {
-        for (auto& c : chunks_) (void)hipFree(c.base);
+        for (auto& c : chunks_) device_free(c.base, c.size);
     }
     // a 256-byte aligned range of `bytes`, or throws ELA3D_E_OOM
     void* alloc(size_t bytes)
@@ -48,10 +48,11 @@ public:
             if (void* p = take(q, bytes)) return p;
         size_t want = std::max(bytes, std::min(kChunk, limit_));
         void* base = nullptr;
-        if (hipMalloc(&base, want) != hipSuccess) {
-            (void)hipGetLastError();
+        try {
+            base = device_alloc(want);
+        } catch (const Error&) {
             want = bytes;  // no room for a whole chunk: just this shot
-            HIP_CHECK(hipMalloc(&base, want));
+            base = device_alloc(want);
         }
         chunks_.push_back(Chunk{static_cast<char*>(base), want, {}});
         chunks_.back().free[0] = want;

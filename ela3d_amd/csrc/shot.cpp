@@ -14,6 +14,7 @@
 #include "geom.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -23,6 +24,40 @@ namespace ela3d {
 
 // axis family of each memory variable (index 0..17 -> field id 9..26)
 static const int kMemAxis[18] = {0, 1, 2, 0, 1, 0, 2, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2};
+
+namespace {
+std::atomic<int64_t> g_device_held{0};
+std::atomic<int64_t> g_device_limit{0};  // 0: no limit
+}  // namespace
+
+void set_device_limit(int64_t bytes) { g_device_limit = bytes > 0 ? bytes : 0; }
+
+void* device_alloc(size_t bytes)
+{
+    const int64_t lim = g_device_limit.load();
+    const int64_t held = g_device_held.fetch_add(int64_t(bytes)) + int64_t(bytes);
+    if (lim > 0 && held > lim) {
+        g_device_held -= int64_t(bytes);
+        throw Error{ELA3D_E_OOM, "device memory limit: " + std::to_string(bytes) + " more bytes would exceed " +
+                                     std::to_string(lim)};
+    }
+    void* p = nullptr;
+    const hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        g_device_held -= int64_t(bytes);
+        throw Error{e == hipErrorOutOfMemory ? ELA3D_E_OOM : ELA3D_E_HIP,
+                    std::string("hipMalloc: ") + hipGetErrorString(e)};
+    }
+    return p;
+}
+
+void device_free(void* p, size_t bytes)
+{
+    if (!p) return;
+    (void)hipFree(p);
+    g_device_held -= int64_t(bytes);
+}
 
 Tuning choose_tuning(const ShotPlan& p)
 {
@@ -257,7 +292,7 @@ private:
         if (aux_) (void)hipStreamSynchronize(aux_);
         for (auto& g : graphs_)
             if (g) (void)hipGraphExecDestroy(g);
-        if (arena_ && !mem_.arena) (void)hipFree(arena_);
+        if (arena_ && !mem_.arena) device_free(arena_, size_t(arena_bytes_));
         if (h_flag_ && h_flag_ != mem_.host_flag) (void)hipHostFree(h_flag_);
         if (aux_) (void)hipStreamDestroy(aux_);
         for (auto& e : fork_)
@@ -734,7 +769,7 @@ private:
                                                std::to_string(mem_.bytes) + " available"};
             arena_ = mem_.arena;
         } else {
-            HIP_CHECK(hipMalloc(&arena_, off));
+            arena_ = device_alloc(off);
         }
         char* b = static_cast<char*>(arena_);
         for (int q = 0; q < 9; ++q) {

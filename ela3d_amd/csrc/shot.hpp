@@ -126,6 +126,14 @@ struct ShotMem {
 std::unique_ptr<Shot> make_shot(ShotPlan plan, const float* cp, const float* cs, const float* rho,
                                 hipStream_t external_stream = nullptr, bool double_buffer = false,
                                 const ShotMem& mem = ShotMem());
+// Device memory the library holds (shot arenas, the server's pool, halo staging)
+// is taken and returned through these.  A process-wide limit on the bytes held at
+// once (ela3d_set_device_limit) lets tests force ELA3D_E_OOM; device_alloc throws
+// Error{ELA3D_E_OOM} past it or when hipMalloc fails.
+void* device_alloc(size_t bytes);
+void device_free(void* p, size_t bytes);
+void set_device_limit(int64_t bytes);  // <= 0: no limit
+
 // device address of the allocation-free entry point's static buffer on `device`
 // (ELA3D_NOALLOC_BYTES long; looked up once per device)
 void* noalloc_scratch(int device);

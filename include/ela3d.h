@@ -153,7 +153,11 @@ typedef struct ela3d_kernel_stats {
 } ela3d_kernel_stats;
 
 /* ---- shot API (replaces main() CPU:17-726 / ZC:487-1506) ---- */
-/* The reference's literal parameter block (CPU:19-78, 146-153) as defaults. */
+/* Defaults: the homogeneous 101^3 cube workload (the reference's medium and C-PML,
+ * CPU:41-78, 146-153, with dt = 1.6e-3, a Ricker force source at f0 = 7 Hz at the
+ * centre and rigid faces).  The reference's own literals (f0 = 35, a derivative-
+ * of-Gaussian sigma source, dt = 1e-3, sampgat = 2, its quirks) are
+ * Config.from_reference in the Python binding and reference=1 in the CLI. */
 void ela3d_config_default(ela3d_config* cfg);
 /* Full config check without a GPU (the reference's only check is CPU:631-634). */
 int ela3d_validate(const ela3d_config* cfg);
@@ -321,6 +325,11 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
 int ela3d_tuning_set(const ela3d_config* cfg, const ela3d_tuning* t);
 int ela3d_tuning_get(const ela3d_config* cfg, ela3d_tuning* t);
 
+/* Failure injection (no reference counterpart): cap the device bytes the library
+ * holds at once (shot memory, the server's pool, halo staging), process-wide; an
+ * allocation past the cap fails with ELA3D_E_OOM as a real out-of-memory would.
+ * bytes <= 0 removes the cap. */
+int ela3d_set_device_limit(int64_t bytes);
 /* Error text (replaces HANDLE_ERROR's printf + exit, HOM:153-162). */
 const char* ela3d_strerror(int code);
 const char* ela3d_last_error(void);
