@@ -326,7 +326,7 @@ class Server:
 
     def __init__(self, device: int = 0, hbm_budget_bytes: Optional[int] = None,
                  max_active: Optional[int] = None, steps_per_round: Optional[int] = None,
-                 use_graph: bool = True, plain_loop: bool = False):
+                 use_graph: bool = True, plain_loop: bool = False, batch_cells: Optional[int] = None):
         from ._lib import ServerOpts
         L = lib()
         o = ServerOpts()
@@ -340,6 +340,8 @@ class Server:
             o.steps_per_round = int(steps_per_round)
         o.use_graph = 1 if use_graph else 0
         o.plain_loop = 1 if plain_loop else 0
+        if batch_cells is not None:
+            o.batch_cells = int(batch_cells)
         self.opts = o
         h = C.c_void_p()
         check(L.ela3d_server_create(C.byref(o), C.byref(h)))
@@ -400,9 +402,10 @@ class Server:
         self._cfg.pop(ticket, None)
 
     def stats(self) -> dict:
-        out = (C.c_int64 * 8)()
+        out = (C.c_int64 * 10)()
         check(lib().ela3d_server_stats(self._h, out))
-        keys = ["submitted", "done", "failed", "active", "queued", "bytes_in_use", "peak_bytes", "rounds"]
+        keys = ["submitted", "done", "failed", "active", "queued", "bytes_in_use", "peak_bytes", "rounds",
+                "batched", "batch_rounds"]
         return dict(zip(keys, list(out)))
 
 

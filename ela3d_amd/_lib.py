@@ -76,7 +76,7 @@ class ServerOpts(C.Structure):
     _fields_ = [
         ("struct_size", C.c_uint32), ("device", C.c_int32), ("hbm_budget_bytes", C.c_int64),
         ("max_active", C.c_int32), ("steps_per_round", C.c_int32), ("use_graph", C.c_int32),
-        ("plain_loop", C.c_int32), ("reserved", C.c_int32 * 8),
+        ("plain_loop", C.c_int32), ("batch_cells", C.c_int32), ("reserved", C.c_int32 * 7),
     ]
 
 

@@ -805,9 +805,8 @@ __device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B
 // each block branches to its box's body (one kernel boundary less per pass).
 // ---------------------------------------------------------------------------
 template <typename T, int MODE, int V>
-__global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
+__device__ __forceinline__ void stress_block(const StepArgs<T>& A, int lb)
 {
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
     const Box B = MODE == kInterior ? A.box[0] : pick_box(A.box, A.nbox, lb);
     int chunk, zs;
     tile_of(B, lb - B.blk0, chunk, zs);
@@ -821,10 +820,32 @@ __global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
     flush_maxima(A, S, 3, 18);
 }
 
-template <typename T, int MODE, bool ENERGY, int V>
-__global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
+template <typename T, int MODE, int V>
+__global__ void __launch_bounds__(256) k_stress(const StepArgs<T> A)
+{
+    stress_block<T, MODE, V>(A, xcd_block(blockIdx.x, gridDim.x));
+}
+
+// shot of logical block lb in a batch launch (blk0 ascending, nshot small)
+__device__ __forceinline__ int batch_shot(const int* blk0, int nshot, int lb)
+{
+    int s = 0;
+    while (s + 1 < nshot && lb >= blk0[s + 1]) ++s;
+    return s;
+}
+
+template <typename T, int V>
+__global__ void __launch_bounds__(256) k_stress_multi(const StepArgs<T>* __restrict__ args,
+                                                      const int* __restrict__ blk0, int nshot)
 {
     const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int s = batch_shot(blk0, nshot, lb);
+    stress_block<T, kMixed, V>(args[s], lb - blk0[s]);
+}
+
+template <typename T, int MODE, bool ENERGY, int V>
+__device__ __forceinline__ void velocity_block(const StepArgs<T>& A, int lb, int eb)
+{
     const Box B = MODE == kInterior ? A.box[0] : pick_box(A.box, A.nbox, lb);
     int chunk, zs;
     tile_of(B, lb - B.blk0, chunk, zs);
@@ -855,11 +876,43 @@ __global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
                 a += red[0][w];
                 b += red[1][w];
             }
-            A.epart[2 * blockIdx.x] = a;
-            A.epart[2 * blockIdx.x + 1] = b;
+            A.epart[2 * eb] = a;
+            A.epart[2 * eb + 1] = b;
         }
     }
 }
+
+template <typename T, int MODE, bool ENERGY, int V>
+__global__ void __launch_bounds__(256) k_velocity(const StepArgs<T> A)
+{
+    velocity_block<T, MODE, ENERGY, V>(A, xcd_block(blockIdx.x, gridDim.x), blockIdx.x);
+}
+
+template <typename T, int V>
+__global__ void __launch_bounds__(256) k_velocity_multi(const StepArgs<T>* __restrict__ args,
+                                                        const int* __restrict__ blk0, int nshot)
+{
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int s = batch_shot(blk0, nshot, lb);
+    velocity_block<T, kMixed, false, V>(args[s], lb - blk0[s], lb - blk0[s]);
+}
+
+template <typename T>
+void launch_stress_multi(const StepArgs<T>* args, const int* blk0, int nshot, int nblocks, int block, hipStream_t s)
+{
+    if (nshot > 0 && nblocks > 0)
+        hipLaunchKernelGGL((k_stress_multi<T, 1>), dim3(nblocks), dim3(block), 0, s, args, blk0, nshot);
+}
+
+template <typename T>
+void launch_velocity_multi(const StepArgs<T>* args, const int* blk0, int nshot, int nblocks, int block,
+                           hipStream_t s)
+{
+    if (nshot > 0 && nblocks > 0)
+        hipLaunchKernelGGL((k_velocity_multi<T, 1>), dim3(nblocks), dim3(block), 0, s, args, blk0, nshot);
+}
+template void launch_stress_multi<float>(const StepArgs<float>*, const int*, int, int, int, hipStream_t);
+template void launch_velocity_multi<float>(const StepArgs<float>*, const int*, int, int, int, hipStream_t);
 
 template <typename T, int M, int V>
 static void launch_stress_v(const StepArgs<T>& a, hipStream_t s)

@@ -122,6 +122,18 @@ template <typename T>
 size_t fused_lds_bytes(int nx, int tile);
 template <typename T>
 void launch_tail(const TailArgs<T>& a, hipStream_t s);
+// A batch of shots stepped together (each a merged launch over all its boxes,
+// in-place, no energy): args[s] in device memory; shot s owns the logical
+// blocks [blk0[s], blk0[s+1]) of one launch of blk0[nshot] blocks of `block`
+// threads, VEC 1.  The tails run one workgroup per shot.
+template <typename T>
+void launch_stress_multi(const StepArgs<T>* args, const int* blk0, int nshot, int nblocks, int block, hipStream_t s);
+template <typename T>
+void launch_velocity_multi(const StepArgs<T>* args, const int* blk0, int nshot, int nblocks, int block,
+                           hipStream_t s);
+template <typename T>
+void launch_tail_multi(const TailArgs<T>* args, int nshot, hipStream_t s);
+constexpr int kMaxBatch = 64;  // shots in one batch launch
 void launch_copy(const float4* src, float4* dst, long n4, int variant, hipStream_t s);
 // zero `bytes` bytes at p (p 16-byte aligned) with a kernel on stream s
 void launch_zero(void* p, size_t bytes, hipStream_t s);

@@ -131,6 +131,8 @@ struct Request {
     std::unique_ptr<Shot> shot;
     void* mem = nullptr;         // its range of the device pool
     int slot = -1;
+    bool batched = false;        // steps in the server's batch launches
+    int inflight = 0;            // batch rounds in flight that include it
     hipEvent_t ev[2] = {nullptr, nullptr};
     int ev_head = 0, ev_count = 0;
     int enqueued = 0;
@@ -148,6 +150,14 @@ public:
                 HIP_CHECK(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
                 for (auto& e : sl.ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
                 HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&sl.flag), sizeof(int), hipHostMallocDefault));
+            }
+            if (batching()) {
+                Slot& b = batch_.slot;
+                HIP_CHECK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+                for (auto& e : b.ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&batch_.staging), 3 * kArgsBytes,
+                                        hipHostMallocDefault));
+                batch_.dargs = pool_.alloc(kArgsBytes);
             }
         } catch (...) {
             free_slots();
@@ -168,9 +178,18 @@ public:
         free_slots();
     }
 
+    bool batching() const { return !opts_.plain_loop && opts_.batch_cells > 0 && opts_.max_active > 1; }
+
     void free_slots()
     {
         (void)hipSetDevice(opts_.device);
+        if (batch_.slot.stream) (void)hipStreamSynchronize(batch_.slot.stream);
+        for (auto& e : batch_.slot.ev)
+            if (e) (void)hipEventDestroy(e);
+        if (batch_.slot.stream) (void)hipStreamDestroy(batch_.slot.stream);
+        if (batch_.staging) (void)hipHostFree(batch_.staging);
+        batch_.slot = Slot();
+        batch_.staging = nullptr;
         for (Slot& sl : slots_) {
             if (sl.stream) (void)hipStreamSynchronize(sl.stream);
             for (auto& e : sl.ev)
@@ -280,6 +299,8 @@ public:
         out[5] = in_use_;
         out[6] = peak_;
         out[7] = rounds_.load();
+        out[8] = batched_.load();
+        out[9] = batch_rounds_.load();
     }
 
 private:
@@ -326,11 +347,15 @@ private:
             }
             for (Request* r : admitted) start(r);
             bool progressed = false;
-            for (size_t q = 0; q < active_.size(); ++q) progressed |= advance(active_[q]);
+            for (size_t q = 0; q < active_.size(); ++q)
+                if (!active_[q]->batched) progressed |= advance(active_[q]);
+            if (batching()) progressed |= advance_batch();
             // retire finished shots
             for (size_t q = 0; q < active_.size();) {
                 Request* r = active_[q];
-                if (r->shot == nullptr || (r->enqueued >= r->cfg.nstep && r->ev_count == 0)) {
+                const bool over = r->batched ? (r->enqueued >= r->cfg.nstep && r->inflight == 0)
+                                             : (r->enqueued >= r->cfg.nstep && r->ev_count == 0);
+                if (r->shot == nullptr || over) {
                     finish(r);
                     std::lock_guard<std::mutex> g(mu_);
                     active_.erase(active_.begin() + q);
@@ -380,6 +405,15 @@ private:
             }
             r->ev[0] = sl.ev[0];
             r->ev[1] = sl.ev[1];
+            if (batching() && !opts_.plain_loop && r->shot->plan().cells <= int64_t(opts_.batch_cells) &&
+                int(batch_.members.size()) < kMaxBatch) {
+                BatchMember probe;
+                if (r->shot->batch_member(kBatchBlock, probe)) {
+                    r->batched = true;
+                    batch_.members.push_back(r);
+                    ++batched_;
+                }
+            }
             std::vector<float>().swap(r->cp);  // the model now lives on the device
             std::vector<float>().swap(r->cs);
             std::vector<float>().swap(r->rho);
@@ -441,6 +475,107 @@ private:
         return moved;
     }
 
+    // Batch rounds: pop completed ones (crediting their members), then keep up to
+    // two in flight.  A round steps every member that still has steps by the
+    // smallest number left among them (at most steps_per_round), so no member
+    // passes its nstep; members that finish leave at the next round.
+    bool advance_batch()
+    {
+        Batch& B = batch_;
+        bool moved = false;
+        try {
+            while (B.ev_count > 0) {
+                const hipError_t q = hipEventQuery(B.slot.ev[B.ev_head]);
+                if (q == hipErrorNotReady) break;
+                HIP_CHECK(q);
+                Round& R = B.rounds[B.ev_head];
+                for (Request* r : R.who) {
+                    --r->inflight;
+                    if (r->shot && r->shot->unstable_seen() && r->status == ELA3D_OK) {
+                        r->status = ELA3D_E_UNSTABLE;  // blown up: no further rounds
+                        r->enqueued = r->cfg.nstep;
+                    }
+                }
+                R.who.clear();
+                B.ev_head ^= 1;
+                --B.ev_count;
+                moved = true;
+            }
+            // members still stepping
+            std::vector<Request*> step;
+            int n = opts_.steps_per_round;
+            for (Request* r : B.members)
+                if (r->shot && r->enqueued < r->cfg.nstep) {
+                    step.push_back(r);
+                    n = std::min(n, r->cfg.nstep - r->enqueued);
+                }
+            B.members = step;
+            if (step.empty() || B.ev_count >= 2) return moved;
+            hipStream_t st = B.slot.stream;
+            std::vector<uint64_t> tickets;
+            for (Request* r : step) tickets.push_back(r->ticket);
+            if (tickets != B.joined) {  // new membership: upload the members' arguments
+                char* h = B.staging + size_t(B.stage) * kArgsBytes;
+                B.stage = (B.stage + 1) % 3;  // a slot is reused only after its copy has run
+                auto* sa = reinterpret_cast<StepArgs<float>*>(h);
+                auto* va = sa + kMaxBatch;
+                auto* ta = reinterpret_cast<TailArgs<float>*>(va + kMaxBatch);
+                auto* blk = reinterpret_cast<int*>(ta + kMaxBatch);
+                int nb = 0;
+                for (size_t q = 0; q < step.size(); ++q) {
+                    BatchMember m;
+                    if (!step[q]->shot->batch_member(kBatchBlock, m)) throw Error{ELA3D_E_STATE, "shot left the batch"};
+                    sa[q] = m.stress;
+                    va[q] = m.velocity;
+                    ta[q] = m.tail;
+                    blk[q] = nb;
+                    nb += m.nblocks;
+                }
+                blk[step.size()] = nb;
+                B.nblocks = nb;
+                HIP_CHECK(hipMemcpyAsync(B.dargs, h, kArgsBytes, hipMemcpyHostToDevice, st));
+                B.joined = tickets;
+            }
+            const auto* sa = static_cast<const StepArgs<float>*>(B.dargs);
+            const auto* va = sa + kMaxBatch;
+            const auto* ta = reinterpret_cast<const TailArgs<float>*>(va + kMaxBatch);
+            const auto* blk = reinterpret_cast<const int*>(ta + kMaxBatch);
+            const int ns = int(step.size());
+            for (int it = 0; it < n; ++it) {
+                launch_stress_multi<float>(sa, blk, ns, B.nblocks, kBatchBlock, st);
+                launch_velocity_multi<float>(va, blk, ns, B.nblocks, kBatchBlock, st);
+                launch_tail_multi<float>(ta, ns, st);
+            }
+            HIP_CHECK(hipGetLastError());
+            for (Request* r : step) {
+                r->shot->enqueue_flag_readback(st);
+                r->shot->set_steps_done(r->enqueued + n);
+                r->enqueued += n;
+                ++r->inflight;
+            }
+            const int slot = (B.ev_head + B.ev_count) & 1;
+            B.rounds[slot].who = step;
+            B.rounds[slot].steps = n;
+            HIP_CHECK(hipEventRecord(B.slot.ev[slot], st));
+            ++B.ev_count;
+            ++batch_rounds_;
+            moved = true;
+        } catch (const Error& e) {  // the batch cannot go on: fail its members
+            (void)hipStreamSynchronize(B.slot.stream);
+            for (Request* r : B.members) {
+                if (r->status == ELA3D_OK) r->status = e.code;
+                r->enqueued = r->cfg.nstep;
+                r->inflight = 0;
+            }
+            for (auto& R : B.rounds) R.who.clear();
+            B.members.clear();
+            B.joined.clear();
+            B.ev_count = 0;
+            moved = true;
+        }
+        return moved;
+    }
+
     void finish(Request* r)
     {
         if (r->shot) {
@@ -476,6 +611,28 @@ private:
     ela3d_server_opts opts_;
     DevicePool pool_;             // device thread only (and the destructor)
     std::vector<Slot> slots_;
+    // Batch of small shots stepped together: per step one stress, one velocity and
+    // one tail launch over all members' boxes (args from device memory), in
+    // rounds on the batch's own stream with up to two rounds in flight.
+    struct Round {
+        std::vector<Request*> who;
+        int steps = 0;
+    };
+    struct Batch {
+        Slot slot;
+        std::vector<Request*> members;  // stepping, in argument order
+        std::vector<uint64_t> joined;   // tickets of the members as of the last argument upload
+        Round rounds[2];
+        int ev_head = 0, ev_count = 0;
+        void* dargs = nullptr;          // device: stress[kMaxBatch], velocity[kMaxBatch], tail[kMaxBatch], blk0
+        char* staging = nullptr;        // pinned: three slots of the same layout
+        int stage = 0;
+        int nblocks = 0;
+    } batch_;
+    static constexpr size_t kArgsBytes =
+        ((2 * kMaxBatch * sizeof(StepArgs<float>) + kMaxBatch * sizeof(TailArgs<float>) + (kMaxBatch + 1) * sizeof(int)) +
+         255) & ~size_t(255);
+    static constexpr int kBatchBlock = 128;
     std::chrono::steady_clock::time_point t0_;
     std::thread th_;
     std::mutex mu_;
@@ -487,6 +644,7 @@ private:
     int64_t in_use_ = 0, peak_ = 0;
     int64_t submitted_ = 0, done_ = 0, failed_ = 0;
     std::atomic<int64_t> rounds_{0};  // device-thread passes that moved any shot
+    std::atomic<int64_t> batched_{0}, batch_rounds_{0};  // shots that joined the batch; batch rounds run
     bool stop_ = false;
 };
 
@@ -525,6 +683,7 @@ void ela3d_server_opts_default(ela3d_server_opts* o)
     o->steps_per_round = 50;
     o->use_graph = 1;
     o->plain_loop = 0;
+    o->batch_cells = 128 * 128 * 128;
 }
 
 int ela3d_server_create(const ela3d_server_opts* o, ela3d_server** out)

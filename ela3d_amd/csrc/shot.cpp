@@ -444,7 +444,33 @@ public:
         HIP_CHECK(hipStreamSynchronize(stream_));
     }
 
-    void enqueue_flag_readback() override
+    bool batch_member(int block, BatchMember& out) override
+    {
+        if constexpr (!std::is_same<T, float>::value) {
+            (void)block;
+            (void)out;
+            return false;
+        } else {
+            if (dbuf_ || plan_.cfg.energy_every > 0 || plan_.koff != 0 || plan_.own_k1 != plan_.cfg.nz) return false;
+            Tuning t = tune_;
+            t.block = block;
+            t.vec = 1;
+            t.merged = 1;
+            t.fused = 0;
+            t.overlap = 0;
+            t.dbuf = 0;
+            LaunchGeom slab, all;
+            geometry(plan_, t, slab, all);
+            if (all.nblocks > epart_cap_) return false;
+            out.stress = step_args(all, d_epart_, 0, 0);
+            out.velocity = step_args(all, d_epart_, 1, 0);
+            out.tail = tail_args(0);
+            out.nblocks = all.nblocks;
+            return true;
+        }
+    }
+
+    void enqueue_flag_readback(hipStream_t on) override
     {
         if (!h_flag_) {
             if (mem_.host_flag)
@@ -453,7 +479,7 @@ public:
                 HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_flag_), sizeof(int), hipHostMallocDefault));
             *h_flag_ = 0;
         }
-        HIP_CHECK(hipMemcpyAsync(h_flag_, d_flag_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        HIP_CHECK(hipMemcpyAsync(h_flag_, d_flag_, sizeof(int), hipMemcpyDeviceToHost, on ? on : stream_));
     }
 
     bool unstable_seen() const override { return h_flag_ && *static_cast<volatile int*>(h_flag_) != 0; }

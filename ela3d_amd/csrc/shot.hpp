@@ -59,6 +59,14 @@ struct ArenaLayout {
 };
 ArenaLayout arena_layout(const ShotPlan& p, int elem_bytes, int nepart, bool dbuf);
 
+// A shot's share of a batch launch (FP32 storage, in place, no energy sums):
+// its merged-launch arguments for both passes, its tail, and its block count.
+struct BatchMember {
+    StepArgs<float> stress, velocity;
+    TailArgs<float> tail;
+    int nblocks = 0;
+};
+
 class Shot {
 public:
     virtual ~Shot() = default;
@@ -70,7 +78,11 @@ public:
     virtual int sync() = 0;                         // ELA3D_OK or ELA3D_E_UNSTABLE
     // queue a copy of the instability flag into pinned host memory; after the
     // stream has passed that point, unstable_seen() reports it without a sync
-    virtual void enqueue_flag_readback() = 0;
+    virtual void enqueue_flag_readback(hipStream_t on = nullptr) = 0;  // on: another stream
+    // Arguments of this shot as a member of a batch launch of `block` threads
+    // (one merged launch per pass over all its boxes); false if the shot cannot
+    // be batched (not FP32, double-buffered or with energy sums).
+    virtual bool batch_member(int block, BatchMember& out) = 0;
     virtual bool unstable_seen() const = 0;
     virtual void seismograms(double* out) = 0;      // nsamples*nrec*3
     virtual void energy(double* out) = 0;           // nenergy*2

@@ -33,7 +33,7 @@ __device__ __forceinline__ void inject(const TailArgs<T>& A, int q, int o, typen
 constexpr int kTailThreads = 1024;  // wide enough to sum ~10^5 energy partials quickly
 
 template <typename T>
-__global__ void __launch_bounds__(kTailThreads) k_tail(const TailArgs<T> A)
+__device__ __forceinline__ void tail_body(const TailArgs<T>& A)
 {
     typedef typename Arith<T>::R R;
     __shared__ double red[2][kTailThreads / 64];
@@ -177,6 +177,20 @@ __global__ void __launch_bounds__(kTailThreads) k_tail(const TailArgs<T> A)
             A.scl[81 + q] = 1.f / sc;
         }
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kTailThreads) k_tail(const TailArgs<T> A)
+{
+    tail_body(A);
+}
+
+// The tails of a batch of shots stepped together: one workgroup per shot, its
+// arguments from device memory.
+template <typename T>
+__global__ void __launch_bounds__(kTailThreads) k_tail_multi(const TailArgs<T>* __restrict__ args)
+{
+    tail_body(args[blockIdx.x]);
 }
 
 // HBM-ceiling calibration copies.  variant 0: grid-stride float4; 1: one
@@ -419,6 +433,12 @@ void launch_copy(const float4* src, float4* dst, long n4, int variant, hipStream
     }
 }
 
+template <typename T>
+void launch_tail_multi(const TailArgs<T>* args, int nshot, hipStream_t s)
+{
+    if (nshot > 0) hipLaunchKernelGGL((k_tail_multi<T>), dim3(nshot), dim3(kTailThreads), 0, s, args);
+}
+template void launch_tail_multi<float>(const TailArgs<float>*, int, hipStream_t);
 template void launch_tail<float>(const TailArgs<float>&, hipStream_t);
 template void launch_tail<double>(const TailArgs<double>&, hipStream_t);
 template void launch_tail<half_t>(const TailArgs<half_t>&, hipStream_t);

@@ -271,7 +271,10 @@ typedef struct ela3d_server_opts {
     int32_t steps_per_round;   /* steps enqueued per shot between completion checks (default 50) */
     int32_t use_graph;         /* replay a hipGraph per step (default 1) */
     int32_t plain_loop;        /* 1: reference-pattern one-shot-at-a-time loop */
-    int32_t reserved[8];
+    int32_t batch_cells;       /* FP32 shots of at most this many cells without energy sums
+                                * step together, one stress, one velocity and one tail launch
+                                * per step for all of them (default 128^3; 0: never) */
+    int32_t reserved[7];
 } ela3d_server_opts;
 typedef struct ela3d_result {
     uint64_t ticket;
@@ -296,7 +299,8 @@ int ela3d_wait(ela3d_server* s, uint64_t ticket, double timeout_s, ela3d_result*
 int ela3d_result_seismograms(ela3d_server* s, uint64_t ticket, double* out, size_t cap);
 int ela3d_result_energy(ela3d_server* s, uint64_t ticket, double* out, size_t cap);
 int ela3d_result_release(ela3d_server* s, uint64_t ticket);
-/* out[8]: submitted, done, failed, active, queued, device bytes in use, peak device bytes, rounds */
+/* out[10]: submitted, done, failed, active, queued, device bytes in use, peak device bytes, rounds,
+ * shots that stepped in the batch launches, batch rounds */
 int ela3d_server_stats(ela3d_server* s, int64_t* out);
 void ela3d_server_destroy(ela3d_server* s);
 
