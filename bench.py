@@ -108,7 +108,9 @@ def load_traffic(workload, kernel):
     with open(path) as f:
         d = json.load(f)
     k = d.get("kernels", {}).get(kernel)
-    return (None if k is None else k.get("hbm_bytes_per_dispatch")), d.get("launch_shape")
+    if k is None:
+        return None, None
+    return k.get("hbm_bytes_per_dispatch"), k.get("launch_shape", d.get("launch_shape"))
 
 
 def accuracy_run(workload, nstep=None):
@@ -218,6 +220,49 @@ def serving_summary(nshots=40, nstep_range=(100, 400), seed=11, max_active=8, st
     out["speedup_vs_plain_loop"] = round(b / out["plain_loop"]["shots_per_s"], 3)
     log(f"[bench] serving: {json.dumps(out)}")
     return out
+
+
+def slabs_main(args):
+    """The workload as args.slabs equal z-slabs on one GPU: each step is the stress
+    phase of every slab, the halo exchanges, the velocity phase, the exchanges."""
+    import ela3d_amd
+    from ela3d_amd import Shot
+    K, W = args.steps, args.warmup
+    cfg, model, desc = make_workload(args.workload, K + W)
+    n = args.slabs
+    cuts = [round(q * cfg.nz / n) for q in range(n + 1)]
+    slabs = [Shot(cfg, *model, slab=(a, b)) for a, b in zip(cuts[:-1], cuts[1:])]
+
+    def step():
+        for phase in (0, 1):
+            for s in slabs:
+                s.step_phase(phase)
+            for lower, upper in zip(slabs[:-1], slabs[1:]):
+                lower.halo_exchange(upper, phase)
+
+    try:
+        for _ in range(W):
+            step()
+        for s in slabs:
+            s.sync()
+        t = time.perf_counter()
+        for _ in range(K):
+            step()
+        for s in slabs:
+            s.sync()
+        wall = time.perf_counter() - t
+    finally:
+        for s in slabs:
+            s.close()
+    ms = wall * 1e3 / K
+    plane = cfg.nx * cfg.ny
+    out = {"metric": "z-slab step time on one GPU", "workload": desc, "slabs": n, "cuts": cuts,
+           "steps": K, "ms_per_step": round(ms, 4),
+           "mcell_updates_per_s": round(cfg.cells / (ms * 1e-3) / 1e6, 1),
+           "halo_bytes_per_interface_per_step": 6 * plane * 4,
+           "timing": "host wall over K steps; every phase's exchange waits for both slabs' streams"}
+    log(f"[bench] slabs: {json.dumps(out)}")
+    print(json.dumps(out), flush=True)
 
 
 def serving_main(args):
@@ -337,6 +382,10 @@ def main():
     ap.add_argument("--shape", default="",
                     help="fixed launch shape block,zseg,vec[,block_slab,zseg_slab,vec_slab"
                          "[,overlap,merged]] (interior, then absorbing layers; skips the tuner)")
+    ap.add_argument("--slabs", type=int, default=0,
+                    help="split the workload into this many z-slabs on the one GPU, step them by phases "
+                         "with halo exchanges, and report the step time (one GPU's share of a z "
+                         "decomposition at its real shape)")
     ap.add_argument("--shots", type=int, default=256, help="serving: shots in the stream")
     ap.add_argument("--seed", type=int, default=7, help="serving: stream seed")
     ap.add_argument("--max-active", type=int, default=8, help="serving: concurrent shots")
@@ -372,6 +421,8 @@ def main():
 
     if args.workload == "serving":
         return serving_main(args)
+    if args.slabs > 0:
+        return slabs_main(args)
     import ela3d_amd
     from ela3d_amd import Shot
     if ela3d_amd.device_count() < 1:
@@ -400,6 +451,7 @@ def main():
         log(f"[bench] autotune: interior block={tuned['block']} zseg={tuned['zseg_interior']} "
             f"vec={tuned['vec']}; layers block={tuned['block_slab']} zseg={tuned['zseg_slab']} "
             f"vec={tuned['vec_slab']} overlap={tuned['overlap']} merged={tuned['merged']} "
+            f"fused={tuned.get('fused', 0)} tile={tuned.get('tile', 0)} "
             f"({tuned['ms_per_step']:.3f} ms/step over "
             f"{tuned['candidates']} shapes, {time.perf_counter() - t:.1f} s)")
     with Shot(cfg, *model) as shot:
@@ -471,7 +523,7 @@ def main():
             "launch_shape": {"block": info["block"], "zseg": info["zseg"], "vec": info["vec"],
                              "block_slab": info["block_slab"], "zseg_slab": info["zseg_slab"],
                              "vec_slab": info["vec_slab"], "overlap": info["overlap"],
-                             "merged": info["merged"]},
+                             "merged": info["merged"], "fused": info["fused"], "tile": info["tile"]},
         },
         "roofline": {
             "bound": "hbm", "kernel": dom["name"], "achieved": round(achieved / 1e9, 2),
