@@ -224,6 +224,15 @@ ShotPlan make_plan(const ela3d_config& in)
     if (c.nrec > 65536) fail("at most 65536 receivers");
     if (c.storage == ELA3D_STORAGE_FP16 && c.fp16_rescale_every < 1)
         fail("fp16_rescale_every must be >= 1");
+    // An explosive source driven by the Gaussian (non-zero mean) leaves a static
+    // stress field around the source in all six stresses, far above the wave's,
+    // which sets every stress field's FP16 scale: the seismograms then miss the
+    // FP16 accuracy bar (4.1e-2 relative L2 against FP64 on a 24x26x46 grid).  The
+    // source cell's own stresses are kept in FP32 (side slot), which is enough for
+    // the reference's sigma source; this combination is refused instead.
+    if (c.storage == ELA3D_STORAGE_FP16 && c.src_kind == ELA3D_SRC_EXPLOSIVE && c.wavelet == ELA3D_WAVELET_GAUSSIAN)
+        fail("FP16 storage cannot hold the static stress field an explosive source with the Gaussian wavelet "
+             "leaves; use FP32 storage or a zero-mean wavelet");
     if (c.block != 0 && c.block != 64 && c.block != 128 && c.block != 256 && c.block != 512)
         fail("block must be 0, 64, 128, 256 or 512");
     if (c.zseg < 0) fail("zseg must be >= 0");

@@ -383,6 +383,17 @@ __device__ __forceinline__ void stress_load(const StepArgs<T>& A, const Lane<V>&
             }
         }
     }
+    if constexpr (Scales<T>::on) {  // the source cell's normal stresses from the side slot
+        if (A.side) {
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+                if (o + unsigned(e) == A.sidx) {
+                    P.sxx[e] = A.side[0];
+                    P.syy[e] = A.side[1];
+                    P.szz[e] = A.side[2];
+                }
+        }
+    }
     if (top) {  // zero ghost plane above the top (only read with the ghost quirk)
 #pragma unroll
         for (int e = 0; e < V; ++e) {
@@ -495,6 +506,20 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
                 }
             }
             nsyz[e] = DT_mu * (dvz_dy + dvy_dz) + P.syz[e];
+        }
+    }
+    if constexpr (Scales<T>::on) {  // the source cell's new normal stresses go to the side slot
+        if (A.side) {
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+                if (L.act[e] && o + unsigned(e) == A.sidx) {
+                    if (onN[e]) {
+                        A.side[0] = nsxx[e];
+                        A.side[1] = nsyy[e];
+                        A.side[2] = nszz[e];
+                    }
+                    onN[e] = false;  // the FP16 arrays keep 0 there and leave it out of their maxima
+                }
         }
     }
     stv<T, V>(fout(A, 3), o, nsxx, onN, S, 3);
@@ -614,6 +639,21 @@ __device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V
                 P.mz[0][e] = ldf(A.mi[11], om, S.r[20]);
                 P.mz[1][e] = ldf(A.mi[14], om, S.r[23]);
                 P.mz[2][e] = ldf(A.mi[17], om, S.r[26]);
+            }
+        }
+    }
+    if constexpr (Scales<T>::on) {  // the source cell's normal stresses from the side slot
+        if (A.side) {
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                const unsigned oe = o + unsigned(e);
+                if (oe == A.sidx) {
+                    P.sxx[e] = A.side[0];
+                    P.syy[e] = A.side[1];
+                }
+                if (oe - 1u == A.sidx) P.sxxW[e] = A.side[0];
+                if (oe + nx == A.sidx) P.syyN[e] = A.side[1];
+                if (dU && oe + dU == A.sidx) P.szz_u[e] = A.side[2];
             }
         }
     }
@@ -784,6 +824,13 @@ __device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B
         }
         ldv<T, V>(A.fi[5], o, S.r[5], Q.szz_c);
         ldm<V>(A.rho, o, Q.rho_c);
+        if constexpr (Scales<T>::on) {
+            if (A.side) {
+#pragma unroll
+                for (int e = 0; e < V; ++e)
+                    if (o + unsigned(e) == A.sidx) Q.szz_c[e] = A.side[2];
+            }
+        }
     }
     for (int k = kb; k < ke; ++k) {
         VPlane<R, V> P;

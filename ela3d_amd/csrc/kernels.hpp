@@ -62,6 +62,13 @@ struct StepArgs {
     int ek0, ek1;            // energy box planes [ek0, ek1) (a slab's local indices of the global box)
     double dt, idx, idy, idz;
     const float* scl;        // FP16 storage scales, kScaleTable floats (layout below)
+    // FP16 storage with a stress source: the source cell's sigma_xx, sigma_yy,
+    // sigma_zz live unscaled in side[0..2] (FP32) instead of the FP16 arrays,
+    // which hold 0 there, so the static stress a non-zero-mean wavelet builds up
+    // at that one cell neither loses precision nor sets the fields' scales.
+    // sidx: the cell's element offset (as cell()).  side == nullptr: off.
+    float* side;
+    unsigned sidx;
     unsigned* maxslot;       // [kMaxSlots][27] |value| maxima as float bits (FP16 storage)
     double* epart;           // energy partials [nblocks][2]
     Box box[kMaxBox];
@@ -98,6 +105,7 @@ struct TailArgs {
     int* step;               // steps completed (device counter)
     int* flag;               // set to 1 when a sample or the energy is not finite
     float* scl;              // FP16 storage scale table (see kScaleTable)
+    float* side;             // FP16 with a stress source: the source cell's normal stresses (see StepArgs)
     unsigned* maxslot;       // [kMaxSlots][27] maxima folded by this step's stencil kernels
     int rescale_every;       // FP16 storage: re-choose every scale each N steps
     double* srctrace;        // [nstep+1][3] sigma_xx, sigma_yy, sigma_zz at the source after injection

@@ -218,6 +218,7 @@ ArenaLayout arena_layout(const ShotPlan& p, int elem, int nepart, bool dbuf)
     L.slot = off; off += al(size_t(kMaxSlots) * 27 * sizeof(unsigned));
     L.srctrace = off; off += al((size_t(c.nstep) + 1) * 3 * sizeof(double));
     L.tabf = off; off += al(6 * size_t(c.nx + c.ny) * sizeof(float));
+    L.side = off; off += al(3 * sizeof(float));
     L.total = off;
     return L;
 }
@@ -548,6 +549,11 @@ public:
             copy_planes(h.data(), d_f_[cur()][field], sizeof(T), hipMemcpyDeviceToHost);
             const double inv = 1.0 / current_scale(field);
             for (size_t q = 0; q < cells; ++q) out[q] = double(h[q]) * inv;
+            if (side_mode() && field >= 3 && field <= 5) {  // the source cell lives in the side slot
+                float v = 0.f;
+                HIP_CHECK(hipMemcpy(&v, d_side_ + (field - 3), sizeof(float), hipMemcpyDeviceToHost));
+                out[src_cell()] = v;
+            }
             return;
         }
         const int m = field - 9, axis = kMemAxis[m];
@@ -567,10 +573,17 @@ public:
         const size_t cells = size_t(plan_.cells);
         if (field < 9) {
             std::vector<T> h(cells);
+            const bool side = side_mode() && field >= 3 && field <= 5;
+            const size_t sc_cell = side ? src_cell() : cells;  // kept out of the FP16 array and its scale
             double mx = 0.0;
-            for (size_t q = 0; q < cells; ++q) mx = std::max(mx, std::fabs(in[q]));
+            for (size_t q = 0; q < cells; ++q)
+                if (q != sc_cell) mx = std::max(mx, std::fabs(in[q]));
             const double sc = host_scale(field, mx);
-            for (size_t q = 0; q < cells; ++q) h[q] = T(in[q] * sc);
+            for (size_t q = 0; q < cells; ++q) h[q] = q == sc_cell ? T(0) : T(in[q] * sc);
+            if (side) {
+                const float v = float(in[sc_cell]);
+                HIP_CHECK(hipMemcpy(d_side_ + (field - 3), &v, sizeof(float), hipMemcpyHostToDevice));
+            }
             for (int b = 0; b < (dbuf_ ? 2 : 1); ++b)  // both sets: cells a step never writes keep it
                 copy_planes(d_f_[b][field], h.data(), sizeof(T), hipMemcpyHostToDevice);
             return;
@@ -820,6 +833,7 @@ private:
         d_maxslot_ = reinterpret_cast<unsigned*>(b + o_slot);
         d_srctrace_ = reinterpret_cast<double*>(b + o_src);
         d_tabf_ = reinterpret_cast<float*>(b + L.tabf);
+        d_side_ = reinterpret_cast<float*>(b + L.side);
         // zero everything: wavefields and memory variables (CPU:107-140), the guards
         // and plane padding (read as neighbours, never written), outputs
         if (mem_.arena) {  // memory the shot does not own may hold a previous user's data
@@ -971,6 +985,8 @@ private:
         a.idz = 1.0 / c.dz;
         a.scl = d_scl_;
         a.maxslot = d_maxslot_;
+        a.side = side_mode() ? d_side_ : nullptr;
+        a.sidx = unsigned(guard_) + unsigned(src_offset());
         a.epart = epart;
         for (int q = 0; q < g.nbox; ++q) a.box[q] = g.box[q];
         a.nbox = g.nbox;
@@ -1035,6 +1051,7 @@ private:
         a.flag = d_flag_;
         a.scl = d_scl_;
         a.maxslot = d_maxslot_;
+        a.side = side_mode() ? d_side_ : nullptr;
         a.rescale_every = std::max(1, c.fp16_rescale_every);
         a.srctrace = d_srctrace_;
         a.nsrctrace = c.nstep + 1;
@@ -1159,6 +1176,25 @@ private:
     double* d_srctrace_ = nullptr;   // [nstep+1][3]
     float* d_tabf_ = nullptr;        // FP32 x and y tables [6][nx], [6][ny]
     float* d_scl_ = nullptr;         // FP16 scale table (kScaleTable floats)
+    float* d_side_ = nullptr;        // FP16 with a stress source: the source cell's normal stresses
+    // FP16 storage with a stress source keeps the source cell's sigma_xx/yy/zz in d_side_
+    bool side_mode() const
+    {
+        const int k = plan_.cfg.src_kind;
+        return std::is_same<T, half_t>::value && (k == ELA3D_SRC_REFERENCE_SIGMA || k == ELA3D_SRC_EXPLOSIVE);
+    }
+    // index of the source cell in an x-fastest nx*ny*nz host array
+    size_t src_cell() const
+    {
+        const ela3d_config& c = plan_.cfg;
+        return size_t(c.isrc) + size_t(c.jsrc) * c.nx + size_t(c.ksrc) * c.nx * c.ny;
+    }
+    // element offset of the source cell from plane 0 of a field
+    int64_t src_offset() const
+    {
+        const ela3d_config& c = plan_.cfg;
+        return int64_t(c.isrc) + int64_t(c.jsrc) * c.nx + int64_t(c.ksrc) * nxyp_;
+    }
     unsigned* d_maxslot_ = nullptr;  // [kMaxSlots][27]
 
     // scale of the data currently stored for field q (1 unless FP16 storage)

@@ -54,7 +54,7 @@ int64_t guard_elems(int nx);
 
 struct ArenaLayout {
     size_t f[9], m[18], f2[9], m2[18], model[3], tab[3];
-    size_t wav, rec, seis, energy, epart, epart2, ctl, scl, slot, srctrace, tabf, total;
+    size_t wav, rec, seis, energy, epart, epart2, ctl, scl, slot, srctrace, tabf, side, total;
     int epart_cap;
 };
 ArenaLayout arena_layout(const ShotPlan& p, int elem_bytes, int nepart, bool dbuf);

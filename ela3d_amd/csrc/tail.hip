@@ -26,6 +26,29 @@ __device__ __forceinline__ void inject(const TailArgs<T>& A, int q, int o, typen
     inj[q] = fabsf((float)v);
 }
 
+// stress injection at the source cell: into the side slot where FP16 storage
+// keeps the source cell's normal stresses there (see StepArgs::side)
+template <typename T>
+__device__ __forceinline__ void inject_sigma(const TailArgs<T>& A, int q, int o, typename Arith<T>::R add, float* inj)
+{
+    if constexpr (std::is_same<T, half_t>::value) {
+        if (A.side) {
+            A.side[q - 3] = A.side[q - 3] + (float)add;
+            return;
+        }
+    }
+    inject(A, q, o, ldf(A.f[q], o, nisc<T>(A, q)) + add, inj);
+}
+
+template <typename T>
+__device__ __forceinline__ double sigma_at_source(const TailArgs<T>& A, int q, int o)
+{
+    if constexpr (std::is_same<T, half_t>::value) {
+        if (A.side) return (double)A.side[q - 3];
+    }
+    return (double)ldf(A.f[q], o, nisc<T>(A, q));
+}
+
 // ---------------------------------------------------------------------------
 // Per-step tail (one workgroup): receivers (before the source, as CPU:593-602
 // precedes CPU:604-622), energy reduction, source injection, step counter.
@@ -95,9 +118,9 @@ __device__ __forceinline__ void tail_body(const TailArgs<T>& A)
             switch (A.src_kind) {
             case 0: {  // CPU:620-622
                 const R fx = (R)(A.sin_t * st), fy = (R)(A.cos_t * st);
-                inject(A, 3, o, ldf(A.f[3], o, nisc<T>(A, 3)) + fx * DT * l2m, inj);
-                inject(A, 4, o, ldf(A.f[4], o, nisc<T>(A, 4)) + fx * DT * l2m, inj);
-                inject(A, 5, o, ldf(A.f[5], o, nisc<T>(A, 5)) + fy * DT * l2m, inj);
+                inject_sigma(A, 3, o, fx * DT * l2m, inj);
+                inject_sigma(A, 4, o, fx * DT * l2m, inj);
+                inject_sigma(A, 5, o, fy * DT * l2m, inj);
                 break;
             }
             case 1: {  // CPU:614-617
@@ -110,9 +133,9 @@ __device__ __forceinline__ void tail_body(const TailArgs<T>& A)
             }
             case 2: {
                 const R s = (R)st;
-                inject(A, 3, o, ldf(A.f[3], o, nisc<T>(A, 3)) + s * DT * l2m, inj);
-                inject(A, 4, o, ldf(A.f[4], o, nisc<T>(A, 4)) + s * DT * l2m, inj);
-                inject(A, 5, o, ldf(A.f[5], o, nisc<T>(A, 5)) + s * DT * l2m, inj);
+                inject_sigma(A, 3, o, s * DT * l2m, inj);
+                inject_sigma(A, 4, o, s * DT * l2m, inj);
+                inject_sigma(A, 5, o, s * DT * l2m, inj);
                 break;
             }
             default: {  // ZC:451-453
@@ -129,9 +152,9 @@ __device__ __forceinline__ void tail_body(const TailArgs<T>& A)
         }
         if (it < A.nsrctrace) {  // stresses at the source cell after injection (UNI:463-465)
             const int o = A.src;
-            A.srctrace[3 * it] = (double)ldf(A.f[3], o, nisc<T>(A, 3));
-            A.srctrace[3 * it + 1] = (double)ldf(A.f[4], o, nisc<T>(A, 4));
-            A.srctrace[3 * it + 2] = (double)ldf(A.f[5], o, nisc<T>(A, 5));
+            A.srctrace[3 * it] = sigma_at_source(A, 3, o);
+            A.srctrace[3 * it + 1] = sigma_at_source(A, 4, o);
+            A.srctrace[3 * it + 2] = sigma_at_source(A, 5, o);
         }
         if (bad) *A.flag = 1;
         *A.step = it;

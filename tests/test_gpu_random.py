@@ -101,14 +101,9 @@ NFP16 = 8
 
 def draw_fp16(seed, storage=STORAGE_FP16):
     """One random multi-step case from rest: (config, launch shape), FP16 storage
-    unless `storage` says otherwise.
-
-    Stress sources (the reference's sigma source, the explosive source) are drawn
-    with zero-mean wavelets only.  A Gaussian injected into the stresses leaves a
-    static stress at the source cell that keeps growing, 10^4-10^5 times the wave's
-    stresses; FP16's relative rounding of it (2^-11) then kicks the neighbouring
-    velocities by more than the wave itself (measured: seismogram rel L2 4.7e-2 on
-    a 24x26x46 grid, the same before and after the growth-aware rescale)."""
+    unless `storage` says otherwise.  Every source with every wavelet: a Gaussian
+    driving a stress source leaves a static stress at the source cell 10^4-10^5
+    times the wave's, which FP16 storage keeps in an FP32 side slot."""
     rng = np.random.default_rng((5000 if storage == STORAGE_FP16 else 6000) + seed)
     pick = lambda xs: xs[int(rng.integers(len(xs)))]
     while True:
@@ -117,11 +112,9 @@ def draw_fp16(seed, storage=STORAGE_FP16):
         lo = npml + 1
         rec = np.stack([rng.integers(1, nx - 1, 8), rng.integers(1, ny - 1, 8), rng.integers(1, nz - 1, 8)], 1)
         src = pick([SRC_REFERENCE_SIGMA, SRC_FORCE, SRC_EXPLOSIVE, SRC_FORCE3])
-        zero_mean = storage == STORAGE_FP16 and src in (SRC_REFERENCE_SIGMA, SRC_EXPLOSIVE)
         cfg = small_cfg(nx=nx, ny=ny, nz=nz, npml=npml, nstep=int(rng.integers(150, 251)),
                         compat=pick([0, COMPAT_REFERENCE]), storage=storage, src_kind=src,
-                        wavelet=pick([WAVELET_DGAUSSIAN, WAVELET_RICKER] if zero_mean else
-                                     [WAVELET_GAUSSIAN, WAVELET_DGAUSSIAN, WAVELET_RICKER]),
+                        wavelet=pick([WAVELET_GAUSSIAN, WAVELET_DGAUSSIAN, WAVELET_RICKER]),
                         isrc=int(rng.integers(lo, nx - lo)), jsrc=int(rng.integers(lo, ny - lo)),
                         ksrc=int(rng.integers(lo, nz - lo)), receivers=rec.astype(np.int32),
                         angle_force=float(rng.uniform(0, 90)), energy_every=pick([1, 5]))
@@ -183,3 +176,44 @@ def test_random_fp32_case_many_steps(gpu, seed):
     eerr = float(np.max(np.abs(e32.sum(1) - e64.sum(1))) / np.max(e64.sum(1)))
     print(f"fp32 case {seed}: seismogram rel L2 {err:.3e}, max energy deviation {eerr:.3e}")
     assert err <= 1e-4 and eerr <= 1e-4
+
+
+def test_fp16_explosive_gaussian_is_refused():
+    """The explosive source with the Gaussian leaves static stresses around the source
+    in all six stresses; FP16 storage refuses the combination (every other source and
+    wavelet is drawn by the randomized FP16 cases)."""
+    cfg = small_cfg(nx=24, ny=26, nz=46, npml=4, storage=STORAGE_FP16, src_kind=SRC_EXPLOSIVE,
+                    wavelet=WAVELET_GAUSSIAN)
+    with pytest.raises(Ela3dError) as e:
+        cfg.validate()
+    assert e.value.code == -1 and "static stress" in str(e.value)
+    cfg.replace(storage=STORAGE_FP32).validate()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("src", [SRC_REFERENCE_SIGMA])
+def test_fp16_stress_source_with_gaussian(gpu, src):
+    """The case FP16 storage once failed: a Gaussian (non-zero mean) driving the
+    reference's sigma source on the 24x26x46 grid (seismogram rel L2 4.7e-2 before
+    the source cell's normal stresses moved to the FP32 side slot; 1.45e-2 after).
+    Against the FP64 oracle within the FP16 bar, and the source cell's stresses in
+    the snapshot are the side slot's, close to the oracle's."""
+    cfg = small_cfg(nx=24, ny=26, nz=46, npml=4, nstep=220, storage=STORAGE_FP16, src_kind=src,
+                    wavelet=WAVELET_GAUSSIAN, isrc=11, jsrc=13, ksrc=20, energy_every=5,
+                    receivers=np.asarray([(i, 13, k) for i in range(2, 22, 3) for k in (10, 30)], np.int32))
+    set_tuning(cfg, block=128, zseg_interior=2, vec=1, block_slab=128, zseg_slab=2, vec_slab=1)
+    model = hetero_model(cfg, seed=77)
+    with Shot(cfg, *model) as shot:
+        shot.run()
+        s16, e16 = shot.seismograms(), shot.energy()
+        snap = [shot.snapshot(q) for q in (3, 4, 5)]
+    s64, e64, st = oracle.run(cfg, model, dtype=np.float64)
+    err = float(np.linalg.norm(s16 - s64) / np.linalg.norm(s64))
+    eerr = float(np.max(np.abs(e16.sum(1) - e64.sum(1))) / np.max(e64.sum(1)))
+    print(f"fp16 stress source {src} with a Gaussian: seismogram rel L2 {err:.3e}, energy {eerr:.3e}")
+    assert err < 3e-2 and eerr < 3e-2
+    k, j, i = cfg.ksrc, cfg.jsrc, cfg.isrc
+    big = max(abs(st.f[q][k, j, i]) for q in (3, 4, 5))  # the injected static stress
+    for q, sn in zip((3, 4, 5), snap):
+        assert abs(sn[k, j, i] - st.f[q][k, j, i]) <= 1e-3 * big, q
+
