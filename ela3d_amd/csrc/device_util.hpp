@@ -50,6 +50,18 @@ __device__ __forceinline__ void stf(T* p, unsigned o, typename Arith<T>::R v, fl
         el(p, o) = v;
 }
 
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also orders
+// global memory at workgroup scope, which on gfx950 waits for every outstanding
+// global load and store of the wave (vmcnt(0)) before the barrier, so loads
+// issued ahead for the next plane would be drained at every plane.  Use only
+// where the workgroup communicates through LDS alone.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll

@@ -89,36 +89,44 @@ struct FSaved {
 // being written where another launch computed them); omi: x memory variables.
 // Every thread issues the same loads every plane (at a safe offset where it
 // needs none), so the wait for one plane's loads never drains the next plane's.
+// The two boundary planes of a z segment need only part of the loads: plane
+// kb-1 only the inputs of sigma_xz and sigma_yz (whose new values the velocity
+// update of plane kb reads), plane ke only those of sigma_zz (read by the vz
+// update of plane ke-1).  The loads a plane does not need are moved by the
+// uniform element offsets dA (not needed at kb-1: moved to plane kb, which the
+// next plane loads) and dB (not needed at ke: moved to plane ke-1, which the
+// previous plane loaded), so they hit the caches instead of reading new lines.
 template <typename T>
 __device__ __forceinline__ void fload(const StepArgs<T>& A, unsigned om, unsigned os, unsigned omi,
-                                      FLoad<typename Arith<T>::R>& L)
+                                      unsigned dA, unsigned dB, FLoad<typename Arith<T>::R>& L)
 {
     typedef typename Arith<T>::R R;
     const unsigned nxyp = A.nxyp, nx = A.nx;
     const unsigned oi = om + A.fdin;
     const T *vx = A.fi[0], *vy = A.fi[1], *vz = A.fi[2];
+    const unsigned dAB = dA + dB;
     L.vz_c = el(vz, oi);
-    L.cp_c = (R)el(A.cp, om);
-    L.vx_u = el(vx, oi + nxyp);
-    L.vy_u = el(vy, oi + nxyp);
-    L.cs_u = (R)el(A.cs, om + nxyp);
-    L.rho_u = (R)el(A.rho, om + nxyp);
-    L.vxE = el(vx, oi + 1);
-    L.vyS = el(vy, oi - nx);
-    L.vyW = el(vy, oi - 1);
-    L.vxN = el(vx, oi + nx);
-    L.vzW = el(vz, oi - 1);
-    L.vzN = el(vz, oi + nx);
-    L.csW = (R)el(A.cs, om - 1);
-    L.csN = (R)el(A.cs, om + nx);
-    L.rhoW = (R)el(A.rho, om - 1);
-    L.rhoN = (R)el(A.rho, om + nx);
-    L.sxx = el(A.fi[3], os);
-    L.syy = el(A.fi[4], os);
-    L.szz = el(A.fi[5], os);
-    L.sxy = el(A.fi[6], os);
-    L.sxz = el(A.fi[7], os);
-    L.syz = el(A.fi[8], os);
+    L.cp_c = (R)el(A.cp, om + dA);
+    L.vx_u = el(vx, oi + nxyp + dB);
+    L.vy_u = el(vy, oi + nxyp + dB);
+    L.cs_u = (R)el(A.cs, om + nxyp + dB);
+    L.rho_u = (R)el(A.rho, om + nxyp + dB);
+    L.vxE = el(vx, oi + dA + 1);
+    L.vyS = el(vy, oi + dA - nx);
+    L.vyW = el(vy, oi + dAB - 1);
+    L.vxN = el(vx, oi + dAB + nx);
+    L.vzW = el(vz, oi + dB - 1);
+    L.vzN = el(vz, oi + dB + nx);
+    L.csW = (R)el(A.cs, om + dB - 1);
+    L.csN = (R)el(A.cs, om + dB + nx);
+    L.rhoW = (R)el(A.rho, om + dB - 1);
+    L.rhoN = (R)el(A.rho, om + dB + nx);
+    L.sxx = el(A.fi[3], os + dAB);
+    L.syy = el(A.fi[4], os + dAB);
+    L.szz = el(A.fi[5], os + dA);
+    L.sxy = el(A.fi[6], os + dAB);
+    L.sxz = el(A.fi[7], os + dB);
+    L.syz = el(A.fi[8], os + dB);
     L.m0 = el(A.mi[0], omi);
     L.m1 = el(A.mi[3], omi);
     L.m2 = el(A.mi[5], omi);
@@ -245,13 +253,15 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
     };
 
     FQueue<R> Q;
+    // (at plane kb-1 only vx and vy are needed; the others are loaded at plane kb)
     auto qinit = [&](int m) {
         const unsigned o = cell_offset(A.g0, A.nxyp, m, p), oi = o + A.fdin;
+        const unsigned ob = m == kb - 1 ? o + unsigned(A.nxyp) : o;
         Q.vx = el(A.fi[0], oi);
         Q.vy = el(A.fi[1], oi);
-        Q.cs = (R)el(A.cs, o);
-        Q.rho = (R)el(A.rho, o);
-        Q.vz_m = el(A.fi[2], oi - unsigned(A.nxyp));
+        Q.cs = (R)el(A.cs, ob);
+        Q.rho = (R)el(A.rho, ob);
+        Q.vz_m = el(A.fi[2], ob + A.fdin - unsigned(A.nxyp));
     };
     // plane m's loads: a computed plane's stress inputs, or this step's stresses
     // where the layer launch computed them (halo rows outside the box, own cells
@@ -264,7 +274,9 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
         const unsigned om = cm ? o : osafe;
         const unsigned os = cm ? o + A.fdin : (needs(m) ? o + A.fdout : osafe + A.fdin);
         const unsigned omi = (cm && inx ? om0 + unsigned(m) * omk : 0u) + A.mdin;
-        fload<T>(A, om, os, omi, L);
+        const unsigned dA = m == kb - 1 ? unsigned(A.nxyp) : 0u;
+        const unsigned dB = m == ke ? 0u - unsigned(A.nxyp) : 0u;
+        fload<T>(A, om, os, omi, dA, dB, L);
     };
     // stress of plane m from its loads: new values into s; stored when `store`;
     // the velocity inputs of plane m into sv; the queue advances to m+1
@@ -403,7 +415,7 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
         if (m >= kb) S = N;
         // plane m's LDS buffer is read by the velocity update of the next
         // iteration; the buffer that iteration writes was last read by this one
-        if (m >= kb && m < ke) __syncthreads();
+        if (m >= kb && m < ke) lds_barrier();
     };
     // Planes kb-1 .. ke, two per trip so that the two load buffers alternate
     // without a copy (a copy of a register a load is still filling waits for

@@ -156,7 +156,13 @@ __device__ __forceinline__ void tail_body(const TailArgs<T>& A)
             A.srctrace[3 * it + 1] = sigma_at_source(A, 4, o);
             A.srctrace[3 * it + 2] = sigma_at_source(A, 5, o);
         }
-        if (bad) *A.flag = 1;
+        // every step, whatever the receivers and energy cadence: the nine fields at
+        // the source cell, where a blow-up of the whole grid shows within a few steps
+        bool fin = true;
+        for (int q = 0; q < 9; ++q)
+            fin = fin && isfinite((q >= 3 && q < 6) ? sigma_at_source(A, q, A.src)
+                                                    : (double)ldf(A.f[q], A.src, nisc<T>(A, q)));
+        if (bad || !fin) *A.flag = 1;
         *A.step = it;
     }
     if constexpr (std::is_same<T, half_t>::value) {

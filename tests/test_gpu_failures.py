@@ -57,3 +57,22 @@ def test_server_fails_only_the_shot_that_does_not_fit(gpu):
     assert infos[0]["status"] == 0 and infos[2]["status"] == 0
     assert stats["failed"] == 1 and stats["done"] == 3
     assert np.array_equal(seis[0], alone[0]) and np.array_equal(seis[2], alone[1])
+
+
+@pytest.mark.parametrize("storage", [0, 1])  # FP32, FP16
+def test_blow_up_is_flagged_without_receivers_or_energy(gpu, storage):
+    """A shot beyond Courant with no receivers and no energy sums is still reported
+    unstable: the tail checks the nine fields at the source cell every step."""
+    from ela3d_amd import Config, homogeneous_model
+    cfg = Config(nx=24, ny=22, nz=20, dx=10.0, dy=10.0, dz=10.0, dt=4.0e-3, nstep=400,
+                 npoints_pml=3, f0=18.0, isrc=12, jsrc=11, ksrc=10, energy_every=0,
+                 storage=storage)
+    assert cfg.nrec == 0
+    model = homogeneous_model(cfg.nx, cfg.ny, cfg.nz, 3300.0, 3300.0 / 1.732, 2800.0)
+    with Shot(cfg, *model) as s:
+        with pytest.raises(Ela3dError) as e:
+            s.run()
+        assert e.value.code == -3  # ELA3D_E_UNSTABLE
+    stable = cfg.replace(dt=1.0e-3, nstep=100)
+    with Shot(stable, *model) as s:
+        s.run()  # a stable shot is not flagged

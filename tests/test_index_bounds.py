@@ -213,11 +213,17 @@ def fused_violations(cfg, tile, zseg, elem=4, ldsreg=None):
             m0 = kb - 1 if computes(kb - 1).any() else kb
             first = computes(kb - 1) if m0 == kb - 1 else computes(kb)
             qinit(pc[first], m0)
+            if m0 == kb - 1:  # the queue loads it does not need there come from plane kb
+                qinit(pc[first], kb)
             # own cells whose plane kb-1 is not computed start their queue at kb
             qinit(pc[own & ~computes(kb - 1)], kb)
             for m in range(kb - 1, ke + 1):
                 cm = computes(m)
-                load(pc[cm], m)
+                # loads a boundary plane does not need move one plane inwards (dA, dB)
+                shift = 1 if m == kb - 1 else (-1 if m == ke else 0)
+                for s in {0, shift}:
+                    load(pc[cm], m + s)
+                    load(pc[act & ~cm], kb + s)  # the safe offset of the others
                 region = kb <= m < ke
                 fresh = ~cm & (own | (act if region else np.zeros(tile, bool)))
                 fld(cell(m, pc[fresh]), ("fresh", m))
