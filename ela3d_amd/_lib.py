@@ -1,7 +1,8 @@
 """ctypes declarations of include/ela3d.h and the library loader.
 
-The library is the in-tree ``ela3d_amd/libela3d.so`` built by ``ela3d_amd.build``.
-There is no CPU fallback: if the library is missing, every call raises.
+The library is the in-tree ``ela3d_amd/libela3d.so`` built by ``ela3d_amd.build``
+(ELA3D_LIB names another build of it, such as the sanitized one).  There is no
+CPU fallback: if the library is missing, every call raises.
 """
 from __future__ import annotations
 
@@ -9,7 +10,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libela3d.so")
+LIB_PATH = os.environ.get("ELA3D_LIB") or os.path.join(HERE, "libela3d.so")
 
 ABI_VERSION = 1
 E_NAMES = {

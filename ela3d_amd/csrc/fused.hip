@@ -208,7 +208,8 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
     const int own_full = fused_own(A.tile, nx);
     const int q0 = B.p0 + chunk * own_full;
     const int n = min(own_full, B.p1 - q0);
-    const int kb = B.k0 + zs * B.zseg, ke = min(kb + B.zseg, B.k1);
+    int kb, ke;
+    segment_planes(B, zs, kb, ke);
     const int t = threadIdx.x;
     const int c = fused_cell(t, n, nx);
     const bool act = c != kNoCell;
@@ -217,7 +218,7 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
     const bool inb = act && p >= B.p0 && p < B.p1;  // recomputed here; else from the layer launch
     const int j = p / nx, i = p - j * nx;
     const bool iN = i <= nx - 2, iS = i >= 1;
-    const int cx = i < A.lo[0] ? i : (i >= A.hi[0] ? A.lo[0] + (i - A.hi[0]) : -1);
+    const int cx = layer_index(i, A.lo[0], A.hi[0]);
     const bool pml = A.w[0] > 0;
     const bool inx = cx >= 0;
     R* xtab = lds + 2 * 5 * LR;  // the x tables, after the two plane buffers
@@ -233,7 +234,7 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
     __syncthreads();
     // x memory-variable offset of this cell at plane k (offset 0 outside the layer)
     const unsigned wx = unsigned(A.w[0]);
-    const unsigned om0 = inx ? unsigned(j) * wx + unsigned(cx) : 0u;
+    const unsigned om0 = inx ? xmem_offset(0, j, cx, ny, A.w[0]) : 0u;
     const unsigned omk = inx ? unsigned(ny) * wx : 0u;  // per plane
     const int li = c + nx;  // LDS index of this cell
     T* const* fw = const_cast<T* const*>(A.fi);  // the same arrays, written at fdout

@@ -1,6 +1,7 @@
 // geom.hpp -- launch geometry shared by the stencil kernels and host-side checks:
-// block renumbering over the XCDs, tile enumeration of a box, cell offsets, and
-// the fused kernel's thread-to-cell map.  Every function is __host__ __device__,
+// block renumbering over the XCDs, tile enumeration of a box, cell offsets, lane
+// geometry, C-PML layer indices and memory-variable offsets, and the fused
+// kernel's thread-to-cell map.  Every function is __host__ __device__,
 // so a CPU program can enumerate exactly the addresses the kernels form.
 #pragma once
 
@@ -16,6 +17,17 @@ __host__ __device__ inline int xcd_block(int b, int n)
     const int q = n / X, r = n % X;
     const int x = b % X, l = b / X;
     return x * q + (x < r ? x : r) + l;
+}
+
+// Box of logical block lb in a launch over nbox boxes (blk0 ascending).
+__host__ __device__ inline Box pick_box(const Box* bx, int nbox, int lb)
+{
+    Box B = bx[0];
+    if (nbox > 1 && lb >= bx[1].blk0) B = bx[1];
+    if (nbox > 2 && lb >= bx[2].blk0) B = bx[2];
+    if (nbox > 3 && lb >= bx[3].blk0) B = bx[3];
+    if (nbox > 4 && lb >= bx[4].blk0) B = bx[4];
+    return B;
 }
 
 // Tile (xy chunk, z segment) of a box's local block index, strip-major: the
@@ -49,6 +61,74 @@ __host__ __device__ inline void tile_of(const Box& B, int local, int& chunk, int
 __host__ __device__ inline unsigned cell_offset(int g0, int nxyp, int k, int p)
 {
     return unsigned(g0) + unsigned(k) * unsigned(nxyp) + unsigned(p);
+}
+
+// Compact index of grid index c along an axis whose absorbing layers are
+// [0, lo) and [hi, n): c below lo, lo + (c - hi) from hi on, -1 in between.
+__host__ __device__ inline int layer_index(int c, int lo, int hi)
+{
+    return c < lo ? c : (c >= hi ? lo + (c - hi) : -1);
+}
+
+// Element offsets of the compact C-PML memory-variable families: x layers
+// [nz][ny][wx] (cx = layer_index of i), y layers [nz][wy][nx] (cy of j), z
+// layers [wz][ny][nx] (cz of k).
+__host__ __device__ inline unsigned xmem_offset(int k, int j, int cx, int ny, int wx)
+{
+    return unsigned((k * ny + j) * wx + cx);
+}
+__host__ __device__ inline unsigned ymem_offset(int k, int cy, int i, int nx, int wy)
+{
+    return unsigned((k * wy + cy) * nx + i);
+}
+__host__ __device__ inline unsigned zmem_offset(int cz, int j, int i, int nx, int ny)
+{
+    return unsigned((cz * ny + j) * nx + i);
+}
+
+// Two-pass kernels: thread t of the workgroup on (chunk, z segment zs) of box B
+// owns the VEC consecutive positions p0 .. p0+vec-1 of the flattened plane and
+// marches planes [kb, ke); nothing to do when p0 >= B.p1.
+__host__ __device__ inline int lane_p0(const Box& B, int chunk, int block, int t, int vec)
+{
+    return B.p0 + (chunk * block + t) * vec;
+}
+__host__ __device__ inline void segment_planes(const Box& B, int zs, int& kb, int& ke)
+{
+    kb = B.k0 + zs * B.zseg;
+    ke = kb + B.zseg < B.k1 ? kb + B.zseg : B.k1;
+}
+
+// Per-lane geometry of the VEC elements p0..p0+VEC-1 of a box ending at p1: grid
+// indices, whether each lies in the box, and its compact x (and, in an
+// absorbing-layer box, y) layer index.  lo/hi: the compact C-PML regions.
+template <int V>
+struct Lane {
+    int p0;
+    int i[V], j[V];
+    bool act[V];   // inside this box
+    int cx[V], cy[V];
+};
+
+template <int V>
+__host__ __device__ inline void lane_geometry(int nx, const int* lo, const int* hi, bool slab, int p0, int p1,
+                                              Lane<V>& L)
+{
+    L.p0 = p0;
+    int j = p0 / nx, i = p0 - j * nx;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        L.i[e] = i;
+        L.j[e] = j;
+        L.act[e] = (p0 + e) < p1;
+        // elements past the box end own no memory variables (their j may exceed ny-1)
+        L.cx[e] = L.act[e] ? layer_index(i, lo[0], hi[0]) : -1;
+        L.cy[e] = (slab && L.act[e]) ? layer_index(j, lo[1], hi[1]) : -1;
+        if (++i == nx) {
+            i = 0;
+            ++j;
+        }
+    }
 }
 
 // Fused interior step: a workgroup of `nt` threads holds one region of the

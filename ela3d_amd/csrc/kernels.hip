@@ -32,16 +32,6 @@
 
 namespace ela3d {
 
-__device__ __forceinline__ Box pick_box(const Box* bx, int nbox, int lb)
-{
-    Box B = bx[0];
-    if (nbox > 1 && lb >= bx[1].blk0) B = bx[1];
-    if (nbox > 2 && lb >= bx[2].blk0) B = bx[2];
-    if (nbox > 3 && lb >= bx[3].blk0) B = bx[3];
-    if (nbox > 4 && lb >= bx[4].blk0) B = bx[4];
-    return B;
-}
-
 // Element offset of cell (p, k) from the start of a field or model allocation.
 // Field and model pointers handed to the stencil kernels point at the start of
 // the allocation, g0 guard elements before plane 0, so every neighbour offset
@@ -212,36 +202,10 @@ __device__ __forceinline__ R cpml(R psi_old, R a, R b, R Kinv, R& d)
 template <typename T>
 using TabP = typename std::conditional<std::is_same<T, double>::value, double, float>::type;
 
-// Per-lane geometry of the VEC elements p0..p0+VEC-1.
-template <int V>
-struct Lane {
-    int p0;
-    int i[V], j[V];
-    bool act[V];   // inside this box
-    int cx[V], cy[V];
-};
-
 template <typename T, int MODE, int V>
 __device__ __forceinline__ void make_lane(const StepArgs<T>& A, int p0, int p1, Lane<V>& L)
 {
-    const int nx = A.nx;
-    L.p0 = p0;
-    int j = p0 / nx, i = p0 - j * nx;
-#pragma unroll
-    for (int e = 0; e < V; ++e) {
-        L.i[e] = i;
-        L.j[e] = j;
-        L.act[e] = (p0 + e) < p1;
-        // elements past the box end own no memory variables (their j may exceed ny-1)
-        L.cx[e] = !L.act[e] ? -1 : (i < A.lo[0] ? i : (i >= A.hi[0] ? A.lo[0] + (i - A.hi[0]) : -1));
-        L.cy[e] = -1;
-        if (MODE == kSlab && L.act[e])
-            L.cy[e] = j < A.lo[1] ? j : (j >= A.hi[1] ? A.lo[1] + (j - A.hi[1]) : -1);
-        if (++i == nx) {
-            i = 0;
-            ++j;
-        }
-    }
+    lane_geometry<V>(A.nx, A.lo, A.hi, MODE == kSlab, p0, p1, L);
 }
 
 // C-PML coefficients of a lane's elements: a, b, 1/K on the full grid then on
@@ -302,7 +266,7 @@ __device__ __forceinline__ ZTab<R> ztab(const StepArgs<T>& A, int k)
     z.az = 0; z.bz = 1; z.Kz = 1; z.azh = 0; z.bzh = 1; z.Kzh = 1;
     if (MODE == kSlab) {
         const int nz = A.nz;
-        z.cz = k < A.lo[2] ? k : (k >= A.hi[2] ? A.lo[2] + (k - A.hi[2]) : -1);
+        z.cz = layer_index(k, A.lo[2], A.hi[2]);
         if (z.cz >= 0) {
             const double* t = A.tab[2];
             z.az = (R)t[k]; z.bz = (R)t[nz + k]; z.Kz = (R)t[2 * nz + k];
@@ -359,24 +323,24 @@ __device__ __forceinline__ void stress_load(const StepArgs<T>& A, const Lane<V>&
     ldv<T, V>(A.fi[6], o, S.r[6], P.sxy);
     ldv<T, V>(A.fi[7], o, S.r[7], P.sxz);
     ldv<T, V>(A.fi[8], o, S.r[8], P.syz);
-    const int cz = MODE == kSlab ? (k < A.lo[2] ? k : (k >= A.hi[2] ? A.lo[2] + (k - A.hi[2]) : -1)) : -1;
+    const int cz = MODE == kSlab ? layer_index(k, A.lo[2], A.hi[2]) : -1;
 #pragma unroll
     for (int e = 0; e < V; ++e) {
         if (L.cx[e] >= 0) {
-            const unsigned om = unsigned((k * A.ny + L.j[e]) * A.w[0] + L.cx[e]);
+            const unsigned om = xmem_offset(k, L.j[e], L.cx[e], A.ny, A.w[0]);
             P.mx[0][e] = ldf(A.mi[0], om, S.r[9]);
             P.mx[1][e] = ldf(A.mi[3], om, S.r[12]);
             P.mx[2][e] = ldf(A.mi[5], om, S.r[14]);
         }
         if (MODE == kSlab) {
             if (L.cy[e] >= 0) {
-                const unsigned om = unsigned((k * A.w[1] + L.cy[e]) * A.nx + L.i[e]);
+                const unsigned om = ymem_offset(k, L.cy[e], L.i[e], A.nx, A.w[1]);
                 P.my[0][e] = ldf(A.mi[1], om, S.r[10]);
                 P.my[1][e] = ldf(A.mi[4], om, S.r[13]);
                 P.my[2][e] = ldf(A.mi[7], om, S.r[16]);
             }
             if (cz >= 0 && L.act[e]) {  // elements past the box end may sit past the last row
-                const unsigned om = unsigned((cz * A.ny + L.j[e]) * A.nx + L.i[e]);
+                const unsigned om = zmem_offset(cz, L.j[e], L.i[e], A.nx, A.ny);
                 P.mz[0][e] = ldf(A.mi[2], om, S.r[11]);
                 P.mz[1][e] = ldf(A.mi[6], om, S.r[15]);
                 P.mz[2][e] = ldf(A.mi[8], om, S.r[17]);
@@ -419,9 +383,9 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
 #pragma unroll
     for (int e = 0; e < V; ++e) {
         const int i = L.i[e], j = L.j[e], cx = L.cx[e], cy = L.cy[e];
-        const unsigned omx = unsigned((k * ny + j) * A.w[0] + cx);
-        const unsigned omy = unsigned((k * A.w[1] + cy) * nx + i);
-        const unsigned omz = unsigned((Z.cz * ny + j) * nx + i);
+        const unsigned omx = xmem_offset(k, j, cx, ny, A.w[0]);
+        const unsigned omy = ymem_offset(k, cy, i, nx, A.w[1]);
+        const unsigned omz = zmem_offset(Z.cz, j, i, nx, ny);
         const R vx_c = Q.vx_c[e], vy_c = Q.vy_c[e];
         // sigma_xx, sigma_yy, sigma_zz: i in [0,nx-2], j in [1,ny-1], k in [1,nz-1] (CPU:377-413)
         {
@@ -534,10 +498,10 @@ template <typename T, int MODE, int V>
 __device__ __forceinline__ void stress_body(const StepArgs<T>& A, const Box& B, int chunk, int zs, Scales<T>& S)
 {
     typedef typename Arith<T>::R R;
-    const int p0 = B.p0 + (chunk * A.block + (int)threadIdx.x) * V;
-    const int kb = B.k0 + zs * B.zseg;
+    const int p0 = lane_p0(B, chunk, A.block, (int)threadIdx.x, V);
+    int kb, ke;
+    segment_planes(B, zs, kb, ke);
     if (p0 >= B.p1 || kb >= B.k1) return;
-    const int ke = min(kb + B.zseg, B.k1);
     Lane<V> L;
     make_lane<T, MODE, V>(A, p0, B.p1, L);
     LaneTabs<TabP<T>, V> LT;
@@ -618,24 +582,24 @@ __device__ __forceinline__ void velocity_load(const StepArgs<T>& A, const Lane<V
         ldm<V>(A.cp, o, P.cp);
         ldm<V>(A.cs, o, P.cs);
     }
-    const int cz = MODE == kSlab ? (k < A.lo[2] ? k : (k >= A.hi[2] ? A.lo[2] + (k - A.hi[2]) : -1)) : -1;
+    const int cz = MODE == kSlab ? layer_index(k, A.lo[2], A.hi[2]) : -1;
 #pragma unroll
     for (int e = 0; e < V; ++e) {
         if (L.cx[e] >= 0) {
-            const unsigned om = unsigned((k * A.ny + L.j[e]) * A.w[0] + L.cx[e]);
+            const unsigned om = xmem_offset(k, L.j[e], L.cx[e], A.ny, A.w[0]);
             P.mx[0][e] = ldf(A.mi[9], om, S.r[18]);
             P.mx[1][e] = ldf(A.mi[12], om, S.r[21]);
             P.mx[2][e] = ldf(A.mi[15], om, S.r[24]);
         }
         if (MODE == kSlab) {
             if (L.cy[e] >= 0) {
-                const unsigned om = unsigned((k * A.w[1] + L.cy[e]) * A.nx + L.i[e]);
+                const unsigned om = ymem_offset(k, L.cy[e], L.i[e], A.nx, A.w[1]);
                 P.my[0][e] = ldf(A.mi[10], om, S.r[19]);
                 P.my[1][e] = ldf(A.mi[13], om, S.r[22]);
                 P.my[2][e] = ldf(A.mi[16], om, S.r[25]);
             }
             if (cz >= 0 && L.act[e]) {  // elements past the box end may sit past the last row
-                const unsigned om = unsigned((cz * A.ny + L.j[e]) * A.nx + L.i[e]);
+                const unsigned om = zmem_offset(cz, L.j[e], L.i[e], A.nx, A.ny);
                 P.mz[0][e] = ldf(A.mi[11], om, S.r[20]);
                 P.mz[1][e] = ldf(A.mi[14], om, S.r[23]);
                 P.mz[2][e] = ldf(A.mi[17], om, S.r[26]);
@@ -680,9 +644,9 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
 #pragma unroll
     for (int e = 0; e < V; ++e) {
         const int i = L.i[e], j = L.j[e], cx = L.cx[e], cy = L.cy[e];
-        const unsigned omx = unsigned((k * ny + j) * A.w[0] + cx);
-        const unsigned omy = unsigned((k * A.w[1] + cy) * nx + i);
-        const unsigned omz = unsigned((Z.cz * ny + j) * nx + i);
+        const unsigned omx = xmem_offset(k, j, cx, ny, A.w[0]);
+        const unsigned omy = ymem_offset(k, cy, i, nx, A.w[1]);
+        const unsigned omz = zmem_offset(Z.cz, j, i, nx, ny);
         const bool face = A.dirichlet && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1 || k == 0 || k == nz - 1);
         // vx: i in [1,nx-1], j in [1,ny-1], k in [1,nz-1] (CPU:502-529)
         const bool onx = L.act[e] && (i >= 1) && (j >= 1) && (k >= 1);
@@ -802,10 +766,10 @@ __device__ __forceinline__ void velocity_body(const StepArgs<T>& A, const Box& B
                                               double& ep, Scales<T>& S)
 {
     typedef typename Arith<T>::R R;
-    const int p0 = B.p0 + (chunk * A.block + (int)threadIdx.x) * V;
-    const int kb = B.k0 + zs * B.zseg;
+    const int p0 = lane_p0(B, chunk, A.block, (int)threadIdx.x, V);
+    int kb, ke;
+    segment_planes(B, zs, kb, ke);
     if (p0 >= B.p1 || kb >= B.k1) return;
-    const int ke = min(kb + B.zseg, B.k1);
     Lane<V> L;
     make_lane<T, MODE, V>(A, p0, B.p1, L);
     LaneTabs<TabP<T>, V> LT;

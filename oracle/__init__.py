@@ -16,7 +16,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libela3d_oracle.so")
+_LIB_PATH = os.environ.get("ELA3D_ORACLE_LIB") or os.path.join(_HERE, "libela3d_oracle.so")
 
 FIELD_NAMES = ["vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz"]
 MEM_NAMES = [
