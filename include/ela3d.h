@@ -167,7 +167,8 @@ int ela3d_create(const ela3d_config* cfg, const float* cp, const float* cs, cons
                  ela3d_shot** out);
 /* Enqueue n time steps (CPU:376-622) on the shot's stream; asynchronous. */
 int ela3d_step(ela3d_shot* shot, int32_t nsteps);
-/* Wait for the shot's queued work; ELA3D_E_UNSTABLE if a non-finite value reached a receiver. */
+/* Wait for the shot's queued work; ELA3D_E_UNSTABLE once a non-finite value was seen at the
+ * source cell (checked every step), a receiver sample or the energy. */
 int ela3d_sync(ela3d_shot* shot);
 /* ela3d_step(nstep - done) + ela3d_sync */
 int ela3d_run(ela3d_shot* shot);
