@@ -81,6 +81,7 @@ class Config:
     use_graph: bool = True
     block: int = 0
     zseg: int = 0
+    free_surface: bool = False   # traction-free surface at the plane k = 0
 
     @property
     def t0_effective(self) -> float:
@@ -132,6 +133,7 @@ class Config:
         s.compat_flags = int(self.compat_flags)
         s.dirichlet = 1 if self.dirichlet else 0
         s.use_graph = 1 if self.use_graph else 0
+        s.free_surface = 1 if self.free_surface else 0
         s.nrec = rec.shape[0]
         s.rec = rec.ctypes.data_as(C.POINTER(C.c_int32)) if rec.size else C.POINTER(C.c_int32)()
         return s, rec

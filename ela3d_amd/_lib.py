@@ -34,7 +34,7 @@ class ConfigStruct(C.Structure):
         ("sample_every", C.c_int32), ("energy_every", C.c_int32), ("storage", C.c_int32),
         ("compat_flags", C.c_uint32), ("dirichlet", C.c_int32), ("fp16_rescale_every", C.c_int32),
         ("device", C.c_int32), ("use_graph", C.c_int32), ("block", C.c_int32), ("zseg", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("free_surface", C.c_int32), ("reserved", C.c_int32 * 5),
     ]
 
 

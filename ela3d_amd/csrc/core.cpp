@@ -53,12 +53,15 @@ PmlAxis make_axis(const ela3d_config& c, int axis)
         K = 1.0 + (c.k_max_pml - 1.0) * p;
         al = amax * (1.0 - s) + 0.1 * amax;
     };
+    // a free surface at k = 0 has no z-min absorbing layer (HOM:853: "disable pml
+    // zmin for free surface condition")
+    const bool min_side = !(axis == 2 && c.free_surface);
     for (int i = 0; i < n; ++i) {
         const double x = h * i;
         double ab = origin_lo - x;
-        if (ab >= 0.0) profile(ab / norm_min_full, A.d[i], A.K[i], A.alpha[i]);
+        if (min_side && ab >= 0.0) profile(ab / norm_min_full, A.d[i], A.K[i], A.alpha[i]);
         ab = zq ? thick_y - (c.dy * (c.ny - 1) + c.dy / 2.0) : origin_lo - (x + h / 2.0);
-        if (ab >= 0.0) profile(ab / norm_min_half, A.d_h[i], A.K_h[i], A.alpha_h[i]);
+        if (min_side && ab >= 0.0) profile(ab / norm_min_half, A.d_h[i], A.K_h[i], A.alpha_h[i]);
         ab = x - origin_hi;
         if (ab >= 0.0) profile(ab / thick, A.d[i], A.K[i], A.alpha[i]);
         ab = x + h / 2.0 - origin_hi;
@@ -118,7 +121,10 @@ int split_boxes(const ShotPlan& p, BoxRange out[5])
 {
     const ela3d_config& c = p.cfg;
     const int nx = c.nx, nxy = c.nx * c.ny;
-    const int ylo = p.ax[1].lo, yhi = p.ax[1].hi, zlo = p.ax[2].lo, zhi = p.ax[2].hi;
+    const int ylo = p.ax[1].lo, yhi = p.ax[1].hi, zhi = p.ax[2].hi;
+    // a free surface is updated by the absorbing-layer kernels: plane 0 is a z-min box
+    // of its own even where no z-min layer is stored
+    const int zlo = (c.free_surface && p.ax[2].lo < 1) ? std::min(1, zhi) : p.ax[2].lo;
     int pa = (ylo * nx + 63) / 64 * 64, pb = (yhi * nx) / 64 * 64;
     if (pb <= pa) pa = pb = nxy;  // too small for an interior box: all absorbing layer
     BoxRange all[5] = {{0, nxy, 0, zlo, 1}, {0, nxy, zhi, c.nz, 1}, {0, pa, zlo, zhi, 1},
@@ -143,7 +149,7 @@ double box_alg_bytes(const ShotPlan& P, const BoxRange& b, int pass)
     double bytes = (pass == 0 ? stress_cell_bytes(c) : velocity_cell_bytes(c)) * cells;
     const PmlAxis &X = P.ax[0], &Y = P.ax[1], &Z = P.ax[2];
     struct V { const std::vector<double>* a; int axis; int i0, i1, j0, j1, k0, k1; int pass; };
-    const V vars[18] = {
+    V vars[18] = {
         {&X.a_h, 0, 0, nx - 2, 1, ny - 1, 1, nz - 1, 0}, {&Y.a, 1, 0, nx - 2, 1, ny - 1, 1, nz - 1, 0},
         {&Z.a, 2, 0, nx - 2, 1, ny - 1, 1, nz - 1, 0},   {&X.a, 0, 1, nx - 1, 0, ny - 2, 0, nz - 1, 0},
         {&Y.a_h, 1, 1, nx - 1, 0, ny - 2, 0, nz - 1, 0}, {&X.a, 0, 1, nx - 1, 0, ny - 1, 0, ks, 0},
@@ -155,6 +161,9 @@ double box_alg_bytes(const ShotPlan& P, const BoxRange& b, int pass)
         {&X.a_h, 0, 0, nx - 2, 1, ny - 1, 0, nz - 2, 1}, {&Y.a, 1, 0, nx - 2, 1, ny - 1, 0, nz - 2, 1},
         {&Z.a_h, 2, 0, nx - 2, 1, ny - 1, 0, nz - 2, 1},
     };
+    // a free surface also updates the normal stresses, vx and vy on plane 0
+    if (c.free_surface && P.koff == 0)
+        for (int q : {0, 1, 2, 9, 10, 11, 12, 13, 14}) vars[q].k0 = 0;
     // 2 accesses (read + write) per (memory variable, cell) where its table a != 0
     int64_t count = 0;
     for (const V& v : vars) {

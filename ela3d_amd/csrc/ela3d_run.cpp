@@ -28,6 +28,7 @@ const char* kUsage =
     "  model     : model=homogeneous [vp vs rho] | model=layered [seed] | model=anticline (cube)\n"
     "              model=raw:CP.bin,CS.bin,RHO.bin[:4|8]  (x-fastest float32 or float64)\n"
     "  run       : sampgat energy_every storage=fp32|fp16|fp64 compat=0..7|reference dirichlet=0|1\n"
+    "              free_surface=0|1 (traction-free plane k = 0, no z-min absorbing layer)\n"
     "              it_output=N (v?%05i.bin) out=DIR checkpoint=PATH checkpoint_every=N resume=PATH\n"
     "              reference=1 (the reference's choices: sigma source, dGaussian wavelet, all\n"
     "              quirks, no Dirichlet faces, sampgat=2) device=N append=0|1\n";
@@ -214,6 +215,7 @@ int main(int argc, char** argv)
     if (compat == "reference") c.compat_flags = ELA3D_COMPAT_REFERENCE;
     else if (!compat.empty()) c.compat_flags = uint32_t(std::atoi(compat.c_str()));
     c.dirichlet = a.integer("dirichlet", c.dirichlet);
+    c.free_surface = a.integer("free_surface", 0);
     c.device = a.integer("device", 0);
 
     std::vector<int32_t> rec = receivers(a, c);

@@ -380,6 +380,7 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
     R nsxx[V], nsyy[V], nszz[V], nsxy[V], nsxz[V], nsyz[V];
     bool onN[V], onXY[V], onXZ[V], onYZ[V];
     const bool shear_k = (k <= A.kmax_shear);
+    const bool surface = MODE == kSlab && A.fs && k == 0;  // free surface (absorbing-layer boxes only)
 #pragma unroll
     for (int e = 0; e < V; ++e) {
         const int i = L.i[e], j = L.j[e], cx = L.cx[e], cy = L.cy[e];
@@ -389,7 +390,7 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
         const R vx_c = Q.vx_c[e], vy_c = Q.vy_c[e];
         // sigma_xx, sigma_yy, sigma_zz: i in [0,nx-2], j in [1,ny-1], k in [1,nz-1] (CPU:377-413)
         {
-            onN[e] = L.act[e] && (i <= nx - 2) && (j >= 1) && (k >= 1);
+            onN[e] = L.act[e] && (i <= nx - 2) && (j >= 1) && (k >= 1 || surface);
             const R vp = P.cp_c[e];
             const R lambda = Q.rho_c[e] * (vp * vp - (R)2 * Q.cs_c[e] * Q.cs_c[e]);
             const R l2m = Q.rho_c[e] * vp * vp;
@@ -411,9 +412,14 @@ __device__ __forceinline__ void stress_update(const StepArgs<T>& A, const Lane<V
                     if (onN[e]) stm(mout(A, 2), omz, m, S, 11);
                 }
             }
+            if (surface) {
+                // traction-free surface (sigma_zz = 0 on plane 0): the vertical strain
+                // rate follows from lambda (exx + eyy) + (lambda + 2 mu) ezz = 0
+                dvz_dz = -(lambda / l2m) * (dvx_dx + dvy_dy);
+            }
             nsxx[e] = DT_l2m * dvx_dx + DT_l * (dvy_dy + dvz_dz) + P.sxx[e];
             nsyy[e] = DT_l * (dvx_dx + dvz_dz) + DT_l2m * dvy_dy + P.syy[e];
-            nszz[e] = DT_l * (dvx_dx + dvy_dy) + DT_l2m * dvz_dz + P.szz[e];
+            nszz[e] = surface ? P.szz[e] : DT_l * (dvx_dx + dvy_dy) + DT_l2m * dvz_dz + P.szz[e];
         }
         // sigma_xy: i in [1,nx-1], j in [0,ny-2]; mu averaged over (i-1,j),(i,j+1) (CPU:415-442)
         {
@@ -638,6 +644,7 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
     const unsigned o = cell(A, k, L.p0);
     const R dt = (R)A.dt, idx = (R)A.idx, idy = (R)A.idy, idz = (R)A.idz;
     const ZTab<R> Z = ztab<T, MODE, R>(A, k);
+    const bool surface = MODE == kSlab && A.fs && k == 0;  // free surface (absorbing-layer boxes only)
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     R nvx[V], nvy[V], nvz[V];
     bool wx[V], wy[V], wz[V];
@@ -647,16 +654,18 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
         const unsigned omx = xmem_offset(k, j, cx, ny, A.w[0]);
         const unsigned omy = ymem_offset(k, cy, i, nx, A.w[1]);
         const unsigned omz = zmem_offset(Z.cz, j, i, nx, ny);
-        const bool face = A.dirichlet && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1 || k == 0 || k == nz - 1);
-        // vx: i in [1,nx-1], j in [1,ny-1], k in [1,nz-1] (CPU:502-529)
-        const bool onx = L.act[e] && (i >= 1) && (j >= 1) && (k >= 1);
+        const bool face = A.dirichlet && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1 ||
+                                          (k == 0 && !surface) || k == nz - 1);
+        // vx: i in [1,nx-1], j in [1,ny-1], k in [1,nz-1] (CPU:502-529); plane 0 too at a
+        // free surface, where the image sigma_xz(k-1/2) = -sigma_xz(k+1/2) gives d/dz
+        const bool onx = L.act[e] && (i >= 1) && (j >= 1) && (k >= 1 || surface);
         nvx[e] = P.vx[e];
         {
             const R rh = (Q.rho_c[e] + P.rhoW[e]) / (R)2;
             const R DT_over_rho = dt / rh;
             R d1 = (P.sxx[e] - P.sxxW[e]) * idx;
             R d2 = (P.sxy[e] - P.sxyS[e]) * idy;
-            R d3 = (P.sxz[e] - Q.sxz_m[e]) * idz;
+            R d3 = (P.sxz[e] - (surface ? -P.sxz[e] : Q.sxz_m[e])) * idz;
             if (cx >= 0) {
                 const R m = cpml(P.mx[0][e], (R)LT.x[0][e], (R)LT.x[1][e], (R)LT.x[2][e], d1);
                 if (onx) stm(mout(A, 9), omx, m, S, 18);
@@ -674,15 +683,15 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             if (onx)
                 nvx[e] = DT_over_rho * (d1 + d2 + d3) + P.vx[e];
         }
-        // vy: i in [0,nx-2], j in [0,ny-2], k in [1,nz-1] (CPU:531-558)
-        const bool ony = L.act[e] && (i <= nx - 2) && (j <= ny - 2) && (k >= 1);
+        // vy: i in [0,nx-2], j in [0,ny-2], k in [1,nz-1] (CPU:531-558), plane 0 at a free surface
+        const bool ony = L.act[e] && (i <= nx - 2) && (j <= ny - 2) && (k >= 1 || surface);
         nvy[e] = P.vy[e];
         {
             const R rh = (Q.rho_c[e] + P.rhoN[e]) / (R)2;
             const R DT_over_rho = dt / rh;
             R d1 = (P.sxyE[e] - P.sxy[e]) * idx;
             R d2 = (P.syyN[e] - P.syy[e]) * idy;
-            R d3 = (P.syz[e] - Q.syz_m[e]) * idz;
+            R d3 = (P.syz[e] - (surface ? -P.syz[e] : Q.syz_m[e])) * idz;
             if (cx >= 0) {
                 const R m = cpml(P.mx[1][e], (R)LT.x[3][e], (R)LT.x[4][e], (R)LT.x[5][e], d1);
                 if (ony) stm(mout(A, 12), omx, m, S, 21);

@@ -58,6 +58,7 @@ struct StepArgs {
     int lo[3], hi[3], w[3];  // compact C-PML regions [0,lo) u [hi,n), width w
     int kmax_shear;          // last plane of the sxz/syz update
     int dirichlet;
+    int fs;                  // traction-free surface on plane 0 (absorbing-layer kernels)
     int emargin;             // energy box [emargin, n-1-emargin) along x and y
     int ek0, ek1;            // energy box planes [ek0, ek1) (a slab's local indices of the global box)
     double dt, idx, idy, idz;

@@ -148,7 +148,8 @@ std::string fused_unsupported(const ShotPlan& p, int tile)
 {
     const ela3d_config& c = p.cfg;
     if (c.storage == ELA3D_STORAGE_FP16) return "the fused schedule runs FP32 and FP64 storage only";
-    if (p.ax[1].lo < 1 || p.ax[2].lo < 1 || p.ax[1].hi > c.ny - 1 || p.ax[2].hi > c.nz - 1)
+    // (a free surface at k = 0 is a one-plane box of the absorbing-layer launches)
+    if (p.ax[1].lo < 1 || (p.ax[2].lo < 1 && !c.free_surface) || p.ax[1].hi > c.ny - 1 || p.ax[2].hi > c.nz - 1)
         return "the fused schedule needs absorbing layers on every y and z face";
     {  // both copies of an array must lie within 4 GiB of its first byte (32-bit offsets)
         const size_t elem = storage_bytes(c.storage);
@@ -976,6 +977,7 @@ private:
         }
         a.kmax_shear = plan_.kmax_shear;
         a.dirichlet = c.dirichlet ? 1 : 0;
+        a.fs = (c.free_surface && plan_.koff == 0) ? 1 : 0;
         a.emargin = std::max(0, c.npoints_pml);
         a.ek0 = plan_.ek0;
         a.ek1 = plan_.ek1;

@@ -22,7 +22,7 @@
 
 namespace ela3d {
 
-typedef std::tuple<int, int, int, int, int> TuneKey;
+typedef std::tuple<int, int, int, int, int, int> TuneKey;
 
 static std::mutex& tune_mutex()
 {
@@ -38,7 +38,8 @@ static std::map<TuneKey, std::pair<Tuning, double>>& tune_cache()
 
 static TuneKey key_of(const ela3d_config& c)
 {
-    return TuneKey(c.nx, c.ny, c.nz, c.storage, c.npoints_pml);
+    // the free surface changes the box split (plane 0 is a box of its own)
+    return TuneKey(c.nx, c.ny, c.nz, c.storage, c.npoints_pml, c.free_surface ? 1 : 0);
 }
 
 bool cached_tuning(const ela3d_config& c, Tuning& out)

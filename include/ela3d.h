@@ -119,7 +119,10 @@ typedef struct ela3d_config {
     int32_t use_graph;      /* 1 (default): replay a hipGraph of one step */
     int32_t block;          /* threads per workgroup, 0 = autotuned */
     int32_t zseg;           /* z planes per workgroup, 0 = autotuned */
-    int32_t reserved[6];
+    int32_t free_surface;   /* 1: traction-free surface at the plane k = 0 (no z-min C-PML
+                               there); the reference only marks the intent (HOM:853,
+                               UNI:1031, ZC:1304: "disable pml zmin for free surface") */
+    int32_t reserved[5];
 } ela3d_config;
 
 typedef struct ela3d_shot ela3d_shot;

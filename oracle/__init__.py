@@ -54,6 +54,7 @@ class OrcCfg(C.Structure):
         ("energy_every", C.c_int),
         ("compat", C.c_uint),
         ("dirichlet", C.c_int),
+        ("free_surface", C.c_int),
         ("threads", C.c_int),
     ]
 
@@ -105,6 +106,7 @@ def make_cfg(cfg, threads: int = 0):
     c.energy_every = int(cfg.energy_every)
     c.compat = int(cfg.compat_flags)
     c.dirichlet = 1 if cfg.dirichlet else 0
+    c.free_surface = 1 if getattr(cfg, "free_surface", False) else 0
     c.threads = int(threads)
     return c, rec
 

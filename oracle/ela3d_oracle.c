@@ -25,6 +25,10 @@
  *              translation invariance; energy conservation with the C-PML
  *              off; C-PML absorption against an enlarged domain.
  *   source / receivers : injection formulas checked on one step from rest.
+ *   free surface (no reference counterpart; the reference states the intent at
+ *              HOM:853) : vertical shear standing waves and horizontal SH waves
+ *              exact with the stress image, the plane-stress law on plane 0,
+ *              the Rayleigh-wave speed (tests/test_oracle_free_surface.py).
  */
 #include "ela3d_oracle.h"
 
@@ -68,11 +72,14 @@ int orc_tables(const orc_cfg* c, int axis, double* out)
     if (zq) { /* CPU:329, 332, 357: z divides by thickness_PML_y */
         Lmin_full = Ly; Lmin_half = Ly; Lmax_half = Ly;
     }
+    /* a free surface at k = 0 has no z-min layer ("disable pml zmin for free
+       surface condition", HOM:853) */
+    int min_side = !(axis == 2 && c->free_surface);
     for (i = 0; i < n; ++i) {
         double x = h * i;
         double s, abscissa;
         abscissa = lo - x;                                    /* min side, full */
-        if (abscissa >= 0.0) {
+        if (min_side && abscissa >= 0.0) {
             s = abscissa / Lmin_full;
             d[i] = d0 * pow(s, c->npower);
             K[i] = 1.0 + (c->k_max - 1.0) * pow(s, c->npower);
@@ -82,7 +89,7 @@ int orc_tables(const orc_cfg* c, int axis, double* out)
             abscissa = Ly - (c->dy * (c->ny - 1) + c->dy / 2.0);
         else
             abscissa = lo - (x + h / 2.0);                    /* min side, half */
-        if (abscissa >= 0.0) {
+        if (min_side && abscissa >= 0.0) {
             s = abscissa / Lmin_half;
             dh[i] = d0 * pow(s, c->npower);
             Kh[i] = 1.0 + (c->k_max - 1.0) * pow(s, c->npower);

@@ -56,6 +56,8 @@ typedef struct orc_cfg {
     int energy_every;         /* 0: no energy */
     unsigned compat;
     int dirichlet;            /* zero vx,vy,vz on the six faces after the v update */
+    int free_surface;         /* traction-free surface at k = 0, no z-min C-PML (the
+                                 reference's stated intent, HOM:853; not implemented there) */
     int threads;              /* OpenMP threads, <=0: runtime default */
 } orc_cfg;
 

@@ -11,7 +11,7 @@
 // stress_body/velocity_body; fused.hip fload/qinit and the velocity update).
 //
 // usage: index_check nx ny nz npml storage compat block zseg vec block_slab
-//                    zseg_slab vec_slab merged fused tile [mutation]
+//                    zseg_slab vec_slab merged fused tile [mutation [free_surface]]
 // mutation 1: offsets formed from plane 0 instead of the allocation start (the
 // wrap of unsigned neighbour offsets at the first row); 2: field arrays one plane
 // short.  Prints "accesses A violations V" and exits 0 (2 on bad arguments).
@@ -165,11 +165,11 @@ int main(int argc, char** argv)
 {
     if (argc < 16) {
         std::fprintf(stderr, "usage: index_check nx ny nz npml storage compat block zseg vec block_slab "
-                             "zseg_slab vec_slab merged fused tile [mutation]\n");
+                             "zseg_slab vec_slab merged fused tile [mutation [free_surface]]\n");
         return 2;
     }
-    int a[16] = {};
-    for (int q = 1; q < argc && q <= 16; ++q) a[q - 1] = std::atoi(argv[q]);
+    int a[17] = {};
+    for (int q = 1; q < argc && q <= 17; ++q) a[q - 1] = std::atoi(argv[q]);
     g_mutation = argc > 16 ? a[15] : 0;
     ela3d_config c;
     ela3d_config_default(&c);
@@ -177,6 +177,7 @@ int main(int argc, char** argv)
     c.storage = a[4]; c.compat_flags = a[5];
     c.isrc = c.nx / 2; c.jsrc = c.ny / 2; c.ksrc = c.nz / 2;
     c.nstep = 2; c.nrec = 0; c.rec = nullptr; c.energy_every = 0;
+    c.free_surface = argc > 17 ? a[16] : 0;
     Tuning t;
     t.block = a[6]; t.zseg_interior = a[7]; t.vec = a[8];
     t.block_slab = a[9]; t.zseg_slab = a[10]; t.vec_slab = a[11];

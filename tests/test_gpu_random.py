@@ -39,6 +39,8 @@ def draw(seed):
                         ksrc=int(rng.integers(1, nz - 1)), receivers=rec, angle_force=float(rng.uniform(0, 90)),
                         energy_every=pick([0, 1, 3]))
         cfg = cfg.replace(dirichlet=bool(rng.random() < 0.5))
+        # a free surface in about a third of the cases (its own generator keeps the other draws)
+        cfg = cfg.replace(free_surface=bool(np.random.default_rng(7000 + seed).random() < 0.35))
         try:
             cfg.validate()
         except Ela3dError:
@@ -78,7 +80,8 @@ def test_random_case_one_step(gpu, seed):
     cfg, sh = draw(seed)
     assert address_violations(cfg, sh) == []
     print(f"case {seed}: {cfg.nx}x{cfg.ny}x{cfg.nz} npml {cfg.npoints_pml} compat {cfg.compat_flags} "
-          f"src {cfg.src_kind} wavelet {cfg.wavelet} storage {cfg.storage} nrec {cfg.nrec} {sh}", flush=True)
+          f"src {cfg.src_kind} wavelet {cfg.wavelet} storage {cfg.storage} nrec {cfg.nrec} "
+          f"free surface {cfg.free_surface} {sh}", flush=True)
     set_tuning(cfg, **sh)
     model = hetero_model(cfg, seed=seed)
     st = random_state(cfg, seed=seed, dtype=DT[cfg.storage])
@@ -118,6 +121,7 @@ def draw_fp16(seed, storage=STORAGE_FP16):
                         isrc=int(rng.integers(lo, nx - lo)), jsrc=int(rng.integers(lo, ny - lo)),
                         ksrc=int(rng.integers(lo, nz - lo)), receivers=rec.astype(np.int32),
                         angle_force=float(rng.uniform(0, 90)), energy_every=pick([1, 5]))
+        cfg = cfg.replace(free_surface=bool(np.random.default_rng(8000 + seed).random() < 0.35))
         try:
             cfg.validate()
         except Ela3dError:

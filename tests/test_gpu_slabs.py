@@ -39,19 +39,22 @@ def run_slabs(cfg, model, cuts):
     return bounds, slabs
 
 
-@pytest.mark.parametrize("storage,cuts,kind,compat", [
-    (STORAGE_FP32, (17,), SRC_EXPLOSIVE, 0),
-    (STORAGE_FP64, (11, 23), SRC_FORCE3, 0),
-    (STORAGE_FP32, (3, 16, 30), SRC_EXPLOSIVE, COMPAT_REFERENCE),  # cuts inside both z layers
+@pytest.mark.parametrize("storage,cuts,kind,compat,fs", [
+    (STORAGE_FP32, (17,), SRC_EXPLOSIVE, 0, False),
+    (STORAGE_FP64, (11, 23), SRC_FORCE3, 0, False),
+    (STORAGE_FP32, (3, 16, 30), SRC_EXPLOSIVE, COMPAT_REFERENCE, False),  # cuts inside both z layers
+    (STORAGE_FP32, (9, 20), SRC_FORCE3, 0, True),  # free surface on the lowest slab
+    (STORAGE_FP64, (2, 17), SRC_EXPLOSIVE, COMPAT_REFERENCE, True),
 ])
-def test_slabs_equal_the_whole_grid(gpu, storage, cuts, kind, compat):
+def test_slabs_equal_the_whole_grid(gpu, storage, cuts, kind, compat, fs):
     cfg = small_cfg(nx=29, ny=23, nz=34, npml=4, compat=compat, storage=storage, nstep=45,
-                    src_kind=kind, ksrc=cuts[0] - 1 if cuts[0] > 2 else 15)
-    rec = np.asarray([(i, 11, k) for k in (2, cuts[0] - 1, cuts[0], 20, 31) for i in range(1, 28, 5)],
+                    src_kind=kind, ksrc=cuts[0] - 1 if cuts[0] > 2 else 15, free_surface=fs)
+    rec = np.asarray([(i, 11, k) for k in (0, 2, cuts[0] - 1, cuts[0], 20, 31) for i in range(1, 28, 5)],
                      dtype=np.int32)
     cfg = cfg.replace(receivers=rec)
     model = hetero_model(cfg, seed=9)
-    print(f"slabs {cuts} storage {storage} source {kind} at k={cfg.ksrc} compat {compat}", flush=True)
+    print(f"slabs {cuts} storage {storage} source {kind} at k={cfg.ksrc} compat {compat} free surface {fs}",
+          flush=True)
     seis, en, snaps = run_whole(cfg, model)
     bounds, slabs = run_slabs(cfg, model, cuts)
     try:

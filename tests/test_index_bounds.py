@@ -41,10 +41,11 @@ def violations(cfg, block, vec, zi, zs, base_at_guard=False, block_slab=None, ve
     pa, pb = (lo[1] * nx + 63) // 64 * 64, (hi[1] * nx) // 64 * 64
     if pb <= pa:
         pa = pb = nxy
+    zl = min(1, hi[2]) if (cfg.free_surface and lo[2] < 1) else lo[2]  # a free surface is a box of its own
     boxes = []
-    for (p0, p1, k0, k1, z, mode) in ((0, nxy, 0, lo[2], zs, 1), (0, nxy, hi[2], nz, zs, 1),
-                                      (0, pa, lo[2], hi[2], zi, 1), (pb, nxy, lo[2], hi[2], zi, 1),
-                                      (pa, pb, lo[2], hi[2], zi, 0)):
+    for (p0, p1, k0, k1, z, mode) in ((0, nxy, 0, zl, zs, 1), (0, nxy, hi[2], nz, zs, 1),
+                                      (0, pa, zl, hi[2], zi, 1), (pb, nxy, zl, hi[2], zi, 1),
+                                      (pa, pb, zl, hi[2], zi, 0)):
         if p1 > p0 and k1 > k0:
             boxes.append((p0, p1, k0, k1, z, mode))
     vec_i = vec
@@ -143,7 +144,9 @@ def fused_violations(cfg, tile, zseg, elem=4, ldsreg=None):
     memsize0 = nz * ny * w0
     pa, pb = (lo[1] * nx + 63) // 64 * 64, (hi[1] * nx) // 64 * 64
     zlo, zhi = lo[2], hi[2]
-    if pb <= pa or zhi <= zlo or lo[1] < 1 or lo[2] < 1 or hi[1] > ny - 1 or hi[2] > nz - 1:
+    if cfg.free_surface and zlo < 1:  # plane 0 is a box of the absorbing-layer launches
+        zlo = 1
+    if pb <= pa or zhi <= zlo or lo[1] < 1 or zlo < 1 or hi[1] > ny - 1 or hi[2] > nz - 1:
         return None  # no interior box, or a face without absorbing layer: refused (fused_unsupported)
     maxt = 512 if elem == 8 else 1024
     own_full = tile - 2 * nx

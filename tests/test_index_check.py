@@ -27,10 +27,10 @@ def checker(tmp_path_factory):
 
 
 def run(exe, grid, storage=0, compat=0, block=256, zseg=4, vec=1, block_slab=128, zseg_slab=2,
-        vec_slab=1, merged=0, fused=0, tile=0, mutation=0):
+        vec_slab=1, merged=0, fused=0, tile=0, mutation=0, fs=0):
     nx, ny, nz, npml = grid
     args = [nx, ny, nz, npml, storage, compat, block, zseg, vec, block_slab, zseg_slab, vec_slab, merged,
-            fused, tile, mutation]
+            fused, tile, mutation, fs]
     out = subprocess.run([exe, *map(str, args)], check=True, stdout=subprocess.PIPE, text=True).stdout
     last = out.strip().splitlines()[-1]
     if last.startswith("refused"):
@@ -44,22 +44,24 @@ SMALL = [(29, 23, 26, 4), (17, 19, 18, 3), (40, 13, 15, 4), (11, 12, 13, 0), (64
 
 
 @pytest.mark.parametrize("grid", SMALL)
-def test_split_launch_shapes(checker, grid):
+@pytest.mark.parametrize("fs", [0, 1])
+def test_split_launch_shapes(checker, grid, fs):
     for storage in (0, 1, 2):
         for compat in (0, 7):
             for block, zseg, vec in ((64, 2, 1), (256, 4, 2), (128, 32, 4), (64, 1, 4)):
                 for block_slab, zseg_slab, vec_slab in ((64, 1, 1), (256, 8, 2)):
                     for merged in (0, 1):
                         r = run(checker, grid, storage, compat, block, zseg, vec, block_slab, zseg_slab,
-                                vec_slab, merged)
+                                vec_slab, merged, fs=fs)
                         assert r is not None and r[0] > 0 and r[1] == 0, (storage, compat, block, zseg, vec, r)
 
 
 @pytest.mark.parametrize("grid", SMALL)
-def test_fused_launch_shapes(checker, grid):
+@pytest.mark.parametrize("fs", [0, 1])
+def test_fused_launch_shapes(checker, grid, fs):
     for storage in (0, 2):
         for tile, zseg in ((256, 2), (512, 4), (1024, 8), (256, 32), (768, 1)):
-            r = run(checker, grid, storage, block=128, zseg=zseg, fused=1, tile=tile)
+            r = run(checker, grid, storage, block=128, zseg=zseg, fused=1, tile=tile, fs=fs)
             assert r is None or r[1] == 0, (storage, tile, zseg, r)
 
 

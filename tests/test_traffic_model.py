@@ -31,9 +31,10 @@ def touched(cfg, seed=4):
 
 @pytest.mark.parametrize("shape", [(29, 23, 26, 4), (40, 13, 15, 4), (17, 19, 18, 3), (64, 64, 9, 3)])
 @pytest.mark.parametrize("compat", [0, 7])
-def test_plan_bytes_match_oracle_update_set(shape, compat):
+@pytest.mark.parametrize("fs", [False, True])
+def test_plan_bytes_match_oracle_update_set(shape, compat, fs):
     nx, ny, nz, npml = shape
-    cfg = small_cfg(nx=nx, ny=ny, nz=nz, npml=npml, compat=compat)
+    cfg = small_cfg(nx=nx, ny=ny, nz=nz, npml=npml, compat=compat, free_surface=fs)
     ns, nv = touched(cfg)
     cells = nx * ny * nz
     for storage, e in ((STORAGE_FP32, 4), (STORAGE_FP16, 2), (STORAGE_FP64, 8)):
