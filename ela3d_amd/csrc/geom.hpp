@@ -146,7 +146,10 @@ __host__ __device__ inline int fused_cell(int t, int n, int nx)
     return kNoCell;
 }
 
-// Own cells of a full fused tile of nt threads (region minus two halo rows).
-__host__ __device__ inline int fused_own(int nt, int nx) { return nt - 2 * nx; }
+// Own cells of a full fused tile of nt threads: the region minus two halo rows,
+// rounded down to a multiple of 32 so that every tile starts on a 128-byte line
+// (box starts are multiples of 64 cells) and each wave's 64 own cells span two
+// lines, not three; the threads left over idle.
+__host__ __device__ inline int fused_own(int nt, int nx) { return (nt - 2 * nx) / 32 * 32; }
 
 }  // namespace ela3d

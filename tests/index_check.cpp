@@ -14,10 +14,12 @@
 //                    zseg_slab vec_slab merged fused tile [mutation [free_surface]]
 // mutation 1: offsets formed from plane 0 instead of the allocation start (the
 // wrap of unsigned neighbour offsets at the first row); 2: field arrays one plane
-// short.  Prints "accesses A violations V" and exits 0 (2 on bad arguments).
+// short; 3: each fused tile one own cell short (a cell no tile owns).  Prints
+// "accesses A violations V" and exits 0 (2 on bad arguments).
 #include <cstdio>
 #include <cstdlib>
 #include <string>
+#include <vector>
 
 #include "core.hpp"
 #include "geom.hpp"
@@ -157,6 +159,39 @@ void fused_launch(const ShotPlan& P, const LaunchGeom& G, const Limits& L, Count
             }
         }
     }
+    // every cell of the box is owned by exactly one tile, and every LDS cell an own
+    // cell's velocity update uses is written by a region thread of its tile
+    std::vector<int> owner(size_t(B.p1 - B.p0), 0);
+    for (int ch = 0; ch < B.nchunk; ++ch) {
+        const int q0 = B.p0 + ch * own_full;
+        const int n = std::min(own_full, B.p1 - q0) - (g_mutation == 3 ? 1 : 0);
+        std::vector<char> written(size_t(G.tile), 0);
+        std::vector<long long> reads;
+        for (int t = 0; t < G.block; ++t) {
+            const int cc = fused_cell(t, n, nx);
+            if (cc == kNoCell) continue;
+            if (cc + nx >= 0 && cc + nx < G.tile) written[size_t(cc + nx)] = 1;
+            if (t < n) {
+                ++owner[size_t(q0 + cc - B.p0)];
+                const int i = (q0 + cc) % nx;
+                // sxx(i-1) is used where i >= 1, sxy/sxz(i+1) where i <= nx-2
+                if (i >= 1) reads.push_back(cc + nx - 1);
+                if (i <= nx - 2) reads.push_back(cc + nx + 1);
+                reads.push_back(cc);           // j-1
+                reads.push_back(cc + 2 * nx);  // j+1
+            }
+        }
+        for (long long r : reads)
+            if (r < 0 || r >= G.tile || !written[size_t(r)]) {
+                if (C.bad < 8) std::printf("  LDS cell %lld used but not written (tile %d)\n", r, ch);
+                ++C.bad;
+            }
+    }
+    for (size_t q = 0; q < owner.size(); ++q)
+        if (owner[q] != 1) {
+            if (C.bad < 8) std::printf("  box cell %zu owned %d times\n", q, owner[q]);
+            ++C.bad;
+        }
 }
 
 }  // namespace

@@ -149,7 +149,7 @@ def fused_violations(cfg, tile, zseg, elem=4, ldsreg=None):
     if pb <= pa or zhi <= zlo or lo[1] < 1 or zlo < 1 or hi[1] > ny - 1 or hi[2] > nz - 1:
         return None  # no interior box, or a face without absorbing layer: refused (fused_unsupported)
     maxt = 512 if elem == 8 else 1024
-    own_full = tile - 2 * nx
+    own_full = (tile - 2 * nx) // 32 * 32  # geom.hpp fused_own: a multiple of 32
     if tile < 64 or tile > maxt or tile % 64 or own_full < 64:
         return None  # the library refuses this tile (fused_lds_bytes)
     LR = tile if ldsreg is None else ldsreg

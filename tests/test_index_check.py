@@ -85,9 +85,11 @@ def test_largest_grids(checker):
 
 def test_checker_catches_wrapped_and_short_arrays(checker):
     """Offsets formed from plane 0 wrap below zero at the first row; arrays one
-    plane short are overrun at the top.  Both must be reported."""
+    plane short are overrun at the top; a fused tile one own cell short leaves a
+    cell no tile owns.  All must be reported."""
     grid = (29, 23, 26, 4)
     assert run(checker, grid, vec=2, mutation=1)[1] > 0
     assert run(checker, grid, vec=2, mutation=2)[1] > 0
     assert run(checker, grid, block=128, zseg=4, fused=1, tile=256, mutation=1)[1] > 0
     assert run(checker, grid, block=128, zseg=4, fused=1, tile=256, mutation=2)[1] > 0
+    assert run(checker, grid, block=128, zseg=4, fused=1, tile=256, mutation=3)[1] > 0
