@@ -404,6 +404,10 @@ uint64_t fingerprint(const ela3d::ShotPlan& p)
     mix(ints, sizeof(ints));
     mix(dbl, sizeof(dbl));
     if (!p.rec.empty()) mix(p.rec.data(), p.rec.size() * sizeof(int32_t));
+    if (c.free_surface) {  // (mixed only when set, so earlier checkpoints keep their fingerprint)
+        const int32_t fs = 1;
+        mix(&fs, sizeof(fs));
+    }
     return h;
 }
 

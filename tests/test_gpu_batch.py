@@ -63,3 +63,16 @@ def test_batch_admits_arrivals_while_running(gpu):
     good = [o for q, o in enumerate(out) if q != 3]
     for (seis, en, info), r in zip(good, ref):
         assert info["status"] == 0 and np.array_equal(seis, r)
+
+
+def test_batched_free_surface_shots_match_alone(gpu):
+    """Members with a free surface (plane 0 a box of the absorbing-layer kind, which
+    the batch launch walks like any other) step bit-identically to running alone."""
+    shots = [s.materialize() for s in specs(5, 31, (32, 40))]
+    shots = [(c.replace(free_surface=(q % 2 == 0), ksrc=3), m) for q, (c, m) in enumerate(shots)]
+    ref = [alone(c, m) for c, m in shots]
+    out, stats = serve(shots, max_active=8, steps_per_round=20, batch_cells=48 ** 3)
+    assert stats["done"] == len(shots) and stats["failed"] == 0 and stats["batched"] == len(shots)
+    for (seis, en, info), r in zip(out, ref):
+        assert info["status"] == 0 and np.any(r)
+        assert np.array_equal(seis, r)

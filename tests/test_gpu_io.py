@@ -20,10 +20,12 @@ pytestmark = pytest.mark.gpu
 CLI = os.path.join(os.path.dirname(ela3d_amd.__file__), "bin", "ela3d_run")
 
 
-@pytest.mark.parametrize("storage,dbuf", [(0, False), (STORAGE_FP16, False), (0, True)])
-def test_checkpoint_resume_bit_identical(gpu, tmp_path, storage, dbuf):
-    """Also with two field sets: the set holding the state follows the step count."""
-    cfg = small_cfg(nx=31 if dbuf else 29, nstep=90, storage=storage, src_kind=SRC_EXPLOSIVE)
+@pytest.mark.parametrize("storage,dbuf,fs", [(0, False, False), (STORAGE_FP16, False, False), (0, True, False),
+                                           (0, True, True)])
+def test_checkpoint_resume_bit_identical(gpu, tmp_path, storage, dbuf, fs):
+    """Also with two field sets (the set holding the state follows the step count) and
+    with a free surface."""
+    cfg = small_cfg(nx=31 if dbuf else 29, nstep=90, storage=storage, src_kind=SRC_EXPLOSIVE, free_surface=fs)
     if dbuf:
         ela3d_amd.set_tuning(cfg, block=256, zseg_interior=4, zseg_slab=4, dbuf=True)
     model = hetero_model(cfg, seed=2)
@@ -53,6 +55,11 @@ def test_checkpoint_rejects_other_config(gpu, tmp_path):
         a.checkpoint(ck)
     other = cfg.replace(f0=cfg.f0 * 1.5)
     with Shot(other, *homo(other)) as b:
+        with pytest.raises(ela3d_amd.Ela3dError) as e:
+            b.restore(ck)
+        assert e.value.code == -1
+    surface = cfg.replace(free_surface=True)  # same sizes, another boundary: refused too
+    with Shot(surface, *homo(surface)) as b:
         with pytest.raises(ela3d_amd.Ela3dError) as e:
             b.restore(ck)
         assert e.value.code == -1
@@ -103,6 +110,26 @@ def test_source_trace_vs_oracle(gpu, src):
 
 
 @pytest.mark.skipif(not os.path.exists(CLI), reason="ela3d_run not built")
+def test_cli_free_surface(gpu, tmp_path):
+    """free_surface=1 on the command line runs the library's free surface."""
+    n, npml, nstep = 28, 4, 30
+    args = [f"nx={n}", f"ny={n}", f"nz={n}", f"npml={npml}", f"nstep={nstep}", "dt=1e-3", "f0=25",
+            "rec=line:x,14,0,4", "free_surface=1", f"out={tmp_path}"]
+    r = subprocess.run([CLI, *args], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rec = np.array([(i, 14, 0) for i in range(0, n, 4)], np.int32)
+    cfg = ela3d_amd.Config(nx=n, ny=n, nz=n, npoints_pml=npml, nstep=nstep, dt=1e-3, f0=25.0,
+                           isrc=n // 2, jsrc=n // 2, ksrc=n // 2, receivers=rec, free_surface=True)
+    model = ela3d_amd.homogeneous_model(n, n, n, 3300.0, 3300.0 / 1.732, 2800.0)
+    with Shot(cfg, *model) as s:
+        s.run()
+        seis = s.seismograms()
+    got = np.array([[float(x) for x in ln.split()] for ln in open(tmp_path / "rechorvx.txt").read().splitlines()])
+    assert got.shape == seis[:, :, 0].shape
+    assert np.allclose(got, seis[:, :, 0].astype(np.float32).astype(np.float64), rtol=5e-6, atol=1e-30)
+    assert np.any(seis[:, :, 0] != 0.0)  # the surface moves (without it, vx on plane 0 stays 0)
+
+
 def test_cli_end_to_end(gpu, tmp_path):
     n, npml, nstep = 32, 4, 40
     args = [f"nx={n}", f"ny={n}", f"nz={n}", f"npml={npml}", f"nstep={nstep}", "dt=1e-3", "f0=25",
