@@ -97,19 +97,33 @@ def cpu_baseline(cfg, model, budget_s=20.0):
 PMC_FILES = {"classic": "pmc_classic_101x641x640_fp32.json"}
 
 
-def load_traffic(workload, kernel):
+# the launch-shape fields a kernel's traffic depends on
+SHAPE_KEYS = {"fused": ("fused", "tile", "zseg"), "interior": ("block", "zseg", "vec", "merged"),
+              "cpml": ("block_slab", "zseg_slab", "vec_slab", "merged")}
+
+
+def load_traffic(workload, kernel, shape=None):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel of known size), with the
-    launch shape it was measured at."""
+    launch shape it was measured at: the measurement at `shape` where there is one
+    (entries "<kernel>" and "<kernel>@<tile>x<zseg>"), else the kernel's main entry."""
     name = PMC_FILES.get(workload)
     path = os.path.join(ROOT, "profiles", name) if name else ""
     if not name or not os.path.exists(path):
         return None, None
     with open(path) as f:
         d = json.load(f)
-    k = d.get("kernels", {}).get(kernel)
-    if k is None:
+    cands = [v for k, v in d.get("kernels", {}).items() if k == kernel or k.startswith(kernel + "@")]
+    if not cands:
         return None, None
+
+    keys = SHAPE_KEYS["fused" if kernel.startswith("fused") else "cpml" if "cpml" in kernel else "interior"]
+
+    def same(v):
+        ls = v.get("launch_shape", d.get("launch_shape")) or {}
+        return shape is not None and all(ls.get(k, 0) == shape.get(k, 0) for k in keys)
+
+    k = next((v for v in cands if same(v)), d["kernels"].get(kernel, cands[0]))
     return k.get("hbm_bytes_per_dispatch"), k.get("launch_shape", d.get("launch_shape"))
 
 
@@ -490,7 +504,10 @@ def main():
     avg_ms = dom["total_ms"] / dom["launches"]
     achieved = dom["alg_bytes_per_launch"] / (avg_ms * 1e-3)
     step_bw = info["alg_bytes_per_step"] / (ms_per_step * 1e-3)
-    traffic, traffic_shape = load_traffic(args.workload, dom["name"])
+    shape_now = {"block": info["block"], "zseg": info["zseg"], "vec": info["vec"],
+                 "block_slab": info["block_slab"], "zseg_slab": info["zseg_slab"], "vec_slab": info["vec_slab"],
+                 "overlap": info["overlap"], "merged": info["merged"], "fused": info["fused"], "tile": info["tile"]}
+    traffic, traffic_shape = load_traffic(args.workload, dom["name"], shape_now)
     log(f"[bench] step: {ms_per_step:.4f} ms, {mcells:.1f} Mcell-updates/s, algorithmic "
         f"{info['alg_bytes_per_step'] / 1e6:.1f} MB/step -> {step_bw / 1e9:.1f} GB/s "
         f"({100 * step_bw / PEAK_HBM:.1f}% of peak)")
@@ -520,10 +537,7 @@ def main():
             "graph_mcell_updates_per_s": None if graph_ms is None else round(cfg.cells / (graph_ms * 1e-3) / 1e6, 2),
             "accuracy": accuracy,
             "serving": serving,
-            "launch_shape": {"block": info["block"], "zseg": info["zseg"], "vec": info["vec"],
-                             "block_slab": info["block_slab"], "zseg_slab": info["zseg_slab"],
-                             "vec_slab": info["vec_slab"], "overlap": info["overlap"],
-                             "merged": info["merged"], "fused": info["fused"], "tile": info["tile"]},
+            "launch_shape": shape_now,
         },
         "roofline": {
             "bound": "hbm", "kernel": dom["name"], "achieved": round(achieved / 1e9, 2),
