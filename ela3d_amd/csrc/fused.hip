@@ -195,8 +195,11 @@ __device__ __forceinline__ void fstress(const StepArgs<T>& A, const FLoad<typena
 // component slots of the LDS plane buffer
 constexpr int kLxx = 0, kLxy = 1, kLyy = 2, kLxz = 3, kLyz = 4;
 
+// FP32: at least 6 waves per SIMD, so two 768-thread workgroups share a CU (at
+// most 80 VGPRs; the energy sums would otherwise take the energy variant to 83)
 template <typename T, bool ENERGY, int NT>
-__global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1)))
+k_fused(const StepArgs<T> A)
 {
     typedef typename Arith<T>::R R;
     extern __shared__ double fused_smem[];
@@ -312,7 +315,7 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
     if (computes(m0)) qinit(m0);
     R pv_sxz = 0, pv_syz = 0;  // new sxz, syz of the own cell one plane below
     FSaved<R> S, N;            // velocity inputs of the plane whose velocity is next
-    double ek = 0.0, ep = 0.0;
+    R ek = 0, ep = 0;  // this thread's energy terms (a segment's planes); summed in FP64 below
     const bool face = A.dirichlet && (i == 0 || i == nx - 1);  // the interior box has no j or k face
     const int em = A.emargin;
     const bool in_e_xy = i >= em && i < nx - 1 - em && j >= em && j < ny - 1 - em;
@@ -399,15 +402,19 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
             if (iN || face) el(fw[2], oo) = nvz;
             if (ENERGY) {
                 if (in_e_xy && k >= A.ek0 && k < A.ek1) {
-                    const double r = S.rho, vp = S.cp, vs = S.cs;
-                    const double mu = r * vs * vs, lam = r * (vp * vp - 2.0 * vs * vs);
-                    const double dxx = sxx, dyy = syy, dzz = S.szz;
-                    const double dxy = sxy, dxz = sxz, dyz = syz;
-                    const double tr = dxx + dyy + dzz;
-                    const double ss = dxx * dxx + dyy * dyy + dzz * dzz + 2.0 * (dxy * dxy + dxz * dxz + dyz * dyz);
-                    const double dvx = nvx, dvy = nvy, dvz = nvz;
-                    ek += 0.5 * r * (dvx * dvx + dvy * dvy + dvz * dvz);
-                    ep += 0.5 * (ss / (2.0 * mu) - lam * tr * tr / (2.0 * mu * (3.0 * lam + 2.0 * mu)));
+                    // kinetic 0.5 rho |v|^2 and strain energy
+                    // 0.5 (s:s / (2 mu) - lambda tr(s)^2 / (2 mu (3 lambda + 2 mu)))
+                    //   = (s:s (3 lambda + 2 mu) - lambda tr(s)^2) / (4 mu (3 lambda + 2 mu)),
+                    // one division per cell.  Cell terms in the arithmetic type (FP32
+                    // storage: FP32, about 1e-7 relative, below the state's own
+                    // rounding), sums in FP64.
+                    const R r = S.rho, vp = S.cp, vs = S.cs;
+                    const R mu = r * vs * vs, lam = r * (vp * vp - (R)2 * vs * vs);
+                    const R tr = sxx + syy + S.szz;
+                    const R ss = sxx * sxx + syy * syy + S.szz * S.szz + (R)2 * (sxy * sxy + sxz * sxz + syz * syz);
+                    const R q = (R)3 * lam + (R)2 * mu;
+                    ek += (R)0.5 * r * (nvx * nvx + nvy * nvy + nvz * nvz);
+                    ep += (ss * q - lam * tr * tr) / ((R)4 * mu * q);
                 }
             }
             pv_sxz = sxz;
@@ -424,12 +431,11 @@ __global__ void __launch_bounds__(NT) k_fused(const StepArgs<T> A)
     for (int m = kb - 1; m <= ke; ++m) plane(m);
     if (ENERGY) {  // per-block FP64 partials, summed in block order by the tail kernel
         __shared__ double red[2][NT / 64];
-        ek = wave_sum(ek);
-        ep = wave_sum(ep);
+        const double eks = wave_sum((double)ek), eps = wave_sum((double)ep);
         const int lane = t & 63, wid = t >> 6;
         if (lane == 0) {
-            red[0][wid] = ek;
-            red[1][wid] = ep;
+            red[0][wid] = eks;
+            red[1][wid] = eps;
         }
         __syncthreads();
         if (t == 0) {
