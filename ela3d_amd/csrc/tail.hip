@@ -150,18 +150,20 @@ __device__ __forceinline__ void tail_body(const TailArgs<T>& A)
             }
             }
         }
-        if (it < A.nsrctrace) {  // stresses at the source cell after injection (UNI:463-465)
-            const int o = A.src;
-            A.srctrace[3 * it] = sigma_at_source(A, 3, o);
-            A.srctrace[3 * it + 1] = sigma_at_source(A, 4, o);
-            A.srctrace[3 * it + 2] = sigma_at_source(A, 5, o);
+        // the velocities and normal stresses at the source cell after injection, all
+        // loaded at once: the stresses are the source trace (UNI:463-465), and every
+        // step, whatever the receivers and energy cadence, a non-finite one marks the
+        // shot unstable (a blow-up of the grid reaches the source cell within a few steps)
+        const int o = A.src;
+        const double sxx = sigma_at_source(A, 3, o), syy = sigma_at_source(A, 4, o), szz = sigma_at_source(A, 5, o);
+        const double vx = ldf(A.f[0], o, nisc<T>(A, 0)), vy = ldf(A.f[1], o, nisc<T>(A, 1));
+        const double vz = ldf(A.f[2], o, nisc<T>(A, 2));
+        if (it < A.nsrctrace) {
+            A.srctrace[3 * it] = sxx;
+            A.srctrace[3 * it + 1] = syy;
+            A.srctrace[3 * it + 2] = szz;
         }
-        // every step, whatever the receivers and energy cadence: the nine fields at
-        // the source cell, where a blow-up of the whole grid shows within a few steps
-        bool fin = true;
-        for (int q = 0; q < 9; ++q)
-            fin = fin && isfinite((q >= 3 && q < 6) ? sigma_at_source(A, q, A.src)
-                                                    : (double)ldf(A.f[q], A.src, nisc<T>(A, q)));
+        const bool fin = isfinite(sxx) & isfinite(syy) & isfinite(szz) & isfinite(vx) & isfinite(vy) & isfinite(vz);
         if (bad || !fin) *A.flag = 1;
         *A.step = it;
     }

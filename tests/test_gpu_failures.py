@@ -62,7 +62,7 @@ def test_server_fails_only_the_shot_that_does_not_fit(gpu):
 @pytest.mark.parametrize("storage", [0, 1])  # FP32, FP16
 def test_blow_up_is_flagged_without_receivers_or_energy(gpu, storage):
     """A shot beyond Courant with no receivers and no energy sums is still reported
-    unstable: the tail checks the nine fields at the source cell every step."""
+    unstable: the tail checks the velocities and normal stresses at the source cell every step."""
     from ela3d_amd import Config, homogeneous_model
     cfg = Config(nx=24, ny=22, nz=20, dx=10.0, dy=10.0, dz=10.0, dt=4.0e-3, nstep=400,
                  npoints_pml=3, f0=18.0, isrc=12, jsrc=11, ksrc=10, energy_every=0,
