@@ -50,6 +50,17 @@ __device__ __forceinline__ void stf(T* p, unsigned o, typename Arith<T>::R v, fl
         el(p, o) = v;
 }
 
+// Streaming (nontemporal) store of element o of p.  For the nine wave fields: a
+// step writes each element once and only a later launch reads it, so the lines
+// need not stay in L2 and leave it to the neighbour and halo re-reads of the
+// launch in flight.  The memory variables keep ordinary stores: their C-PML runs
+// cover part of a line, and streaming them measured slower.
+template <typename T, typename V>
+__device__ __forceinline__ void stn(T* p, unsigned o, V v)
+{
+    __builtin_nontemporal_store(static_cast<T>(v), &el(p, o));
+}
+
 // Workgroup barrier that orders LDS accesses only.  __syncthreads() also orders
 // global memory at workgroup scope, which on gfx950 waits for every outstanding
 // global load and store of the wave (vmcnt(0)) before the barrier, so loads

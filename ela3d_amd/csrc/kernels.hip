@@ -152,7 +152,8 @@ __device__ __forceinline__ void stm(T* p, unsigned o, typename Arith<T>::R v, Sc
     S.track(q, (float)v);
 }
 
-// store the enabled elements; one vector store when all are enabled
+// store the enabled elements of a wave field, streaming (stn); one vector store
+// when all are enabled
 template <typename T, int V>
 __device__ __forceinline__ void stv(T* p, unsigned o, const typename Arith<T>::R (&val)[V], const bool (&en)[V],
                                     Scales<T>& S, int q)
@@ -175,13 +176,18 @@ __device__ __forceinline__ void stv(T* p, unsigned o, const typename Arith<T>::R
                 else
                     v[e] = val[e];
             }
-            __builtin_memcpy(&el(p, o), &v, sizeof(vt));
+            __builtin_nontemporal_store(v, reinterpret_cast<vt*>(&el(p, o)));
             return;
         }
     }
 #pragma unroll
-    for (int e = 0; e < V; ++e)
-        if (en[e]) stf(p, o + e, val[e], sc);
+    for (int e = 0; e < V; ++e) {
+        if constexpr (std::is_same<T, half_t>::value) {
+            if (en[e]) stn(p, o + e, val[e] * sc);
+        } else {
+            if (en[e]) stn(p, o + e, val[e]);
+        }
+    }
 }
 
 // memory-variable recurrence psi <- b psi + a D;  D <- D/K + psi  (CPU:395-405),
