@@ -366,7 +366,7 @@ k_fused(const StepArgs<T> A)
             R nvx = S.vx, nvy = S.vy, nvz = S.vz;
             {  // vx: i >= 1 (CPU:502-529)
                 const R rh = (S.rho + S.rhoW) / (R)2;
-                const R DT_over_rho = dt / rh;
+                const R DT_over_rho = dt_over(dt, rh);
                 R d1 = (sxx - sxxW) * idx;
                 const R d2 = (sxy - sxyS) * idy;
                 const R d3 = (sxz - pv_sxz) * idz;
@@ -376,7 +376,7 @@ k_fused(const StepArgs<T> A)
             }
             {  // vy: i <= nx-2 (CPU:531-558)
                 const R rh = (S.rho + S.rhoN) / (R)2;
-                const R DT_over_rho = dt / rh;
+                const R DT_over_rho = dt_over(dt, rh);
                 R d1 = (sxyE - sxy) * idx;
                 const R d2 = (syyN - syy) * idy;
                 const R d3 = (syz - pv_syz) * idz;
@@ -386,7 +386,7 @@ k_fused(const StepArgs<T> A)
             }
             {  // vz: i <= nx-2 (CPU:562-590)
                 const R rh = (S.rho + S.rho_u) / (R)2;
-                const R DT_over_rho = dt / rh;
+                const R DT_over_rho = dt_over(dt, rh);
                 R d1 = (sxzE - sxz) * idx;
                 const R d2 = (syz - syzS) * idy;
                 const R d3 = (szz_u - S.szz) * idz;
