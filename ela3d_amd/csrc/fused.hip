@@ -427,9 +427,8 @@ k_fused(const StepArgs<T> A)
         // iteration; the buffer that iteration writes was last read by this one
         if (m >= kb && m < ke) lds_barrier();
     };
-    // Planes kb-1 .. ke, two per trip so that the two load buffers alternate
-    // without a copy (a copy of a register a load is still filling waits for
-    // that load); an odd count starts with one idle plane kb-2.
+    // planes kb-1 .. ke: plane kb-1 only computes sigma_xz and sigma_yz below the
+    // segment, plane ke only sigma_zz above it (own cells, where inside the box)
     for (int m = kb - 1; m <= ke; ++m) plane(m);
     if (ENERGY) {  // per-block FP64 partials, summed in block order by the tail kernel
         __shared__ double red[2][NT / 64];
