@@ -63,6 +63,33 @@ __device__ __forceinline__ R dt_over(R dt, R r)
         return dt / r;
 }
 
+// Cell energy terms of the total-energy sum, in the arithmetic type R (FP32
+// storage: FP32, about 1e-7 relative, below the state's own rounding; the sums
+// are FP64).  Kinetic 0.5 rho |v|^2.  Strain energy
+//   0.5 (s:s / (2 mu) - lambda tr(s)^2 / (2 mu (3 lambda + 2 mu)))
+//     = s:s / (4 mu) - tr(s)^2 lambda / (4 mu q),   q = 3 lambda + 2 mu,
+// with one division, c = 1 / (4 mu q), and every stress scaled by a coefficient
+// before it is squared, so no intermediate exceeds the squared stresses times
+// 1/(4 mu): FP32 holds the terms for stresses up to about 1e24 Pa (the unscaled
+// products s:s q overflow near 3e13 Pa, which the reference's own sigma source
+// parameters pass within a few dozen steps).
+template <typename R>
+__device__ __forceinline__ R kinetic_energy(R rho, R vx, R vy, R vz)
+{
+    return ((R)0.5 * rho * vx) * vx + ((R)0.5 * rho * vy) * vy + ((R)0.5 * rho * vz) * vz;
+}
+template <typename R>
+__device__ __forceinline__ R strain_energy(R rho, R vp, R vs, R sxx, R syy, R szz, R sxy, R sxz, R syz)
+{
+    const R mu = rho * vs * vs, lam = rho * (vp * vp - (R)2 * vs * vs);
+    const R q = (R)3 * lam + (R)2 * mu;
+    const R c = (R)1 / ((R)4 * mu * q);
+    const R a = q * c, a2 = (R)2 * a, b = lam * c;
+    const R tr = sxx + syy + szz;
+    return (a * sxx) * sxx + (a * syy) * syy + (a * szz) * szz +
+           (a2 * sxy) * sxy + (a2 * sxz) * sxz + (a2 * syz) * syz - (b * tr) * tr;
+}
+
 // Streaming (nontemporal) store of element o of p.  For the nine wave fields: a
 // step writes each element once and only a later launch reads it, so the lines
 // need not stay in L2 and leave it to the neighbour and halo re-reads of the

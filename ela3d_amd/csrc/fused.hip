@@ -404,19 +404,8 @@ k_fused(const StepArgs<T> A)
             if (iN || face) stn(fw[2], oo, nvz);
             if (ENERGY) {
                 if (in_e_xy && k >= A.ek0 && k < A.ek1) {
-                    // kinetic 0.5 rho |v|^2 and strain energy
-                    // 0.5 (s:s / (2 mu) - lambda tr(s)^2 / (2 mu (3 lambda + 2 mu)))
-                    //   = (s:s (3 lambda + 2 mu) - lambda tr(s)^2) / (4 mu (3 lambda + 2 mu)),
-                    // one division per cell.  Cell terms in the arithmetic type (FP32
-                    // storage: FP32, about 1e-7 relative, below the state's own
-                    // rounding), sums in FP64.
-                    const R r = S.rho, vp = S.cp, vs = S.cs;
-                    const R mu = r * vs * vs, lam = r * (vp * vp - (R)2 * vs * vs);
-                    const R tr = sxx + syy + S.szz;
-                    const R ss = sxx * sxx + syy * syy + S.szz * S.szz + (R)2 * (sxy * sxy + sxz * sxz + syz * syz);
-                    const R q = (R)3 * lam + (R)2 * mu;
-                    ek += (R)0.5 * r * (nvx * nvx + nvy * nvy + nvz * nvz);
-                    ep += (ss * q - lam * tr * tr) / ((R)4 * mu * q);
+                    ek += kinetic_energy(S.rho, nvx, nvy, nvz);
+                    ep += strain_energy(S.rho, S.cp, S.cs, sxx, syy, S.szz, sxy, sxz, syz);
                 }
             }
             pv_sxz = sxz;

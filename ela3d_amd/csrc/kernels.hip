@@ -769,18 +769,10 @@ __device__ __forceinline__ void velocity_update(const StepArgs<T>& A, const Lane
             const bool in_e = L.act[e] && i >= em && i < nx - 1 - em && j >= em && j < ny - 1 - em && k >= A.ek0 &&
                               k < A.ek1;
             if (in_e) {
-                // state after this velocity update: cell-local values, FP64 terms
-                const double r = Q.rho_c[e], vp = P.cp[e], vs = P.cs[e];
-                const double mu = r * vs * vs, lam = r * (vp * vp - 2.0 * vs * vs);
-                const double sxx = P.sxx[e], syy = P.syy[e], szz = Q.szz_c[e];
-                const double sxy = P.sxy[e], sxz = P.sxz[e], syz = P.syz[e];
-                const double tr = sxx + syy + szz;
-                const double ss = sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + sxz * sxz + syz * syz);
-                const double dvx = nvx[e], dvy = nvy[e], dvz = nvz[e];
-                const double tk = 0.5 * r * (dvx * dvx + dvy * dvy + dvz * dvz);
-                const double tp = 0.5 * (ss / (2.0 * mu) - lam * tr * tr / (2.0 * mu * (3.0 * lam + 2.0 * mu)));
-                ek += tk;
-                ep += tp;
+                // state after this velocity update, cell-local: kinetic and strain energy
+                ek += (double)kinetic_energy(Q.rho_c[e], nvx[e], nvy[e], nvz[e]);
+                ep += (double)strain_energy(Q.rho_c[e], P.cp[e], P.cs[e], P.sxx[e], P.syy[e], Q.szz_c[e],
+                                            P.sxy[e], P.sxz[e], P.syz[e]);
             }
         }
     }
