@@ -98,7 +98,7 @@ PMC_FILES = {"classic": "pmc_classic_101x641x640_fp32.json"}
 
 
 # the launch-shape fields a kernel's traffic depends on
-SHAPE_KEYS = {"fused": ("fused", "tile", "zseg"), "interior": ("block", "zseg", "vec", "merged"),
+SHAPE_KEYS = {"fused": ("fused", "tile", "zseg", "strips"), "interior": ("block", "zseg", "vec", "merged"),
               "cpml": ("block_slab", "zseg_slab", "vec_slab", "merged")}
 
 
@@ -382,6 +382,9 @@ def main():
     ap.add_argument("--fused-tile", type=int, default=0,
                     help="with --shape: run the fused interior step with this many cells per workgroup "
                          "(its z-segment is the shape's interior zseg)")
+    ap.add_argument("--fused-strips", type=int, default=0,
+                    help="with --shape and --fused-tile: xy strips of the fused step's workgroup order "
+                         "(0: the default)")
     ap.add_argument("--profile-every", type=int, default=10,
                     help="within the K timed steps, every N-th step runs eagerly with hipEvents around each "
                          "launch (the per-kernel roofline); the others replay the step graphs (default 10; "
@@ -456,7 +459,7 @@ def main():
         b, zi, pp, bs, zs, ps, ov, mg = v
         ela3d_amd.set_tuning(cfg, block=b, zseg_interior=zi, vec=pp, block_slab=bs, zseg_slab=zs,
                              vec_slab=ps, overlap=bool(ov), merged=bool(mg), fused=args.fused_tile > 0,
-                             tile=args.fused_tile if args.fused_tile > 0 else 1024)
+                             tile=args.fused_tile if args.fused_tile > 0 else 1024, strips=args.fused_strips)
         log(f"[bench] fixed launch shape interior block={b} zseg={zi} vec={pp}; "
             f"layers block={bs or b} zseg={zs} vec={ps}")
     elif not args.no_tune:
@@ -465,7 +468,7 @@ def main():
         log(f"[bench] autotune: interior block={tuned['block']} zseg={tuned['zseg_interior']} "
             f"vec={tuned['vec']}; layers block={tuned['block_slab']} zseg={tuned['zseg_slab']} "
             f"vec={tuned['vec_slab']} overlap={tuned['overlap']} merged={tuned['merged']} "
-            f"fused={tuned.get('fused', 0)} tile={tuned.get('tile', 0)} "
+            f"fused={tuned.get('fused', 0)} tile={tuned.get('tile', 0)} strips={tuned.get('strips', 0)} "
             f"({tuned['ms_per_step']:.3f} ms/step over "
             f"{tuned['candidates']} shapes, {time.perf_counter() - t:.1f} s)")
     with Shot(cfg, *model) as shot:
@@ -506,7 +509,8 @@ def main():
     step_bw = info["alg_bytes_per_step"] / (ms_per_step * 1e-3)
     shape_now = {"block": info["block"], "zseg": info["zseg"], "vec": info["vec"],
                  "block_slab": info["block_slab"], "zseg_slab": info["zseg_slab"], "vec_slab": info["vec_slab"],
-                 "overlap": info["overlap"], "merged": info["merged"], "fused": info["fused"], "tile": info["tile"]}
+                 "overlap": info["overlap"], "merged": info["merged"], "fused": info["fused"], "tile": info["tile"],
+                 "strips": info["strips"]}
     traffic, traffic_shape = load_traffic(args.workload, dom["name"], shape_now)
     log(f"[bench] step: {ms_per_step:.4f} ms, {mcells:.1f} Mcell-updates/s, algorithmic "
         f"{info['alg_bytes_per_step'] / 1e6:.1f} MB/step -> {step_bw / 1e9:.1f} GB/s "

@@ -425,7 +425,7 @@ def autotune(cfg: Config, model, steps: int = 8) -> dict:
 
 def set_tuning(cfg: Config, block: int, zseg_interior: int, zseg_slab: int, vec: int = 1,
                block_slab: int = 0, vec_slab: int = 1, overlap: bool = False, merged: bool = False,
-               dbuf: bool = False, fused: bool = False, tile: int = 1024):
+               dbuf: bool = False, fused: bool = False, tile: int = 1024, strips: int = 0):
     """Pin the launch shape later shots of this grid use (as if the tuner had chosen it).
 
     ``block``/``zseg_interior``/``vec`` shape the interior launches,
@@ -435,7 +435,8 @@ def set_tuning(cfg: Config, block: int, zseg_interior: int, zseg_slab: int, vec:
     interior ones; ``dbuf`` keeps two field sets (each step reads one, writes the
     other; FP16 storage always updates in place and ignores it); ``fused`` (implies
     ``dbuf``) runs the fused interior step with ``tile``
-    cells per workgroup and ``zseg_interior`` planes per segment.  A shape a grid
+    cells per workgroup and ``zseg_interior`` planes per segment, its workgroups
+    walking ``strips`` xy strips of tiles (0: the default).  A shape a grid
     cannot run (a fused tile that does not fit, FP16 storage with ``fused``) falls
     back to the two-pass schedule when the shot is created."""
     from ._lib import TuningStruct
@@ -443,7 +444,8 @@ def set_tuning(cfg: Config, block: int, zseg_interior: int, zseg_slab: int, vec:
     t = TuningStruct(block=block, zseg_interior=zseg_interior, zseg_slab=zseg_slab, vec=vec,
                      candidates=0, ms_per_step=0.0, block_slab=block_slab, vec_slab=vec_slab,
                      overlap=1 if overlap else 0, merged=1 if merged else 0,
-                     dbuf=1 if (dbuf or fused) else 0, fused=1 if fused else 0, tile=int(tile))
+                     dbuf=1 if (dbuf or fused) else 0, fused=1 if fused else 0, tile=int(tile),
+                     strips=int(strips))
     check(lib().ela3d_tuning_set(C.byref(s), C.byref(t)))
 
 

@@ -48,6 +48,7 @@ class InfoStruct(C.Structure):
         ("region_lo", C.c_int32 * 3), ("region_hi", C.c_int32 * 3),
         ("block_slab", C.c_int32), ("vec_slab", C.c_int32), ("overlap", C.c_int32),
         ("merged", C.c_int32), ("dbuf", C.c_int32), ("fused", C.c_int32), ("tile", C.c_int32),
+        ("strips", C.c_int32),
     ]
 
 
@@ -60,7 +61,8 @@ class TuningStruct(C.Structure):
     _fields_ = [("block", C.c_int32), ("zseg_interior", C.c_int32), ("zseg_slab", C.c_int32),
                 ("vec", C.c_int32), ("candidates", C.c_int32), ("ms_per_step", C.c_double),
                 ("block_slab", C.c_int32), ("vec_slab", C.c_int32), ("overlap", C.c_int32),
-                ("merged", C.c_int32), ("dbuf", C.c_int32), ("fused", C.c_int32), ("tile", C.c_int32)]
+                ("merged", C.c_int32), ("dbuf", C.c_int32), ("fused", C.c_int32), ("tile", C.c_int32),
+                ("strips", C.c_int32)]
 
 
 class LayeredSpec(C.Structure):

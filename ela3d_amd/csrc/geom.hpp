@@ -31,8 +31,9 @@ __host__ __device__ inline Box pick_box(const Box* bx, int nbox, int lb)
 }
 
 // Tile (xy chunk, z segment) of a box's local block index, strip-major: the
-// chunks are cut into up to 8 strips of at least 8 chunks, and each strip's
-// chunks are enumerated for every z segment before the next strip starts.
+// chunks are cut into B.nstrip strips (0: up to 8 strips of at least 8 chunks),
+// and each strip's chunks are enumerated for every z segment before the next
+// strip starts.
 // With the contiguous per-XCD block ranges of xcd_block, each XCD then marches
 // one xy strip through z (a narrow box, such as a y absorbing layer, is one
 // strip: each XCD takes a run of its whole planes), so the plane above a z
@@ -42,8 +43,12 @@ __host__ __device__ inline Box pick_box(const Box* bx, int nbox, int lb)
 __host__ __device__ inline void tile_of(const Box& B, int local, int& chunk, int& zs)
 {
     const int nzs = (B.k1 - B.k0 + B.zseg - 1) / B.zseg;
-    int ns = B.nchunk / 8;
-    ns = ns < 1 ? 1 : (ns > 8 ? 8 : ns);
+    int ns = B.nstrip;
+    if (ns <= 0) {
+        ns = B.nchunk / 8;
+        ns = ns > 8 ? 8 : ns;
+    }
+    ns = ns < 1 ? 1 : (ns > B.nchunk ? B.nchunk : ns);
     const int w = B.nchunk / ns, r = B.nchunk % ns;  // strips 0..r-1 hold w+1 chunks
     const int cp = local / nzs;
     const int s = cp < r * (w + 1) ? cp / (w + 1) : r + (cp - r * (w + 1)) / w;

@@ -34,6 +34,7 @@ struct Box {
     int zseg;    // planes per block
     int blk0;    // first logical block of this box
     int mode;    // kInterior or kSlab
+    int nstrip;  // xy strips of the tile order (0: up to 8 of at least 8 chunks, see tile_of)
 };
 
 // Fields are read through fi and written through fo, memory variables read

@@ -45,6 +45,7 @@ struct Tuning {
     int fused = 0;          // 1: one fused stress+velocity launch over the interior box
                             //    (needs dbuf; zseg_interior planes per workgroup)
     int tile = 1024;        // fused: interior cells of the flattened plane per workgroup
+    int strips = 0;         // fused: xy strips of the tile order (0: the default of tile_of)
 };
 
 // Device layout of the 9 fields and the 3 model arrays: planes of plane_stride

@@ -277,6 +277,7 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
             out->dbuf = best.dbuf;
             out->fused = best.fused;
             out->tile = best.tile;
+            out->strips = best.strips;
             out->ms_per_step = best_ms;
             out->candidates = tried;
         }
@@ -311,6 +312,8 @@ int ela3d_tuning_set(const ela3d_config* cfg, const ela3d_tuning* t)
     u.dbuf = t->dbuf ? 1 : 0;
     u.fused = t->fused ? 1 : 0;
     u.tile = t->tile > 0 ? t->tile : 1024;
+    if (t->strips < 0 || t->strips > 64) return ELA3D_E_INVALID;
+    u.strips = t->strips;
     if (u.fused) u.dbuf = 1;
     ela3d::store_tuning(*cfg, u, t->ms_per_step);
     return ELA3D_OK;
@@ -332,6 +335,7 @@ int ela3d_tuning_get(const ela3d_config* cfg, ela3d_tuning* t)
     t->dbuf = u.dbuf;
     t->fused = u.fused;
     t->tile = u.tile;
+    t->strips = u.strips;
     t->ms_per_step = 0.0;
     t->candidates = 0;
     return ELA3D_OK;

@@ -145,6 +145,7 @@ typedef struct ela3d_info {
     int32_t merged;                /* one mixed launch per pass */
     int32_t dbuf;                  /* two field sets (double buffered) */
     int32_t fused, tile;           /* fused interior schedule and its tile */
+    int32_t strips;                /* fused: xy strips of its workgroup order in effect (0: up to 8) */
 } ela3d_info;
 
 #define ELA3D_MAX_KERNELS 8
@@ -327,6 +328,10 @@ typedef struct ela3d_tuning {
                            * (FP32 and FP64 storage; FP16 storage always updates in place) */
     int32_t fused;        /* 1: one fused stress+velocity launch over the interior box (needs dbuf) */
     int32_t tile;         /* fused: interior cells per workgroup (multiple of 64, <= 1024) */
+    int32_t strips;       /* fused: xy strips the workgroup order walks, each through every z
+                           * segment before the next (0 = the default: 4 for tiles of up to 768
+                           * threads and at least 32 tiles per plane, else up to 8 strips of at
+                           * least 8 tiles) */
 } ela3d_tuning;
 int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, const float* rho,
                    int32_t steps, ela3d_tuning* out);

@@ -11,7 +11,7 @@
 // stress_body/velocity_body; fused.hip fload/qinit and the velocity update).
 //
 // usage: index_check nx ny nz npml storage compat block zseg vec block_slab
-//                    zseg_slab vec_slab merged fused tile [mutation [free_surface]]
+//                    zseg_slab vec_slab merged fused tile [mutation [free_surface [strips]]]
 // mutation 1: offsets formed from plane 0 instead of the allocation start (the
 // wrap of unsigned neighbour offsets at the first row); 2: field arrays one plane
 // short; 3: each fused tile one own cell short (a cell no tile owns).  Prints
@@ -113,9 +113,12 @@ void fused_launch(const ShotPlan& P, const LaunchGeom& G, const Limits& L, Count
     const Box B = G.box[0];
     const int w0 = P.ax[0].width();
     const int own_full = fused_own(G.tile, nx);
+    const int nzs = (B.k1 - B.k0 + B.zseg - 1) / B.zseg;
+    std::vector<int> tiles(size_t(B.nchunk) * size_t(nzs), 0);  // blocks per (chunk, z segment)
     for (int b = 0; b < G.nblocks; ++b) {
         int chunk, zs;
         tile_of(B, xcd_block(b, G.nblocks), chunk, zs);
+        if (chunk >= 0 && chunk < B.nchunk && zs >= 0 && zs < nzs) ++tiles[size_t(zs) * B.nchunk + chunk];
         int kb, ke;
         segment_planes(B, zs, kb, ke);
         const int q0 = B.p0 + chunk * own_full;
@@ -159,6 +162,17 @@ void fused_launch(const ShotPlan& P, const LaunchGeom& G, const Limits& L, Count
             }
         }
     }
+    // every (chunk, z segment) is one block's (tile_of is a bijection for the strip
+    // count in use)
+    if (G.nblocks != B.nchunk * nzs) {
+        std::printf("  %d blocks for %d tiles\n", G.nblocks, B.nchunk * nzs);
+        ++C.bad;
+    }
+    for (size_t q = 0; q < tiles.size(); ++q)
+        if (tiles[q] != 1) {
+            if (C.bad < 8) std::printf("  tile %zu taken by %d blocks\n", q, tiles[q]);
+            ++C.bad;
+        }
     // every cell of the box is owned by exactly one tile, and every LDS cell an own
     // cell's velocity update uses is written by a region thread of its tile
     std::vector<int> owner(size_t(B.p1 - B.p0), 0);
@@ -200,11 +214,11 @@ int main(int argc, char** argv)
 {
     if (argc < 16) {
         std::fprintf(stderr, "usage: index_check nx ny nz npml storage compat block zseg vec block_slab "
-                             "zseg_slab vec_slab merged fused tile [mutation [free_surface]]\n");
+                             "zseg_slab vec_slab merged fused tile [mutation [free_surface [strips]]]\n");
         return 2;
     }
-    int a[17] = {};
-    for (int q = 1; q < argc && q <= 17; ++q) a[q - 1] = std::atoi(argv[q]);
+    int a[18] = {};
+    for (int q = 1; q < argc && q <= 18; ++q) a[q - 1] = std::atoi(argv[q]);
     g_mutation = argc > 16 ? a[15] : 0;
     ela3d_config c;
     ela3d_config_default(&c);
@@ -217,6 +231,7 @@ int main(int argc, char** argv)
     t.block = a[6]; t.zseg_interior = a[7]; t.vec = a[8];
     t.block_slab = a[9]; t.zseg_slab = a[10]; t.vec_slab = a[11];
     t.merged = a[12]; t.fused = a[13]; t.tile = a[14];
+    t.strips = argc > 18 ? a[17] : 0;
     t.dbuf = t.fused;
     try {
         const ShotPlan P = make_plan(c);

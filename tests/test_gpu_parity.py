@@ -110,6 +110,7 @@ SHAPES = [  # interior launch, absorbing-layer launch, schedule
     dict(block=256, zseg_interior=8, vec=1, block_slab=256, zseg_slab=4, vec_slab=1, fused=True, tile=512),
     dict(block=256, zseg_interior=3, vec=1, block_slab=128, zseg_slab=2, vec_slab=2, fused=True, tile=256),
     dict(block=256, zseg_interior=32, vec=1, block_slab=256, zseg_slab=4, vec_slab=1, fused=True, tile=1024),
+    dict(block=256, zseg_interior=4, vec=1, block_slab=128, zseg_slab=2, vec_slab=1, fused=True, tile=256, strips=3),
 ]
 
 
@@ -133,6 +134,7 @@ def test_launch_shapes_agree(gpu, storage):
             assert inf["fused"] == int(sh.get("fused", False))
             assert inf["dbuf"] == int(sh.get("dbuf", False) or sh.get("fused", False))
             assert inf["merged"] == int(sh.get("merged", False))
+            assert inf["strips"] == (sh.get("strips", 0) if sh.get("fused") else 0)
             if not sh.get("fused"):
                 assert (inf["block"], inf["vec"], inf["vec_slab"]) == (sh["block"], sh["vec"], sh["vec_slab"])
             shot.run()

@@ -27,10 +27,10 @@ def checker(tmp_path_factory):
 
 
 def run(exe, grid, storage=0, compat=0, block=256, zseg=4, vec=1, block_slab=128, zseg_slab=2,
-        vec_slab=1, merged=0, fused=0, tile=0, mutation=0, fs=0):
+        vec_slab=1, merged=0, fused=0, tile=0, mutation=0, fs=0, strips=0):
     nx, ny, nz, npml = grid
     args = [nx, ny, nz, npml, storage, compat, block, zseg, vec, block_slab, zseg_slab, vec_slab, merged,
-            fused, tile, mutation, fs]
+            fused, tile, mutation, fs, strips]
     out = subprocess.run([exe, *map(str, args)], check=True, stdout=subprocess.PIPE, text=True).stdout
     last = out.strip().splitlines()[-1]
     if last.startswith("refused"):
@@ -60,9 +60,10 @@ def test_split_launch_shapes(checker, grid, fs):
 @pytest.mark.parametrize("fs", [0, 1])
 def test_fused_launch_shapes(checker, grid, fs):
     for storage in (0, 2):
-        for tile, zseg in ((256, 2), (512, 4), (1024, 8), (256, 32), (768, 1)):
-            r = run(checker, grid, storage, block=128, zseg=zseg, fused=1, tile=tile, fs=fs)
-            assert r is None or r[1] == 0, (storage, tile, zseg, r)
+        for tile, zseg, strips in ((256, 2, 0), (512, 4, 2), (1024, 8, 0), (256, 32, 3), (768, 1, 0),
+                                   (256, 3, 64)):
+            r = run(checker, grid, storage, block=128, zseg=zseg, fused=1, tile=tile, fs=fs, strips=strips)
+            assert r is None or r[1] == 0, (storage, tile, zseg, strips, r)
 
 
 def test_workload_grids(checker):
@@ -72,6 +73,8 @@ def test_workload_grids(checker):
         assert run(checker, grid, block=256, zseg=16, vec=2, merged=1)[1] == 0
     for tile, zseg in ((1024, 8), (1024, 16), (768, 32), (512, 4)):
         assert run(checker, classic, block=128, zseg=zseg, fused=1, tile=tile)[1] == 0
+    for strips in (1, 2, 3, 4, 6):  # the fused step's tile orders the tuner tries
+        assert run(checker, classic, block=128, zseg=16, fused=1, tile=768, strips=strips)[1] == 0
     assert run(checker, layered, block=128, zseg=16, fused=1, tile=1024)[1] == 0
 
 
