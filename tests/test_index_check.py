@@ -73,8 +73,10 @@ def test_workload_grids(checker):
         assert run(checker, grid, block=256, zseg=16, vec=2, merged=1)[1] == 0
     for tile, zseg in ((1024, 8), (1024, 16), (768, 32), (512, 4)):
         assert run(checker, classic, block=128, zseg=zseg, fused=1, tile=tile)[1] == 0
-    for strips in (1, 2, 3, 4, 6):  # the fused step's tile orders the tuner tries
+    for strips in (1, 2, 3, 4, 6):  # tile orders of the fused step
         assert run(checker, classic, block=128, zseg=16, fused=1, tile=768, strips=strips)[1] == 0
+    for zseg in (12, 24):  # the tuner's other fused segment lengths, in the default order
+        assert run(checker, classic, block=128, zseg=zseg, fused=1, tile=768)[1] == 0
     assert run(checker, layered, block=128, zseg=16, fused=1, tile=1024)[1] == 0
 
 
