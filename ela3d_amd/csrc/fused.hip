@@ -224,7 +224,6 @@ k_fused(const StepArgs<T> A)
     const int j = p / nx, i = p - j * nx;
     const bool iN = i <= nx - 2, iS = i >= 1;
     const int cx = layer_index(i, A.lo[0], A.hi[0]);
-    const bool pml = A.w[0] > 0;
     const bool inx = cx >= 0;
     R* xtab = lds + 2 * 5 * LR;  // the x tables, after the two plane buffers
     for (int q = t; q < 6 * nx; q += blockDim.x) {

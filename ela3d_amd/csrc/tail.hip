@@ -163,7 +163,7 @@ __device__ __forceinline__ void tail_body(const TailArgs<T>& A)
             A.srctrace[3 * it + 1] = syy;
             A.srctrace[3 * it + 2] = szz;
         }
-        const bool fin = isfinite(sxx) & isfinite(syy) & isfinite(szz) & isfinite(vx) & isfinite(vy) & isfinite(vz);
+        const bool fin = isfinite(sxx) && isfinite(syy) && isfinite(szz) && isfinite(vx) && isfinite(vy) && isfinite(vz);
         if (bad || !fin) *A.flag = 1;
         *A.step = it;
     }
