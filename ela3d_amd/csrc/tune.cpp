@@ -257,12 +257,16 @@ int ela3d_autotune(const ela3d_config* cfg, const float* cp, const float* cs, co
                     fc.emplace_back(w, t);
                 }
             }
-            // the three fastest again, three rounds in turn, each kept at its best
-            // round: their differences (under 1 %) are below one short timing's noise
+            // the three fastest again, three rounds in turn with the two-pass pick,
+            // each kept at its best round: their differences (under 1 %) are below
+            // one short timing's noise
             std::sort(fc.begin(), fc.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
             if (fc.size() > 3) fc.resize(3);
-            for (int r = 0; r < 3 && fc.size() > 1; ++r)
+            const Tuning split = best;
+            for (int r = 0; r < 3 && !fc.empty(); ++r) {
+                best_wall = std::min(best_wall, wall_of(split));
                 for (auto& c : fc) c.first = std::min(c.first, wall_of(c.second));
+            }
             Tuning fbest = best;
             double fwall = 1e300;
             for (const auto& c : fc) {
