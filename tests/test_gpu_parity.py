@@ -154,6 +154,29 @@ def test_launch_shapes_agree(gpu, storage):
     set_tuning(cfg, **SHAPES[0])
 
 
+@pytest.mark.parametrize("storage,tile", [(STORAGE_FP32, 768), (STORAGE_FP64, 512)])
+def test_fused_strip_order_default(gpu, storage, tile):
+    """A grid with enough tiles per plane (at least 32) that the fused step walks its
+    default 4-strip order: the same cells as the two-pass schedule, to rounding."""
+    from ela3d_amd import set_tuning
+    cfg = small_cfg(nx=48, ny=600, nz=30, nstep=30, storage=storage)
+    model = hetero_model(cfg, seed=8)
+    out = []
+    for fused in (False, True):
+        set_tuning(cfg, block=128, zseg_interior=8, vec=1, block_slab=128, zseg_slab=2, vec_slab=1, fused=fused,
+                   tile=tile)
+        with Shot(cfg, *model) as shot:
+            inf = shot.info()
+            assert inf["fused"] == int(fused) and inf["strips"] == (4 if fused else 0)
+            shot.run()
+            out.append((shot.seismograms(), shot.snapshot(0), shot.snapshot(5), shot.snapshot(8)))
+    set_tuning(cfg, block=128, zseg_interior=8, vec=1, block_slab=128, zseg_slab=2, vec_slab=1)
+    tol = 1e-6 if storage == STORAGE_FP32 else 1e-13
+    assert np.abs(out[0][0]).max() > 0
+    for x, y in zip(out[0], out[1]):
+        assert rel_l2(y, x) <= tol
+
+
 @pytest.mark.parametrize("storage", [STORAGE_FP32, STORAGE_FP64])
 @pytest.mark.parametrize("tile,zseg", [(512, 4), (256, 7), (1024, 1)])
 def test_fused_one_step_random_state(gpu, storage, tile, zseg):
