@@ -49,8 +49,12 @@ def draw(seed):
                      block_slab=pick([64, 128, 256]), zseg_slab=pick([1, 2, 4]), vec_slab=pick([1, 2]),
                      overlap=bool(rng.random() < 0.3), merged=bool(rng.random() < 0.25),
                      dbuf=bool(rng.random() < 0.3))
-        if rng.random() < 0.2 and cfg.storage == STORAGE_FP32 and npml > 0:
-            shape.update(fused=True, tile=pick([256, 512, 1024]), dbuf=True, merged=False)
+        if rng.random() < 0.2 and cfg.storage in (STORAGE_FP32, STORAGE_FP64) and npml > 0:
+            # FP64 tiles hold at most 512 cells; the tile order's strip count from a
+            # generator of its own (0: the default order)
+            tiles = [256, 512, 1024] if cfg.storage == STORAGE_FP32 else [256, 512, 512]
+            strips = int(np.random.default_rng(9000 + seed).integers(0, 5))
+            shape.update(fused=True, tile=pick(tiles), dbuf=True, merged=False, strips=strips)
         return cfg, shape
 
 
