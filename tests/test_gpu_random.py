@@ -16,7 +16,7 @@ from ela3d_amd import (COMPAT_GHOST, COMPAT_REFERENCE, COMPAT_XHALF_A, COMPAT_ZH
 from helpers import compare_state, hetero_model, max_rel, push_state, random_state, small_cfg
 from test_index_bounds import fused_violations, violations
 
-NCASES = 24
+NCASES = 48
 TOL = {STORAGE_FP32: 2e-6, STORAGE_FP64: 1e-12}
 DT = {STORAGE_FP32: np.float32, STORAGE_FP64: np.float64}
 
