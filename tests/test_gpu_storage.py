@@ -102,6 +102,29 @@ def test_layered_full_size_fp32_and_fp16_vs_gpu_fp64(gpu):
         assert err <= bar and eerr <= bar
 
 
+def test_classic_shape_fused_equals_two_pass_fp64(gpu):
+    """At the full 101 x 641 x 640 size, FP64 storage, 300 steps: the fused step (512-cell
+    tiles in its default 4-strip order) and the two-pass schedule agree to FP64 rounding."""
+    from ela3d_amd import set_tuning
+    cfg, model = workloads.classic(nstep=300, storage=STORAGE_FP64)
+    out = []
+    try:
+        for fused in (False, True):
+            set_tuning(cfg, block=128, zseg_interior=8, vec=1, block_slab=256, zseg_slab=2, vec_slab=1,
+                       fused=fused, tile=512)
+            with Shot(cfg, *model) as shot:
+                inf = shot.info()
+                assert inf["fused"] == int(fused) and inf["strips"] == (4 if fused else 0)
+                shot.run()
+                out.append((shot.seismograms(), shot.energy(), shot.snapshot(2), shot.snapshot(7)))
+    finally:
+        set_tuning(cfg, block=128, zseg_interior=8, vec=1, block_slab=256, zseg_slab=2, vec_slab=1)
+    # (the wave reaches the receivers after about 590 steps: the seismograms are still 0)
+    assert np.abs(out[0][1]).max() > 0 and np.abs(out[0][2]).max() > 0 and np.abs(out[0][3]).max() > 0
+    for x, y in zip(out[0], out[1]):
+        assert rel_l2(y, x) <= 1e-12
+
+
 @pytest.mark.parametrize("storage", [STORAGE_FP32, STORAGE_FP64])
 def test_classic_shape_exact_linearity(gpu, storage):
     """A size-independent property at the full 101 x 641 x 640 size: the loop is linear
