@@ -100,3 +100,23 @@ def test_layered_model_seeded_and_gardner():
     first_c = np.argmax(col_c > 2000 * 1.04)
     first_e = np.argmax(col_e > 2000 * 1.04)
     assert first_c < first_e
+
+
+def test_tuning_set_and_get_round_trip():
+    """A pinned launch shape is kept per grid and read back field for field (no GPU
+    needed); shapes outside the documented ranges are refused."""
+    from ela3d_amd._lib import TuningStruct
+    cfg = Config(nx=37, ny=41, nz=23)
+    ela3d_amd.set_tuning(cfg, block=128, zseg_interior=12, vec=2, block_slab=64, zseg_slab=2, vec_slab=1,
+                         fused=True, tile=512, strips=3)
+    s, keep = cfg.to_struct()
+    t = TuningStruct()
+    assert lib().ela3d_tuning_get(C.byref(s), C.byref(t)) == 0
+    assert (t.block, t.zseg_interior, t.vec, t.block_slab, t.zseg_slab, t.vec_slab) == (128, 12, 2, 64, 2, 1)
+    assert (t.fused, t.dbuf, t.tile, t.strips) == (1, 1, 512, 3)
+    for bad in (dict(strips=-1), dict(strips=65), dict(block=96), dict(vec=3)):
+        kw = dict(block=128, zseg_interior=12, vec=2, block_slab=64, zseg_slab=2, vec_slab=1, fused=True,
+                  tile=512, strips=3)
+        kw.update(bad)
+        with pytest.raises(Ela3dError):
+            ela3d_amd.set_tuning(cfg, **kw)
